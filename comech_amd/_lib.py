@@ -1,0 +1,82 @@
+"""ctypes binding of libcomech_hip.so (declared in include/comech_fe.h).
+
+There is no CPU fallback: if the library is missing or fails to load, importing the solver raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcomech_hip.so")
+
+c_int, c_ll, c_double, c_void_p = ctypes.c_int, ctypes.c_longlong, ctypes.c_double, ctypes.c_void_p
+P_int, P_ll, P_double, P_u8 = (ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_longlong),
+                               ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_uint8))
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iters", c_int), ("flag", c_int), ("outer", c_int), ("levels", c_int), ("relres", c_double),
+                ("t_solve_ms", c_double), ("t_setup_ms", c_double)]
+
+
+# name -> argtypes (all return int status unless listed in _RET)
+SIGNATURES = {
+    "cm_last_error": [],
+    "cm_version": [],
+    "cm_device_count": [P_int],
+    "cm_sys_create": [ctypes.POINTER(c_void_p), c_void_p, c_int, c_int, c_int, P_double, P_double, P_double, c_int],
+    "cm_sys_destroy": [c_void_p],
+    "cm_sys_set_labels": [c_void_p, c_int, P_double, P_double, P_u8],
+    "cm_sys_set_cellmat": [c_void_p, P_double, P_double, P_u8],
+    "cm_sys_set_split": [c_void_p, c_ll, P_ll, c_ll, P_ll, P_ll],
+    "cm_sys_set_extra": [c_void_p, c_ll, P_ll, P_ll, P_double],
+    "cm_sys_set_used": [c_void_p, c_ll, P_u8],
+    "cm_sys_set_rigid": [c_void_p, c_int, c_ll, P_ll, P_ll, P_double, P_double],
+    "cm_sys_build": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double, c_int, c_int],
+    "cm_sys_levels": [c_void_p, P_int, P_ll, c_int],
+    "cm_sys_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p],
+    "cm_sys_vcycle": [c_void_p, c_int, c_void_p, c_void_p],
+    "cm_sys_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_int, c_double, ctypes.POINTER(SolveInfo)],
+    "cm_cell_stress": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_void_p,
+                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "cm_element_forces": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_ll, c_void_p,
+                          c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "cm_sys_time_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, P_double],
+    "cm_sys_set_kz": [c_void_p, c_int],
+    "cm_sys_sync": [c_void_p],
+    "cm_sample_fe": [c_void_p] + [c_int] * 7 + [c_void_p] * 9 + [c_void_p] * 5 + [c_void_p] * 4,
+}
+_RET = {"cm_last_error": ctypes.c_char_p, "cm_version": c_int}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the library (once).  Raises OSError with a clear message when it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError(f"{path} is missing: build it with `python -m comech_amd.build` (hipcc, gfx950); "
+                      "the solver has no CPU fallback")
+    lib = ctypes.CDLL(path)
+    for name, args in SIGNATURES.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = _RET.get(name, c_int)
+    _lib = lib
+    return lib
+
+
+class CmError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc != 0:
+        raise CmError(load().cm_last_error().decode())
+    return rc
+
+
+def ptr(a, ctype):
+    """Host numpy array -> ctypes pointer (array must stay alive during the call)."""
+    return a.ctypes.data_as(ctypes.POINTER(ctype))

@@ -1,0 +1,1325 @@
+// Voxel FE system on one GPU: geometric-multigrid hierarchy (matrix-free fine level, exact
+// Galerkin moment cells on coarse levels, dense FP64 coarsest inverse), V-cycle, PCG drivers and
+// the extern "C" boundary declared in include/comech_fe.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <unordered_map>
+
+#include "../../include/comech_fe.h"
+#include "kernels_fe.hpp"
+#include "kernels_post.hpp"
+#include "kernels_setup.hpp"
+#include "kernels_vec.hpp"
+
+namespace cm {
+
+static thread_local std::string g_last_error;
+
+// ---------------------------------------------------------------------------------------------
+// Per-level, per-precision device data
+template <typename T>
+struct LevelT {
+  DevBuf<T> hx, hy, hz;
+  DevBuf<T> tabL, tabM;     // level 0, label mode (256 entries)
+  DevBuf<T> cellL, cellM;   // level 0, value mode (x-fastest, irregular cells zeroed)
+  DevBuf<T> irrL, irrM;     // level 0 irregular cells
+  DevBuf<T> mom;            // coarse levels: [54][ncell]
+  DevBuf<T> Dinv;           // [nn][9]
+  DevBuf<T> pw[3];          // prolongation weights, this level as the fine side
+  DevBuf<T> rw[3];          // restriction weights of the next level's coarse nodes
+  DevBuf<T> xval;           // extra stiffness values
+  DevBuf<T> x, b, r, t, d;  // work vectors
+  double lmax = 0.0, lmin = 0.0;
+  bool ready = false;
+};
+
+struct Level {
+  int nx = 0, ny = 0, nz = 0;
+  long long nng = 0, nn = 0, ncell = 0;
+  std::vector<double> e[3];
+  std::vector<uint8_t> used_h;
+  DevBuf<uint8_t> used;
+  // transfer to the next level
+  std::vector<int> pia_h[3], rs_h[3];
+  std::vector<double> pw_h[3], rw_h[3];
+  DevBuf<int> pia[3], rs[3];
+  // nesting of this level's cells in the next level's cells (for moments)
+  std::vector<int> ncs_h[3], ncn_h[3];     // stored on the coarse level: per coarse cell
+  std::vector<double> na_h[3], nb_h[3];    // stored on the coarse level: per fine cell
+  DevBuf<int> ncs[3], ncn[3];
+  DevBuf<double> na[3], nb[3];
+  // extra stiffness (CSR over unique rows, full DOF index of this level)
+  std::vector<int> xrow_h, xptr_h, xcol_h;
+  std::vector<double> xval_h;
+  DevBuf<int> xrow, xptr, xcol;
+  // level 0 only
+  bool label_mode = false;
+  DevBuf<uint8_t> lab;                      // apply labels (x-fastest, irregular -> 255)
+  std::vector<double> tabL_h, tabM_h;       // 256
+  DevBuf<double> matL, matM;                // real per-cell materials, x-fastest (setup)
+  DevBuf<double> applyL, applyM;            // per-cell apply materials (value mode / diag setup)
+  int ncopy = 0, nirr = 0;
+  std::vector<int> copy_orig_h;
+  DevBuf<int> copy_orig, irr_nodes, irr_ijk;
+  std::vector<double> irrL_h, irrM_h;
+  // coarse-level moments (FP64 master)
+  DevBuf<double> mom64;
+  LevelT<double> d64;
+  LevelT<float> d32;
+  template <typename T>
+  LevelT<T>& dat();
+};
+template <>
+LevelT<double>& Level::dat<double>() { return d64; }
+template <>
+LevelT<float>& Level::dat<float>() { return d32; }
+
+struct Options {
+  int smoother = 0;   // 0 chebyshev (fe/gmg.py:61-76), 1 damped block Jacobi
+  int degree = 3;
+  double omega = 0.6;
+  int npre = 2, npost = 2;
+};
+
+template <typename T>
+static void upload_cast(DevBuf<T>& dst, const std::vector<double>& src, hipStream_t s) {
+  std::vector<T> tmp(src.begin(), src.end());
+  dst.upload(tmp, s);
+}
+
+static inline dim3 grid1(long long n, int bs = 256) { return dim3((unsigned)div_up(n, bs)); }
+
+struct System {
+  hipStream_t stream = nullptr;
+  int sri = 1;
+  std::vector<std::unique_ptr<Level>> L;
+  Options opt;
+  bool built = false;
+  // PCG state
+  DevBuf<double> S, partial;
+  DevBuf<double> r64, z64, p64, Ap64;
+  DevBuf<float> r32, z32, p32, Ap32, d32;
+  // rigid-mode correction
+  int zcol = 0;
+  DevBuf<int> zcptr, zrows_c, zrow, zptr, zcols;
+  DevBuf<double> zvals_c, zvals, Einv, zw, zv;
+  // coarsest dense inverse
+  DevBuf<double> Ainv;
+  int nc_dense = 0;
+  double t_setup_ms = 0.0;
+  // staged level-0 input (converted in prepare_level0)
+  int st_nmat = -1;
+  std::vector<double> st_tabL, st_tabM, st_lam, st_mu;
+  std::vector<uint8_t> st_lab, st_active;
+  std::vector<long long> st_copy_orig, st_irr_cell, st_irr_nodes;
+  std::vector<long long> st_xr, st_xc;
+  std::vector<double> st_xv;
+  std::vector<uint8_t> st_used_extra;
+
+  // Level-0 data in the layouts the kernels use.
+  void prepare_level0() {
+    Level& v = lv(0);
+    const int nx = v.nx, ny = v.ny, nz = v.nz;
+    const long long ncell = v.ncell;
+    CM_CHECK(st_nmat >= 0 || !st_lam.empty(), "materials not set (cm_sys_set_labels or cm_sys_set_cellmat)");
+    v.label_mode = st_nmat >= 0;
+    std::vector<double> mL(ncell, 0.0), mM(ncell, 0.0);
+    std::vector<uint8_t> act(ncell, 0), labx;
+    if (v.label_mode) {
+      labx.assign(ncell, LAB_NONE);
+      v.tabL_h.assign(256, 0.0);
+      v.tabM_h.assign(256, 0.0);
+      for (int m = 0; m < st_nmat; ++m) { v.tabL_h[m] = st_tabL[m]; v.tabM_h[m] = st_tabM[m]; }
+    }
+    auto xf = [&](long long c) {
+      const long long k = c % nz, j = (c / nz) % ny, i = c / ((long long)ny * nz);
+      return i + (long long)nx * (j + (long long)ny * k);
+    };
+    for (long long c = 0; c < ncell; ++c) {
+      const long long x = xf(c);
+      if (v.label_mode) {
+        const uint8_t l = st_lab[c];
+        if (l == LAB_NONE) continue;
+        CM_CHECK((int)l < st_nmat, "label outside the material table");
+        act[x] = 1; labx[x] = l; mL[x] = st_tabL[l]; mM[x] = st_tabM[l];
+      } else {
+        if (!st_active[c]) continue;
+        act[x] = 1; mL[x] = st_lam[c]; mM[x] = st_mu[c];
+      }
+    }
+    // split copies and irregular cells
+    v.ncopy = (int)st_copy_orig.size();
+    v.nn = v.nng + v.ncopy;
+    v.copy_orig_h.resize(v.ncopy);
+    for (int q = 0; q < v.ncopy; ++q) {
+      CM_CHECK(st_copy_orig[q] >= 0 && st_copy_orig[q] < v.nng, "copy origin outside the grid");
+      v.copy_orig_h[q] = (int)st_copy_orig[q];
+    }
+    v.nirr = (int)st_irr_cell.size();
+    std::vector<int> ijk(3 * (size_t)v.nirr), nodes(8 * (size_t)v.nirr);
+    v.irrL_h.resize(v.nirr); v.irrM_h.resize(v.nirr);
+    std::vector<uint8_t> is_irr(ncell, 0);
+    for (int e = 0; e < v.nirr; ++e) {
+      const long long c = st_irr_cell[e];
+      CM_CHECK(c >= 0 && c < ncell, "irregular cell id out of range");
+      const long long x = xf(c);
+      CM_CHECK(act[x], "irregular cell is not active");
+      is_irr[x] = 1;
+      ijk[3 * e] = (int)(c / ((long long)ny * nz)); ijk[3 * e + 1] = (int)((c / nz) % ny); ijk[3 * e + 2] = (int)(c % nz);
+      v.irrL_h[e] = mL[x]; v.irrM_h[e] = mM[x];
+      for (int l = 0; l < 8; ++l) {
+        const long long nd = st_irr_nodes[8 * (size_t)e + l];
+        CM_CHECK(nd >= 0 && nd < v.nn, "irregular node id out of range");
+        nodes[8 * (size_t)e + l] = (int)nd;
+      }
+    }
+    // used nodes (fe/solver.py:141, rb/splitfe.py:73): corners of active cells
+    v.used_h.assign(v.nn, 0);
+    const long long nxn = nx + 1, nyn = ny + 1;
+    for (int k = 0; k < nz; ++k)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const long long x = i + (long long)nx * (j + (long long)ny * k);
+          if (!act[x] || is_irr[x]) continue;
+          for (int l = 0; l < 8; ++l)
+            v.used_h[(i + (l & 1)) + nxn * ((j + ((l >> 1) & 1)) + nyn * (k + (l >> 2)))] = 1;
+        }
+    for (size_t q = 0; q < nodes.size(); ++q) v.used_h[nodes[q]] = 1;
+    if (!st_used_extra.empty()) {
+      CM_CHECK((long long)st_used_extra.size() == v.nn, "used mask length must equal grid nodes + copies");
+      for (long long q = 0; q < v.nn; ++q) v.used_h[q] |= st_used_extra[q];
+    }
+    // apply arrays: irregular cells are handled by their own kernel
+    std::vector<double> aL(mL), aM(mM);
+    for (long long x = 0; x < ncell; ++x)
+      if (is_irr[x]) { aL[x] = 0.0; aM[x] = 0.0; if (v.label_mode) labx[x] = LAB_NONE; }
+    v.matL.upload(mL, stream); v.matM.upload(mM, stream);
+    v.applyL.upload(aL, stream); v.applyM.upload(aM, stream);
+    if (v.label_mode) v.lab.upload(labx, stream);
+    v.irr_nodes.upload(nodes, stream);
+    v.irr_ijk.upload(ijk, stream);
+    v.copy_orig.upload(v.copy_orig_h, stream);
+    // extra stiffness COO -> unique-row CSR (duplicates summed)
+    const long long nd3 = 3 * v.nn;
+    std::vector<std::pair<long long, double>> ent;
+    ent.reserve(st_xr.size());
+    for (size_t q = 0; q < st_xr.size(); ++q) {
+      CM_CHECK(st_xr[q] >= 0 && st_xr[q] < nd3 && st_xc[q] >= 0 && st_xc[q] < nd3, "extra entry out of range");
+      ent.emplace_back(st_xr[q] * nd3 + st_xc[q], st_xv[q]);
+    }
+    std::sort(ent.begin(), ent.end(), [](auto& a, auto& b) { return a.first < b.first; });
+    std::vector<std::pair<long long, double>> merged;
+    for (auto& e : ent) {
+      if (!merged.empty() && merged.back().first == e.first) merged.back().second += e.second;
+      else merged.push_back(e);
+    }
+    set_extra_sorted(v, merged, nd3);
+  }
+
+  Level& lv(int l) { return *L[l]; }
+  int nlev() const { return (int)L.size(); }
+
+  // ------------------------------------------------------------------ construction helpers
+  static void axis_transfer(const std::vector<double>& ef, Level& fine, Level& coarse, int ax) {
+    const int nfn = (int)ef.size();          // fine nodes
+    std::vector<int> cidx;
+    for (int i = 0; i < nfn; i += 2) cidx.push_back(i);
+    if (cidx.back() != nfn - 1) cidx.push_back(nfn - 1);
+    const int ncn = (int)cidx.size();
+    std::vector<double> ec(ncn);
+    for (int I = 0; I < ncn; ++I) ec[I] = ef[cidx[I]];
+    coarse.e[ax] = ec;
+    // prolongation (fe/gmg.py:7-26)
+    std::vector<int> ia(nfn);
+    std::vector<double> w(nfn, 0.0);
+    for (int i = 0, a = 0; i < nfn; ++i) {
+      while (a + 1 < ncn && cidx[a + 1] <= i) ++a;
+      ia[i] = a;
+      if (cidx[a] != i) w[i] = (ef[i] - ef[cidx[a]]) / (ef[cidx[a + 1]] - ef[cidx[a]]);
+    }
+    // restriction: coarse I gathers fine cidx[I]-1 .. cidx[I]+1
+    std::vector<int> rs(ncn);
+    std::vector<double> rw(3 * ncn, 0.0);
+    for (int I = 0; I < ncn; ++I) {
+      const int c = cidx[I];
+      rs[I] = c - 1;
+      rw[3 * I + 1] = 1.0;
+      if (I > 0 && c - 1 > cidx[I - 1]) rw[3 * I] = w[c - 1];
+      if (I + 1 < ncn && c + 1 < cidx[I + 1]) rw[3 * I + 2] = 1.0 - w[c + 1];
+    }
+    fine.pia_h[ax] = ia; fine.pw_h[ax] = w; fine.rs_h[ax] = rs; fine.rw_h[ax] = rw;
+    // nesting of fine cells in coarse cells
+    const int ncc = ncn - 1, nfc = nfn - 1;
+    std::vector<int> cs(ncc), cn(ncc);
+    std::vector<double> na(nfc), nb(nfc);
+    for (int C = 0; C < ncc; ++C) {
+      cs[C] = cidx[C];
+      cn[C] = cidx[C + 1] - cidx[C];
+      const double X0 = ec[C], X1 = ec[C + 1], H = X1 - X0;
+      for (int i = cidx[C]; i < cidx[C + 1]; ++i) {
+        na[i] = (ef[i + 1] - ef[i]) / H;
+        nb[i] = (ef[i] + ef[i + 1] - X0 - X1) / H;
+      }
+    }
+    coarse.ncs_h[ax] = cs; coarse.ncn_h[ax] = cn; coarse.na_h[ax] = na; coarse.nb_h[ax] = nb;
+  }
+
+  // Kept coarse nodes: coarse nodes with a nonzero prolongation weight from a used fine node
+  // (fe/gmg.py:94-96, keep = columns of P[act] with nonzero sum).
+  static void coarse_used(Level& f, Level& c) {
+    const int fx = f.nx + 1, fy = f.ny + 1, fz = f.nz + 1;
+    const int cx = c.nx + 1, cy = c.ny + 1, cz = c.nz + 1;
+    c.used_h.assign((size_t)cx * cy * cz, 0);
+    // fold copies onto their originals: a copy is interpolated like its original
+    std::vector<uint8_t> fu(f.used_h.begin(), f.used_h.begin() + f.nng);
+    for (long long q = 0; q < (long long)f.nn - f.nng; ++q)
+      if (f.used_h[f.nng + q]) fu[f.copy_orig_h[q]] = 1;
+    auto sup = [](const std::vector<int>& ia, const std::vector<double>& w, int i, int* out) {
+      int n = 0;
+      out[n++] = ia[i];
+      if (w[i] != 0.0) out[n++] = ia[i] + 1;
+      return n;
+    };
+    for (int k = 0; k < fz; ++k)
+      for (int j = 0; j < fy; ++j)
+        for (int i = 0; i < fx; ++i) {
+          if (!fu[(size_t)i + (size_t)fx * (j + (size_t)fy * k)]) continue;
+          int si[2], sj[2], sk[2];
+          const int ni = sup(f.pia_h[0], f.pw_h[0], i, si);
+          const int nj = sup(f.pia_h[1], f.pw_h[1], j, sj);
+          const int nk = sup(f.pia_h[2], f.pw_h[2], k, sk);
+          for (int c3 = 0; c3 < nk; ++c3)
+            for (int b = 0; b < nj; ++b)
+              for (int a = 0; a < ni; ++a) c.used_h[(size_t)si[a] + (size_t)cx * (sj[b] + (size_t)cy * sk[c3])] = 1;
+        }
+  }
+
+  // Galerkin P^T S P of the extra stiffness (entries with nonzero weights only)
+  static void coarse_extra(Level& f, Level& c) {
+    const int fx = f.nx + 1, fy = f.ny + 1;
+    const int cx = c.nx + 1, cy = c.ny + 1;
+    auto support = [&](int fnode, long long* nodes, double* w) {
+      int g = fnode;
+      if (g >= f.nng) g = f.copy_orig_h[g - f.nng];
+      const int i = g % fx, j = (g / fx) % fy, k = g / (fx * fy);
+      int n = 0;
+      for (int dz = 0; dz < 2; ++dz) {
+        const double wz = dz ? f.pw_h[2][k] : 1.0 - f.pw_h[2][k];
+        if (wz == 0.0) continue;
+        for (int dy = 0; dy < 2; ++dy) {
+          const double wy = dy ? f.pw_h[1][j] : 1.0 - f.pw_h[1][j];
+          if (wy == 0.0) continue;
+          for (int dx = 0; dx < 2; ++dx) {
+            const double wx = dx ? f.pw_h[0][i] : 1.0 - f.pw_h[0][i];
+            if (wx == 0.0) continue;
+            nodes[n] = (f.pia_h[0][i] + dx) + (long long)cx * ((f.pia_h[1][j] + dy) + (long long)cy * (f.pia_h[2][k] + dz));
+            w[n] = wx * wy * wz;
+            ++n;
+          }
+        }
+      }
+      return n;
+    };
+    std::unordered_map<long long, double> acc;
+    const long long ncd = 3LL * (long long)(c.nx + 1) * (c.ny + 1) * (c.nz + 1);
+    for (size_t r = 0; r < f.xrow_h.size(); ++r) {
+      const int row = f.xrow_h[r];
+      long long rn[8], cnod[8];
+      double rw[8], cw[8];
+      const int nr = support(row / 3, rn, rw);
+      for (int q = f.xptr_h[r]; q < f.xptr_h[r + 1]; ++q) {
+        const int col = f.xcol_h[q];
+        const int ncs = support(col / 3, cnod, cw);
+        const double v = f.xval_h[q];
+        for (int a = 0; a < nr; ++a)
+          for (int b = 0; b < ncs; ++b) {
+            const long long R = 3 * rn[a] + row % 3, C = 3 * cnod[b] + col % 3;
+            acc[R * ncd + C] += rw[a] * cw[b] * v;
+          }
+      }
+    }
+    std::vector<std::pair<long long, double>> ent(acc.begin(), acc.end());
+    std::sort(ent.begin(), ent.end());
+    set_extra_sorted(c, ent, ncd);
+  }
+
+  static void set_extra_sorted(Level& lv, const std::vector<std::pair<long long, double>>& ent, long long ncd) {
+    lv.xrow_h.clear(); lv.xptr_h.clear(); lv.xcol_h.clear(); lv.xval_h.clear();
+    long long last = -1;
+    for (auto& e : ent) {
+      const long long R = e.first / ncd, C = e.first % ncd;
+      if (R != last) {
+        lv.xrow_h.push_back((int)R);
+        lv.xptr_h.push_back((int)lv.xcol_h.size());
+        last = R;
+      }
+      lv.xcol_h.push_back((int)C);
+      lv.xval_h.push_back(e.second);
+    }
+    lv.xptr_h.push_back((int)lv.xcol_h.size());
+  }
+
+  long long compact_dofs(int l) {
+    Level& v = lv(l);
+    long long n = 0;
+    for (auto u : v.used_h) n += u;
+    return 3 * n;
+  }
+
+  // ------------------------------------------------------------------ build
+  void build(int min_dofs, int max_levels, int fixed_levels, const Options& o) {
+    CM_CHECK(!L.empty(), "system has no fine level");
+    opt = o;
+    hipEvent_t e0, e1;
+    CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
+    CM_HIP(hipEventRecord(e0, stream));
+    CM_CHECK(!built, "system already built");
+    prepare_level0();
+    Level& f0 = lv(0);
+    f0.used.upload(f0.used_h, stream);
+    // level structure (fe/gmg.py:86-101, rb/gpusolve.py:44-46)
+    const int nl_max = fixed_levels > 0 ? fixed_levels : max_levels;
+    while (true) {
+      const int l = nlev() - 1;
+      const bool last = fixed_levels > 0 ? (l == fixed_levels - 1) : (compact_dofs(l) <= min_dofs || l == nl_max - 1);
+      if (last) break;
+      Level& f = lv(l);
+      CM_CHECK(f.nx >= 2 && f.ny >= 2 && f.nz >= 2, "grid too small for another multigrid level");
+      auto c = std::make_unique<Level>();
+      for (int ax = 0; ax < 3; ++ax) axis_transfer(f.e[ax], f, *c, ax);
+      c->nx = (int)c->e[0].size() - 1; c->ny = (int)c->e[1].size() - 1; c->nz = (int)c->e[2].size() - 1;
+      c->nng = c->nn = (long long)(c->nx + 1) * (c->ny + 1) * (c->nz + 1);
+      c->ncell = (long long)c->nx * c->ny * c->nz;
+      coarse_used(f, *c);
+      coarse_extra(f, *c);
+      L.push_back(std::move(c));
+    }
+    // device transfer tables and per-level constant data
+    for (int l = 0; l < nlev(); ++l) {
+      Level& v = lv(l);
+      if (l > 0) v.used.upload(v.used_h, stream);
+      if (l + 1 < nlev()) {
+        for (int ax = 0; ax < 3; ++ax) {
+          v.pia[ax].upload(v.pia_h[ax], stream);
+          v.rs[ax].upload(v.rs_h[ax], stream);
+        }
+      }
+      if (l > 0) {
+        for (int ax = 0; ax < 3; ++ax) {
+          v.ncs[ax].upload(v.ncs_h[ax], stream);
+          v.ncn[ax].upload(v.ncn_h[ax], stream);
+          v.na[ax].upload(v.na_h[ax], stream);
+          v.nb[ax].upload(v.nb_h[ax], stream);
+        }
+      }
+      v.xrow.upload(v.xrow_h, stream);
+      v.xptr.upload(v.xptr_h, stream);
+      v.xcol.upload(v.xcol_h, stream);
+      prepare_prec<double>(l, /*dinv later*/ false);
+    }
+    // moments of coarse cells (FP64)
+    for (int l = 1; l < nlev(); ++l) {
+      Level& c = lv(l);
+      Level& f = lv(l - 1);
+      c.mom64.alloc(54 * (size_t)c.ncell);
+      AxisNest ax{c.ncs[0].p, c.ncn[0].p, c.na[0].p, c.nb[0].p};
+      AxisNest ay{c.ncs[1].p, c.ncn[1].p, c.na[1].p, c.nb[1].p};
+      AxisNest az{c.ncs[2].p, c.ncn[2].p, c.na[2].p, c.nb[2].p};
+      if (l == 1) {
+        hipLaunchKernelGGL(k_moments_from_fine, grid1(c.ncell), dim3(256), 0, stream, c.nx, c.ny, c.nz, f.nx, f.ny,
+                           ax, ay, az, f.d64.hx.p, f.d64.hy.p, f.d64.hz.p, f.matL.p, f.matM.p, sri, c.mom64.p);
+      } else {
+        hipLaunchKernelGGL(k_moments_from_moments, grid1(c.ncell), dim3(256), 0, stream, c.nx, c.ny, c.nz, f.nx, f.ny,
+                           f.nz, ax, ay, az, f.mom64.p, c.mom64.p);
+      }
+      CM_LAUNCH_CHECK();
+    }
+    // diagonal blocks (FP64) -> Dinv per precision is derived lazily
+    for (int l = 0; l + 1 < nlev(); ++l) compute_diag(l);
+    build_coarse_dense();
+    CM_HIP(hipEventRecord(e1, stream));
+    CM_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    t_setup_ms = ms;
+    CM_HIP(hipEventDestroy(e0)); CM_HIP(hipEventDestroy(e1));
+    S.alloc(64);
+    partial.alloc(RED_BLOCKS);
+    built = true;
+  }
+
+  std::vector<DevBuf<double>> D64;   // per level FP64 diagonal blocks
+
+  void compute_diag(int l) {
+    Level& v = lv(l);
+    if ((int)D64.size() < nlev()) D64.resize(nlev());
+    DevBuf<double>& D = D64[l];
+    D.alloc(9 * (size_t)v.nn);
+    hipLaunchKernelGGL(k_fill<double>, grid1(9 * v.nn), dim3(256), 0, stream, D.p, 9 * v.nn, 0.0);
+    if (l == 0) {
+      if (sri)
+        hipLaunchKernelGGL((k_diag_structured<false, true>), grid1(v.nng), dim3(256), 0, stream, v.nx, v.ny, v.nz,
+                           v.d64.hx.p, v.d64.hy.p, v.d64.hz.p, v.applyL.p, v.applyM.p, (const double*)nullptr, D.p);
+      else
+        hipLaunchKernelGGL((k_diag_structured<false, false>), grid1(v.nng), dim3(256), 0, stream, v.nx, v.ny, v.nz,
+                           v.d64.hx.p, v.d64.hy.p, v.d64.hz.p, v.applyL.p, v.applyM.p, (const double*)nullptr, D.p);
+      CM_LAUNCH_CHECK();
+      if (v.nirr) {
+        if (sri)
+          hipLaunchKernelGGL((k_diag_irregular<true>), grid1(v.nirr), dim3(256), 0, stream, v.nirr, v.irr_nodes.p,
+                             v.irr_ijk.p, v.d64.irrL.p, v.d64.irrM.p, v.d64.hx.p, v.d64.hy.p, v.d64.hz.p, D.p);
+        else
+          hipLaunchKernelGGL((k_diag_irregular<false>), grid1(v.nirr), dim3(256), 0, stream, v.nirr, v.irr_nodes.p,
+                             v.irr_ijk.p, v.d64.irrL.p, v.d64.irrM.p, v.d64.hx.p, v.d64.hy.p, v.d64.hz.p, D.p);
+        CM_LAUNCH_CHECK();
+      }
+    } else {
+      hipLaunchKernelGGL((k_diag_structured<true, false>), grid1(v.nng), dim3(256), 0, stream, v.nx, v.ny, v.nz,
+                         v.d64.hx.p, v.d64.hy.p, v.d64.hz.p, (const double*)nullptr, (const double*)nullptr,
+                         v.mom64.p, D.p);
+      CM_LAUNCH_CHECK();
+    }
+    const int nrow = (int)v.xrow_h.size();
+    if (nrow) {
+      DevBuf<double> xv;
+      xv.upload(v.xval_h, stream);
+      hipLaunchKernelGGL(k_diag_extra, grid1(nrow), dim3(256), 0, stream, nrow, v.xrow.p, v.xptr.p, v.xcol.p, xv.p,
+                         D.p);
+      CM_LAUNCH_CHECK();
+      CM_HIP(hipStreamSynchronize(stream));
+    }
+  }
+
+  void build_coarse_dense() {
+    const int l = nlev() - 1;
+    Level& c = lv(l);
+    const long long n = 3 * c.nn;
+    CM_CHECK(n <= 30000, "coarsest level too large for the dense direct solve; use more levels");
+    nc_dense = (int)n;
+    Ainv.alloc((size_t)n * n);
+    hipLaunchKernelGGL(k_fill<double>, grid1(n * n), dim3(256), 0, stream, Ainv.p, n * n, 0.0);
+    CM_LAUNCH_CHECK();
+    if (l == 0) {
+      // single-level hierarchy: the fine operator is assembled column by column
+      DevBuf<double> e(n), y(n);
+      for (long long j = 0; j < n; ++j) {
+        hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, stream, e.p, n, 0.0);
+        const double one = 1.0;
+        CM_HIP(hipMemcpyAsync(e.p + j, &one, sizeof(double), hipMemcpyHostToDevice, stream));
+        apply<double>(0, e.p, y.p, MODE_APPLY, nullptr);
+        CM_HIP(hipMemcpyAsync(Ainv.p + j * n, y.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        CM_HIP(hipStreamSynchronize(stream));
+      }
+    } else {
+      hipLaunchKernelGGL(k_dense_assemble, dim3((unsigned)c.ncell), dim3(64), 0, stream, c.nx, c.ny, c.nz,
+                         c.d64.hx.p, c.d64.hy.p, c.d64.hz.p, c.mom64.p, (int)n, Ainv.p);
+      CM_LAUNCH_CHECK();
+      const int nrow = (int)c.xrow_h.size();
+      if (nrow) {
+        DevBuf<double> xv;
+        xv.upload(c.xval_h, stream);
+        hipLaunchKernelGGL(k_dense_add_extra, grid1(nrow), dim3(256), 0, stream, nrow, c.xrow.p, c.xptr.p, c.xcol.p,
+                           xv.p, (int)n, Ainv.p);
+        CM_LAUNCH_CHECK();
+        CM_HIP(hipStreamSynchronize(stream));
+      }
+    }
+    hipLaunchKernelGGL(k_dense_fix_empty, dim3((unsigned)n), dim3(256), 0, stream, (int)n, Ainv.p);
+    CM_LAUNCH_CHECK();
+    DevBuf<double> colk(n);
+    for (int k = 0; k < n; ++k) {
+      hipLaunchKernelGGL(k_gj_savecol, grid1(n), dim3(256), 0, stream, (int)n, k, Ainv.p, colk.p);
+      hipLaunchKernelGGL(k_gj_scale, grid1(n), dim3(256), 0, stream, (int)n, k, Ainv.p, colk.p);
+      hipLaunchKernelGGL(k_gj_update, grid1(n * n), dim3(256), 0, stream, (int)n, k, Ainv.p, colk.p);
+    }
+    CM_LAUNCH_CHECK();
+    CM_HIP(hipStreamSynchronize(stream));
+  }
+
+  // Level data in precision T (geometry, materials, moments, Dinv, weights, work vectors).
+  template <typename T>
+  void prepare_prec(int l, bool with_dinv) {
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    if (!d.ready) {
+      std::vector<double> h[3];
+      for (int ax = 0; ax < 3; ++ax) {
+        h[ax].resize(v.e[ax].size() - 1);
+        for (size_t i = 0; i + 1 < v.e[ax].size(); ++i) h[ax][i] = v.e[ax][i + 1] - v.e[ax][i];
+      }
+      upload_cast(d.hx, h[0], stream); upload_cast(d.hy, h[1], stream); upload_cast(d.hz, h[2], stream);
+      if (l == 0) {
+        if (v.label_mode) {
+          upload_cast(d.tabL, v.tabL_h, stream);
+          upload_cast(d.tabM, v.tabM_h, stream);
+        } else {
+          d.cellL.alloc(v.ncell); d.cellM.alloc(v.ncell);
+          hipLaunchKernelGGL((k_cast<T, double>), grid1(v.ncell), dim3(256), 0, stream, v.ncell, d.cellL.p, v.applyL.p);
+          hipLaunchKernelGGL((k_cast<T, double>), grid1(v.ncell), dim3(256), 0, stream, v.ncell, d.cellM.p, v.applyM.p);
+          CM_LAUNCH_CHECK();
+        }
+        if (v.nirr) { upload_cast(d.irrL, v.irrL_h, stream); upload_cast(d.irrM, v.irrM_h, stream); }
+      } else if (v.mom64.n) {
+        d.mom.alloc(v.mom64.n);
+        hipLaunchKernelGGL((k_cast<T, double>), grid1((long long)v.mom64.n), dim3(256), 0, stream,
+                           (long long)v.mom64.n, d.mom.p, v.mom64.p);
+        CM_LAUNCH_CHECK();
+      }
+      if (l + 1 < nlev()) {
+        for (int ax = 0; ax < 3; ++ax) { upload_cast(d.pw[ax], v.pw_h[ax], stream); upload_cast(d.rw[ax], v.rw_h[ax], stream); }
+      }
+      if (!v.xval_h.empty()) upload_cast(d.xval, v.xval_h, stream);
+      const size_t nv = 3 * (size_t)v.nn;
+      d.x.alloc(nv); d.b.alloc(nv); d.r.alloc(nv); d.t.alloc(nv); d.d.alloc(nv);
+      d.ready = true;
+    }
+    if (with_dinv && !d.Dinv.n && l + 1 < nlev()) {
+      d.Dinv.alloc(9 * (size_t)v.nn);
+      hipLaunchKernelGGL(k_block_inverse<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, D64[l].p, d.Dinv.p);
+      CM_LAUNCH_CHECK();
+      if (opt.smoother == 0) power_iteration<T>(l);
+    }
+  }
+
+  template <typename T>
+  void ensure_prec() {
+    for (int l = 0; l < nlev(); ++l) prepare_prec<T>(l, true);
+  }
+
+  // ------------------------------------------------------------------ operator
+  template <typename T>
+  void apply(int l, const T* u, T* y, int mode, const T* b) {
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    GridArgs<T> g{v.nx, v.ny, v.nz, d.hx.p, d.hy.p, d.hz.p};
+    const int kz = pick_kz(v);
+    dim3 grid(div_up(v.nx + 1, TOUT), div_up(v.ny + 1, TOUT), div_up(v.nz + 1, kz));
+    const T sign = mode == MODE_APPLY ? T(1) : T(-1);
+    if (l == 0) {
+      launch_fine<T>(v, d, g, grid, kz, u, y, mode, b);
+      const long long nc3 = 3 * (v.nn - v.nng);
+      if (nc3) {
+        if (mode == MODE_APPLY)
+          hipLaunchKernelGGL(k_fill<T>, grid1(nc3), dim3(256), 0, stream, y + 3 * v.nng, nc3, T(0));
+        else if (b != y)
+          hipLaunchKernelGGL(k_copy_range<T>, grid1(nc3), dim3(256), 0, stream, y + 3 * v.nng, b + 3 * v.nng, nc3);
+      }
+      if (v.nirr) {
+        if (sri)
+          hipLaunchKernelGGL((k_apply_irregular<T, true>), grid1(v.nirr), dim3(256), 0, stream, v.nirr,
+                             v.irr_nodes.p, v.irr_ijk.p, d.irrL.p, d.irrM.p, g, u, y, sign);
+        else
+          hipLaunchKernelGGL((k_apply_irregular<T, false>), grid1(v.nirr), dim3(256), 0, stream, v.nirr,
+                             v.irr_nodes.p, v.irr_ijk.p, d.irrL.p, d.irrM.p, g, u, y, sign);
+      }
+    } else {
+      MomentCells<T> mc{d.mom.p};
+      if (mode == MODE_APPLY)
+        hipLaunchKernelGGL((k_apply_mom<T, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, mc, u, y, b, kz);
+      else
+        hipLaunchKernelGGL((k_apply_mom<T, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, mc, u, y, b, kz);
+    }
+    const int nrow = (int)v.xrow_h.size();
+    if (nrow)
+      hipLaunchKernelGGL(k_apply_extra<T>, grid1(nrow), dim3(256), 0, stream, nrow, v.xrow.p, v.xptr.p, v.xcol.p,
+                         d.xval.p, u, y, sign);
+    CM_LAUNCH_CHECK();
+  }
+
+  // structured part only (timing)
+  template <typename T>
+  void apply_structured(int l, const T* u, T* y) {
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    GridArgs<T> g{v.nx, v.ny, v.nz, d.hx.p, d.hy.p, d.hz.p};
+    const int kz = pick_kz(v);
+    dim3 grid(div_up(v.nx + 1, TOUT), div_up(v.ny + 1, TOUT), div_up(v.nz + 1, kz));
+    if (l == 0) {
+      launch_fine<T>(v, d, g, grid, kz, u, y, MODE_APPLY, nullptr);
+    } else {
+      MomentCells<T> mc{d.mom.p};
+      hipLaunchKernelGGL((k_apply_mom<T, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, mc, u, y, (const T*)nullptr, kz);
+    }
+    CM_LAUNCH_CHECK();
+  }
+
+  int kz_override = 0;
+  int pick_kz(const Level& v) const {
+    if (kz_override > 0) return kz_override;
+    const long long xy = (long long)div_up(v.nx + 1, TOUT) * div_up(v.ny + 1, TOUT);
+    // enough blocks to fill 256 CUs several times over, but keep the z halo small
+    for (int kz : {32, 16, 8}) {
+      if (xy * div_up(v.nz + 1, kz) >= 2048) return kz;
+    }
+    return 8;
+  }
+
+  template <typename T>
+  void launch_fine(Level& v, LevelT<T>& d, GridArgs<T> g, dim3 grid, int kz, const T* u, T* y, int mode, const T* b) {
+    FineLabelCells<T, true> cells_lab_t{v.lab.p, d.tabL.p, d.tabM.p};
+    FineLabelCells<T, false> cells_lab_f{v.lab.p, d.tabL.p, d.tabM.p};
+    FineValueCells<T, true> cells_val_t{d.cellL.p, d.cellM.p};
+    FineValueCells<T, false> cells_val_f{d.cellL.p, d.cellM.p};
+    if (v.label_mode) {
+      if (sri) {
+        if (mode == MODE_APPLY) hipLaunchKernelGGL((k_apply_fine<T, FineLabelCells<T, true>, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, cells_lab_t, u, y, b, kz);
+        else hipLaunchKernelGGL((k_apply_fine<T, FineLabelCells<T, true>, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, cells_lab_t, u, y, b, kz);
+      } else {
+        if (mode == MODE_APPLY) hipLaunchKernelGGL((k_apply_fine<T, FineLabelCells<T, false>, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, cells_lab_f, u, y, b, kz);
+        else hipLaunchKernelGGL((k_apply_fine<T, FineLabelCells<T, false>, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, cells_lab_f, u, y, b, kz);
+      }
+    } else {
+      if (sri) {
+        if (mode == MODE_APPLY) hipLaunchKernelGGL((k_apply_fine<T, FineValueCells<T, true>, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, cells_val_t, u, y, b, kz);
+        else hipLaunchKernelGGL((k_apply_fine<T, FineValueCells<T, true>, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, cells_val_t, u, y, b, kz);
+      } else {
+        if (mode == MODE_APPLY) hipLaunchKernelGGL((k_apply_fine<T, FineValueCells<T, false>, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, cells_val_f, u, y, b, kz);
+        else hipLaunchKernelGGL((k_apply_fine<T, FineValueCells<T, false>, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, cells_val_f, u, y, b, kz);
+      }
+    }
+  }
+
+  // ------------------------------------------------------------------ smoothers, V-cycle
+  template <typename T>
+  void power_iteration(int l) {
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    const long long n = 3 * v.nn;
+    hipLaunchKernelGGL(k_randn<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, v.used.p, 0ULL, d.x.p);
+    DevBuf<double> s(4);
+    double lam = 1.0;
+    for (int it = 0; it < 12; ++it) {
+      apply<T>(l, d.x.p, d.t.p, MODE_APPLY, nullptr);
+      hipLaunchKernelGGL(k_block_apply<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, d.Dinv.p, d.t.p, d.d.p);
+      hipLaunchKernelGGL((k_dot<T, T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, d.d.p, d.d.p, partial.p);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, s.p, 0);
+      hipLaunchKernelGGL((k_dot<T, T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, d.x.p, d.x.p, partial.p);
+      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, s.p, 1);
+      CM_LAUNCH_CHECK();
+      double hs[2];
+      CM_HIP(hipMemcpyAsync(hs, s.p, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      CM_HIP(hipStreamSynchronize(stream));
+      lam = std::sqrt(hs[0]) / std::sqrt(hs[1]);
+      // x = y / ||y||
+      CM_HIP(hipMemcpyAsync(d.x.p, d.d.p, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+      hipLaunchKernelGGL(k_scale<T>, grid1(n), dim3(256), 0, stream, n, d.x.p, s.p, 0);
+      CM_LAUNCH_CHECK();
+    }
+    d.lmax = 1.1 * lam;
+    d.lmin = d.lmax / 30.0;
+  }
+
+  template <typename T>
+  void smooth(int l, const T* b, T* x, bool zero_start, int nsweep) {
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    const long long nn = v.nn;
+    if (opt.smoother == 1) {
+      int s0 = 0;
+      if (zero_start) {
+        hipLaunchKernelGGL(k_jacobi<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, (const T*)nullptr, d.Dinv.p,
+                           T(opt.omega), 1);
+        s0 = 1;
+      }
+      for (int s = s0; s < nsweep; ++s) {
+        apply<T>(l, x, d.t.p, MODE_APPLY, nullptr);
+        hipLaunchKernelGGL(k_jacobi<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.t.p, d.Dinv.p, T(opt.omega), 0);
+      }
+      CM_LAUNCH_CHECK();
+      return;
+    }
+    // Chebyshev, degree opt.degree (fe/gmg.py:61-76)
+    const double theta = 0.5 * (d.lmax + d.lmin), delta = 0.5 * (d.lmax - d.lmin);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    T* r = d.r.p;
+    T* dd = d.d.p;
+    const long long n3 = 3 * nn;
+    if (zero_start)
+      hipLaunchKernelGGL(k_copy_range<T>, grid1(n3), dim3(256), 0, stream, r, b, n3);
+    else
+      apply<T>(l, x, r, MODE_RESID, b);
+    // d = M r / theta ; x += d
+    if (zero_start)
+      hipLaunchKernelGGL(k_cheb<T>, grid1(nn), dim3(256), 0, stream, nn, x, dd, r, d.Dinv.p, T(0), T(1.0 / theta), 1);
+    else {
+      hipLaunchKernelGGL(k_cheb<T>, grid1(nn), dim3(256), 0, stream, nn, d.t.p, dd, r, d.Dinv.p, T(0), T(1.0 / theta), 1);
+      hipLaunchKernelGGL((k_add<T, T>), grid1(n3), dim3(256), 0, stream, n3, x, dd);
+    }
+    for (int k = 1; k < opt.degree; ++k) {
+      apply<T>(l, dd, r, MODE_RESID, r);   // r -= A d
+      const double rho_new = 1.0 / (2.0 * sigma - rho);
+      hipLaunchKernelGGL(k_cheb<T>, grid1(nn), dim3(256), 0, stream, nn, x, dd, r, d.Dinv.p, T(rho_new * rho),
+                         T(2.0 * rho_new / delta), 0);
+      rho = rho_new;
+    }
+    CM_LAUNCH_CHECK();
+  }
+
+  template <typename T>
+  void vcycle(int l, const T* b, T* x) {
+    if (l == nlev() - 1) {
+      hipLaunchKernelGGL(k_dense_gemv<T>, grid1(64LL * nc_dense), dim3(256), 0, stream, nc_dense, Ainv.p, b, x);
+      CM_LAUNCH_CHECK();
+      return;
+    }
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    Level& c = lv(l + 1);
+    LevelT<T>& dc = c.dat<T>();
+    smooth<T>(l, b, x, true, opt.npre);
+    apply<T>(l, x, d.r.p, MODE_RESID, b);
+    AxisR<T> rx{v.rs[0].p, d.rw[0].p}, ry{v.rs[1].p, d.rw[1].p}, rz{v.rs[2].p, d.rw[2].p};
+    hipLaunchKernelGGL(k_restrict<T>, grid1(c.nn), dim3(256), 0, stream, v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1,
+                       c.ny + 1, c.nz + 1, rx, ry, rz, d.r.p, dc.b.p);
+    AxisP<T> px{v.pia[0].p, d.pw[0].p}, py{v.pia[1].p, d.pw[1].p}, pz{v.pia[2].p, d.pw[2].p};
+    const int ncopy = (int)(v.nn - v.nng);
+    if (ncopy)
+      hipLaunchKernelGGL(k_restrict_copies<T>, grid1(ncopy), dim3(256), 0, stream, ncopy, v.nng, v.copy_orig.p,
+                         v.nx + 1, v.ny + 1, c.nx + 1, c.ny + 1, px, py, pz, d.r.p, dc.b.p);
+    CM_LAUNCH_CHECK();
+    vcycle<T>(l + 1, dc.b.p, dc.x.p);
+    hipLaunchKernelGGL(k_prolong_add<T>, grid1(v.nng), dim3(256), 0, stream, v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1,
+                       c.ny + 1, px, py, pz, dc.x.p, v.used.p, x);
+    if (ncopy)
+      hipLaunchKernelGGL(k_prolong_copies<T>, grid1(ncopy), dim3(256), 0, stream, ncopy, v.nng, v.copy_orig.p,
+                         v.nx + 1, v.ny + 1, c.nx + 1, c.ny + 1, px, py, pz, dc.x.p, x);
+    CM_LAUNCH_CHECK();
+    smooth<T>(l, b, x, false, opt.npost);
+  }
+
+  // ------------------------------------------------------------------ PCG
+  template <typename T>
+  void finalize_to(int slot) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, S.p, slot);
+  }
+  template <typename Ta, typename Tb>
+  void dot(long long n, const Ta* a, const Tb* b, int slot) {
+    hipLaunchKernelGGL((k_dot<Ta, Tb>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, a, b, partial.p);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, S.p, slot);
+  }
+  double read_slot(int slot) {
+    double v;
+    CM_HIP(hipMemcpyAsync(&v, S.p + slot, sizeof(double), hipMemcpyDeviceToHost, stream));
+    CM_HIP(hipStreamSynchronize(stream));
+    return v;
+  }
+
+  // rigid-mode correction z += Z Einv Z^T r (FP64)
+  void rigid_correct(const double* r, double* z) {
+    if (!zcol) return;
+    hipLaunchKernelGGL(k_zt_r, dim3(zcol), dim3(RED_THREADS), 0, stream, zcol, zcptr.p, zrows_c.p, zvals_c.p, r, zw.p);
+    hipLaunchKernelGGL(k_small_gemv, dim3(div_up(zcol, 64)), dim3(64), 0, stream, zcol, Einv.p, zw.p, zv.p);
+    const int nr = (int)(zrow.n);
+    hipLaunchKernelGGL(k_z_v, grid1(nr), dim3(256), 0, stream, nr, zrow.p, zptr.p, zcols.p, zvals.p, zv.p, z);
+    CM_LAUNCH_CHECK();
+  }
+
+  // z = prec(r), FP64 in/out; vcycle in precision VP
+  void precondition64(int vprec, const double* r, double* z) {
+    Level& v0 = lv(0);
+    const long long n = 3 * v0.nn;
+    if (vprec == 0) {
+      vcycle<double>(0, r, z);
+    } else {
+      hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, r32.p, r);
+      vcycle<float>(0, r32.p, z32.p);
+      hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z, z32.p);
+    }
+    rigid_correct(r, z);
+  }
+
+  void alloc_pcg64() {
+    const size_t n = 3 * (size_t)lv(0).nn;
+    if (r64.n != n) { r64.alloc(n); z64.alloc(n); p64.alloc(n); Ap64.alloc(n); }
+  }
+  void alloc_pcg32() {
+    const size_t n = 3 * (size_t)lv(0).nn;
+    if (r32.n != n) { r32.alloc(n); z32.alloc(n); p32.alloc(n); Ap32.alloc(n); d32.alloc(n); }
+  }
+
+  // Slots of S
+  enum { S_RR = 0, S_BB = 1, S_PAP = 2, S_RZ0 = 4, S_RZ1 = 5, S_IR_RR = 8 };
+
+  // FP64 PCG (rb/gpusolve.py:229-254), V-cycle in FP64 (vprec 0) or FP32 (vprec 1)
+  void pcg64(int vprec, const double* b, double* x, double tol, int maxiter, int check_every, cm_solve_info* info) {
+    Level& v0 = lv(0);
+    const long long n = 3 * v0.nn;
+    alloc_pcg64();
+    if (vprec) { alloc_pcg32(); ensure_prec<float>(); }
+    ensure_prec<double>();
+    apply<double>(0, x, r64.p, MODE_RESID, b);
+    dot(n, b, b, S_BB);
+    const double bn = std::sqrt(read_slot(S_BB));
+    const double thr = tol * (bn > 0 ? bn : 1.0);
+    precondition64(vprec, r64.p, z64.p);
+    CM_HIP(hipMemcpyAsync(p64.p, z64.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    int cur = S_RZ0, nxt = S_RZ1;
+    dot(n, r64.p, z64.p, cur);
+    dot(n, r64.p, r64.p, S_RR);
+    double rn = std::sqrt(read_slot(S_RR));
+    int it = 0, flag = rn <= thr ? 0 : 1;
+    while (flag && it < maxiter) {
+      apply<double>(0, p64.p, Ap64.p, MODE_APPLY, nullptr);
+      dot(n, p64.p, Ap64.p, S_PAP);
+      hipLaunchKernelGGL((k_pcg_xr<double, double, float>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, x,
+                         r64.p, p64.p, Ap64.p, S.p, cur, S_PAP, partial.p, (float*)nullptr);
+      finalize_to<double>(S_RR);
+      ++it;
+      if (it < check_every || it % check_every == 0) {
+        rn = std::sqrt(read_slot(S_RR));
+        if (rn <= thr) { flag = 0; break; }
+      }
+      precondition64(vprec, r64.p, z64.p);
+      dot(n, r64.p, z64.p, nxt);
+      hipLaunchKernelGGL(k_pcg_p<double>, grid1(n), dim3(256), 0, stream, n, p64.p, z64.p, S.p, nxt, cur);
+      CM_LAUNCH_CHECK();
+      std::swap(cur, nxt);
+    }
+    if (flag) {
+      rn = std::sqrt(read_slot(S_RR));
+      flag = rn <= thr ? 0 : maxiter;
+    }
+    info->iters = it;
+    info->flag = flag;
+    info->outer = 1;
+    info->relres = rn / (bn > 0 ? bn : 1.0);
+  }
+
+  // Mixed precision: FP64 outer residual / iterative refinement, FP32 inner PCG whose alpha/beta
+  // reductions accumulate in FP64, FP32 V-cycle with an FP64 coarse solve.
+  void pcg_mixed(const double* b, double* x, double tol, int maxiter, int check_every, double inner_tol,
+                 cm_solve_info* info) {
+    Level& v0 = lv(0);
+    const long long n = 3 * v0.nn;
+    alloc_pcg64();
+    alloc_pcg32();
+    ensure_prec<float>();
+    ensure_prec<double>();
+    dot(n, b, b, S_BB);
+    const double bn = std::sqrt(read_slot(S_BB));
+    const double thr = tol * (bn > 0 ? bn : 1.0);
+    int it = 0, outer = 0, flag = 1;
+    double rn = 0.0;
+    while (true) {
+      // true FP64 residual
+      apply<double>(0, x, Ap64.p, MODE_APPLY, nullptr);
+      hipLaunchKernelGGL(k_resid_cast<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, r64.p, b, Ap64.p,
+                         partial.p, (float*)nullptr, 1.0);
+      finalize_to<double>(S_IR_RR);
+      rn = std::sqrt(read_slot(S_IR_RR));
+      if (rn <= thr) { flag = 0; break; }
+      if (it >= maxiter) { flag = maxiter; break; }
+      ++outer;
+      // inner FP32 PCG on A d = r / ||r||
+      const double sc = 1.0 / rn;
+      hipLaunchKernelGGL((k_resid_cast<float>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, r64.p, b, Ap64.p,
+                         partial.p, r32.p, sc);
+      hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(256), 0, stream, d32.p, n, 0.f);
+      vcycle<float>(0, r32.p, z32.p);
+      if (zcol) {
+        hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z64.p, z32.p);
+        hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, Ap64.p, r32.p);
+        rigid_correct(Ap64.p, z64.p);
+        hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, z32.p, z64.p);
+      }
+      CM_HIP(hipMemcpyAsync(p32.p, z32.p, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
+      int cur = S_RZ0, nxt = S_RZ1;
+      dot(n, r32.p, z32.p, cur);
+      int inner = 0;
+      const double ithr = std::max(inner_tol, thr / rn);
+      while (it < maxiter) {
+        apply<float>(0, p32.p, Ap32.p, MODE_APPLY, nullptr);
+        dot(n, p32.p, Ap32.p, S_PAP);
+        hipLaunchKernelGGL((k_pcg_xr<float, float, float>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, d32.p,
+                           r32.p, p32.p, Ap32.p, S.p, cur, S_PAP, partial.p, (float*)nullptr);
+        finalize_to<double>(S_RR);
+        ++it; ++inner;
+        if (inner % check_every == 0 || inner < check_every) {
+          const double rin = std::sqrt(read_slot(S_RR));
+          if (rin <= ithr) break;
+        }
+        vcycle<float>(0, r32.p, z32.p);
+        if (zcol) {
+          hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z64.p, z32.p);
+          hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, Ap64.p, r32.p);
+          rigid_correct(Ap64.p, z64.p);
+          hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, z32.p, z64.p);
+        }
+        dot(n, r32.p, z32.p, nxt);
+        hipLaunchKernelGGL(k_pcg_p<float>, grid1(n), dim3(256), 0, stream, n, p32.p, z32.p, S.p, nxt, cur);
+        CM_LAUNCH_CHECK();
+        std::swap(cur, nxt);
+      }
+      hipLaunchKernelGGL(k_axpy_scaled<float>, grid1(n), dim3(256), 0, stream, n, x, d32.p, rn);
+      CM_LAUNCH_CHECK();
+    }
+    info->iters = it;
+    info->flag = flag;
+    info->outer = outer;
+    info->relres = rn / (bn > 0 ? bn : 1.0);
+  }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Level-0 description from the boundary
+static void set_fine_common(System& s, int nx, int ny, int nz, const double* xe, const double* ye, const double* ze) {
+  auto lv = std::make_unique<Level>();
+  lv->nx = nx; lv->ny = ny; lv->nz = nz;
+  lv->e[0].assign(xe, xe + nx + 1);
+  lv->e[1].assign(ye, ye + ny + 1);
+  lv->e[2].assign(ze, ze + nz + 1);
+  lv->nng = lv->nn = (long long)(nx + 1) * (ny + 1) * (nz + 1);
+  lv->ncell = (long long)nx * ny * nz;
+  s.L.clear();
+  s.L.push_back(std::move(lv));
+}
+
+}  // namespace cm
+
+
+using cm::System;
+
+#define CM_API_BEGIN try {
+#define CM_API_END                                        \
+  return 0;                                               \
+  }                                                       \
+  catch (const std::exception& e) {                       \
+    cm::g_last_error = e.what();                          \
+    return -1;                                            \
+  }                                                       \
+  catch (...) {                                           \
+    cm::g_last_error = "comech: unknown error";           \
+    return -1;                                            \
+  }
+
+struct cm_sys {
+  System s;
+};
+
+namespace {
+template <typename T>
+const T* cp(const void* p) { return static_cast<const T*>(p); }
+template <typename T>
+T* mp(void* p) { return static_cast<T*>(p); }
+void need_built(cm_sys* h) {
+  CM_CHECK(h != nullptr, "null system");
+  CM_CHECK(h->s.built, "system not built (call cm_sys_build)");
+}
+void need_level(cm_sys* h, int level) {
+  need_built(h);
+  CM_CHECK(level >= 0 && level < h->s.nlev(), "level out of range");
+}
+}  // namespace
+
+extern "C" {
+
+const char* cm_last_error(void) { return cm::g_last_error.c_str(); }
+
+int cm_version(void) { return CM_ABI_VERSION; }
+
+int cm_device_count(int* n) {
+  CM_API_BEGIN
+  CM_HIP(hipGetDeviceCount(n));
+  CM_API_END
+}
+
+int cm_sys_create(cm_sys** out, void* stream, int nx, int ny, int nz, const double* xe, const double* ye,
+                  const double* ze, int sri) {
+  CM_API_BEGIN
+  CM_CHECK(out != nullptr, "out is null");
+  CM_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "grid must have at least one cell per axis");
+  for (int q = 0; q < nx; ++q) CM_CHECK(xe[q + 1] > xe[q], "xe must increase");
+  for (int q = 0; q < ny; ++q) CM_CHECK(ye[q + 1] > ye[q], "ye must increase");
+  for (int q = 0; q < nz; ++q) CM_CHECK(ze[q + 1] > ze[q], "ze must increase");
+  CM_CHECK((long long)(nx + 1) * (ny + 1) * (nz + 1) * 3 < (1LL << 31), "grid too large for 32-bit DOF indices");
+  auto* h = new cm_sys();
+  h->s.stream = (hipStream_t)stream;
+  h->s.sri = sri ? 1 : 0;
+  cm::set_fine_common(h->s, nx, ny, nz, xe, ye, ze);
+  *out = h;
+  CM_API_END
+}
+
+int cm_sys_destroy(cm_sys* h) {
+  CM_API_BEGIN
+  if (h) {
+    (void)hipStreamSynchronize(h->s.stream);
+    delete h;
+  }
+  CM_API_END
+}
+
+int cm_sys_set_labels(cm_sys* h, int nmat, const double* lam_tab, const double* mu_tab, const uint8_t* lab_ref) {
+  CM_API_BEGIN
+  CM_CHECK(h && !h->s.built, "system missing or already built");
+  CM_CHECK(nmat >= 1 && nmat <= 255, "label mode holds 1..255 materials");
+  System& s = h->s;
+  const long long nc = s.lv(0).ncell;
+  s.st_nmat = nmat;
+  s.st_tabL.assign(lam_tab, lam_tab + nmat);
+  s.st_tabM.assign(mu_tab, mu_tab + nmat);
+  s.st_lab.assign(lab_ref, lab_ref + nc);
+  s.st_lam.clear(); s.st_mu.clear(); s.st_active.clear();
+  CM_API_END
+}
+
+int cm_sys_set_cellmat(cm_sys* h, const double* lam_ref, const double* mu_ref, const uint8_t* active_ref) {
+  CM_API_BEGIN
+  CM_CHECK(h && !h->s.built, "system missing or already built");
+  System& s = h->s;
+  const long long nc = s.lv(0).ncell;
+  s.st_nmat = -1;
+  s.st_lam.assign(lam_ref, lam_ref + nc);
+  s.st_mu.assign(mu_ref, mu_ref + nc);
+  s.st_active.assign(active_ref, active_ref + nc);
+  CM_API_END
+}
+
+int cm_sys_set_split(cm_sys* h, int64_t ncopy, const int64_t* copy_orig, int64_t nirr, const int64_t* irr_cell_ref,
+                     const int64_t* irr_nodes) {
+  CM_API_BEGIN
+  CM_CHECK(h && !h->s.built, "system missing or already built");
+  CM_CHECK(ncopy >= 0 && nirr >= 0, "negative counts");
+  System& s = h->s;
+  s.st_copy_orig.assign(copy_orig, copy_orig + ncopy);
+  s.st_irr_cell.assign(irr_cell_ref, irr_cell_ref + nirr);
+  s.st_irr_nodes.assign(irr_nodes, irr_nodes + 8 * nirr);
+  CM_API_END
+}
+
+int cm_sys_set_extra(cm_sys* h, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals) {
+  CM_API_BEGIN
+  CM_CHECK(h && !h->s.built, "system missing or already built");
+  CM_CHECK(nnz >= 0, "negative nnz");
+  System& s = h->s;
+  s.st_xr.assign(rows, rows + nnz);
+  s.st_xc.assign(cols, cols + nnz);
+  s.st_xv.assign(vals, vals + nnz);
+  CM_API_END
+}
+
+int cm_sys_set_used(cm_sys* h, int64_t nn, const uint8_t* used) {
+  CM_API_BEGIN
+  CM_CHECK(h && !h->s.built, "system missing or already built");
+  h->s.st_used_extra.assign(used, used + nn);
+  CM_API_END
+}
+
+int cm_sys_set_rigid(cm_sys* h, int ncol, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals,
+                     const double* Einv) {
+  CM_API_BEGIN
+  need_built(h);
+  System& s = h->s;
+  const long long nd3 = 3 * s.lv(0).nn;
+  s.zcol = 0;
+  if (ncol <= 0 || nnz <= 0) return 0;
+  // CSC for Z^T r, unique-row CSR for Z v
+  std::vector<int> cptr(ncol + 1, 0), crow(nnz);
+  std::vector<double> cval(nnz);
+  for (long long q = 0; q < nnz; ++q) {
+    CM_CHECK(cols[q] >= 0 && cols[q] < ncol && rows[q] >= 0 && rows[q] < nd3, "rigid-mode entry out of range");
+    cptr[cols[q] + 1]++;
+  }
+  for (int j = 0; j < ncol; ++j) cptr[j + 1] += cptr[j];
+  std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+  for (long long q = 0; q < nnz; ++q) {
+    const int pos = fill[cols[q]]++;
+    crow[pos] = (int)rows[q];
+    cval[pos] = vals[q];
+  }
+  std::vector<long long> ord(nnz);
+  std::iota(ord.begin(), ord.end(), 0);
+  std::sort(ord.begin(), ord.end(), [&](long long a, long long b) { return rows[a] < rows[b] || (rows[a] == rows[b] && cols[a] < cols[b]); });
+  std::vector<int> rrow, rptr, rcol;
+  std::vector<double> rval;
+  long long last = -1;
+  for (long long q : ord) {
+    if (rows[q] != last) { rrow.push_back((int)rows[q]); rptr.push_back((int)rcol.size()); last = rows[q]; }
+    rcol.push_back((int)cols[q]);
+    rval.push_back(vals[q]);
+  }
+  rptr.push_back((int)rcol.size());
+  s.zcptr.upload(cptr, s.stream); s.zrows_c.upload(crow, s.stream); s.zvals_c.upload(cval, s.stream);
+  s.zrow.upload(rrow, s.stream); s.zptr.upload(rptr, s.stream); s.zcols.upload(rcol, s.stream);
+  s.zvals.upload(rval, s.stream);
+  std::vector<double> E(Einv, Einv + (size_t)ncol * ncol);
+  s.Einv.upload(E, s.stream);
+  s.zw.alloc(ncol); s.zv.alloc(ncol);
+  s.zcol = ncol;
+  CM_API_END
+}
+
+int cm_sys_build(cm_sys* h, int min_dofs, int max_levels, int fixed_levels, int smoother, int degree, double omega,
+                 int npre, int npost) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  CM_CHECK(smoother == 0 || smoother == 1, "smoother must be 0 (Chebyshev) or 1 (Jacobi)");
+  CM_CHECK(max_levels >= 1 && degree >= 1 && npre >= 1 && npost >= 1, "bad multigrid options");
+  cm::Options o;
+  o.smoother = smoother; o.degree = degree; o.omega = omega; o.npre = npre; o.npost = npost;
+  h->s.build(min_dofs, max_levels, fixed_levels, o);
+  CM_API_END
+}
+
+int cm_sys_levels(cm_sys* h, int* nlev, int64_t* desc, int cap) {
+  CM_API_BEGIN
+  need_built(h);
+  System& s = h->s;
+  *nlev = s.nlev();
+  for (int l = 0; l < s.nlev() && l < cap; ++l) {
+    cm::Level& v = s.lv(l);
+    desc[6 * l] = v.nx; desc[6 * l + 1] = v.ny; desc[6 * l + 2] = v.nz;
+    desc[6 * l + 3] = v.nn; desc[6 * l + 4] = s.compact_dofs(l); desc[6 * l + 5] = (int64_t)v.xrow_h.size();
+  }
+  CM_API_END
+}
+
+int cm_sys_apply(cm_sys* h, int level, int prec, const void* u, void* y) {
+  CM_API_BEGIN
+  need_level(h, level);
+  System& s = h->s;
+  if (prec == 0) {
+    s.prepare_prec<double>(level, false);
+    s.apply<double>(level, cp<double>(u), mp<double>(y), cm::MODE_APPLY, nullptr);
+  } else {
+    s.prepare_prec<float>(level, false);
+    s.apply<float>(level, cp<float>(u), mp<float>(y), cm::MODE_APPLY, nullptr);
+  }
+  CM_API_END
+}
+
+int cm_sys_vcycle(cm_sys* h, int prec, const void* b, void* x) {
+  CM_API_BEGIN
+  need_built(h);
+  System& s = h->s;
+  if (prec == 0) {
+    s.ensure_prec<double>();
+    s.vcycle<double>(0, cp<double>(b), mp<double>(x));
+  } else {
+    s.ensure_prec<float>();
+    s.vcycle<float>(0, cp<float>(b), mp<float>(x));
+  }
+  CM_API_END
+}
+
+int cm_sys_solve(cm_sys* h, int mode, const double* b, double* x, double tol, int maxiter, int check_every,
+                 double inner_tol, cm_solve_info* info) {
+  CM_API_BEGIN
+  need_built(h);
+  CM_CHECK(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+  CM_CHECK(info != nullptr, "info is null");
+  System& s = h->s;
+  if (check_every < 1) check_every = 5;
+  hipEvent_t e0, e1;
+  CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
+  CM_HIP(hipEventRecord(e0, s.stream));
+  std::memset(info, 0, sizeof(*info));
+  if (mode == 2) s.pcg_mixed(b, x, tol, maxiter, check_every, inner_tol, info);
+  else s.pcg64(mode, b, x, tol, maxiter, check_every, info);
+  CM_HIP(hipEventRecord(e1, s.stream));
+  CM_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  CM_HIP(hipEventDestroy(e0)); CM_HIP(hipEventDestroy(e1));
+  info->t_solve_ms = ms;
+  info->t_setup_ms = s.t_setup_ms;
+  info->levels = s.nlev();
+  CM_API_END
+}
+
+int cm_cell_stress(void* stream, int nx, int ny, int nz, const double* hx, const double* hy, const double* hz,
+                   int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* lam_ref,
+                   const double* mu_ref, const double* U, double* sig, double* vm, double* eps) {
+  CM_API_BEGIN
+  CM_CHECK(nx > 0 && ny > 0 && nz > 0, "bad grid");
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(cm::k_cell_stress, cm::grid1(n), dim3(256), 0, (hipStream_t)stream, (long long)n,
+                     (const long long*)cells_ref, (const long long*)nodes8, nx, ny, nz, hx, hy, hz, lam_ref, mu_ref, U,
+                     sig, vm, eps);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
+int cm_element_forces(void* stream, int nx, int ny, int nz, const double* hx, const double* hy, const double* hz,
+                      int sri, int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* lam_ref,
+                      const double* mu_ref, const double* U, double* out) {
+  CM_API_BEGIN
+  CM_CHECK(nx > 0 && ny > 0 && nz > 0, "bad grid");
+  if (n <= 0) return 0;
+  if (sri)
+    hipLaunchKernelGGL(cm::k_element_forces<true>, cm::grid1(n), dim3(256), 0, (hipStream_t)stream, (long long)n,
+                       (const long long*)cells_ref, (const long long*)nodes8, nx, ny, nz, hx, hy, hz, lam_ref, mu_ref,
+                       U, out);
+  else
+    hipLaunchKernelGGL(cm::k_element_forces<false>, cm::grid1(n), dim3(256), 0, (hipStream_t)stream, (long long)n,
+                       (const long long*)cells_ref, (const long long*)nodes8, nx, ny, nz, hx, hy, hz, lam_ref, mu_ref,
+                       U, out);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
+int cm_sys_time_apply(cm_sys* h, int level, int prec, const void* u, void* y, int nrep, int structured_only,
+                      double* avg_ms) {
+  CM_API_BEGIN
+  need_level(h, level);
+  CM_CHECK(nrep >= 1, "nrep must be positive");
+  System& s = h->s;
+  if (prec == 0) s.prepare_prec<double>(level, false);
+  else s.prepare_prec<float>(level, false);
+  hipEvent_t e0, e1;
+  CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
+  auto once = [&]() {
+    if (prec == 0) {
+      if (structured_only) s.apply_structured<double>(level, cp<double>(u), mp<double>(y));
+      else s.apply<double>(level, cp<double>(u), mp<double>(y), cm::MODE_APPLY, nullptr);
+    } else {
+      if (structured_only) s.apply_structured<float>(level, cp<float>(u), mp<float>(y));
+      else s.apply<float>(level, cp<float>(u), mp<float>(y), cm::MODE_APPLY, nullptr);
+    }
+  };
+  once();
+  CM_HIP(hipEventRecord(e0, s.stream));
+  for (int r = 0; r < nrep; ++r) once();
+  CM_HIP(hipEventRecord(e1, s.stream));
+  CM_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+  CM_HIP(hipEventDestroy(e0)); CM_HIP(hipEventDestroy(e1));
+  *avg_ms = ms / nrep;
+  CM_API_END
+}
+
+int cm_sys_set_kz(cm_sys* h, int kz) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  CM_CHECK(kz >= 0 && kz <= 4096, "kz out of range");
+  h->s.kz_override = kz;
+  CM_API_END
+}
+
+int cm_sys_sync(cm_sys* h) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  CM_HIP(hipStreamSynchronize(h->s.stream));
+  CM_API_END
+}
+
+int cm_sample_fe(void* stream, int mx, int my, int mz, int sub, int nx, int ny, int nz, const int* si, const double* st,
+                 const uint8_t* sin_, const int* sj, const double* su, const uint8_t* sjn, const int* sk,
+                 const double* sv, const uint8_t* skn, const uint8_t* active, const int64_t* cn, const float* VM,
+                 const float* SIG, const double* Urel, float* vm_out, float* u_out, double* cover, float* sig_out) {
+  CM_API_BEGIN
+  CM_CHECK(mx > 0 && my > 0 && mz > 0 && sub > 0, "bad analysis grid");
+  const long long ncm = (long long)mx * my * mz;
+  hipLaunchKernelGGL(cm::k_sample_fe, cm::grid1(ncm), dim3(256), 0, (hipStream_t)stream, mx, my, mz, sub, nx, ny, nz,
+                     si, st, sin_, sj, su, sjn, sk, sv, skn, active, (const long long*)cn, VM, SIG, Urel, vm_out,
+                     u_out, cover, sig_out);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
+}  // extern "C"

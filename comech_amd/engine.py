@@ -1,0 +1,173 @@
+"""Host logic shared by the drop-in solvers: reference inputs -> GPU system -> reference outputs.
+
+The reference assembles a CSR matrix on compact DOFs ``3*nmap[node]+c`` (fe/solver.py:137-162,
+rb/splitfe.py:72-91) and hands it to ``pcg_gmg``.  Here nothing is assembled: the cells stay as
+labels/materials on the GPU, and only the genuinely sparse terms (springs, penalties, ligament and
+interface springs, the ``extra(nmap, n)`` callback) become a small COO over level-0 node DOFs
+``3*node+c``.  Node ids are the reference's own (grid ids, then split copies), so the level-0
+vector *is* the reference's ``U`` array (nnode_total, 3) without the NaN of unused nodes.
+"""
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from .system import VoxelSystem, cell_stress_gpu, require_gpu
+
+
+def split_cells(grid, active, cn):
+    """Active cells whose connectivity differs from the grid corners (split-copy cells).
+    Returns (irr_cells, irr_nodes(n,8))."""
+    cells = np.flatnonzero(active)
+    out_c, out_n = [], []
+    for a in range(0, len(cells), 1 << 20):
+        c = cells[a:a + (1 << 20)]
+        row = cn[c]
+        diff = (row != grid.corners(c)).any(1)
+        out_c.append(c[diff]); out_n.append(row[diff])
+    if not out_c:
+        return np.zeros(0, np.int64), np.zeros((0, 8), np.int64)
+    return np.concatenate(out_c).astype(np.int64), np.concatenate(out_n).astype(np.int64)
+
+
+def used_nodes(grid, active, nn, irr_cells=None, irr_nodes=None):
+    """Mask of nodes touched by active cells (fe/solver.py:141, rb/splitfe.py:73)."""
+    used = np.zeros(nn, bool)
+    cells = np.flatnonzero(active)
+    if irr_cells is not None and len(irr_cells):
+        keep = np.ones(grid.ncell, bool)
+        keep[irr_cells] = False
+        cells = cells[keep[cells]]
+        used[np.asarray(irr_nodes).ravel()] = True
+    for a in range(0, len(cells), 1 << 20):
+        used[grid.corners(cells[a:a + (1 << 20)]).ravel()] = True
+    return used
+
+
+def compact_to_full_coo(M, nused):
+    """CSR/COO on compact DOFs -> COO on node DOFs 3*node+c."""
+    M = sp.coo_matrix(M)
+    r = 3 * nused[M.row // 3] + M.row % 3
+    c = 3 * nused[M.col // 3] + M.col % 3
+    return r.astype(np.int64), c.astype(np.int64), M.data.astype(np.float64)
+
+
+def diag_coo(nodes, kvals, comps=(0, 1, 2)):
+    """Diagonal springs: nodes (m,), kvals (m, len(comps)) -> COO."""
+    rows = (3 * np.asarray(nodes, np.int64)[:, None] + np.asarray(comps)[None, :]).ravel()
+    return rows, rows.copy(), np.asarray(kvals, float).ravel()
+
+
+def rank1_coo(dofs, w, k):
+    """k w w^T on the given DOFs (duplicates in dofs are summed first)."""
+    dofs = np.asarray(dofs, np.int64)
+    w = np.asarray(w, float)
+    ud, inv = np.unique(dofs, return_inverse=True)
+    wu = np.zeros(len(ud))
+    np.add.at(wu, inv, w)
+    nz = wu != 0
+    ud, wu = ud[nz], wu[nz]
+    return np.repeat(ud, len(ud)), np.tile(ud, len(ud)), (k * np.outer(wu, wu)).ravel()
+
+
+def cat_coo(parts):
+    parts = [p for p in parts if p is not None and len(p[0])]
+    if not parts:
+        return None
+    return tuple(np.concatenate([p[q] for p in parts]) for q in range(3))
+
+
+def rigid_coo(xyz, used, node_groups):
+    """Rigid-body modes per node group (fe/solver.py:177-199, rb/splitfe.py:105-120) as COO over
+    node DOFs: 3 translations and 3 rotations with r/20, groups of fewer than 4 used nodes skipped."""
+    rows, cols, vals = [], [], []
+    col = 0
+    for nodes in node_groups:
+        nodes = np.asarray(nodes, np.int64)
+        nodes = nodes[used[nodes]]
+        if len(nodes) < 4:
+            continue
+        r = (xyz[nodes] - xyz[nodes].mean(0)) / 20.0
+        one = np.ones(len(nodes))
+        modes = [((0, one),), ((1, one),), ((2, one),),
+                 ((1, -r[:, 2]), (2, r[:, 1])), ((0, r[:, 2]), (2, -r[:, 0])), ((0, -r[:, 1]), (1, r[:, 0]))]
+        for m in modes:
+            for comp, v in m:
+                rows.append(3 * nodes + comp); cols.append(np.full(len(nodes), col)); vals.append(v)
+            col += 1
+    if col == 0:
+        return None, 0
+    return (np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)), col
+
+
+MG_KEYS = ("min_dofs", "max_levels", "levels", "smoother", "degree", "omega", "npre", "npost")
+SOLVE_KEYS = ("mode", "check_every", "inner_tol")
+
+
+def split_opts(kw):
+    mg = {k: kw.pop(k) for k in list(kw) if k in MG_KEYS}
+    so = {k: kw.pop(k) for k in list(kw) if k in SOLVE_KEYS}
+    if kw:
+        raise TypeError(f"unexpected options: {sorted(kw)}")
+    return mg, so
+
+
+class Problem:
+    """One voxel FE problem on the GPU in reference terms."""
+
+    def __init__(self, grid, lam, mu, active, *, sri, cn=None, orig=None, extra=None, node_groups=None,
+                 device=None, **mg):
+        self.grid = grid
+        self.lam = np.asarray(lam, float)
+        self.mu = np.asarray(mu, float)
+        self.active = np.asarray(active, bool)
+        self.sri = sri
+        self.cn = cn
+        if cn is not None:
+            orig = np.asarray(orig, np.int64)
+            self.copy_orig = orig[grid.nnode:]
+            self.irr_cells, self.irr_nodes = split_cells(grid, self.active, cn)
+        else:
+            self.copy_orig = np.zeros(0, np.int64)
+            self.irr_cells, self.irr_nodes = np.zeros(0, np.int64), np.zeros((0, 8), np.int64)
+        self.nn = grid.nnode + len(self.copy_orig)
+        self.used = used_nodes(grid, self.active, self.nn, self.irr_cells, self.irr_nodes)
+        self.nused = np.flatnonzero(self.used)
+        self.nmap = -np.ones(self.nn, np.int64)
+        self.nmap[self.nused] = np.arange(len(self.nused))
+        self.ndof = 3 * len(self.nused)
+        self.sys = VoxelSystem(grid, sri=sri, active=self.active, lam=self.lam, mu=self.mu,
+                               copy_orig=self.copy_orig, irr_cells=self.irr_cells, irr_nodes=self.irr_nodes,
+                               extra=extra, device=device, **mg)
+        self.device = self.sys.device
+        if node_groups:
+            xyz = grid.node_xyz if cn is None else grid.node_xyz[np.asarray(orig)]
+            Z, ncol = rigid_coo(xyz, self.used, node_groups)
+            if ncol:
+                self.sys.set_rigid(*Z, ncol)
+
+    def full_rhs(self, f_compact):
+        b = np.zeros(3 * self.nn)
+        b.reshape(-1, 3)[self.nused] = np.asarray(f_compact).reshape(-1, 3)
+        return torch.from_numpy(b).to(self.device)
+
+    def solve(self, b_full, x0_full=None, tol=1e-8, maxiter=800, **so):
+        x, info = self.sys.solve(b_full, x0_full, tol=tol, maxiter=maxiter, **so)
+        U = x.view(-1, 3).cpu().numpy().copy()
+        U[~self.used] = np.nan
+        return U, info, x
+
+    def close(self):
+        self.sys.close()
+
+
+def matvec_operator(problem):
+    """scipy LinearOperator on compact DOFs backed by the GPU operator (used for info['K'])."""
+    import scipy.sparse.linalg as spla
+    P = problem
+
+    def mv(v):
+        full = np.zeros(3 * P.nn)
+        full.reshape(-1, 3)[P.nused] = np.asarray(v).reshape(-1, 3)
+        y = P.sys.apply(torch.from_numpy(full).to(P.device)).view(-1, 3).cpu().numpy()
+        return y[P.nused].ravel()
+    return spla.LinearOperator((P.ndof, P.ndof), matvec=mv, dtype=np.float64)

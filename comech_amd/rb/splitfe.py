@@ -1,0 +1,189 @@
+"""Drop-in for the reference's ``rb/splitfe.py`` (voxel FE with split nodes between bones), GPU path.
+
+Names and arguments as ``rb/splitfe.py``: ``split_connectivity_fast`` (and ``split_connectivity``),
+``KL_family``, ``face_nodes_cn``, ``rigid_modes_cn``, ``solve_cn``, ``cell_stress_cn``,
+``element_forces_cn``.  The volumetric term uses selective reduced integration (SRI = True,
+rb/splitfe.py:58).  ``solve_cn`` runs matrix-free GMG-PCG on the GPU; by default it mirrors the
+reference GPU solver (FP64 PCG with an FP32 V-cycle, rb/gpusolve.py:265) with the residual checked
+at the reference cadence.  Extra keyword options as ``comech_amd.fe.solver.solve``.  The laminate
+``aniso`` option is not supported on the GPU path and raises ``NotImplementedError``.
+"""
+import numpy as np
+import scipy.sparse as sp
+import torch
+
+from .. import engine as E
+from ..grid import LOC
+from ..system import _lib, require_gpu
+
+SRI = True
+
+
+def split_connectivity_fast(g, active, part, bone_share):
+    """Split connectivity (rb/splitfe.py:261-305, same result as the loop form :17-55).
+
+    A grid node touched by cells of two or more bones gets one id per bone: the lowest bone keeps
+    the grid id, every further (node, bone) pair gets ``nnode + rank`` in (node, bone) order.  A
+    part-0 (joint/soft) cell at a split node uses the bone with the largest share in that cell, or,
+    when the cell holds none of those bones, the bone with most cells at the node.
+    Returns (cn (ncell,8), orig (nnode+ncopy,), number of split nodes)."""
+    cells = np.flatnonzero(active)
+    corn = g.corners(cells)
+    pc = np.asarray(part)[cells].astype(np.int64)
+    nb = bone_share.shape[1]
+    # incidence of (node, bone) pairs over bone cells, with multiplicity
+    isb = pc > 0
+    pair_keys = (corn[isb] * nb + pc[isb][:, None]).ravel()
+    pairs, mult = np.unique(pair_keys, return_counts=True)
+    p_node, p_bone = np.divmod(pairs, nb)
+    bones_at = np.bincount(p_node, minlength=g.nnode)
+    is_split = bones_at >= 2
+    lead = np.ones(len(pairs), bool)
+    lead[1:] = p_node[1:] != p_node[:-1]
+    gets_copy = is_split[p_node] & ~lead
+    copy_id = np.full(len(pairs), -1, np.int64)
+    copy_id[gets_copy] = g.nnode + np.arange(int(gets_copy.sum()))
+    orig = np.concatenate([np.arange(g.nnode, dtype=np.int64), p_node[gets_copy]])
+    first_pair = np.searchsorted(p_node, np.arange(g.nnode))
+    cn = g.cell_nodes.copy()
+    r, l = np.nonzero(is_split[corn])
+    if len(r):
+        node = corn[r, l]
+        cell = cells[r]
+        bone = pc[r].copy()
+        soft = np.flatnonzero(bone == 0)
+        if len(soft):
+            sn = node[soft]
+            sc = cell[soft]
+            k = bones_at[sn]
+            best_share, best_by_share = np.full(len(soft), -1.0), np.zeros(len(soft), np.int64)
+            best_mult, best_by_mult = np.full(len(soft), -1), np.zeros(len(soft), np.int64)
+            for j in range(int(k.max())):
+                m = np.flatnonzero(j < k)
+                pidx = first_pair[sn[m]] + j
+                b = p_bone[pidx]
+                share = bone_share[sc[m], b]
+                up = share > best_share[m]           # strict: the first maximum wins, as argmax
+                best_share[m[up]] = share[up]; best_by_share[m[up]] = b[up]
+                up = mult[pidx] > best_mult[m]
+                best_mult[m[up]] = mult[pidx][up]; best_by_mult[m[up]] = b[up]
+            bone[soft] = np.where(best_share > 0, best_by_share, best_by_mult)
+        pos = np.searchsorted(pairs, node * nb + bone)
+        cid = copy_id[pos]
+        sel = cid >= 0
+        cn[cell[sel], l[sel]] = cid[sel]
+    return cn, orig, int(is_split.sum())
+
+
+split_connectivity = split_connectivity_fast
+
+
+def KL_family(g):
+    """Volumetric element matrices per dims family: one-point (SRI) or 2x2x2 (rb/splitfe.py:61-69)."""
+    return g.KL_sri if SRI else g.KL
+
+
+def face_nodes_cn(g, cn, cells, face):
+    """rb/splitfe.py:94-102."""
+    from ..grid import face_node_areas
+    return face_node_areas(g, cells, face, cn=cn)
+
+
+def rigid_modes_cn(xyz, nmap, n, node_groups):
+    """Sparse Z on compact DOFs (rb/splitfe.py:105-120)."""
+    used = nmap >= 0
+    Z, ncol = E.rigid_coo(xyz, used, node_groups)
+    if not ncol:
+        return sp.csr_matrix((n, 0))
+    r, c, v = Z
+    return sp.csr_matrix((v, (3 * nmap[r // 3] + r % 3, c)), shape=(n, ncol))
+
+
+def solve_cn(g, lam, mu, active, cn, orig, fixed, loads, extra=None, node_groups=None, tol=1e-6, maxiter=800,
+             aniso=None, device=None, **opts):
+    """rb/splitfe.py:123-158 on the GPU.  fixed: (node_ids, area) ground faces with k = 1e6 N/mm^3
+    per area on 3 components; loads: [(node_ids, area, t[3])]; extra(nmap, n): sparse stiffness on
+    compact DOFs.  Returns U (nnode+ncopy, 3) with NaN for unused nodes, info, reactions R."""
+    if aniso is not None:
+        raise NotImplementedError("laminate (aniso) cells are not supported by the GPU solver")
+    mg, so = E.split_opts(dict(opts))
+    so.setdefault("mode", "ref")
+    so.setdefault("check_every", 5)
+    active = np.asarray(active, bool)
+    orig = np.asarray(orig, np.int64)
+    nn = len(orig)
+    irr_c, irr_n = E.split_cells(g, active, cn)
+    used = E.used_nodes(g, active, nn, irr_c, irr_n)
+    nused = np.flatnonzero(used)
+    nmap = -np.ones(nn, np.int64)
+    nmap[nused] = np.arange(len(nused))
+    n = 3 * len(nused)
+    nb_, ab_ = fixed
+    nb_ = np.asarray(nb_, np.int64)
+    ab_ = np.asarray(ab_, float)
+    okf = used[nb_]
+    parts = [E.diag_coo(nb_[okf], np.repeat((ab_[okf] * 1e6)[:, None], 3, 1))]
+    if extra is not None:
+        parts.append(E.compact_to_full_coo(extra(nmap, n), nused))
+    f = np.zeros(n)
+    for nodes, area, t in loads:
+        ln = nmap[np.asarray(nodes, np.int64)]
+        ok = ln >= 0
+        for c in range(3):
+            np.add.at(f, 3 * ln[ok] + c, np.asarray(area, float)[ok] * t[c])
+    P = E.Problem(g, lam, mu, active, sri=SRI, cn=cn, orig=orig, extra=E.cat_coo(parts), node_groups=node_groups,
+                  device=device, **mg)
+    U, info, _ = P.solve(P.full_rhs(f), None, tol=tol, maxiter=maxiter, **so)
+    R = np.zeros((nn, 3))
+    np.add.at(R, nb_, -(ab_[:, None] * 1e6) * U[nb_])
+    info.update(ndof=n)
+    P.close()
+    return U, info, R
+
+
+def _post(kind, g, lam, mu, U, cells, cn, device):
+    dev = require_gpu(device)
+    lib = _lib.load()
+    cells = np.asarray(cells, np.int64)
+    n = len(cells)
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    Ud = t(np.asarray(U, float).reshape(-1), torch.float64)
+    hx, hy, hz = (t(d, torch.float64) for d in (g.dx, g.dy, g.dz))
+    cd = t(cells, torch.int64)
+    nd = t(np.asarray(cn)[cells].reshape(n, 8), torch.int64)
+    la, ma = t(np.asarray(lam, float), torch.float64), t(np.asarray(mu, float), torch.float64)
+    vp = lambda x: _lib.c_void_p(x.data_ptr())
+    st = _lib.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    if kind == "stress":
+        sig = torch.empty(n * 6, dtype=torch.float64, device=dev)
+        vm = torch.empty(n, dtype=torch.float64, device=dev)
+        eps = torch.empty(n * 6, dtype=torch.float64, device=dev)
+        _lib.check(lib.cm_cell_stress(st, g.nx, g.ny, g.nz, vp(hx), vp(hy), vp(hz), n, vp(cd), vp(nd), vp(la), vp(ma),
+                                      vp(Ud), vp(sig), vp(vm), vp(eps)))
+        return sig.view(n, 6).cpu().numpy(), vm.cpu().numpy(), eps.view(n, 6).cpu().numpy()
+    out = torch.empty(n * 24, dtype=torch.float64, device=dev)
+    _lib.check(lib.cm_element_forces(st, g.nx, g.ny, g.nz, vp(hx), vp(hy), vp(hz), int(SRI), n, vp(cd), vp(nd),
+                                     vp(la), vp(ma), vp(Ud), vp(out)))
+    return out.view(n, 8, 3).cpu().numpy()
+
+
+def cell_stress_cn(g, lam, mu, U, cells, cn, aniso=None, device=None):
+    """rb/splitfe.py:161-175 on the GPU (NaN displacements count as 0)."""
+    if aniso is not None:
+        raise NotImplementedError("laminate (aniso) cells are not supported by the GPU solver")
+    if len(cells) == 0:
+        return np.zeros((0, 6)), np.zeros(0), np.zeros((0, 6))
+    return _post("stress", g, lam, mu, U, cells, cn, device)
+
+
+def element_forces_cn(g, lam, mu, U, cells, cn, aniso=None, device=None):
+    """rb/splitfe.py:178-193 on the GPU: (n, 8, 3) nodal forces K_e u_e."""
+    if aniso is not None:
+        raise NotImplementedError("laminate (aniso) cells are not supported by the GPU solver")
+    if len(cells) == 0:
+        return np.zeros((0, 8, 3))
+    return _post("forces", g, lam, mu, U, cells, cn, device)
+
+
+__all__ = ["split_connectivity_fast", "split_connectivity", "KL_family", "face_nodes_cn", "rigid_modes_cn",
+           "solve_cn", "cell_stress_cn", "element_forces_cn", "SRI", "LOC"]

@@ -1,0 +1,254 @@
+"""Seeded synthetic voxel rasters for the benchmark configurations (inputs in the reference's API).
+
+``config1``  32^3 cube at 1 mm, six random ellipsoidal inclusions (E 17000 in / 700 out, nu 0.3),
+             bottom face held by stiff Winkler springs (k = 1e6 N/mm^3 x tributary area, the
+             reference's "fixed" convention, rb/splitfe.py:143), 1 MPa compression on the top face.
+             fe/solver.py ``solve`` form (2x2x2 integration).
+``config2``  hindfoot-like raster (default 160x96x96 at 0.5 mm): soft-tissue envelope (E 1, nu 0.45)
+             holding three ellipsoid bones (cortical shell 17000 / cancellous core 700, nu 0.3), a
+             coalition cylinder (r = 4 voxels, E 700) bridging two of them, split nodes between
+             bones (rb/splitfe.py:261-305) joined by isotropic springs k = 1e3 N/mm, ~2000 plantar
+             ground springs k = 50 N/mm, 200 ligament springs k = 100 N/mm between bone-surface
+             nodes, 700 N downward over the top 10 cell layers.  rb/splitfe.py ``solve_cn`` form
+             (SRI).  The same generator at other sizes gives the batched and the large
+             configurations (anatomy is relative to the box; shell thickness in voxels).
+``config3``  64 feet x 8 surgical actions with grids between 96x64x64 and 192x112x112.
+
+Seeds: ``numpy.random.default_rng(config_id * 1000 + item)``.
+"""
+import numpy as np
+from scipy import ndimage
+
+from .grid import Grid, face_node_areas, lame
+from .rb.replica3 import spring_pairs
+from .rb.splitfe import split_connectivity_fast
+
+
+def _rotation(rng, max_angle=None):
+    if max_angle is None:
+        q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+        q = q * np.sign(np.diag(r))
+        if np.linalg.det(q) < 0:
+            q[:, 0] = -q[:, 0]
+        return q
+    ang = rng.uniform(-max_angle, max_angle, 3)
+    cx, cy, cz = np.cos(ang)
+    sx, sy, sz = np.sin(ang)
+    Rx = np.array([[1, 0, 0], [0, cx, -sx], [0, sx, cx]])
+    Ry = np.array([[cy, 0, sy], [0, 1, 0], [-sy, 0, cy]])
+    Rz = np.array([[cz, -sz, 0], [sz, cz, 0], [0, 0, 1]])
+    return Rz @ Ry @ Rx
+
+
+def _ellipsoid_radius(P, c, axes, R):
+    """Normalised radius of points P (n,3) in an ellipsoid (centre c, semi-axes, rotation R)."""
+    q = (P - c) @ R
+    return np.sqrt(((q / axes) ** 2).sum(1))
+
+
+def _cell_centres(g):
+    return np.stack([g.xc[g.cI], g.yc[g.cJ], g.zc[g.cK]], 1)
+
+
+# --------------------------------------------------------------------------------------------
+def config1(n=32, h=1.0, seed=1000, n_incl=6):
+    rng = np.random.default_rng(seed)
+    e = np.arange(n + 1) * h
+    g = Grid(e, e, e)
+    P = _cell_centres(g)
+    inside = np.zeros(g.ncell, bool)
+    for _ in range(n_incl):
+        c = rng.uniform(0, n * h, 3)
+        axes = rng.uniform(3, 8, 3) * h
+        R = _rotation(rng)
+        inside |= _ellipsoid_radius(P, c, axes, R) <= 1.0
+    labels = inside.astype(np.uint8)                       # 0 matrix, 1 inclusion
+    E_tab, nu_tab = np.array([700.0, 17000.0]), np.array([0.3, 0.3])
+    lam_tab, mu_tab = lame(E_tab, nu_tab)
+    lam, mu = lam_tab[labels], mu_tab[labels]
+    active = np.ones(g.ncell, bool)
+    bot = np.flatnonzero(g.cK == 0)
+    top = np.flatnonzero(g.cK == g.nz - 1)
+    nb, ab = face_node_areas(g, bot, "z-")
+    nt, at = face_node_areas(g, top, "z+")
+    springs = [(nb, ab, np.array([1e6, 1e6, 1e6]))]
+    loads = [(nt, at, np.array([0.0, 0.0, -1.0]))]
+    return dict(name="config1", grid=g, lam=lam, mu=mu, active=active, labels=labels, lam_tab=lam_tab,
+                mu_tab=mu_tab, springs=springs, loads=loads, sri=False)
+
+
+def solve_inputs_config1(case):
+    """config1 as level-0 GPU inputs: extra COO (diagonal springs) and nodal force vector."""
+    g = case["grid"]
+    nb, ab, kv = case["springs"][0]
+    kd = np.zeros((g.nnode, 3))
+    np.add.at(kd, nb, ab[:, None] * kv[None, :])
+    nodes = np.flatnonzero(kd.any(1))
+    rows = (3 * nodes[:, None] + np.arange(3)).ravel()
+    extra = (rows, rows.copy(), kd[nodes].ravel())
+    f = np.zeros((g.nnode, 3))
+    nt, at, t = case["loads"][0]
+    np.add.at(f, nt, at[:, None] * t[None, :])
+    return extra, f.ravel()
+
+
+# --------------------------------------------------------------------------------------------
+FOOT_E = dict(soft=(1.0, 0.45), coalition=(700.0, 0.3), cortical=(17000.0, 0.3), cancellous=(700.0, 0.3))
+
+
+def config2(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000, n_lig=200, k_iface=1e3, k_ground=50.0,
+            k_lig=100.0, load=700.0, action="baseline", angle_seed=None):
+    """Hindfoot-like raster.  action: baseline | resection | osteotomy | interposition | fusion |
+    load0.5 | load1.5 | load2 (config 3)."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    g = Grid(np.arange(nx + 1) * h, np.arange(ny + 1) * h, np.arange(nz + 1) * h)
+    L = np.array([nx, ny, nz]) * h
+    P = _cell_centres(g)
+    env = _ellipsoid_radius(P, L * [0.5, 0.5, 0.45], L * [0.48, 0.45, 0.6], np.eye(3)) <= 1.0
+    jit = lambda: rng.uniform(-0.03, 0.03, 3) * L
+    bones = [  # calcaneus-, talus-, navicular-like
+        (L * [0.30, 0.50, 0.33] + jit(), L * [0.20, 0.22, 0.20] * rng.uniform(0.95, 1.05, 3)),
+        (L * [0.52, 0.50, 0.62] + jit(), L * [0.15, 0.20, 0.15] * rng.uniform(0.95, 1.05, 3)),
+        (L * [0.70, 0.50, 0.47] + jit(), L * [0.12, 0.18, 0.15] * rng.uniform(0.95, 1.05, 3)),
+    ]
+    rots = [_rotation(rng, 0.25) for _ in bones]
+    rad = np.stack([_ellipsoid_radius(P, c, a, R) for (c, a), R in zip(bones, rots)], 1)
+    inb = (rad <= 1.0) & env[:, None]
+    bone = np.where(inb.any(1), np.argmin(np.where(inb, rad, np.inf), 1) + 1, 0)
+    # coalition: cylinder of radius 4 voxels bridging bones 1 and 2 (cells outside every bone)
+    a0, a1 = bones[0][0], bones[1][0]
+    d = a1 - a0
+    t = np.clip(((P - a0) @ d) / (d @ d), 0, 1)
+    dist = np.linalg.norm(P - (a0 + t[:, None] * d), axis=1)
+    coal = (dist <= 4 * h) & env & (bone == 0)
+    # cortical shell: bone cells within `shell` voxels of the bone boundary
+    shape = (nx, ny, nz)
+    cort = np.zeros(g.ncell, bool)
+    for b in (1, 2, 3):
+        m = (bone == b).reshape(shape)
+        core = ndimage.binary_erosion(m, iterations=shell)
+        cort |= (m & ~core).ravel()
+    labels = np.zeros(g.ncell, np.uint8)           # 0 soft, 1 coalition, 2+2(b-1) cortical, 3+2(b-1) cancellous
+    labels[coal] = 1
+    isb = bone > 0
+    labels[isb] = (2 + 2 * (bone[isb] - 1) + (~cort[isb])).astype(np.uint8)
+    active = env.copy()
+    E_lab = [FOOT_E["soft"], FOOT_E["coalition"]] + [FOOT_E["cortical"], FOOT_E["cancellous"]] * 3
+    E_tab = np.array([e for e, _ in E_lab], float)
+    nu_tab = np.array([v for _, v in E_lab], float)
+    part = np.where(isb, bone, 0).astype(np.int64)
+    k_if = k_iface
+    scale = 1.0
+    if action == "resection":                       # coalition relabelled to soft tissue
+        labels[coal] = 0
+    elif action in ("osteotomy", "interposition"):  # 3-voxel planar slab through bone 1 removed
+        arng = np.random.default_rng(seed if angle_seed is None else angle_seed)
+        nrm = _rotation(arng, 0.5) @ np.array([1.0, 0.0, 0.0])
+        slab = (np.abs((P - bones[0][0]) @ nrm) <= 1.5 * h) & (bone == 1)
+        if action == "osteotomy":
+            active &= ~slab
+        else:                                      # gap filled with E = 5 MPa
+            E_tab = np.append(E_tab, 5.0); nu_tab = np.append(nu_tab, 0.3)
+            labels[slab] = len(E_tab) - 1
+        part[slab] = 0
+    elif action == "fusion":
+        k_if = 1e6
+    elif action.startswith("load"):
+        scale = float(action[4:])
+    labels[~active] = 255
+    lam_tab, mu_tab = lame(E_tab, nu_tab)
+    lab_safe = np.where(active, labels, 0)
+    lam = np.where(active, lam_tab[lab_safe], 0.0)
+    mu = np.where(active, mu_tab[lab_safe], 0.0)
+    part[~active] = 0
+    share = np.zeros((g.ncell, 4), np.float32)
+    share[np.arange(g.ncell), part] = 1.0
+    share[:, 0] = 0.0
+    cn, orig, nsplit = split_connectivity_fast(g, active, part, share)
+    nn = len(orig)
+    # nodes used by active cells
+    used = np.zeros(nn, bool)
+    used[cn[active].ravel()] = True
+    xyz = g.node_xyz[orig]
+    # interface springs: copy <-> its original grid node
+    copies = np.arange(g.nnode, nn)
+    iface = spring_pairs(xyz, np.stack([copies, orig[copies]], 1), k_if, directional=False)
+    # plantar ground springs on used bottom grid nodes
+    bottom = np.flatnonzero(used[: g.nnode] & (np.arange(g.nnode) // ((nx + 1) * (ny + 1)) == 0))
+    gsel = np.sort(rng.choice(bottom, size=min(n_ground, len(bottom)), replace=False))
+    # ligament springs between bone-surface nodes of different bones
+    soft_touch = np.zeros(g.nnode, bool)
+    soft_touch[g.corners(np.flatnonzero(active & (part == 0))).ravel()] = True
+    surf = []
+    for b in (1, 2, 3):
+        nodes_b = np.unique(cn[np.flatnonzero(active & (part == b))].ravel())
+        surf.append(nodes_b[soft_touch[orig[nodes_b]]])
+    pairs = []
+    for _ in range(n_lig):
+        b1, b2 = rng.choice(3, 2, replace=False)
+        if len(surf[b1]) == 0 or len(surf[b2]) == 0:
+            continue
+        a = rng.choice(surf[b1])
+        dd = np.linalg.norm(xyz[surf[b2]] - xyz[a], axis=1)
+        near = surf[b2][np.argsort(dd)[:50]]
+        b_ = rng.choice(near)
+        if np.linalg.norm(xyz[b_] - xyz[a]) > 0:
+            pairs.append((a, b_))
+    lig = spring_pairs(xyz, np.asarray(pairs, np.int64), k_lig, directional=True)
+    # load: downward total over active cells of the top 10 layers, 1/8 per corner
+    topc = np.flatnonzero(active & (g.cK >= nz - 10))
+    ln = cn[topc].ravel()
+    la = np.full(len(ln), scale * load / len(ln))
+    loads = [(ln, la, np.array([0.0, 0.0, -1.0]))]
+    fixed = (gsel, np.full(len(gsel), k_ground / 1e6))
+    extra = (np.concatenate([iface[0], lig[0]]), np.concatenate([iface[1], lig[1]]),
+             np.concatenate([iface[2], lig[2]]))
+
+    def extra_cb(nmap, n):
+        import scipy.sparse as sp
+        r, c, v = extra
+        ok = (nmap[r // 3] >= 0) & (nmap[c // 3] >= 0)
+        rr = 3 * nmap[r[ok] // 3] + r[ok] % 3
+        cc = 3 * nmap[c[ok] // 3] + c[ok] % 3
+        return sp.csr_matrix((v[ok], (rr, cc)), shape=(n, n))
+
+    return dict(name="config2", grid=g, lam=lam, mu=mu, active=active, labels=labels, lam_tab=lam_tab,
+                mu_tab=mu_tab, part=part, bone_share=share, cn=cn, orig=orig, nsplit=nsplit, fixed=fixed,
+                loads=loads, extra=extra, extra_cb=extra_cb, sri=True, action=action, used=used,
+                n_ground=len(gsel), n_lig=len(pairs), n_copies=len(copies))
+
+
+def solve_inputs_config2(case):
+    """config2 as level-0 GPU inputs: extra COO over node DOFs (ground + interface + ligament) and
+    the nodal force vector (node layout, grid nodes then copies)."""
+    g = case["grid"]
+    nn = len(case["orig"])
+    gs, ga = case["fixed"]
+    rows = (3 * np.asarray(gs)[:, None] + np.arange(3)).ravel()
+    ground = (rows, rows.copy(), np.repeat(ga * 1e6, 3))
+    r, c, v = case["extra"]
+    extra = (np.concatenate([ground[0], r]), np.concatenate([ground[1], c]), np.concatenate([ground[2], v]))
+    f = np.zeros((nn, 3))
+    for nodes, area, t in case["loads"]:
+        np.add.at(f, nodes, area[:, None] * t[None, :])
+    return extra, f.ravel()
+
+
+ACTIONS = ["baseline", "resection", "osteotomy", "interposition", "fusion", "load0.5", "load1.5", "load2"]
+
+
+def config3_sizes(n_feet=64, seed=3000):
+    """Grid sizes of the batched configuration: uniform between 96x64x64 and 192x112x112 (even)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.array([96, 64, 64]), np.array([192, 112, 112])
+    return [tuple(int(2 * round(v / 2)) for v in rng.uniform(lo, hi)) for _ in range(n_feet)]
+
+
+def config3_case(foot, action, n_feet=64, h=0.5):
+    dims = config3_sizes(n_feet)[foot]
+    return config2(dims=dims, h=h, seed=3000 + foot, action=action, angle_seed=3500 + foot)
+
+
+def config5(dims=(384, 256, 256), h=0.25, seed=5000):
+    return config2(dims=dims, h=h, seed=seed, shell=4)

@@ -1,0 +1,235 @@
+"""Host wrapper of one GPU voxel system (ctypes over libcomech_hip.so).
+
+A ``VoxelSystem`` owns the device hierarchy of one problem: the fine level is matrix-free
+(per-cell labels or materials, split copies, irregular cells, sparse extra stiffness), coarse levels
+hold exact Galerkin moment cells, the coarsest level a dense FP64 inverse.  Vectors are torch CUDA
+tensors in the level-0 node layout (3 values per node, grid nodes in the reference's numbering
+followed by split copies).  PyTorch only provides device memory and the stream.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import P_double, P_ll, P_u8, check, ptr
+
+MODES = {"fp64": 0, "ref": 1, "mixed": 2}
+SMOOTHERS = {"chebyshev": 0, "jacobi": 1}
+PREC = {"f64": 0, "fp64": 0, torch.float64: 0, "f32": 1, "fp32": 1, torch.float32: 1}
+
+
+def require_gpu(device=None):
+    if not torch.cuda.is_available():
+        raise RuntimeError("comech_amd: no HIP device visible; the solver runs on the GPU only (no CPU fallback)")
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    if dev.type != "cuda":
+        raise RuntimeError(f"comech_amd: device {dev} is not a GPU")
+    return dev
+
+
+def compress_materials(lam, mu, active, max_labels=255):
+    """Per-cell (lam, mu) -> (table_lam, table_mu, labels) when the active cells use at most
+    ``max_labels`` distinct pairs (exact: the table holds the cells' own values), else None."""
+    act = np.asarray(active, bool)
+    la, ma = np.asarray(lam, float)[act], np.asarray(mu, float)[act]
+    ul, il = np.unique(la, return_inverse=True)
+    if len(ul) > max_labels:
+        return None
+    um, im = np.unique(ma, return_inverse=True)
+    if len(um) > max_labels:
+        return None
+    code = il.astype(np.int64) * len(um) + im
+    uc, ic = np.unique(code, return_inverse=True)
+    if len(uc) > max_labels:
+        return None
+    lab = np.full(len(act), 255, np.uint8)
+    lab[act] = ic.astype(np.uint8)
+    return ul[uc // len(um)], um[uc % len(um)], lab
+
+
+class VoxelSystem:
+    def __init__(self, grid, *, sri, active, lam=None, mu=None, labels=None, lam_tab=None, mu_tab=None,
+                 copy_orig=None, irr_cells=None, irr_nodes=None, extra=None, used=None, min_dofs=6000, max_levels=6,
+                 levels=None, smoother="chebyshev", degree=3, omega=0.6, npre=2, npost=2, device=None):
+        """grid: Grid; active: (ncell,) bool in reference cell order; materials either per cell
+        (lam, mu) or as labels + tables; split connectivity as copy originals and irregular cells
+        with their 8 node ids; extra: (rows, cols, vals) COO over level-0 DOFs."""
+        self.device = require_gpu(device)
+        self.lib = _lib.load()
+        self.grid = grid
+        self.sri = bool(sri)
+        self.stream = torch.cuda.current_stream(self.device)
+        self.nng = grid.nnode
+        self.ncopy = 0 if copy_orig is None else len(copy_orig)
+        self.nn = self.nng + self.ncopy
+        self.active = np.asarray(active, bool)
+        h = ctypes.c_void_p()
+        xe, ye, ze = (np.ascontiguousarray(e, np.float64) for e in (grid.xe, grid.ye, grid.ze))
+        with torch.cuda.device(self.device):
+            check(self.lib.cm_sys_create(ctypes.byref(h), ctypes.c_void_p(self.stream.cuda_stream), grid.nx, grid.ny,
+                                         grid.nz, ptr(xe, ctypes.c_double), ptr(ye, ctypes.c_double),
+                                         ptr(ze, ctypes.c_double), int(self.sri)))
+        self.h = h
+        try:
+            self._describe(labels, lam_tab, mu_tab, lam, mu, copy_orig, irr_cells, irr_nodes, extra)
+            if used is not None:
+                um = np.ascontiguousarray(used, np.uint8)
+                check(self.lib.cm_sys_set_used(self.h, len(um), ptr(um, ctypes.c_uint8)))
+            fixed = 0 if levels is None else int(levels)
+            with torch.cuda.device(self.device):
+                check(self.lib.cm_sys_build(self.h, int(min_dofs), int(max_levels), fixed, SMOOTHERS[smoother],
+                                            int(degree), float(omega), int(npre), int(npost)))
+        except Exception:
+            self.close()
+            raise
+        self.smoother = smoother
+
+    # ------------------------------------------------------------------ description
+    def _describe(self, labels, lam_tab, mu_tab, lam, mu, copy_orig, irr_cells, irr_nodes, extra):
+        lib = self.lib
+        if labels is None:
+            comp = compress_materials(lam, mu, self.active)
+            if comp is not None:
+                lam_tab, mu_tab, labels = comp
+        if labels is not None:
+            lt = np.ascontiguousarray(lam_tab, np.float64)
+            mt = np.ascontiguousarray(mu_tab, np.float64)
+            lb = np.ascontiguousarray(labels, np.uint8).copy()
+            lb[~self.active] = 255
+            check(lib.cm_sys_set_labels(self.h, len(lt), ptr(lt, ctypes.c_double), ptr(mt, ctypes.c_double),
+                                        ptr(lb, ctypes.c_uint8)))
+            self.label_mode = True
+            self.lam_tab, self.mu_tab, self.labels = lt, mt, lb
+        else:
+            la = np.ascontiguousarray(lam, np.float64)
+            ma = np.ascontiguousarray(mu, np.float64)
+            ac = np.ascontiguousarray(self.active, np.uint8)
+            check(lib.cm_sys_set_cellmat(self.h, ptr(la, ctypes.c_double), ptr(ma, ctypes.c_double),
+                                         ptr(ac, ctypes.c_uint8)))
+            self.label_mode = False
+        co = np.ascontiguousarray(np.zeros(0) if copy_orig is None else copy_orig, np.int64)
+        ic = np.ascontiguousarray(np.zeros(0) if irr_cells is None else irr_cells, np.int64)
+        inod = np.ascontiguousarray(np.zeros((0, 8)) if irr_nodes is None else irr_nodes, np.int64)
+        check(lib.cm_sys_set_split(self.h, len(co), ptr(co, ctypes.c_longlong), len(ic), ptr(ic, ctypes.c_longlong),
+                                   ptr(inod, ctypes.c_longlong)))
+        self.irr_cells, self.irr_nodes = ic, inod
+        if extra is not None:
+            r, c, v = (np.ascontiguousarray(a, t) for a, t in zip(extra, (np.int64, np.int64, np.float64)))
+            check(lib.cm_sys_set_extra(self.h, len(r), ptr(r, ctypes.c_longlong), ptr(c, ctypes.c_longlong),
+                                       ptr(v, ctypes.c_double)))
+
+    # ------------------------------------------------------------------ info
+    def levels(self):
+        cap = 16
+        desc = np.zeros(6 * cap, np.int64)
+        n = ctypes.c_int()
+        check(self.lib.cm_sys_levels(self.h, ctypes.byref(n), ptr(desc, ctypes.c_longlong), cap))
+        d = desc[: 6 * n.value].reshape(-1, 6)
+        return [dict(nx=int(r[0]), ny=int(r[1]), nz=int(r[2]), nodes=int(r[3]), ndof=int(r[4]), extra_rows=int(r[5]))
+                for r in d]
+
+    def zeros(self, dtype=torch.float64, level=0):
+        nn = self.nn if level == 0 else self.levels()[level]["nodes"]
+        return torch.zeros(3 * nn, dtype=dtype, device=self.device)
+
+    # ------------------------------------------------------------------ compute
+    def _ck(self, t, dtype, n):
+        if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == dtype and t.is_contiguous() and t.numel() == n):
+            raise ValueError(f"expected a contiguous CUDA {dtype} tensor of {n} values")
+
+    def apply(self, u, y=None, level=0):
+        prec = PREC[u.dtype]
+        n = u.numel()
+        y = torch.empty_like(u) if y is None else y
+        self._ck(u, u.dtype, n)
+        self._ck(y, u.dtype, n)
+        check(self.lib.cm_sys_apply(self.h, level, prec, ctypes.c_void_p(u.data_ptr()), ctypes.c_void_p(y.data_ptr())))
+        return y
+
+    def vcycle(self, b, x=None):
+        prec = PREC[b.dtype]
+        x = torch.empty_like(b) if x is None else x
+        self._ck(b, b.dtype, 3 * self.nn)
+        self._ck(x, b.dtype, 3 * self.nn)
+        check(self.lib.cm_sys_vcycle(self.h, prec, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr())))
+        return x
+
+    def solve(self, b, x0=None, tol=1e-8, maxiter=800, mode="fp64", check_every=5, inner_tol=1e-4):
+        """PCG on A x = b (FP64 device vectors, level-0 layout).  Returns (x, info)."""
+        n = 3 * self.nn
+        self._ck(b, torch.float64, n)
+        x = torch.zeros(n, dtype=torch.float64, device=self.device) if x0 is None else x0.clone()
+        self._ck(x, torch.float64, n)
+        info = _lib.SolveInfo()
+        check(self.lib.cm_sys_solve(self.h, MODES[mode], ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                    float(tol), int(maxiter), int(check_every), float(inner_tol), ctypes.byref(info)))
+        out = dict(iters=info.iters, flag=info.flag, outer=info.outer, levels=info.levels, relres=info.relres,
+                   t_solve_ms=info.t_solve_ms, t_setup_ms=info.t_setup_ms, mode=mode, device=str(self.device))
+        return x, out
+
+    def time_apply(self, u, y, nrep=20, structured_only=False, level=0):
+        ms = ctypes.c_double()
+        check(self.lib.cm_sys_time_apply(self.h, level, PREC[u.dtype], ctypes.c_void_p(u.data_ptr()),
+                                         ctypes.c_void_p(y.data_ptr()), int(nrep), int(bool(structured_only)),
+                                         ctypes.byref(ms)))
+        return ms.value
+
+    def set_kz(self, kz):
+        check(self.lib.cm_sys_set_kz(self.h, int(kz)))
+
+    def set_rigid(self, rows, cols, vals, ncol):
+        """Rigid-mode correction Z Einv Z^T (fe/gmg.py:127-135): Z as COO over level-0 DOFs.
+        E = sym(Z^T A Z) is formed with the GPU operator, Einv = pinv(E) (ncol x ncol, host)."""
+        rows = np.ascontiguousarray(rows, np.int64)
+        cols = np.ascontiguousarray(cols, np.int64)
+        vals = np.ascontiguousarray(vals, np.float64)
+        n = 3 * self.nn
+        Zd = torch.zeros((ncol, n), dtype=torch.float64, device=self.device)
+        Zd[torch.from_numpy(cols).to(self.device), torch.from_numpy(rows).to(self.device)] = \
+            torch.from_numpy(vals).to(self.device)
+        AZ = torch.stack([self.apply(Zd[j].contiguous()) for j in range(ncol)])
+        E = (Zd @ AZ.T).cpu().numpy()
+        Einv = np.ascontiguousarray(np.linalg.pinv(0.5 * (E + E.T)))
+        check(self.lib.cm_sys_set_rigid(self.h, int(ncol), len(rows), ptr(rows, ctypes.c_longlong),
+                                        ptr(cols, ctypes.c_longlong), ptr(vals, ctypes.c_double),
+                                        ptr(Einv, ctypes.c_double)))
+
+    def sync(self):
+        check(self.lib.cm_sys_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self.lib.cm_sys_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def cell_stress_gpu(grid, lam, mu, U, cells, nodes8=None, device=None):
+    """Cell-centre stress/strain/von Mises on the GPU (fe/solver.py:257-272).  U: host (nnode,3)
+    or device tensor; returns host arrays (sig, vm, eps)."""
+    dev = require_gpu(device)
+    lib = _lib.load()
+    cells = np.asarray(cells, np.int64)
+    n = len(cells)
+    if n == 0:
+        return np.zeros((0, 6)), np.zeros(0), np.zeros((0, 6))
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    Ud = U if isinstance(U, torch.Tensor) else t(np.asarray(U, float).reshape(-1), torch.float64)
+    hx, hy, hz = t(grid.dx, torch.float64), t(grid.dy, torch.float64), t(grid.dz, torch.float64)
+    cd = t(cells, torch.int64)
+    nd = None if nodes8 is None else t(np.asarray(nodes8, np.int64).reshape(n, 8), torch.int64)
+    la, ma = t(np.asarray(lam, float), torch.float64), t(np.asarray(mu, float), torch.float64)
+    sig = torch.empty(n * 6, dtype=torch.float64, device=dev)
+    vm = torch.empty(n, dtype=torch.float64, device=dev)
+    eps = torch.empty(n * 6, dtype=torch.float64, device=dev)
+    vp = lambda x: ctypes.c_void_p(0 if x is None else x.data_ptr())
+    check(lib.cm_cell_stress(ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream), grid.nx, grid.ny, grid.nz,
+                             vp(hx), vp(hy), vp(hz), n, vp(cd), vp(nd), vp(la), vp(ma), vp(Ud), vp(sig), vp(vm),
+                             vp(eps)))
+    return sig.view(n, 6).cpu().numpy(), vm.cpu().numpy(), eps.view(n, 6).cpu().numpy()

@@ -1,0 +1,150 @@
+/* C ABI of the MI355X voxel finite-element solve path (libcomech_hip.so).
+ *
+ * The reference (xtli12/CoMech) is pure Python; its GPU solver plugs in by replacing the module
+ * attribute fe.gmg.pcg_gmg with rb.gpusolve.pcg_gmg_gpu (rb/rbfe.py:20-27).  This library is what
+ * the Python drop-ins of that surface (comech_amd.fe.solver, comech_amd.fe.gmg,
+ * comech_amd.rb.splitfe, comech_amd.rb.gpusolve) bind through ctypes.  Each entry point below names
+ * the reference interface it serves.
+ *
+ * Conventions
+ *  - Return value 0 on success, -1 on failure; cm_last_error() gives the message (thread-local).
+ *  - Host pointers are read during the call only.  "device" pointers are HIP device memory owned
+ *    by the caller.
+ *  - Node vectors are AoS with 3 components per node.  Level-0 node ids are the reference's grid
+ *    ids i + (nx+1)(j + (ny+1)k) (fe/solver.py:107) followed by split-node copies nnode + r
+ *    (rb/splitfe.py:268-280).  Cell ids passed in are the reference's z-fastest ids
+ *    (i*ny + j)*nz + k (fe/solver.py:104-105,128).
+ *  - Precision codes: 0 = FP64, 1 = FP32.
+ */
+#ifndef COMECH_FE_H
+#define COMECH_FE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CM_ABI_VERSION 1
+
+typedef struct cm_sys cm_sys;
+
+/* Result of cm_sys_solve; mirrors the info dict of rb/gpusolve.py:259-260,285. */
+typedef struct cm_solve_info {
+  int iters;        /* PCG iterations (all inner iterations in mixed mode) */
+  int flag;         /* 0 converged, else maxiter (rb/gpusolve.py:252-254) */
+  int outer;        /* refinement steps (mixed mode), 1 otherwise */
+  int levels;       /* multigrid levels */
+  double relres;    /* ||b - A x|| / ||b|| (FP64) at exit */
+  double t_solve_ms;
+  double t_setup_ms;
+} cm_solve_info;
+
+const char* cm_last_error(void);
+int cm_version(void);
+int cm_device_count(int* n);
+
+/* ---- system description (replaces the CSR assembly of fe/solver.py:137-162 and
+ *      rb/splitfe.py:72-91 / rb/gpusolve.py:306-347: the GPU path is matrix-free) ------------- */
+
+/* Grid of nx*ny*nz cells with edges xe[nx+1], ye[ny+1], ze[nz+1] (fe/solver.py:93-100).
+ * sri = 1: volumetric term at the cell centre (rb/splitfe.py:58-69); 0: 2x2x2 Gauss
+ * (fe/solver.py:77-90).  stream: hipStream_t for every launch (0 = default stream). */
+int cm_sys_create(cm_sys** out, void* stream, int nx, int ny, int nz, const double* xe, const double* ye,
+                  const double* ze, int sri);
+int cm_sys_destroy(cm_sys* sys);
+
+/* Materials as labels: lam_tab/mu_tab[nmat] (nmat <= 255), lab_ref[ncell] in reference cell order,
+ * 255 = inactive cell.  Equivalent to lam[cells], mu[cells] of fe/solver.py:152. */
+int cm_sys_set_labels(cm_sys* sys, int nmat, const double* lam_tab, const double* mu_tab, const uint8_t* lab_ref);
+
+/* Materials per cell: lam_ref, mu_ref, active_ref[ncell] (fe/solver.py:131-152). */
+int cm_sys_set_cellmat(cm_sys* sys, const double* lam_ref, const double* mu_ref, const uint8_t* active_ref);
+
+/* Split connectivity (rb/splitfe.py:261-305): copy_orig[ncopy] = original grid node of copy r;
+ * irregular cells (active cells whose cn row differs from the grid corners) with their 8 node
+ * ids in local order l = a + 2b + 4c (fe/solver.py:11-12). */
+int cm_sys_set_split(cm_sys* sys, int64_t ncopy, const int64_t* copy_orig, int64_t nirr, const int64_t* irr_cell_ref,
+                     const int64_t* irr_nodes);
+
+/* Extra stiffness in COO over level-0 node DOFs 3*node+c: diagonal springs (fe/solver.py:210-220,
+ * rb/splitfe.py:139-148), rank-1 penalties (fe/solver.py:222-232), ligament springs
+ * (rb/replica3.py:68-86) and interface springs.  Duplicates are summed. */
+int cm_sys_set_extra(cm_sys* sys, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals);
+
+/* Mark nodes as unknowns in addition to the corners of active cells (nn = grid nodes + copies).
+ * Used when the stiffness comes only from cm_sys_set_extra, as for the CSR input of
+ * fe/gmg.py:120 pcg_gmg(A, b, used_nodes, ...). */
+int cm_sys_set_used(cm_sys* sys, int64_t nn, const uint8_t* used);
+
+/* Additive rigid-mode correction z += Z Einv Z^T r of fe/gmg.py:127-135 / rb/gpusolve.py:217-228:
+ * Z in COO over level-0 DOFs (ncol columns), Einv dense ncol x ncol (row-major). */
+int cm_sys_set_rigid(cm_sys* sys, int ncol, int64_t nnz, const int64_t* rows, const int64_t* cols,
+                     const double* vals, const double* Einv);
+
+/* Build the multigrid hierarchy (fe/gmg.py:79-102, rb/gpusolve.py:39-65): coarsen 2x (every other
+ * node plus the last) until the level has <= min_dofs active DOFs or max_levels levels
+ * (fixed_levels > 0 forces that count).  smoother 0 = Chebyshev of degree `degree` with 3x3
+ * block Jacobi (fe/gmg.py:47-76), 1 = damped block Jacobi with npre/npost sweeps. */
+int cm_sys_build(cm_sys* sys, int min_dofs, int max_levels, int fixed_levels, int smoother, int degree, double omega,
+                 int npre, int npost);
+
+/* Level table: 6 int64 per level: nx, ny, nz, nodes (incl. copies), compact DOFs, extra rows. */
+int cm_sys_levels(cm_sys* sys, int* nlev, int64_t* desc, int cap);
+
+/* ---- compute (device pointers) ------------------------------------------------------------- */
+
+/* y = A_level u (A64 @ p of rb/gpusolve.py:239; L.A @ x of :104). */
+int cm_sys_apply(cm_sys* sys, int level, int prec, const void* u, void* y);
+
+/* x = V-cycle(b) on level 0 (GPUMG.vcycle, rb/gpusolve.py:163-171). */
+int cm_sys_vcycle(cm_sys* sys, int prec, const void* b, void* x);
+
+/* PCG to ||b - A x|| <= tol ||b|| (rb/gpusolve.py:206-262, fe/gmg.py:120-137).  b, x: FP64
+ * device, level-0 layout; x holds the initial guess (x0) on entry.
+ * mode 0: FP64 PCG, FP64 V-cycle.  mode 1: FP64 PCG, FP32 V-cycle (mg_dtype="float32" of
+ * rb/gpusolve.py:265).  mode 2: FP64 iterative refinement around FP32 PCG with FP64 reductions,
+ * inner relative tolerance inner_tol.  The residual is checked at iterations < check_every and
+ * at multiples of check_every (rb/gpusolve.py:244). */
+int cm_sys_solve(cm_sys* sys, int mode, const double* b, double* x, double tol, int maxiter, int check_every,
+                 double inner_tol, cm_solve_info* info);
+
+/* Cell-centre strain/stress/von Mises (fe/solver.py:257-272, rb/splitfe.py:161-175) for n listed
+ * cells.  hx/hy/hz: cell sizes per axis; cells_ref: reference cell ids; nodes8: 8 node ids per
+ * listed cell or NULL for the grid's own corners; lam_ref/mu_ref indexed by reference cell id;
+ * U: nodal displacement (3 per node, NaN -> 0); outputs sig[n*6], vm[n], eps[n*6].  All device. */
+int cm_cell_stress(void* stream, int nx, int ny, int nz, const double* hx, const double* hy, const double* hz,
+                   int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* lam_ref,
+                   const double* mu_ref, const double* U, double* sig, double* vm, double* eps);
+
+/* Nodal element forces K_e u_e (rb/splitfe.py:178-193) of n listed cells, out[n*24] in element
+ * DOF order 3*l + c; arguments as cm_cell_stress, sri as cm_sys_create.  All device. */
+int cm_element_forces(void* stream, int nx, int ny, int nz, const double* hx, const double* hy, const double* hz,
+                      int sri, int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* lam_ref,
+                      const double* mu_ref, const double* U, double* out);
+
+/* Average kernel time of nrep launches of the level-0 operator (structured_only = 1: the
+ * structured stencil kernel alone), measured with HIP events on the system's stream. */
+int cm_sys_time_apply(cm_sys* sys, int level, int prec, const void* u, void* y, int nrep, int structured_only,
+                      double* avg_ms);
+
+/* Override the z extent of a structured block (0 = automatic). */
+int cm_sys_set_kz(cm_sys* sys, int kz);
+
+int cm_sys_sync(cm_sys* sys);
+
+/* Analysis-grid resampling of rb/rbsim.py:110-140 (sample_fe).  Per axis sample tables (FE cell
+ * index, local coordinate, inside flag) for mx*sub, my*sub, mz*sub samples; FE grid nx,ny,nz;
+ * active[ncell], cn[ncell*8] (reference cell order), VM[ncell], SIG[ncell*6] (float32),
+ * Urel[nodes*3] (FP64).  Outputs per analysis cell (reference order).  All device pointers. */
+int cm_sample_fe(void* stream, int mx, int my, int mz, int sub, int nx, int ny, int nz, const int* si,
+                 const double* st, const uint8_t* sin_, const int* sj, const double* su, const uint8_t* sjn,
+                 const int* sk, const double* sv, const uint8_t* skn, const uint8_t* active, const int64_t* cn,
+                 const float* VM, const float* SIG, const double* Urel, float* vm_out, float* u_out, double* cover,
+                 float* sig_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COMECH_FE_H */

@@ -41,6 +41,7 @@ SIGNATURES = {
     "cm_element_forces": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_ll, c_void_p,
                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "cm_sys_time_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, P_double],
+    "cm_sys_autotune": [c_void_p, c_int, c_int],
     "cm_sys_set_kz": [c_void_p, c_int],
     "cm_sys_sync": [c_void_p],
     "cm_sample_fe": [c_void_p] + [c_int] * 7 + [c_void_p] * 9 + [c_void_p] * 5 + [c_void_p] * 4,

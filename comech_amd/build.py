@@ -17,6 +17,8 @@ FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared", "-mun
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
 
 SOURCES = [os.path.join(CSRC, "system.hip")]
+HOST_LIB = os.path.join(HERE, "libcomech_host.so")
+HOST_SOURCES = [os.path.join(CSRC, "host_element.cpp")]
 
 
 def build(verbose=False):
@@ -30,7 +32,19 @@ def build(verbose=False):
     if verbose and err.strip():
         print(err, file=sys.stderr)
     os.replace(tmp, LIB)
+    build_host()
     return LIB
+
+
+def build_host():
+    """CPU build of the element math (g++), for tests without a GPU."""
+    tmp = HOST_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-o", tmp] + HOST_SOURCES
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("g++ failed:\n" + res.stderr)
+    os.replace(tmp, HOST_LIB)
+    return HOST_LIB
 
 
 if __name__ == "__main__":

@@ -101,7 +101,7 @@ struct cell_traits<FineValueCells<T, SRI>> {
 // Structured apply, constant-material (level 0) cells.
 template <typename T, class Cells, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_apply_fine(GridArgs<T> g, Cells cells, const T* __restrict__ u,
-                                                       T* __restrict__ y, const T* __restrict__ b, int kz) {
+                                                       T* y, const T* b, int kz) {
   constexpr bool SRI = cell_traits<Cells>::sri;
   __shared__ T s_u[2][TN * TN * 3];
   __shared__ T s_g[BLOCK * 9];
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(BLOCK) void k_apply_fine(GridArgs<T> g, Cells cells
 // Structured apply, coarse levels (moment cells).
 template <typename T, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_apply_mom(GridArgs<T> g, MomentCells<T> cells, const T* __restrict__ u,
-                                                      T* __restrict__ y, const T* __restrict__ b, int kz) {
+                                                      T* y, const T* b, int kz) {
   __shared__ T s_u[2][TN * TN * 3];
   __shared__ T s_g[BLOCK * 9];
   const int tid = threadIdx.x;
@@ -313,16 +313,22 @@ __global__ void k_apply_irregular(int nirr, const int* __restrict__ nodes, const
 }
 
 // Sparse extra stiffness (springs, penalties, ligaments; Galerkin-coarsened on coarse levels):
-// y[row] += sign * sum_k val[k] u[col[k]] over the listed rows (rows unique).
-template <typename T>
+// y[row] += sign * sum_k val[k] u[col[k]] over the listed rows (rows unique).  W lanes share a row
+// (strided over its entries, shuffle-reduced), W chosen per level from the mean row length.
+template <typename T, int W>
 __global__ void k_apply_extra(int nrow, const int* __restrict__ rows, const int* __restrict__ ptr,
                               const int* __restrict__ cols, const T* __restrict__ vals, const T* __restrict__ u,
-                              T* __restrict__ y, T sign) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrow) return;
+                              T* y, T sign) {
+  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = (int)(gid / W), lane = (int)(gid % W);
+  if (r >= nrow) return;   // the W lanes of a row leave together
   T s = T(0);
-  for (int q = ptr[r]; q < ptr[r + 1]; ++q) s += vals[q] * u[cols[q]];
-  y[rows[r]] += sign * s;
+  for (int q = ptr[r] + lane; q < ptr[r + 1]; q += W) s += vals[q] * u[cols[q]];
+  if (W > 1) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) s += __shfl_down(s, off, W);
+  }
+  if (lane == 0) y[rows[r]] += sign * s;
 }
 
 template <typename T>

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <memory>
 #include <numeric>
+#include <type_traits>
 #include <unordered_map>
 
 #include "../../include/comech_fe.h"
@@ -55,6 +56,8 @@ struct Level {
   std::vector<int> xrow_h, xptr_h, xcol_h;
   std::vector<double> xval_h;
   DevBuf<int> xrow, xptr, xcol;
+  int xw = 1;                 // lanes per extra row
+  int kz[2] = {0, 0};         // z extent of a structured block per precision (0 = heuristic)
   // level 0 only
   bool label_mode = false;
   DevBuf<uint8_t> lab;                      // apply labels (x-fastest, irregular -> 255)
@@ -348,8 +351,14 @@ struct System {
 
   static void set_extra_sorted(Level& lv, const std::vector<std::pair<long long, double>>& ent, long long ncd) {
     lv.xrow_h.clear(); lv.xptr_h.clear(); lv.xcol_h.clear(); lv.xval_h.clear();
+    // Galerkin products of node-pair springs between a copy and its original cancel exactly on
+    // coarse levels (both ends are interpolated alike); drop such entries
+    double vmax = 0.0;
+    for (auto& e : ent) vmax = std::max(vmax, std::fabs(e.second));
+    const double drop = 1e-14 * vmax;
     long long last = -1;
     for (auto& e : ent) {
+      if (std::fabs(e.second) <= drop) continue;
       const long long R = e.first / ncd, C = e.first % ncd;
       if (R != last) {
         lv.xrow_h.push_back((int)R);
@@ -360,6 +369,11 @@ struct System {
       lv.xval_h.push_back(e.second);
     }
     lv.xptr_h.push_back((int)lv.xcol_h.size());
+    const size_t nr = lv.xrow_h.size();
+    const double avg = nr ? (double)lv.xcol_h.size() / nr : 1.0;
+    int w = 1;
+    while (w < 64 && w < avg) w <<= 1;
+    lv.xw = w;
   }
 
   long long compact_dofs(int l) {
@@ -387,7 +401,6 @@ struct System {
       const bool last = fixed_levels > 0 ? (l == fixed_levels - 1) : (compact_dofs(l) <= min_dofs || l == nl_max - 1);
       if (last) break;
       Level& f = lv(l);
-      CM_CHECK(f.nx >= 2 && f.ny >= 2 && f.nz >= 2, "grid too small for another multigrid level");
       auto c = std::make_unique<Level>();
       for (int ax = 0; ax < 3; ++ax) axis_transfer(f.e[ax], f, *c, ax);
       c->nx = (int)c->e[0].size() - 1; c->ny = (int)c->e[1].size() - 1; c->nz = (int)c->e[2].size() - 1;
@@ -418,8 +431,8 @@ struct System {
       v.xrow.upload(v.xrow_h, stream);
       v.xptr.upload(v.xptr_h, stream);
       v.xcol.upload(v.xcol_h, stream);
-      prepare_prec<double>(l, /*dinv later*/ false);
     }
+    prepare_prec<double>(0, false);
     // moments of coarse cells (FP64)
     for (int l = 1; l < nlev(); ++l) {
       Level& c = lv(l);
@@ -436,6 +449,7 @@ struct System {
                            f.nz, ax, ay, az, f.mom64.p, c.mom64.p);
       }
       CM_LAUNCH_CHECK();
+      prepare_prec<double>(l, false);   // casts the moments just computed
     }
     // diagonal blocks (FP64) -> Dinv per precision is derived lazily
     for (int l = 0; l + 1 < nlev(); ++l) compute_diag(l);
@@ -595,7 +609,7 @@ struct System {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
     GridArgs<T> g{v.nx, v.ny, v.nz, d.hx.p, d.hy.p, d.hz.p};
-    const int kz = pick_kz(v);
+    const int kz = pick_kz<T>(v);
     dim3 grid(div_up(v.nx + 1, TOUT), div_up(v.ny + 1, TOUT), div_up(v.nz + 1, kz));
     const T sign = mode == MODE_APPLY ? T(1) : T(-1);
     if (l == 0) {
@@ -623,10 +637,21 @@ struct System {
         hipLaunchKernelGGL((k_apply_mom<T, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, mc, u, y, b, kz);
     }
     const int nrow = (int)v.xrow_h.size();
-    if (nrow)
-      hipLaunchKernelGGL(k_apply_extra<T>, grid1(nrow), dim3(256), 0, stream, nrow, v.xrow.p, v.xptr.p, v.xcol.p,
-                         d.xval.p, u, y, sign);
+    if (nrow) launch_extra<T>(v, d, u, y, sign);
     CM_LAUNCH_CHECK();
+  }
+
+  template <typename T>
+  void launch_extra(Level& v, LevelT<T>& d, const T* u, T* y, T sign) {
+    const int nrow = (int)v.xrow_h.size();
+    const dim3 g = grid1((long long)nrow * v.xw);
+    switch (v.xw) {
+#define CM_XW(W) \
+  case W: hipLaunchKernelGGL((k_apply_extra<T, W>), g, dim3(256), 0, stream, nrow, v.xrow.p, v.xptr.p, v.xcol.p, d.xval.p, u, y, sign); break;
+      CM_XW(1) CM_XW(2) CM_XW(4) CM_XW(8) CM_XW(16) CM_XW(32) CM_XW(64)
+#undef CM_XW
+      default: throw Error("bad extra width");
+    }
   }
 
   // structured part only (timing)
@@ -635,7 +660,7 @@ struct System {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
     GridArgs<T> g{v.nx, v.ny, v.nz, d.hx.p, d.hy.p, d.hz.p};
-    const int kz = pick_kz(v);
+    const int kz = pick_kz<T>(v);
     dim3 grid(div_up(v.nx + 1, TOUT), div_up(v.ny + 1, TOUT), div_up(v.nz + 1, kz));
     if (l == 0) {
       launch_fine<T>(v, d, g, grid, kz, u, y, MODE_APPLY, nullptr);
@@ -647,14 +672,46 @@ struct System {
   }
 
   int kz_override = 0;
+  template <typename T>
   int pick_kz(const Level& v) const {
     if (kz_override > 0) return kz_override;
+    const int tuned = v.kz[std::is_same<T, float>::value ? 1 : 0];
+    if (tuned > 0) return tuned;
     const long long xy = (long long)div_up(v.nx + 1, TOUT) * div_up(v.ny + 1, TOUT);
-    // enough blocks to fill 256 CUs several times over, but keep the z halo small
-    for (int kz : {32, 16, 8}) {
-      if (xy * div_up(v.nz + 1, kz) >= 2048) return kz;
+    // enough blocks to fill 256 CUs several times over; short z chains on small levels
+    for (int kz : {32, 16, 8, 4, 2}) {
+      if (xy * div_up(v.nz + 1, kz) >= 1024) return kz;
     }
-    return 8;
+    return 1;
+  }
+
+  // Tile autotuner: time the structured kernel of every level for each z extent and keep the
+  // fastest (HIP events, on this system's stream).
+  template <typename T>
+  void autotune_kz(int nrep) {
+    hipEvent_t e0, e1;
+    CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
+    for (int l = 0; l < nlev() - 1; ++l) {
+      Level& v = lv(l);
+      LevelT<T>& d = v.dat<T>();
+      const int slot = std::is_same<T, float>::value ? 1 : 0;
+      float best = 1e30f;
+      int best_kz = 0;
+      for (int kz : {1, 2, 4, 8, 16, 32}) {
+        if (kz > v.nz + 1 && kz != 1) continue;
+        v.kz[slot] = kz;
+        apply_structured<T>(l, d.x.p, d.t.p);
+        CM_HIP(hipEventRecord(e0, stream));
+        for (int r = 0; r < nrep; ++r) apply_structured<T>(l, d.x.p, d.t.p);
+        CM_HIP(hipEventRecord(e1, stream));
+        CM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) { best = ms; best_kz = kz; }
+      }
+      v.kz[slot] = best_kz;
+    }
+    CM_HIP(hipEventDestroy(e0)); CM_HIP(hipEventDestroy(e1));
   }
 
   template <typename T>
@@ -720,8 +777,7 @@ struct System {
     if (opt.smoother == 1) {
       int s0 = 0;
       if (zero_start) {
-        hipLaunchKernelGGL(k_jacobi<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, (const T*)nullptr, d.Dinv.p,
-                           T(opt.omega), 1);
+        hipLaunchKernelGGL(k_jacobi<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, b, d.Dinv.p, T(opt.omega), 1);
         s0 = 1;
       }
       for (int s = s0; s < nsweep; ++s) {
@@ -1290,6 +1346,21 @@ int cm_sys_time_apply(cm_sys* h, int level, int prec, const void* u, void* y, in
   CM_HIP(hipEventElapsedTime(&ms, e0, e1));
   CM_HIP(hipEventDestroy(e0)); CM_HIP(hipEventDestroy(e1));
   *avg_ms = ms / nrep;
+  CM_API_END
+}
+
+int cm_sys_autotune(cm_sys* h, int prec, int nrep) {
+  CM_API_BEGIN
+  need_built(h);
+  CM_CHECK(nrep >= 1, "nrep must be positive");
+  System& s = h->s;
+  if (prec == 0) {
+    s.ensure_prec<double>();
+    s.autotune_kz<double>(nrep);
+  } else {
+    s.ensure_prec<float>();
+    s.autotune_kz<float>(nrep);
+  }
   CM_API_END
 }
 

@@ -51,7 +51,8 @@ def compress_materials(lam, mu, active, max_labels=255):
 class VoxelSystem:
     def __init__(self, grid, *, sri, active, lam=None, mu=None, labels=None, lam_tab=None, mu_tab=None,
                  copy_orig=None, irr_cells=None, irr_nodes=None, extra=None, used=None, min_dofs=6000, max_levels=6,
-                 levels=None, smoother="chebyshev", degree=3, omega=0.6, npre=2, npost=2, device=None):
+                 levels=None, smoother="chebyshev", degree=3, omega=0.6, npre=2, npost=2, materials="auto",
+                 device=None):
         """grid: Grid; active: (ncell,) bool in reference cell order; materials either per cell
         (lam, mu) or as labels + tables; split connectivity as copy originals and irregular cells
         with their 8 node ids; extra: (rows, cols, vals) COO over level-0 DOFs."""
@@ -71,6 +72,7 @@ class VoxelSystem:
                                          grid.nz, ptr(xe, ctypes.c_double), ptr(ye, ctypes.c_double),
                                          ptr(ze, ctypes.c_double), int(self.sri)))
         self.h = h
+        self.materials = materials
         try:
             self._describe(labels, lam_tab, mu_tab, lam, mu, copy_orig, irr_cells, irr_nodes, extra)
             if used is not None:
@@ -88,8 +90,10 @@ class VoxelSystem:
     # ------------------------------------------------------------------ description
     def _describe(self, labels, lam_tab, mu_tab, lam, mu, copy_orig, irr_cells, irr_nodes, extra):
         lib = self.lib
-        if labels is None:
+        if labels is None and self.materials != "values":
             comp = compress_materials(lam, mu, self.active)
+            if comp is None and self.materials == "labels":
+                raise ValueError("more than 255 distinct (lam, mu) pairs: use materials='values'")
             if comp is not None:
                 lam_tab, mu_tab, labels = comp
         if labels is not None:
@@ -174,6 +178,11 @@ class VoxelSystem:
                                          ctypes.c_void_p(y.data_ptr()), int(nrep), int(bool(structured_only)),
                                          ctypes.byref(ms)))
         return ms.value
+
+    def autotune(self, precisions=("f32", "f64"), nrep=5):
+        """Pick the fastest structured-block z extent per level (project autotuner, HIP events)."""
+        for p in precisions:
+            check(self.lib.cm_sys_autotune(self.h, PREC[p], int(nrep)))
 
     def set_kz(self, kz):
         check(self.lib.cm_sys_set_kz(self.h, int(kz)))

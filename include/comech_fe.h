@@ -128,6 +128,10 @@ int cm_element_forces(void* stream, int nx, int ny, int nz, const double* hx, co
 int cm_sys_time_apply(cm_sys* sys, int level, int prec, const void* u, void* y, int nrep, int structured_only,
                       double* avg_ms);
 
+/* Tile autotuner: for every level, time the structured stencil kernel in precision `prec` for
+ * z extents 1..32 (nrep launches each, HIP events) and keep the fastest. */
+int cm_sys_autotune(cm_sys* sys, int prec, int nrep);
+
 /* Override the z extent of a structured block (0 = automatic). */
 int cm_sys_set_kz(cm_sys* sys, int kz);
 

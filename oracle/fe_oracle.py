@@ -452,6 +452,9 @@ def solve(grid, lam, mu, active, springs, loads, method="cg", tol=1e-8, x0=None,
         K = K + sp.csr_matrix((k * np.outer(w[nz], w[nz]).ravel(), (np.repeat(nz, len(nz)), np.tile(nz, len(nz)))),
                               shape=K.shape)
     info = {}
+    if method == "none":                                            # assembled system only
+        info.update(ndof=n, K=K, f=f, used=used, nmap=nmap)
+        return None, info
     if method == "gmg":
         Z = rigid_modes(grid.node_xyz, nmap, n, node_groups) if node_groups else None
         u, inf = pcg_gmg(K, f, used, (grid.xe, grid.ye, grid.ze), tol=tol, x0=x0, Z=Z, maxiter=maxiter, **mg_kw)
@@ -624,6 +627,9 @@ def solve_cn(grid, lam, mu, active, cn, orig, fixed, loads, extra=None, node_gro
     xyz = grid.node_xyz[orig]
     Z = rigid_modes(xyz, nmap, n, node_groups) if node_groups else None
     info = {}
+    if method == "none":                                            # assembled system only
+        info.update(ndof=n, K=K, f=f, used=used, nmap=nmap, Z=Z)
+        return None, info, None
     if method == "direct":
         u = spla.spsolve(K.tocsc(), f, permc_spec="MMD_AT_PLUS_A")
         info.update(iters=0, flag=0)

@@ -1,0 +1,200 @@
+"""CPU tests of the product's host logic and of the kernel math (CPU build of element.hpp), plus
+the C-ABI library: it loads and exports every symbol include/comech_fe.h declares (no compute calls
+without a GPU)."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from comech_amd import _lib, synth
+from comech_amd import engine as E
+from comech_amd.grid import Grid
+from comech_amd.rb import gpusolve as GS
+from comech_amd.rb import splitfe as SF
+from comech_amd.system import compress_materials
+from oracle import fe_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "comech_fe.h")).read()
+    names = set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+(cm_\w+)\s*\(", hdr, re.M))
+    assert len(names) >= 20
+    lib = _lib.load()
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
+    assert lib.cm_version() == 1
+
+
+def test_library_error_path_without_gpu():
+    """Argument errors come back as status -1 with a message (no device touched)."""
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    e = np.zeros(1)
+    rc = lib.cm_sys_create(ctypes.byref(h), None, 0, 1, 1, _lib.ptr(e, ctypes.c_double), _lib.ptr(e, ctypes.c_double),
+                           _lib.ptr(e, ctypes.c_double), 1)
+    assert rc == -1 and b"at least one cell" in lib.cm_last_error()
+
+
+def _host():
+    lib = ctypes.CDLL(os.path.join(ROOT, "comech_amd", "libcomech_host.so"))
+    return lib
+
+
+P = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (0.5, 0.7, 1.3), (2.0, 3.0, 0.25)])
+@pytest.mark.parametrize("sri", [0, 1])
+def test_kernel_element_operator_matches_oracle(dims, sri):
+    """The Walsh-Hadamard-basis operator of the kernels equals lam*KL + mu*KM (fe/solver.py:77-90,
+    rb/splitfe.py:58-69), FP64 to rounding and FP32 to 1e-6."""
+    lib = _host()
+    KL, KM, _ = O.element_matrices(dims)
+    if sri:
+        KL = O.kl_sri(dims)
+    K = np.zeros((24, 24))
+    lib.cm_host_elem_const(P(np.asarray(dims, float)), ctypes.c_double(1.7), ctypes.c_double(2.3), sri, P(K))
+    ref = 1.7 * KL + 2.3 * KM
+    assert np.abs(K - ref).max() <= 1e-14 * np.abs(ref).max()
+    u = np.random.default_rng(0).standard_normal(24)
+    f = np.zeros(24)
+    lib.cm_host_elem_const_f32(P(np.asarray(dims, float)), ctypes.c_double(1.7), ctypes.c_double(2.3), sri, P(u), P(f))
+    assert np.abs(f - ref @ u).max() <= 1e-5 * np.abs(ref @ u).max()
+
+
+@pytest.mark.parametrize("sri", [0, 1])
+def test_kernel_moments_are_exact_galerkin(sri):
+    """Coarse cell operator from shifted child moments == P^T K P of its 2x2x2 children with random
+    materials on a stretched cell (fe/gmg.py:91-97)."""
+    lib = _host()
+    rng = np.random.default_rng(sri)
+    xe, ye, ze = np.r_[0, 0.7, 2.0], np.r_[0, 1.1, 1.5], np.r_[0, 0.4, 1.2]
+    g = O.Grid(xe, ye, ze)
+    lam, mu = rng.uniform(0.1, 3, 8), rng.uniform(0.1, 3, 8)
+    KLf = g.KL_sri if sri else g.KL
+    Kf = np.zeros((81, 81))
+    mL, mM = np.zeros(27), np.zeros(27)
+    H = np.array([xe[-1], ye[-1], ze[-1]])
+    for c in range(8):
+        d = (3 * g.cell_nodes[c][:, None] + np.arange(3)).ravel()
+        Kf[np.ix_(d, d)] += lam[c] * KLf[g.dim_id[c]] + mu[c] * g.KM[g.dim_id[c]]
+        lo = np.array([xe[g.cI[c]], ye[g.cJ[c]], ze[g.cK[c]]])
+        hi = np.array([xe[g.cI[c] + 1], ye[g.cJ[c] + 1], ze[g.cK[c] + 1]])
+        a = (hi - lo) / H
+        b = (lo + hi - H) / H
+        t = np.zeros(27)
+        lib.cm_host_mom_const(ctypes.c_double(lam[c]), ctypes.c_double(np.prod(hi - lo)), sri, P(t))
+        lib.cm_host_mom_shift_add(P(t), P(a), P(b), P(mL))
+        lib.cm_host_mom_const(ctypes.c_double(mu[c]), ctypes.c_double(np.prod(hi - lo)), 0, P(t))
+        lib.cm_host_mom_shift_add(P(t), P(a), P(b), P(mM))
+    Px, _ = O.p1d(xe); Py, _ = O.p1d(ye); Pz, _ = O.p1d(ze)
+    import scipy.sparse as sp
+    Pd = np.kron(sp.kron(Pz, sp.kron(Py, Px)).toarray(), np.eye(3))
+    ref = Pd.T @ Kf @ Pd
+    K = np.zeros((24, 24))
+    lib.cm_host_elem_mom(P(mL), P(mM), P(H), P(K))
+    assert np.abs(K - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+def test_kernel_cell_stress_matches_oracle():
+    lib = _host()
+    dims = np.array([0.5, 0.7, 1.3])
+    u = np.random.default_rng(2).standard_normal(24)
+    eps, sig, vm = np.zeros(6), np.zeros(6), np.zeros(1)
+    lib.cm_host_cell_stress(P(u), P(dims), ctypes.c_double(3.0), ctypes.c_double(2.0), P(eps), P(sig), P(vm))
+    Bc = O.element_matrices(dims)[2]
+    e_ref = Bc @ u
+    tr = e_ref[:3].sum()
+    s_ref = np.r_[3.0 * tr + 4.0 * e_ref[:3], 2.0 * e_ref[3:]]
+    assert np.allclose(eps, e_ref) and np.allclose(sig, s_ref) and np.isclose(vm[0], O.von_mises(s_ref[None])[0])
+
+
+def test_product_grid_matches_oracle():
+    xe, ye, ze = np.r_[0, 1, 3, 4, 7, 10], np.r_[0, 2, 5, 10], np.r_[0, 1, 2, 4, 6, 10]
+    g, o = Grid(xe, ye, ze), O.Grid(xe, ye, ze)
+    assert np.array_equal(g.cell_nodes, o.cell_nodes) and np.array_equal(g.dim_id, o.dim_id)
+    assert np.array_equal(g.node_xyz, o.node_xyz) and np.array_equal(g.cI, o.cI)
+    for a, b in ((g.KL, o.KL), (g.KM, o.KM), (g.Bc, o.Bc), (g.KL_sri, o.KL_sri), (g.KB, o.KB)):
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_product_split_connectivity_bit_exact(seed):
+    """Node numbering, split copies and orig: product == oracle loop form, bit for bit."""
+    rng = np.random.default_rng(100 + seed)
+    shape = (7, 5, 6)
+    g, og = Grid(*(np.arange(n + 1.0) for n in shape)), O.Grid(*(np.arange(n + 1.0) for n in shape))
+    part = rng.integers(0, 4, g.ncell)
+    active = rng.uniform(size=g.ncell) > 0.1
+    share = rng.uniform(size=(g.ncell, 4)).astype(np.float32)
+    share[rng.uniform(size=g.ncell) < 0.3] = 0
+    share[:, 0] = 0
+    cn, orig, ns = SF.split_connectivity_fast(g, active, part, share)
+    cno, origo, nso = O.split_connectivity(og, active, part, share)
+    assert ns == nso and np.array_equal(orig, origo) and np.array_equal(cn[active], cno[active])
+    assert cn.dtype == np.int64 and orig.dtype == np.int64
+
+
+def test_used_nodes_and_irregular_cells():
+    c = synth.config2(dims=(20, 12, 12), h=4.0)
+    g = c["grid"]
+    irr_c, irr_n = E.split_cells(g, c["active"], c["cn"])
+    used = E.used_nodes(g, c["active"], len(c["orig"]), irr_c, irr_n)
+    ref = np.zeros(len(c["orig"]), bool)
+    ref[np.unique(c["cn"][c["active"]])] = True
+    assert np.array_equal(used, ref)
+    assert np.array_equal(irr_c, np.flatnonzero(c["active"] & (c["cn"] != g.cell_nodes).any(1)))
+
+
+def test_compress_materials_exact():
+    rng = np.random.default_rng(3)
+    lam = rng.choice([1.0, 2.0, 3.0], 100)
+    mu = rng.choice([0.5, 0.25], 100)
+    act = rng.uniform(size=100) > 0.3
+    lt, mt, lab = compress_materials(lam, mu, act)
+    assert np.array_equal(lt[lab[act]], lam[act]) and np.array_equal(mt[lab[act]], mu[act])
+    assert (lab[~act] == 255).all()
+    assert compress_materials(rng.uniform(size=1000), rng.uniform(size=1000), np.ones(1000, bool)) is None
+
+
+def test_synth_deterministic_and_sized():
+    a, b = synth.config2(dims=(24, 16, 16), h=3.0), synth.config2(dims=(24, 16, 16), h=3.0)
+    for k in ("labels", "cn", "orig", "active"):
+        assert np.array_equal(a[k], b[k])
+    assert a["n_lig"] > 0 and a["n_copies"] > 0
+    r, c_, v = a["extra"]
+    assert len(r) == len(c_) == len(v)
+    c1 = synth.config1(n=32)
+    assert c1["grid"].ncell == 32 ** 3 and set(np.unique(c1["labels"])) == {0, 1}
+    sizes = synth.config3_sizes()
+    assert len(sizes) == 64 and all(96 <= s[0] <= 192 and 64 <= s[1] <= 112 and 64 <= s[2] <= 112 for s in sizes)
+
+
+def test_rigid_modes_match_oracle():
+    g = Grid(np.arange(4.0), np.arange(3.0), np.arange(3.0))
+    og = O.Grid(g.xe, g.ye, g.ze)
+    used = np.ones(g.nnode, bool)
+    used[5] = False
+    nmap = -np.ones(g.nnode, np.int64)
+    nmap[used] = np.arange(used.sum())
+    groups = [np.arange(0, 20), np.arange(15, 30), np.arange(3)]
+    from comech_amd.fe.solver import rigid_modes
+    Zp = rigid_modes(g, nmap, 3 * used.sum(), groups).toarray()
+    Zo = O.rigid_modes(og.node_xyz, nmap, 3 * used.sum(), groups).toarray()
+    assert np.allclose(Zp, Zo)
+
+
+def test_gpusolve_failure_hook_raises_without_cpu_fallback(monkeypatch):
+    """COMECH_TEST_GPU_FAIL fires before any device call (rb/gpusolve.py:207-208); after the
+    retries the product raises instead of solving on the CPU."""
+    import scipy.sparse as sp
+    monkeypatch.setenv("COMECH_TEST_GPU_FAIL", "1")
+    monkeypatch.setenv("COMECH_RETRY_SLEEP", "0")
+    with pytest.raises(RuntimeError, match="simulated GPU failure"):
+        GS.pcg_gmg_gpu(sp.identity(3, format="csr"), np.ones(3), np.array([0]), (np.arange(2.0),) * 3,
+                       device="cuda:0")

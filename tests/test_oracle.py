@@ -1,0 +1,246 @@
+"""Pins the FP64 oracle (CPU).  Every check is chosen so that a plausible mistake -- a dropped
+term, a wrong sign, a swapped index, a transposed operand -- fails: closed forms, invariants,
+brute force on tiny inputs and library routines (scipy splu), never a retyped formula."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+from oracle import fe_oracle as O
+
+
+def test_grid_numbering_closed_forms():
+    """fe/solver.py:104-109,128: nodes x-fastest, cells z-fastest, corner order a + 2b + 4c."""
+    g = O.Grid(np.arange(4.0), np.arange(3.0) * 2, np.arange(5.0))   # 3 x 2 x 4 cells
+    assert g.nnode == 4 * 3 * 5 and g.ncell == 24
+    c = g.cid(2, 1, 3)
+    assert c == (2 * 2 + 1) * 4 + 3
+    assert (g.cI[c], g.cJ[c], g.cK[c]) == (2, 1, 3)
+    # corner (a,b,c) of cell (2,1,3) is node (2+a) + 4 ((1+b) + 3 (3+c))
+    exp = [(2 + a) + 4 * ((1 + b) + 3 * (3 + cc)) for cc in (0, 1) for b in (0, 1) for a in (0, 1)]
+    assert list(g.cell_nodes[c]) == exp
+    assert np.allclose(g.node_xyz[exp[7]], [3.0, 4.0, 4.0])
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (0.5, 0.7, 1.3), (2.0, 3.0, 0.25)])
+def test_element_matrices_invariants(dims):
+    KL, KM, Bc = O.element_matrices(dims)
+    for K in (KL, KM):
+        assert np.allclose(K, K.T, atol=1e-13)
+    # six rigid-body modes are in the null space of lam KL + mu KM
+    X = np.array([[a * dims[0], b * dims[1], c * dims[2]] for c in (0, 1) for b in (0, 1) for a in (0, 1)])
+    R = []
+    for d in range(3):
+        v = np.zeros((8, 3)); v[:, d] = 1; R.append(v.ravel())
+    for (i, j) in ((0, 1), (1, 2), (0, 2)):
+        v = np.zeros((8, 3)); v[:, i] = -X[:, j]; v[:, j] = X[:, i]; R.append(v.ravel())
+    K = 1.3 * KL + 0.7 * KM
+    for v in R:
+        assert np.abs(K @ v).max() < 1e-12 * np.abs(K).max()
+    assert np.linalg.matrix_rank(K, tol=1e-10 * np.abs(K).max()) == 18
+    # energy of a linear displacement u = G x equals V * eps:D:eps (closed form, exact integration)
+    rng = np.random.default_rng(0)
+    G = rng.standard_normal((3, 3))
+    u = (X @ G.T).ravel()
+    eps = 0.5 * (G + G.T)
+    V = np.prod(dims)
+    lam, mu = 1.3, 0.7
+    e_exact = V * (lam * np.trace(eps) ** 2 + 2 * mu * (eps ** 2).sum())
+    assert np.isclose(u @ K @ u, e_exact, rtol=1e-12)
+    # centre B gives the strain of the linear field in engineering order
+    e6 = Bc @ u
+    assert np.allclose(e6, [G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[1, 2] + G[2, 1], G[0, 2] + G[2, 0]])
+
+
+def test_kl_sri_rank_one():
+    dims = (0.5, 0.7, 1.3)
+    K = O.kl_sri(dims)
+    assert np.linalg.matrix_rank(K, tol=1e-10 * np.abs(K).max()) == 1
+    Bc = O.element_matrices(dims)[2]
+    g = Bc[:3].sum(0)                      # divergence at the centre
+    assert np.allclose(K, np.prod(dims) * np.outer(g, g))
+
+
+def test_assemble_brute_force():
+    g = O.Grid([0, 1, 2.5], [0, 1], [0, 0.5, 1.5])
+    rng = np.random.default_rng(1)
+    lam, mu = rng.uniform(1, 2, g.ncell), rng.uniform(1, 2, g.ncell)
+    active = np.array([True, True, False, True])
+    K, nmap, used, cells, _ = O.assemble(g, lam, mu, active)
+    D = np.zeros((3 * g.nnode, 3 * g.nnode))
+    for c in np.flatnonzero(active):
+        d = g.dim_id[c]
+        dofs = (3 * g.cell_nodes[c][:, None] + np.arange(3)).ravel()
+        D[np.ix_(dofs, dofs)] += lam[c] * g.KL[d] + mu[c] * g.KM[d]
+    keep = (3 * used[:, None] + np.arange(3)).ravel()
+    assert np.allclose(K.toarray(), D[np.ix_(keep, keep)], atol=1e-13)
+
+
+def _random_bones(seed, shape=(6, 5, 4), nb=3):
+    rng = np.random.default_rng(seed)
+    g = O.Grid(*(np.arange(n + 1.0) for n in shape))
+    part = rng.integers(0, nb + 1, g.ncell)
+    active = rng.uniform(size=g.ncell) > 0.15
+    share = rng.uniform(size=(g.ncell, nb + 1)).astype(np.float32)
+    share[rng.uniform(size=g.ncell) < 0.3] = 0.0                 # cells holding none of the bones
+    share[:, 0] = 0
+    return g, active, part, share
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_split_connectivity_fast_equals_loop(seed):
+    """rb/splitfe.py:262 says the vectorised form gives the same result as the loop form."""
+    g, active, part, share = _random_bones(seed)
+    a = O.split_connectivity(g, active, part, share)
+    b = O.split_connectivity_fast(g, active, part, share)
+    assert np.array_equal(a[0][active], b[0][active]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+
+
+def test_split_connectivity_two_bones_by_hand():
+    """Two cells of different bones share a face: its 4 nodes are split, the second bone gets
+    copies nnode + 0..3 in node order; a part-0 cell attaches to the bone it holds most of."""
+    g = O.Grid([0, 1, 2, 3], [0, 1], [0, 1])          # cells 0,1,2 along x
+    part = np.array([1, 2, 0])
+    share = np.zeros((3, 3), np.float32)
+    share[2, 1], share[2, 2] = 0.2, 0.6
+    cn, orig, ns = O.split_connectivity(g, np.ones(3, bool), part, share)
+    face = [1, 5, 9, 13]                               # x = 1 nodes
+    assert ns == 4 and list(orig[g.nnode:]) == face
+    assert list(cn[1][[0, 2, 4, 6]]) == [g.nnode + q for q in range(4)]
+    assert list(cn[0]) == list(g.cell_nodes[0])
+    # cell 2 (part 0) touches x=2 nodes only: none split, untouched
+    assert list(cn[2]) == list(g.cell_nodes[2])
+
+
+def test_p1d_linear_and_partition_of_unity():
+    x = np.array([0.0, 0.3, 1.0, 1.1, 2.5, 2.6])
+    P, xc = O.p1d(x)
+    assert list(xc) == [0.0, 1.0, 2.5, 2.6]            # every other node plus the last
+    assert np.allclose(P.sum(1), 1.0)
+    assert np.allclose(P @ (3 * xc - 1), 3 * x - 1)
+
+
+def _spd(n, seed):
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((n, n))
+    return sp.csr_matrix(A @ A.T + n * np.eye(n))
+
+
+def test_chebyshev_smoother_polynomial():
+    """x_new = x + (I - p(MA)) (x* - x), p(t) = T_3((theta - t)/delta) / T_3(theta/delta)."""
+    A = _spd(12, 2)
+    L = O.Level(A)
+    rng = np.random.default_rng(3)
+    b, x = rng.standard_normal(12), rng.standard_normal(12)
+    xs = np.linalg.solve(A.toarray(), b)
+    Minv = np.zeros((12, 12))
+    for i in range(4):
+        Minv[3 * i:3 * i + 3, 3 * i:3 * i + 3] = L.Dinv[i]
+    MA = Minv @ A.toarray()
+    th, de = 0.5 * (L.lmax + L.lmin), 0.5 * (L.lmax - L.lmin)
+    T3 = lambda s: 4 * s ** 3 - 3 * s
+    w, V = np.linalg.eig(MA)
+    P = (V @ np.diag(T3((th - w) / de) / T3(th / de)) @ np.linalg.inv(V)).real
+    assert np.allclose(xs - L.smooth(x, b), P @ (xs - x), atol=1e-10)
+
+
+def test_block_jacobi_sweeps():
+    A = _spd(9, 4)
+    L = O.Level(A, smoother="jacobi", omega=0.6, sweeps=2)
+    b, x = np.ones(9), np.zeros(9)
+    Minv = np.zeros((9, 9))
+    for i in range(3):
+        Minv[3 * i:3 * i + 3, 3 * i:3 * i + 3] = np.linalg.inv(A.toarray()[3 * i:3 * i + 3, 3 * i:3 * i + 3])
+    E = np.eye(9) - 0.6 * Minv @ A.toarray()
+    xs = np.linalg.solve(A.toarray(), b)
+    assert np.allclose(xs - L.smooth(x, b), E @ E @ (xs - x))
+
+
+def test_two_grid_error_propagation():
+    """V-cycle on 2 levels: E = S (I - P Ac^-1 P^T A) S, with S the smoother error operator."""
+    g = O.Grid(np.arange(5.0), np.arange(3.0), np.arange(3.0))
+    lam, mu = np.full(g.ncell, 1.0), np.full(g.ncell, 1.0)
+    K, nmap, used, _, _ = O.assemble(g, lam, mu, np.ones(g.ncell, bool))
+    K = K + sp.identity(K.shape[0]) * 0.5
+    mg = O.GMG(K, used, g.xe, g.ye, g.ze, levels=2, smoother="jacobi", omega=0.6, sweeps=1)
+    n = K.shape[0]
+    A = K.toarray()
+    Minv = np.zeros((n, n))
+    for i in range(n // 3):
+        Minv[3 * i:3 * i + 3, 3 * i:3 * i + 3] = mg.levels[0].Dinv[i]
+    S = np.eye(n) - 0.6 * Minv @ A
+    P = mg.P[0].toarray()
+    E = S @ (np.eye(n) - P @ np.linalg.solve(P.T @ A @ P, P.T @ A)) @ S
+    rng = np.random.default_rng(5)
+    e = rng.standard_normal(n)
+    b = A @ e
+    # vcycle(b) from x=0 approximates e: error e - vcycle(b) = E e
+    assert np.allclose(e - mg.vcycle(b), E @ e, atol=1e-10)
+
+
+def test_pcg_gmg_matches_splu_and_cadence():
+    g = O.Grid(np.arange(9.0), np.arange(7.0), np.arange(6.0))
+    rng = np.random.default_rng(6)
+    E = np.where(rng.uniform(size=g.ncell) < 0.3, 17000.0, 700.0)
+    lam, mu = O.lame(E, np.full(g.ncell, 0.3))
+    bot = np.flatnonzero(g.cK == 0); top = np.flatnonzero(g.cK == g.nz - 1)
+    nb, ab = O.face_node_areas(g, bot, "z-"); nt, at = O.face_node_areas(g, top, "z+")
+    args = (g, lam, mu, np.ones(g.ncell, bool), [(nb, ab, np.array([1e6] * 3))], [(nt, at, np.array([0, 0, -1.0]))])
+    Ud, _ = O.solve(*args, method="direct")
+    Ug, info = O.solve(*args, method="gmg", tol=1e-12)
+    assert np.linalg.norm(Ug - Ud) / np.linalg.norm(Ud) < 1e-9 and info["flag"] == 0
+    _, inf0 = O.solve(*args, method="none")
+    K, f = inf0["K"], inf0["f"]
+    mg = O.GMG(K, inf0["used"], g.xe, g.ye, g.ze)
+    x, it, flag, rr = O.pcg_cadence(K, f, mg.vcycle, 1e-10, 500)
+    assert flag == 0 and rr <= 1e-10 and (it < 5 or it % 5 == 0)
+    assert np.linalg.norm(K @ x - f) <= 1e-10 * np.linalg.norm(f)
+
+
+def test_patch_test_closed_forms():
+    """fe/test_patch.py:4-14."""
+    g = O.Grid(np.r_[0, 1, 3, 4, 7, 10], np.r_[0, 2, 5, 10], np.r_[0, 1, 2, 4, 6, 10])
+    E, nu = 1000.0, 0.3
+    lam, mu = O.lame(np.full(g.ncell, E), np.full(g.ncell, nu))
+    bot, top = np.flatnonzero(g.cK == 0), np.flatnonzero(g.cK == g.nz - 1)
+    nb, ab = O.face_node_areas(g, bot, "z-")
+    nt, at = O.face_node_areas(g, top, "z+")
+    U, info = O.solve(g, lam, mu, np.ones(g.ncell, bool), [(nb, ab, np.array([1e-6, 1e-6, 1e9]))],
+                      [(nt, at, np.array([0, 0, -1.0]))], method="direct")
+    sig, vm, eps = O.cell_stress(g, lam, mu, U, np.arange(g.ncell))
+    top_nodes = np.flatnonzero(g.node_xyz[:, 2] == 10)
+    assert np.allclose(U[top_nodes, 2], -10 / E, rtol=1e-6)
+    assert np.allclose(sig[:, 2], -1.0, atol=1e-8) and np.allclose(vm, 1.0, atol=1e-8)
+    assert abs(np.ptp(U[:, 0]) - nu * 10 / E) < 1e-9
+
+
+def test_von_mises_closed_forms():
+    s = np.array([[-1, 0, 0, 0, 0, 0], [5, 5, 5, 0, 0, 0], [0, 0, 0, 2, 0, 0], [1, 2, 3, 0, 0, 0.0]])
+    vm = O.von_mises(s)
+    assert np.allclose(vm[:3], [1.0, 0.0, 2 * np.sqrt(3)])
+    assert np.isclose(vm[3], np.sqrt(3.0))
+
+
+def test_spring_matrix_closed_form():
+    """One ligament between single-node patches: K = k w w^T with w = +d at a, -d at b."""
+    xyz = np.array([[0, 0, 0], [10.0, 0, 0], [0, 20, 0], [20, 20, 20.0]])
+    springs = [dict(a=[0, 0, 0.0], b=[3.0, 4.0, 0], k=5.0, A="x", B="y")]
+    build = O.spring_matrix_cn(xyz, springs, {"x": np.array([0]), "y": np.array([2])})
+    nmap = np.arange(4)
+    K = build(nmap, 12).toarray()
+    d = np.array([0.6, 0.8, 0.0])
+    w = np.zeros(12); w[0:3] = d; w[6:9] = -d
+    assert np.allclose(K, 5.0 * np.outer(w, w))
+
+
+def test_sample_fe_constant_and_linear():
+    g = O.Grid(np.arange(5.0), np.arange(5.0), np.arange(5.0))
+    gm = O.Grid(np.arange(0, 5, 2.0), np.arange(0, 5, 2.0), np.arange(0, 5, 2.0))
+    active = np.ones(g.ncell, bool)
+    VM = np.full(g.ncell, 3.0, np.float32)
+    SIG = np.tile(np.arange(6, dtype=np.float32), (g.ncell, 1))
+    Ur = g.node_xyz @ np.array([[1, 0, 0], [0, 2, 0], [0, 0, 3.0]])      # linear field
+    vm, u, cov, sig = O.sample_fe(g, g.cell_nodes, active, VM, SIG, Ur, gm)
+    assert np.allclose(vm, 3.0) and np.allclose(cov, 1.0) and np.allclose(sig, np.arange(6))
+    centres = gm.cell_xyz * np.array([1, 2, 3.0])
+    assert np.allclose(u, centres, atol=1e-6)

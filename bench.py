@@ -32,16 +32,16 @@ def build_inputs(dims=(160, 96, 96), h=0.5):
     from comech_amd import synth
     from comech_amd.engine import split_cells
     c = synth.config2(dims=dims, h=h)
-    extra, f = synth.solve_inputs_config2(c)
+    extra, terms, f = synth.solve_inputs_config2(c)
     irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
-    return c, extra, f, irr_c, irr_n
+    return c, (extra, terms), f, irr_c, irr_n
 
 
 def make_system(c, extra, irr_c, irr_n, kz_cache=None):
     from comech_amd.system import VoxelSystem
     s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
                     mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c, irr_nodes=irr_n,
-                    extra=extra, device="cuda:0", smoother="chebyshev")
+                    extra=extra[0], rank1=extra[1], device="cuda:0", smoother="chebyshev")
     return s
 
 
@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--mode", default="mixed", choices=["mixed", "ref", "fp64"])
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="for rocprofv3 --pmc runs: 10 launches of the FP32 fine stencil on the workload, then "
+                         "5 plain device copies of 256 MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; every solve is independent (batched solves "
@@ -111,6 +114,21 @@ def main():
     c, extra, f, irr_c, irr_n = build_inputs()
     f_dev = torch.from_numpy(f).to(dev)
     log(f"inputs ready in {time.time() - t0:.1f} s: ncell {c['grid'].ncell}, active {int(c['active'].sum())}")
+
+    if args.kernel_only:
+        s = make_system(c, extra, irr_c, irr_n)
+        u = torch.randn(3 * s.nn, dtype=torch.float32, device=dev)
+        y = torch.empty_like(u)
+        for _ in range(5):
+            s.time_apply(u, y, nrep=1, structured_only=True)          # 2 launches each
+        a = torch.randn(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        s.close()
+        log("kernel-only probe done")
+        return
 
     for _ in range(args.warmup):
         info = one_solve(c, extra, f_dev, irr_c, irr_n, args.mode, args.tol)

@@ -29,6 +29,7 @@ SIGNATURES = {
     "cm_sys_set_cellmat": [c_void_p, P_double, P_double, P_u8],
     "cm_sys_set_split": [c_void_p, c_ll, P_ll, c_ll, P_ll, P_ll],
     "cm_sys_set_extra": [c_void_p, c_ll, P_ll, P_ll, P_double],
+    "cm_sys_set_rank1": [c_void_p, c_ll, P_ll, P_ll, P_double, P_double],
     "cm_sys_set_used": [c_void_p, c_ll, P_u8],
     "cm_sys_set_rigid": [c_void_p, c_int, c_ll, P_ll, P_ll, P_double, P_double],
     "cm_sys_build": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double, c_int, c_int],
@@ -42,6 +43,7 @@ SIGNATURES = {
                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "cm_sys_time_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, P_double],
     "cm_sys_autotune": [c_void_p, c_int, c_int],
+    "cm_sys_timings": [c_void_p, P_double, c_int],
     "cm_sys_set_kz": [c_void_p, c_int],
     "cm_sys_sync": [c_void_p],
     "cm_sample_fe": [c_void_p] + [c_int] * 7 + [c_void_p] * 9 + [c_void_p] * 5 + [c_void_p] * 4,

@@ -331,6 +331,26 @@ __global__ void k_apply_extra(int nrow, const int* __restrict__ rows, const int*
   if (lane == 0) y[rows[r]] += sign * s;
 }
 
+// Rank-one extra stiffness sum_t k_t w_t w_t^T (springs, penalties, ligaments and their exact
+// Galerkin coarsening): y += sign * k_t (w_t . u) w_t.  One wavefront per term: lanes stride over
+// the term's entries, the dot product is shuffle-reduced, each lane scatters its entries (atomics).
+template <typename T>
+__global__ __launch_bounds__(256) void k_apply_rank1(int nterm, const int* __restrict__ ptr,
+                                                     const int* __restrict__ dof, const T* __restrict__ w,
+                                                     const T* __restrict__ k, const T* __restrict__ u, T* y,
+                                                     T sign) {
+  const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (t >= nterm) return;     // the 64 lanes of a term leave together
+  const int q0 = ptr[t], q1 = ptr[t + 1];
+  T s = T(0);
+  for (int q = q0 + lane; q < q1; q += 64) s += w[q] * u[dof[q]];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  s *= sign * k[t];
+  for (int q = q0 + lane; q < q1; q += 64) atomicAdd(&y[dof[q]], s * w[q]);
+}
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, long long n, T v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

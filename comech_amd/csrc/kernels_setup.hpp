@@ -176,6 +176,29 @@ __global__ void k_diag_extra(int nrow, const int* __restrict__ rows, const int* 
   }
 }
 
+// Diagonal blocks of rank-one terms: D[node] += k w_i w_j for entry pairs on the same node.
+__global__ void k_diag_rank1(int nterm, const int* __restrict__ ptr, const int* __restrict__ dof,
+                             const double* __restrict__ w, const double* __restrict__ k, double* __restrict__ D) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nterm) return;
+  for (int a = ptr[t]; a < ptr[t + 1]; ++a)
+    for (int b = ptr[t]; b < ptr[t + 1]; ++b) {
+      const int na = dof[a] / 3, nb = dof[b] / 3;
+      if (na != nb) continue;
+      atomicAdd(&D[9LL * na + 3 * (dof[a] % 3) + dof[b] % 3], k[t] * w[a] * w[b]);
+    }
+}
+
+// Rank-one terms into the dense coarsest matrix.
+__global__ void k_dense_add_rank1(int nterm, const int* __restrict__ ptr, const int* __restrict__ dof,
+                                  const double* __restrict__ w, const double* __restrict__ k, int n,
+                                  double* __restrict__ A) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nterm) return;
+  for (int a = ptr[t]; a < ptr[t + 1]; ++a)
+    for (int b = ptr[t]; b < ptr[t + 1]; ++b) atomicAdd(&A[(long long)dof[a] * n + dof[b]], k[t] * w[a] * w[b]);
+}
+
 // Inverse of the 3x3 blocks; blocks with |det| < 1e-30 become the identity (fe/gmg.py:39-41).
 template <typename T>
 __global__ void k_block_inverse(long long nn, const double* __restrict__ D, T* __restrict__ Dinv) {
@@ -247,30 +270,116 @@ __global__ void k_dense_fix_empty(int n, double* __restrict__ A) {
   if (threadIdx.x == 0 && s == 0.0) A[(long long)r * n + r] = 1.0;
 }
 
-// In-place Gauss-Jordan inversion of an SPD matrix (no pivoting), step k in three kernels:
-// (a) save column k; (b) scale row k by 1/p and set A[k][k] = 1/p; (c) eliminate.
-__global__ void k_gj_savecol(int n, int k, const double* __restrict__ A, double* __restrict__ colk) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) colk[i] = A[(long long)i * n + k];
+// In-place blocked Gauss-Jordan inversion of an SPD matrix (no pivoting), pivot blocks of GJB
+// rows.  For the pivot block K (all other rows/columns I, J):
+//   Dk = A_KK^-1;  R_KJ = Dk A_KJ;  A_IJ -= A_IK R_KJ;  A_IK = -A_IK Dk;  A_KJ = R_KJ;  A_KK = Dk
+// which is GJB consecutive steps of scalar Gauss-Jordan.
+constexpr int GJB = 64;
+
+// (1) Dk = inverse of the nb x nb diagonal block at k0 (one block of 256 threads, LDS).
+__global__ __launch_bounds__(256) void k_gjb_diag(int n, int k0, int nb, const double* __restrict__ A,
+                                                  double* __restrict__ Dk) {
+  __shared__ double M[GJB][GJB + 1];
+  __shared__ double col[GJB];
+  for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
+    const int i = t / GJB, j = t % GJB;
+    M[i][j] = (i < nb && j < nb) ? A[(long long)(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int k = 0; k < nb; ++k) {
+    if (threadIdx.x < GJB) col[threadIdx.x] = M[threadIdx.x][k];
+    __syncthreads();
+    const double ip = 1.0 / col[k];
+    for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
+      const int i = t / GJB, j = t % GJB;
+      if (i == k) {
+        M[i][j] = (j == k) ? ip : M[i][j] * ip;
+      }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
+      const int i = t / GJB, j = t % GJB;
+      if (i != k) M[i][j] = (j == k) ? -col[i] * ip : M[i][j] - col[i] * M[k][j];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
+    const int i = t / nb, j = t % nb;
+    Dk[i * GJB + j] = M[i][j];
+  }
 }
-__global__ void k_gj_scale(int n, int k, double* __restrict__ A, const double* __restrict__ colk) {
+
+// (2) R[:, j] = Dk A[K, j] for every column j (R is GJB x n; columns inside K are skipped later).
+__global__ __launch_bounds__(256) void k_gjb_rowpanel(int n, int k0, int nb, const double* __restrict__ A,
+                                                      const double* __restrict__ Dk, double* __restrict__ R) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
-  const double ip = 1.0 / colk[k];
-  A[(long long)k * n + j] = (j == k) ? ip : A[(long long)k * n + j] * ip;
-}
-// (b) A[i][j] -= colk[i] * A[k][j] for i != k; column k: A[i][k] = -colk[i] / p.
-__global__ void k_gj_update(int n, int k, double* __restrict__ A, const double* __restrict__ colk) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long long)n * n) return;
-  const int i = t / n, j = t % n;
-  if (i == k) return;
-  const double ci = colk[i];
-  if (j == k) {
-    A[t] = -ci / colk[k];
-  } else {
-    A[t] -= ci * A[(long long)k * n + j];
+  double a[GJB];
+  for (int q = 0; q < nb; ++q) a[q] = A[(long long)(k0 + q) * n + j];
+  for (int i = 0; i < nb; ++i) {
+    double s = 0.0;
+    for (int q = 0; q < nb; ++q) s += Dk[i * GJB + q] * a[q];
+    R[(long long)i * n + j] = s;
   }
+}
+
+// (3) A_IJ -= A_IK R_KJ for I, J outside K: 64x64 output tiles, K-panel staged in LDS.
+__global__ __launch_bounds__(256) void k_gjb_update(int n, int k0, int nb, double* __restrict__ A,
+                                                    const double* __restrict__ R) {
+  __shared__ double sa[64][GJB + 1];   // A[I-tile, K]
+  __shared__ double sr[GJB][64 + 1];   // R[K, J-tile]
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  for (int t = threadIdx.x; t < 64 * GJB; t += blockDim.x) {
+    const int r = t / GJB, q = t % GJB;
+    const int i = i0 + r;
+    sa[r][q] = (i < n && q < nb) ? A[(long long)i * n + k0 + q] : 0.0;
+    const int q2 = t / 64, c = t % 64;
+    const int j = j0 + c;
+    sr[q2][c] = (j < n && q2 < nb) ? R[(long long)q2 * n + j] : 0.0;
+  }
+  __syncthreads();
+  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;   // 16x16 threads, 4x4 outputs each
+  double acc[4][4];
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
+  for (int q = 0; q < nb; ++q) {
+    double x[4], y[4];
+    for (int a = 0; a < 4; ++a) x[a] = sa[tr + 16 * a][q];
+    for (int b = 0; b < 4; ++b) y[b] = sr[q][tc + 16 * b];
+    for (int a = 0; a < 4; ++a)
+      for (int b = 0; b < 4; ++b) acc[a][b] += x[a] * y[b];
+  }
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + tr + 16 * a;
+    if (i >= n || (i >= k0 && i < k0 + nb)) continue;
+    for (int b = 0; b < 4; ++b) {
+      const int j = j0 + tc + 16 * b;
+      if (j >= n || (j >= k0 && j < k0 + nb)) continue;
+      A[(long long)i * n + j] -= acc[a][b];
+    }
+  }
+}
+
+// (4) A_IK = -A_IK Dk for rows outside K; (5) A_KJ = R_KJ for columns outside K, A_KK = Dk.
+__global__ __launch_bounds__(256) void k_gjb_colpanel(int n, int k0, int nb, double* __restrict__ A,
+                                                      const double* __restrict__ Dk) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n || (i >= k0 && i < k0 + nb)) return;
+  double a[GJB];
+  for (int q = 0; q < nb; ++q) a[q] = A[(long long)i * n + k0 + q];
+  for (int j = 0; j < nb; ++j) {
+    double s = 0.0;
+    for (int q = 0; q < nb; ++q) s += a[q] * Dk[q * GJB + j];
+    A[(long long)i * n + k0 + j] = -s;
+  }
+}
+__global__ __launch_bounds__(256) void k_gjb_finish(int n, int k0, int nb, double* __restrict__ A,
+                                                    const double* __restrict__ R, const double* __restrict__ Dk) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const bool inK = (j >= k0 && j < k0 + nb);
+  for (int q = 0; q < nb; ++q)
+    A[(long long)(k0 + q) * n + j] = inK ? Dk[q * GJB + (j - k0)] : R[(long long)q * n + j];
 }
 
 // Coarse solve x = Ainv b (dense FP64), one wavefront per row.

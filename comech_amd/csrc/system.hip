@@ -2,11 +2,13 @@
 // Galerkin moment cells on coarse levels, dense FP64 coarsest inverse), V-cycle, PCG drivers and
 // the extern "C" boundary declared in include/comech_fe.h.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <numeric>
 #include <type_traits>
+#include <tuple>
 #include <unordered_map>
 
 #include "../../include/comech_fe.h"
@@ -32,6 +34,7 @@ struct LevelT {
   DevBuf<T> pw[3];          // prolongation weights, this level as the fine side
   DevBuf<T> rw[3];          // restriction weights of the next level's coarse nodes
   DevBuf<T> xval;           // extra stiffness values
+  DevBuf<T> r1w, r1k;       // rank-one terms
   DevBuf<T> x, b, r, t, d;  // work vectors
   double lmax = 0.0, lmin = 0.0;
   bool ready = false;
@@ -57,6 +60,17 @@ struct Level {
   std::vector<double> xval_h;
   DevBuf<int> xrow, xptr, xcol;
   int xw = 1;                 // lanes per extra row
+  // rank-one extra stiffness sum_t k_t w_t w_t^T (springs; exact coarsening w -> P^T w):
+  // t* = all terms of this level (host, coarsening); r1* = the terms applied as rank-one (support
+  // wider than one 3x3x3 node neighbourhood), the rest is folded into the CSR
+  std::vector<int> tptr_h, tdof_h;
+  std::vector<double> tw_h, tk_h;
+  // generic CSR part (off-diagonal input entries and their Galerkin products), host
+  std::vector<std::tuple<long long, double, double>> gen_h;
+  std::vector<int> r1ptr_h, r1dof_h;
+  std::vector<double> r1w_h, r1k_h;
+  DevBuf<int> r1ptr, r1dof;
+  DevBuf<double> r1w64, r1k64;
   int kz[2] = {0, 0};         // z extent of a structured block per precision (0 = heuristic)
   // level 0 only
   bool label_mode = false;
@@ -113,6 +127,9 @@ struct System {
   DevBuf<double> Ainv;
   int nc_dense = 0;
   double t_setup_ms = 0.0;
+  // build stage times (ms): level-0 prep, hierarchy structure, uploads, moments, diagonals,
+  // coarsest inverse, total
+  double t_stage[7] = {0, 0, 0, 0, 0, 0, 0};
   // staged level-0 input (converted in prepare_level0)
   int st_nmat = -1;
   std::vector<double> st_tabL, st_tabM, st_lam, st_mu;
@@ -121,6 +138,8 @@ struct System {
   std::vector<long long> st_xr, st_xc;
   std::vector<double> st_xv;
   std::vector<uint8_t> st_used_extra;
+  std::vector<long long> st_r1ptr, st_r1dof;
+  std::vector<double> st_r1w, st_r1k;
 
   // Level-0 data in the layouts the kernels use.
   void prepare_level0() {
@@ -219,7 +238,40 @@ struct System {
       if (!merged.empty() && merged.back().first == e.first) merged.back().second += e.second;
       else merged.push_back(e);
     }
-    set_extra_sorted(v, merged, nd3);
+    // diagonal entries are rank-one terms k e_r e_r^T (cheap exact coarsening); off-diagonal
+    // entries form the generic part
+    v.tptr_h.assign(1, 0); v.tdof_h.clear(); v.tw_h.clear(); v.tk_h.clear(); v.gen_h.clear();
+    for (auto& e : merged) {
+      const long long R = e.first / nd3, C = e.first % nd3;
+      if (R == C) {
+        if (e.second != 0.0) {
+          v.tdof_h.push_back((int)R); v.tw_h.push_back(1.0); v.tk_h.push_back(e.second);
+          v.tptr_h.push_back((int)v.tdof_h.size());
+        }
+      } else {
+        v.gen_h.emplace_back(e.first, e.second, std::fabs(e.second));
+      }
+    }
+    const long long nt = st_r1ptr.empty() ? 0 : (long long)st_r1ptr.size() - 1;
+    for (long long t = 0; t < nt; ++t) {
+      for (long long q = st_r1ptr[t]; q < st_r1ptr[t + 1]; ++q) {
+        CM_CHECK(st_r1dof[q] >= 0 && st_r1dof[q] < nd3, "rank-one entry out of range");
+        v.tdof_h.push_back((int)st_r1dof[q]);
+        v.tw_h.push_back(st_r1w[q]);
+      }
+      v.tk_h.push_back(st_r1k[t]);
+      v.tptr_h.push_back((int)v.tdof_h.size());
+    }
+    // level 0 applies everything as CSR (small: springs touch few nodes)
+    std::vector<std::tuple<long long, double, double>> ent0(v.gen_h);
+    for (size_t t = 0; t + 1 < v.tptr_h.size(); ++t)
+      for (int a = v.tptr_h[t]; a < v.tptr_h[t + 1]; ++a)
+        for (int b = v.tptr_h[t]; b < v.tptr_h[t + 1]; ++b) {
+          const double val = v.tk_h[t] * v.tw_h[a] * v.tw_h[b];
+          ent0.emplace_back((long long)v.tdof_h[a] * nd3 + v.tdof_h[b], val, std::fabs(val));
+        }
+    set_extra_sorted(v, ent0, nd3);
+    v.r1ptr_h.assign(1, 0); v.r1dof_h.clear(); v.r1w_h.clear(); v.r1k_h.clear();
   }
 
   Level& lv(int l) { return *L[l]; }
@@ -300,73 +352,210 @@ struct System {
         }
   }
 
-  // Galerkin P^T S P of the extra stiffness (entries with nonzero weights only)
-  static void coarse_extra(Level& f, Level& c) {
+  // Coarse support of a fine node (copies interpolate like their originals): <= 8 coarse nodes.
+  static int node_support(const Level& f, const Level& c, int fnode, long long* nodes, double* w) {
     const int fx = f.nx + 1, fy = f.ny + 1;
     const int cx = c.nx + 1, cy = c.ny + 1;
-    auto support = [&](int fnode, long long* nodes, double* w) {
-      int g = fnode;
-      if (g >= f.nng) g = f.copy_orig_h[g - f.nng];
-      const int i = g % fx, j = (g / fx) % fy, k = g / (fx * fy);
-      int n = 0;
-      for (int dz = 0; dz < 2; ++dz) {
-        const double wz = dz ? f.pw_h[2][k] : 1.0 - f.pw_h[2][k];
-        if (wz == 0.0) continue;
-        for (int dy = 0; dy < 2; ++dy) {
-          const double wy = dy ? f.pw_h[1][j] : 1.0 - f.pw_h[1][j];
-          if (wy == 0.0) continue;
-          for (int dx = 0; dx < 2; ++dx) {
-            const double wx = dx ? f.pw_h[0][i] : 1.0 - f.pw_h[0][i];
-            if (wx == 0.0) continue;
-            nodes[n] = (f.pia_h[0][i] + dx) + (long long)cx * ((f.pia_h[1][j] + dy) + (long long)cy * (f.pia_h[2][k] + dz));
-            w[n] = wx * wy * wz;
-            ++n;
+    int g = fnode;
+    if (g >= f.nng) g = f.copy_orig_h[g - f.nng];
+    const int i = g % fx, j = (g / fx) % fy, k = g / (fx * fy);
+    int n = 0;
+    for (int dz = 0; dz < 2; ++dz) {
+      const double wz = dz ? f.pw_h[2][k] : 1.0 - f.pw_h[2][k];
+      if (wz == 0.0) continue;
+      for (int dy = 0; dy < 2; ++dy) {
+        const double wy = dy ? f.pw_h[1][j] : 1.0 - f.pw_h[1][j];
+        if (wy == 0.0) continue;
+        for (int dx = 0; dx < 2; ++dx) {
+          const double wx = dx ? f.pw_h[0][i] : 1.0 - f.pw_h[0][i];
+          if (wx == 0.0) continue;
+          nodes[n] = (f.pia_h[0][i] + dx) + (long long)cx * ((f.pia_h[1][j] + dy) + (long long)cy * (f.pia_h[2][k] + dz));
+          w[n] = wx * wy * wz;
+          ++n;
+        }
+      }
+    }
+    return n;
+  }
+
+  // Galerkin coarsening of the extra stiffness.  Generic part: P^T S P by sorted accumulation (an
+  // entry whose contributions cancel to 1e-12 of their magnitude is dropped).  Rank-one terms:
+  // w -> P^T w (entries cancelling to 1e-12 dropped; node-pair springs between a copy and its
+  // original vanish exactly).  For applying, terms whose support fits one 3x3x3 node
+  // neighbourhood are summed into 27-point node blocks and emitted as CSR rows; wider terms
+  // (ligaments) stay rank-one.
+  static void coarse_extra(Level& f, Level& c) {
+    const int cx = c.nx + 1, cy = c.ny + 1;
+    const long long ncd = 3LL * (long long)cx * cy * (c.nz + 1);
+    c.gen_h.clear();
+    for (auto& e : f.gen_h) {
+      const long long key = std::get<0>(e);
+      const long long row = key / (3 * f.nn), col = key % (3 * f.nn);
+      long long rn[8], cnod[8];
+      double rw[8], cw[8];
+      const int nr = node_support(f, c, (int)(row / 3), rn, rw);
+      const int ncs = node_support(f, c, (int)(col / 3), cnod, cw);
+      for (int a2 = 0; a2 < nr; ++a2)
+        for (int b2 = 0; b2 < ncs; ++b2) {
+          const double val = rw[a2] * cw[b2] * std::get<1>(e);
+          c.gen_h.emplace_back((3 * rn[a2] + row % 3) * ncd + 3 * cnod[b2] + col % 3, val, std::fabs(val));
+        }
+    }
+    if (!c.gen_h.empty()) {
+      // merge duplicates now so that the next level starts from unique entries
+      std::vector<std::tuple<long long, double, double>> m;
+      std::sort(c.gen_h.begin(), c.gen_h.end(), [](auto& x, auto& y) { return std::get<0>(x) < std::get<0>(y); });
+      for (size_t q = 0; q < c.gen_h.size();) {
+        const long long key = std::get<0>(c.gen_h[q]);
+        double sv = 0.0, sa = 0.0;
+        while (q < c.gen_h.size() && std::get<0>(c.gen_h[q]) == key) { sv += std::get<1>(c.gen_h[q]); sa += std::get<2>(c.gen_h[q]); ++q; }
+        if (std::fabs(sv) > 1e-12 * sa) m.emplace_back(key, sv, std::fabs(sv));
+      }
+      c.gen_h.swap(m);
+    }
+    // coarsen the rank-one terms
+    c.tptr_h.assign(1, 0); c.tdof_h.clear(); c.tw_h.clear(); c.tk_h.clear();
+    std::vector<std::tuple<int, double, double>> tw;
+    for (size_t t = 0; t + 1 < f.tptr_h.size(); ++t) {
+      tw.clear();
+      for (int q = f.tptr_h[t]; q < f.tptr_h[t + 1]; ++q) {
+        const int d = f.tdof_h[q];
+        long long cn[8];
+        double cw[8];
+        const int ns = node_support(f, c, d / 3, cn, cw);
+        for (int a2 = 0; a2 < ns; ++a2) {
+          const double val = cw[a2] * f.tw_h[q];
+          tw.emplace_back((int)(3 * cn[a2] + d % 3), val, std::fabs(val));
+        }
+      }
+      std::sort(tw.begin(), tw.end(), [](auto& x, auto& y) { return std::get<0>(x) < std::get<0>(y); });
+      const size_t before = c.tdof_h.size();
+      for (size_t q = 0; q < tw.size();) {
+        const int d = std::get<0>(tw[q]);
+        double sv = 0.0, sa = 0.0;
+        while (q < tw.size() && std::get<0>(tw[q]) == d) { sv += std::get<1>(tw[q]); sa += std::get<2>(tw[q]); ++q; }
+        if (std::fabs(sv) > 1e-12 * sa) { c.tdof_h.push_back(d); c.tw_h.push_back(sv); }
+      }
+      if (c.tdof_h.size() > before) {
+        c.tk_h.push_back(f.tk_h[t]);
+        c.tptr_h.push_back((int)c.tdof_h.size());
+      }
+    }
+    // apply structures: local terms -> 27-point node blocks -> CSR; wide terms -> rank-one
+    c.r1ptr_h.assign(1, 0); c.r1dof_h.clear(); c.r1w_h.clear(); c.r1k_h.clear();
+    std::unordered_map<long long, int> slot;
+    std::vector<double> blk;          // per slot: [3 comps][27 offsets][3 comps]
+    std::vector<long long> slot_node;
+    auto ijk = [&](long long nd, int& i, int& j, int& k) {
+      i = (int)(nd % cx); j = (int)((nd / cx) % cy); k = (int)(nd / ((long long)cx * cy));
+    };
+    for (size_t t = 0; t + 1 < c.tptr_h.size(); ++t) {
+      const int q0 = c.tptr_h[t], q1 = c.tptr_h[t + 1];
+      int lo[3] = {1 << 30, 1 << 30, 1 << 30}, hi[3] = {-1, -1, -1};
+      for (int q = q0; q < q1; ++q) {
+        int p3[3];
+        ijk(c.tdof_h[q] / 3, p3[0], p3[1], p3[2]);
+        for (int a2 = 0; a2 < 3; ++a2) { lo[a2] = std::min(lo[a2], p3[a2]); hi[a2] = std::max(hi[a2], p3[a2]); }
+      }
+      const bool local = (hi[0] - lo[0] <= 1) && (hi[1] - lo[1] <= 1) && (hi[2] - lo[2] <= 1);
+      if (!local) {
+        for (int q = q0; q < q1; ++q) { c.r1dof_h.push_back(c.tdof_h[q]); c.r1w_h.push_back(c.tw_h[q]); }
+        c.r1k_h.push_back(c.tk_h[t]);
+        c.r1ptr_h.push_back((int)c.r1dof_h.size());
+        continue;
+      }
+      for (int qa = q0; qa < q1; ++qa) {
+        const long long na = c.tdof_h[qa] / 3;
+        auto it = slot.find(na);
+        int sl;
+        if (it == slot.end()) {
+          sl = (int)slot_node.size();
+          slot.emplace(na, sl);
+          slot_node.push_back(na);
+          blk.resize(blk.size() + 243, 0.0);
+        } else {
+          sl = it->second;
+        }
+        int ia, ja, ka;
+        ijk(na, ia, ja, ka);
+        for (int qb = q0; qb < q1; ++qb) {
+          int ib, jb, kb;
+          ijk(c.tdof_h[qb] / 3, ib, jb, kb);
+          const int off = (ib - ia + 1) + 3 * (jb - ja + 1) + 9 * (kb - ka + 1);
+          blk[(size_t)sl * 243 + (c.tdof_h[qa] % 3) * 81 + off * 3 + c.tdof_h[qb] % 3] +=
+              c.tk_h[t] * c.tw_h[qa] * c.tw_h[qb];
+        }
+      }
+    }
+    std::vector<std::tuple<long long, double, double>> ent(c.gen_h);
+    std::vector<int> order(slot_node.size());
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](int x, int y) { return slot_node[x] < slot_node[y]; });
+    if (ent.empty()) {
+      // rows straight from the node blocks, columns in increasing order (offsets z, y, x)
+      c.xrow_h.clear(); c.xptr_h.clear(); c.xcol_h.clear(); c.xval_h.clear();
+      for (int sl : order) {
+        const long long na = slot_node[sl];
+        int ia, ja, ka;
+        ijk(na, ia, ja, ka);
+        for (int ca = 0; ca < 3; ++ca) {
+          const size_t start = c.xcol_h.size();
+          for (int off = 0; off < 27; ++off) {
+            const int ib = ia + off % 3 - 1, jb = ja + (off / 3) % 3 - 1, kb = ka + off / 9 - 1;
+            for (int cb = 0; cb < 3; ++cb) {
+              const double v2 = blk[(size_t)sl * 243 + ca * 81 + off * 3 + cb];
+              if (v2 == 0.0) continue;
+              c.xcol_h.push_back((int)(3 * ((long long)ib + (long long)cx * (jb + (long long)cy * kb)) + cb));
+              c.xval_h.push_back(v2);
+            }
+          }
+          if (c.xcol_h.size() > start) {
+            c.xrow_h.push_back((int)(3 * na + ca));
+            c.xptr_h.push_back((int)start);
           }
         }
       }
-      return n;
-    };
-    std::unordered_map<long long, double> acc;
-    const long long ncd = 3LL * (long long)(c.nx + 1) * (c.ny + 1) * (c.nz + 1);
-    for (size_t r = 0; r < f.xrow_h.size(); ++r) {
-      const int row = f.xrow_h[r];
-      long long rn[8], cnod[8];
-      double rw[8], cw[8];
-      const int nr = support(row / 3, rn, rw);
-      for (int q = f.xptr_h[r]; q < f.xptr_h[r + 1]; ++q) {
-        const int col = f.xcol_h[q];
-        const int ncs = support(col / 3, cnod, cw);
-        const double v = f.xval_h[q];
-        for (int a = 0; a < nr; ++a)
-          for (int b = 0; b < ncs; ++b) {
-            const long long R = 3 * rn[a] + row % 3, C = 3 * cnod[b] + col % 3;
-            acc[R * ncd + C] += rw[a] * cw[b] * v;
-          }
+      c.xptr_h.push_back((int)c.xcol_h.size());
+      const size_t nr = c.xrow_h.size();
+      const double avg = nr ? (double)c.xcol_h.size() / nr : 1.0;
+      int w = 1;
+      while (w < 64 && w < avg) w <<= 1;
+      c.xw = w;
+    } else {
+      for (int sl : order) {
+        const long long na = slot_node[sl];
+        int ia, ja, ka;
+        ijk(na, ia, ja, ka);
+        for (int ca = 0; ca < 3; ++ca)
+          for (int off = 0; off < 27; ++off)
+            for (int cb = 0; cb < 3; ++cb) {
+              const double v2 = blk[(size_t)sl * 243 + ca * 81 + off * 3 + cb];
+              if (v2 == 0.0) continue;
+              const long long nb2 = (ia + off % 3 - 1) + (long long)cx * ((ja + (off / 3) % 3 - 1) + (long long)cy * (ka + off / 9 - 1));
+              ent.emplace_back((3 * na + ca) * ncd + 3 * nb2 + cb, v2, std::fabs(v2));
+            }
       }
+      set_extra_sorted(c, ent, ncd);
     }
-    std::vector<std::pair<long long, double>> ent(acc.begin(), acc.end());
-    std::sort(ent.begin(), ent.end());
-    set_extra_sorted(c, ent, ncd);
   }
 
-  static void set_extra_sorted(Level& lv, const std::vector<std::pair<long long, double>>& ent, long long ncd) {
+  static void set_extra_sorted(Level& lv, std::vector<std::tuple<long long, double, double>>& ent, long long ncd) {
     lv.xrow_h.clear(); lv.xptr_h.clear(); lv.xcol_h.clear(); lv.xval_h.clear();
-    // Galerkin products of node-pair springs between a copy and its original cancel exactly on
-    // coarse levels (both ends are interpolated alike); drop such entries
-    double vmax = 0.0;
-    for (auto& e : ent) vmax = std::max(vmax, std::fabs(e.second));
-    const double drop = 1e-14 * vmax;
+    std::sort(ent.begin(), ent.end(), [](auto& x, auto& y) { return std::get<0>(x) < std::get<0>(y); });
     long long last = -1;
-    for (auto& e : ent) {
-      if (std::fabs(e.second) <= drop) continue;
-      const long long R = e.first / ncd, C = e.first % ncd;
+    for (size_t q = 0; q < ent.size();) {
+      const long long key = std::get<0>(ent[q]);
+      double sv = 0.0, sa = 0.0;
+      while (q < ent.size() && std::get<0>(ent[q]) == key) { sv += std::get<1>(ent[q]); sa += std::get<2>(ent[q]); ++q; }
+      if (!(std::fabs(sv) > 1e-12 * sa)) continue;     // cancelled (or zero)
+      const long long R = key / ncd, C = key % ncd;
       if (R != last) {
         lv.xrow_h.push_back((int)R);
         lv.xptr_h.push_back((int)lv.xcol_h.size());
         last = R;
       }
       lv.xcol_h.push_back((int)C);
-      lv.xval_h.push_back(e.second);
+      lv.xval_h.push_back(sv);
     }
     lv.xptr_h.push_back((int)lv.xcol_h.size());
     const size_t nr = lv.xrow_h.size();
@@ -391,7 +580,17 @@ struct System {
     CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
     CM_HIP(hipEventRecord(e0, stream));
     CM_CHECK(!built, "system already built");
+    using clk = std::chrono::steady_clock;
+    auto tick = clk::now();
+    auto lap = [&](int k) {
+      CM_HIP(hipStreamSynchronize(stream));
+      const auto now = clk::now();
+      t_stage[k] = std::chrono::duration<double, std::milli>(now - tick).count();
+      tick = now;
+    };
+    const auto t_begin = clk::now();
     prepare_level0();
+    lap(0);
     Level& f0 = lv(0);
     f0.used.upload(f0.used_h, stream);
     // level structure (fe/gmg.py:86-101, rb/gpusolve.py:44-46)
@@ -410,6 +609,7 @@ struct System {
       coarse_extra(f, *c);
       L.push_back(std::move(c));
     }
+    lap(1);
     // device transfer tables and per-level constant data
     for (int l = 0; l < nlev(); ++l) {
       Level& v = lv(l);
@@ -431,8 +631,13 @@ struct System {
       v.xrow.upload(v.xrow_h, stream);
       v.xptr.upload(v.xptr_h, stream);
       v.xcol.upload(v.xcol_h, stream);
+      v.r1ptr.upload(v.r1ptr_h, stream);
+      v.r1dof.upload(v.r1dof_h, stream);
+      v.r1w64.upload(v.r1w_h, stream);
+      v.r1k64.upload(v.r1k_h, stream);
     }
     prepare_prec<double>(0, false);
+    lap(2);
     // moments of coarse cells (FP64)
     for (int l = 1; l < nlev(); ++l) {
       Level& c = lv(l);
@@ -451,9 +656,13 @@ struct System {
       CM_LAUNCH_CHECK();
       prepare_prec<double>(l, false);   // casts the moments just computed
     }
+    lap(3);
     // diagonal blocks (FP64) -> Dinv per precision is derived lazily
     for (int l = 0; l + 1 < nlev(); ++l) compute_diag(l);
+    lap(4);
     build_coarse_dense();
+    lap(5);
+    t_stage[6] = std::chrono::duration<double, std::milli>(clk::now() - t_begin).count();
     CM_HIP(hipEventRecord(e1, stream));
     CM_HIP(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -505,6 +714,12 @@ struct System {
       CM_LAUNCH_CHECK();
       CM_HIP(hipStreamSynchronize(stream));
     }
+    const int nterm = (int)v.r1k_h.size();
+    if (nterm) {
+      hipLaunchKernelGGL(k_diag_rank1, grid1(nterm), dim3(256), 0, stream, nterm, v.r1ptr.p, v.r1dof.p, v.r1w64.p,
+                         v.r1k64.p, D.p);
+      CM_LAUNCH_CHECK();
+    }
   }
 
   void build_coarse_dense() {
@@ -540,14 +755,24 @@ struct System {
         CM_LAUNCH_CHECK();
         CM_HIP(hipStreamSynchronize(stream));
       }
+      const int nterm = (int)c.r1k_h.size();
+      if (nterm) {
+        hipLaunchKernelGGL(k_dense_add_rank1, grid1(nterm), dim3(256), 0, stream, nterm, c.r1ptr.p, c.r1dof.p,
+                           c.r1w64.p, c.r1k64.p, (int)n, Ainv.p);
+        CM_LAUNCH_CHECK();
+      }
     }
     hipLaunchKernelGGL(k_dense_fix_empty, dim3((unsigned)n), dim3(256), 0, stream, (int)n, Ainv.p);
     CM_LAUNCH_CHECK();
-    DevBuf<double> colk(n);
-    for (int k = 0; k < n; ++k) {
-      hipLaunchKernelGGL(k_gj_savecol, grid1(n), dim3(256), 0, stream, (int)n, k, Ainv.p, colk.p);
-      hipLaunchKernelGGL(k_gj_scale, grid1(n), dim3(256), 0, stream, (int)n, k, Ainv.p, colk.p);
-      hipLaunchKernelGGL(k_gj_update, grid1(n * n), dim3(256), 0, stream, (int)n, k, Ainv.p, colk.p);
+    DevBuf<double> Dk(GJB * GJB), R((size_t)GJB * n);
+    const dim3 tiles(div_up(n, 64), div_up(n, 64));
+    for (int k0 = 0; k0 < n; k0 += GJB) {
+      const int nb = (int)std::min<long long>(GJB, n - k0);
+      hipLaunchKernelGGL(k_gjb_diag, dim3(1), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
+      hipLaunchKernelGGL(k_gjb_rowpanel, grid1(n), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p, R.p);
+      hipLaunchKernelGGL(k_gjb_update, tiles, dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, R.p);
+      hipLaunchKernelGGL(k_gjb_colpanel, grid1(n), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
+      hipLaunchKernelGGL(k_gjb_finish, grid1(n), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, R.p, Dk.p);
     }
     CM_LAUNCH_CHECK();
     CM_HIP(hipStreamSynchronize(stream));
@@ -586,6 +811,7 @@ struct System {
         for (int ax = 0; ax < 3; ++ax) { upload_cast(d.pw[ax], v.pw_h[ax], stream); upload_cast(d.rw[ax], v.rw_h[ax], stream); }
       }
       if (!v.xval_h.empty()) upload_cast(d.xval, v.xval_h, stream);
+      if (!v.r1k_h.empty()) { upload_cast(d.r1w, v.r1w_h, stream); upload_cast(d.r1k, v.r1k_h, stream); }
       const size_t nv = 3 * (size_t)v.nn;
       d.x.alloc(nv); d.b.alloc(nv); d.r.alloc(nv); d.t.alloc(nv); d.d.alloc(nv);
       d.ready = true;
@@ -638,6 +864,10 @@ struct System {
     }
     const int nrow = (int)v.xrow_h.size();
     if (nrow) launch_extra<T>(v, d, u, y, sign);
+    const int nterm = (int)v.r1k_h.size();
+    if (nterm)
+      hipLaunchKernelGGL(k_apply_rank1<T>, grid1(64LL * nterm), dim3(256), 0, stream, nterm, v.r1ptr.p, v.r1dof.p,
+                         d.r1w.p, d.r1k.p, u, y, sign);
     CM_LAUNCH_CHECK();
   }
 
@@ -740,6 +970,9 @@ struct System {
   }
 
   // ------------------------------------------------------------------ smoothers, V-cycle
+  // lambda_max of M A by 12 power steps from a fixed pseudo-normal start (fe/gmg.py:51-59):
+  // lam = |y| / |x| of the last step, then lmax = 1.1 lam, lmin = lmax / 30.  The iterate is
+  // normalised on the device; the host reads the result once.
   template <typename T>
   void power_iteration(int l) {
     Level& v = lv(l);
@@ -747,24 +980,25 @@ struct System {
     const long long n = 3 * v.nn;
     hipLaunchKernelGGL(k_randn<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, v.used.p, 0ULL, d.x.p);
     DevBuf<double> s(4);
-    double lam = 1.0;
     for (int it = 0; it < 12; ++it) {
       apply<T>(l, d.x.p, d.t.p, MODE_APPLY, nullptr);
       hipLaunchKernelGGL(k_block_apply<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, d.Dinv.p, d.t.p, d.d.p);
       hipLaunchKernelGGL((k_dot<T, T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, d.d.p, d.d.p, partial.p);
       hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, s.p, 0);
-      hipLaunchKernelGGL((k_dot<T, T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, d.x.p, d.x.p, partial.p);
-      hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, s.p, 1);
-      CM_LAUNCH_CHECK();
-      double hs[2];
-      CM_HIP(hipMemcpyAsync(hs, s.p, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
-      CM_HIP(hipStreamSynchronize(stream));
-      lam = std::sqrt(hs[0]) / std::sqrt(hs[1]);
-      // x = y / ||y||
-      CM_HIP(hipMemcpyAsync(d.x.p, d.d.p, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
-      hipLaunchKernelGGL(k_scale<T>, grid1(n), dim3(256), 0, stream, n, d.x.p, s.p, 0);
+      if (it == 11) {
+        hipLaunchKernelGGL((k_dot<T, T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, d.x.p, d.x.p, partial.p);
+        hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, s.p, 1);
+      } else {
+        // x = y / |y|
+        CM_HIP(hipMemcpyAsync(d.x.p, d.d.p, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+        hipLaunchKernelGGL(k_scale<T>, grid1(n), dim3(256), 0, stream, n, d.x.p, s.p, 0);
+      }
       CM_LAUNCH_CHECK();
     }
+    double hs[2];
+    CM_HIP(hipMemcpyAsync(hs, s.p, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    CM_HIP(hipStreamSynchronize(stream));
+    const double lam = std::sqrt(hs[0]) / std::sqrt(hs[1]);
     d.lmax = 1.1 * lam;
     d.lmin = d.lmax / 30.0;
   }
@@ -945,8 +1179,24 @@ struct System {
     info->relres = rn / (bn > 0 ? bn : 1.0);
   }
 
-  // Mixed precision: FP64 outer residual / iterative refinement, FP32 inner PCG whose alpha/beta
-  // reductions accumulate in FP64, FP32 V-cycle with an FP64 coarse solve.
+  // z32 = prec(r32) in FP32 (+ the FP64 rigid-mode correction when present)
+  void precondition32(const float* r, float* z) {
+    const long long n = 3 * lv(0).nn;
+    vcycle<float>(0, r, z);
+    if (zcol) {
+      hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z64.p, z);
+      hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, Ap64.p, r);
+      rigid_correct(Ap64.p, z64.p);
+      hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, z, z64.p);
+    }
+  }
+
+  // Mixed precision as iterative refinement: FP64 outer residual r = b - A64 x; inner FP32 PCG
+  // (operator, V-cycle and Krylov vectors in FP32, alpha/beta reductions accumulated in FP64,
+  // coarse solve FP64) on A d = r/||r|| until its recurrence residual falls below inner_tol (or
+  // the outer target); x += ||r|| d in FP64.  Convergence is decided by the FP64 true residual.
+  // With an FP32 operator each outer step contracts the true residual by at most about
+  // eps32 * cond(A), so inner tolerances much below that only cost iterations.
   void pcg_mixed(const double* b, double* x, double tol, int maxiter, int check_every, double inner_tol,
                  cm_solve_info* info) {
     Level& v0 = lv(0);
@@ -961,7 +1211,6 @@ struct System {
     int it = 0, outer = 0, flag = 1;
     double rn = 0.0;
     while (true) {
-      // true FP64 residual
       apply<double>(0, x, Ap64.p, MODE_APPLY, nullptr);
       hipLaunchKernelGGL(k_resid_cast<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, r64.p, b, Ap64.p,
                          partial.p, (float*)nullptr, 1.0);
@@ -970,18 +1219,11 @@ struct System {
       if (rn <= thr) { flag = 0; break; }
       if (it >= maxiter) { flag = maxiter; break; }
       ++outer;
-      // inner FP32 PCG on A d = r / ||r||
       const double sc = 1.0 / rn;
-      hipLaunchKernelGGL((k_resid_cast<float>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, r64.p, b, Ap64.p,
+      hipLaunchKernelGGL(k_resid_cast<float>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, r64.p, b, Ap64.p,
                          partial.p, r32.p, sc);
       hipLaunchKernelGGL(k_fill<float>, grid1(n), dim3(256), 0, stream, d32.p, n, 0.f);
-      vcycle<float>(0, r32.p, z32.p);
-      if (zcol) {
-        hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z64.p, z32.p);
-        hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, Ap64.p, r32.p);
-        rigid_correct(Ap64.p, z64.p);
-        hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, z32.p, z64.p);
-      }
+      precondition32(r32.p, z32.p);
       CM_HIP(hipMemcpyAsync(p32.p, z32.p, n * sizeof(float), hipMemcpyDeviceToDevice, stream));
       int cur = S_RZ0, nxt = S_RZ1;
       dot(n, r32.p, z32.p, cur);
@@ -994,17 +1236,10 @@ struct System {
                            r32.p, p32.p, Ap32.p, S.p, cur, S_PAP, partial.p, (float*)nullptr);
         finalize_to<double>(S_RR);
         ++it; ++inner;
-        if (inner % check_every == 0 || inner < check_every) {
-          const double rin = std::sqrt(read_slot(S_RR));
-          if (rin <= ithr) break;
+        if (inner < check_every || inner % check_every == 0) {
+          if (std::sqrt(read_slot(S_RR)) <= ithr) break;
         }
-        vcycle<float>(0, r32.p, z32.p);
-        if (zcol) {
-          hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z64.p, z32.p);
-          hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, Ap64.p, r32.p);
-          rigid_correct(Ap64.p, z64.p);
-          hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, z32.p, z64.p);
-        }
+        precondition32(r32.p, z32.p);
         dot(n, r32.p, z32.p, nxt);
         hipLaunchKernelGGL(k_pcg_p<float>, grid1(n), dim3(256), 0, stream, n, p32.p, z32.p, S.p, nxt, cur);
         CM_LAUNCH_CHECK();
@@ -1158,6 +1393,22 @@ int cm_sys_set_extra(cm_sys* h, int64_t nnz, const int64_t* rows, const int64_t*
   CM_API_END
 }
 
+int cm_sys_set_rank1(cm_sys* h, int64_t nterm, const int64_t* ptr, const int64_t* dofs, const double* w,
+                     const double* k) {
+  CM_API_BEGIN
+  CM_CHECK(h && !h->s.built, "system missing or already built");
+  CM_CHECK(nterm >= 0, "negative term count");
+  System& s = h->s;
+  s.st_r1ptr.assign(ptr, ptr + nterm + 1);
+  CM_CHECK(s.st_r1ptr[0] == 0, "rank-one ptr must start at 0");
+  for (int64_t t = 0; t < nterm; ++t) CM_CHECK(s.st_r1ptr[t + 1] >= s.st_r1ptr[t], "rank-one ptr must not decrease");
+  const int64_t nnz = s.st_r1ptr[nterm];
+  s.st_r1dof.assign(dofs, dofs + nnz);
+  s.st_r1w.assign(w, w + nnz);
+  s.st_r1k.assign(k, k + nterm);
+  CM_API_END
+}
+
 int cm_sys_set_used(cm_sys* h, int64_t nn, const uint8_t* used) {
   CM_API_BEGIN
   CM_CHECK(h && !h->s.built, "system missing or already built");
@@ -1229,7 +1480,7 @@ int cm_sys_levels(cm_sys* h, int* nlev, int64_t* desc, int cap) {
   for (int l = 0; l < s.nlev() && l < cap; ++l) {
     cm::Level& v = s.lv(l);
     desc[6 * l] = v.nx; desc[6 * l + 1] = v.ny; desc[6 * l + 2] = v.nz;
-    desc[6 * l + 3] = v.nn; desc[6 * l + 4] = s.compact_dofs(l); desc[6 * l + 5] = (int64_t)v.xrow_h.size();
+    desc[6 * l + 3] = v.nn; desc[6 * l + 4] = s.compact_dofs(l); desc[6 * l + 5] = (int64_t)(v.xrow_h.size() + v.r1k_h.size());
   }
   CM_API_END
 }
@@ -1361,6 +1612,13 @@ int cm_sys_autotune(cm_sys* h, int prec, int nrep) {
     s.ensure_prec<float>();
     s.autotune_kz<float>(nrep);
   }
+  CM_API_END
+}
+
+int cm_sys_timings(cm_sys* h, double* out, int cap) {
+  CM_API_BEGIN
+  need_built(h);
+  for (int k = 0; k < 7 && k < cap; ++k) out[k] = h->s.t_stage[k];
   CM_API_END
 }
 

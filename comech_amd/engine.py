@@ -11,6 +11,7 @@ import numpy as np
 import scipy.sparse as sp
 import torch
 
+from .grid import corners_of
 from .system import VoxelSystem, cell_stress_gpu, require_gpu
 
 
@@ -22,7 +23,7 @@ def split_cells(grid, active, cn):
     for a in range(0, len(cells), 1 << 20):
         c = cells[a:a + (1 << 20)]
         row = cn[c]
-        diff = (row != grid.corners(c)).any(1)
+        diff = (row != corners_of(grid, c)).any(1)
         out_c.append(c[diff]); out_n.append(row[diff])
     if not out_c:
         return np.zeros(0, np.int64), np.zeros((0, 8), np.int64)
@@ -39,7 +40,7 @@ def used_nodes(grid, active, nn, irr_cells=None, irr_nodes=None):
         cells = cells[keep[cells]]
         used[np.asarray(irr_nodes).ravel()] = True
     for a in range(0, len(cells), 1 << 20):
-        used[grid.corners(cells[a:a + (1 << 20)]).ravel()] = True
+        used[corners_of(grid, cells[a:a + (1 << 20)]).ravel()] = True
     return used
 
 

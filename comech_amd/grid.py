@@ -229,6 +229,19 @@ class Grid:
         return self._lazy("cell_xyz", lambda: np.stack([self.xc[self.cI], self.yc[self.cJ], self.zc[self.cK]], 1))
 
 
+def corners_of(grid, cells):
+    """(n, 8) grid node ids of cells; works for this Grid and for the reference's fe/solver.Grid."""
+    if hasattr(grid, "corners"):
+        return grid.corners(cells)
+    cells = np.asarray(cells, np.int64)
+    k = cells % grid.nz
+    j = (cells // grid.nz) % grid.ny
+    i = cells // (grid.ny * grid.nz)
+    base = i + (grid.nx + 1) * (j + (grid.ny + 1) * k)
+    off = LOC[:, 0] + (grid.nx + 1) * (LOC[:, 1] + (grid.ny + 1) * LOC[:, 2])
+    return base[:, None] + off[None, :]
+
+
 def face_node_areas(grid, cells, face, cn=None):
     """Four face nodes per cell and tributary area face/4 (fe/solver.py:165-174,
     rb/splitfe.py:94-102 with ``cn``)."""
@@ -236,7 +249,7 @@ def face_node_areas(grid, cells, face, cn=None):
     side = 1 if face[1] == "+" else 0
     sel = LOC[:, ax] == side
     cells = np.asarray(cells, np.int64)
-    nodes = (grid.corners(cells) if cn is None else cn[cells])[:, sel]
+    nodes = (corners_of(grid, cells) if cn is None else cn[cells])[:, sel]
     k = cells % grid.nz
     j = (cells // grid.nz) % grid.ny
     i = cells // (grid.ny * grid.nz)

@@ -64,3 +64,35 @@ def spring_pairs(xyz, pairs, k, directional=True):
         return z, z, np.zeros(0)
     return (np.concatenate(rows).astype(np.int64), np.concatenate(cols).astype(np.int64),
             np.concatenate(vals).astype(float))
+
+
+def spring_terms(xyz, pairs, k, directional=True):
+    """Springs between single nodes as rank-one terms (ptr, dofs, w, k) over node DOFs: directional
+    -> one term w = (+d at a, -d at b); isotropic -> one term per component, w = e_a - e_b."""
+    pairs = np.asarray(pairs, np.int64).reshape(-1, 2)
+    kk = np.broadcast_to(np.asarray(k, float), (len(pairs),))
+    a, b = pairs[:, 0], pairs[:, 1]
+    if directional:
+        d = xyz[b] - xyz[a]
+        d = d / np.linalg.norm(d, axis=1, keepdims=True)
+        dofs = np.concatenate([3 * a[:, None] + np.arange(3), 3 * b[:, None] + np.arange(3)], 1)
+        w = np.concatenate([d, -d], 1)
+        ptr = np.arange(len(pairs) + 1) * 6
+        return ptr.astype(np.int64), dofs.ravel().astype(np.int64), w.ravel(), kk.copy()
+    dofs = np.stack([3 * a[:, None] + np.arange(3), 3 * b[:, None] + np.arange(3)], 2)   # (m, 3, 2)
+    w = np.broadcast_to(np.array([1.0, -1.0]), dofs.shape)
+    ptr = np.arange(3 * len(pairs) + 1) * 2
+    return ptr.astype(np.int64), dofs.reshape(-1).astype(np.int64), w.reshape(-1).copy(), np.repeat(kk, 3)
+
+
+def cat_terms(*terms):
+    """Concatenate rank-one term sets (ptr, dofs, w, k)."""
+    terms = [t for t in terms if t is not None and len(t[3])]
+    if not terms:
+        return None
+    ptr, off = [np.zeros(1, np.int64)], 0
+    for t in terms:
+        ptr.append(t[0][1:] + off)
+        off += t[0][-1]
+    return (np.concatenate(ptr), np.concatenate([t[1] for t in terms]), np.concatenate([t[2] for t in terms]),
+            np.concatenate([t[3] for t in terms]))

@@ -13,7 +13,7 @@ import scipy.sparse as sp
 import torch
 
 from .. import engine as E
-from ..grid import LOC
+from ..grid import LOC, corners_of
 from ..system import _lib, require_gpu
 
 SRI = True
@@ -28,7 +28,7 @@ def split_connectivity_fast(g, active, part, bone_share):
     when the cell holds none of those bones, the bone with most cells at the node.
     Returns (cn (ncell,8), orig (nnode+ncopy,), number of split nodes)."""
     cells = np.flatnonzero(active)
-    corn = g.corners(cells)
+    corn = corners_of(g, cells)
     pc = np.asarray(part)[cells].astype(np.int64)
     nb = bone_share.shape[1]
     # incidence of (node, bone) pairs over bone cells, with multiplicity

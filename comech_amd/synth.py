@@ -19,8 +19,8 @@ Seeds: ``numpy.random.default_rng(config_id * 1000 + item)``.
 import numpy as np
 from scipy import ndimage
 
-from .grid import Grid, face_node_areas, lame
-from .rb.replica3 import spring_pairs
+from .grid import Grid, corners_of, face_node_areas, lame
+from .rb.replica3 import cat_terms, spring_pairs, spring_terms
 from .rb.splitfe import split_connectivity_fast
 
 
@@ -179,7 +179,7 @@ def config2(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000, n_lig=
     gsel = np.sort(rng.choice(bottom, size=min(n_ground, len(bottom)), replace=False))
     # ligament springs between bone-surface nodes of different bones
     soft_touch = np.zeros(g.nnode, bool)
-    soft_touch[g.corners(np.flatnonzero(active & (part == 0))).ravel()] = True
+    soft_touch[corners_of(g, np.flatnonzero(active & (part == 0))).ravel()] = True
     surf = []
     for b in (1, 2, 3):
         nodes_b = np.unique(cn[np.flatnonzero(active & (part == b))].ravel())
@@ -196,6 +196,8 @@ def config2(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000, n_lig=
         if np.linalg.norm(xyz[b_] - xyz[a]) > 0:
             pairs.append((a, b_))
     lig = spring_pairs(xyz, np.asarray(pairs, np.int64), k_lig, directional=True)
+    terms = cat_terms(spring_terms(xyz, np.stack([copies, orig[copies]], 1), k_if, directional=False),
+                      spring_terms(xyz, np.asarray(pairs, np.int64), k_lig, directional=True))
     # load: downward total over active cells of the top 10 layers, 1/8 per corner
     topc = np.flatnonzero(active & (g.cK >= nz - 10))
     ln = cn[topc].ravel()
@@ -215,24 +217,26 @@ def config2(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000, n_lig=
 
     return dict(name="config2", grid=g, lam=lam, mu=mu, active=active, labels=labels, lam_tab=lam_tab,
                 mu_tab=mu_tab, part=part, bone_share=share, cn=cn, orig=orig, nsplit=nsplit, fixed=fixed,
-                loads=loads, extra=extra, extra_cb=extra_cb, sri=True, action=action, used=used,
+                loads=loads, extra=extra, extra_cb=extra_cb, terms=terms, sri=True, action=action, used=used,
                 n_ground=len(gsel), n_lig=len(pairs), n_copies=len(copies))
 
 
-def solve_inputs_config2(case):
-    """config2 as level-0 GPU inputs: extra COO over node DOFs (ground + interface + ligament) and
-    the nodal force vector (node layout, grid nodes then copies)."""
-    g = case["grid"]
+def solve_inputs_config2(case, as_terms=True):
+    """config2 as level-0 GPU inputs: extra COO over node DOFs (ground springs; with
+    ``as_terms=False`` also the interface and ligament springs as COO), rank-one terms (interface
+    and ligament springs) and the nodal force vector (grid nodes then copies)."""
     nn = len(case["orig"])
     gs, ga = case["fixed"]
     rows = (3 * np.asarray(gs)[:, None] + np.arange(3)).ravel()
     ground = (rows, rows.copy(), np.repeat(ga * 1e6, 3))
-    r, c, v = case["extra"]
-    extra = (np.concatenate([ground[0], r]), np.concatenate([ground[1], c]), np.concatenate([ground[2], v]))
     f = np.zeros((nn, 3))
     for nodes, area, t in case["loads"]:
         np.add.at(f, nodes, area[:, None] * t[None, :])
-    return extra, f.ravel()
+    if as_terms:
+        return ground, case["terms"], f.ravel()
+    r, c, v = case["extra"]
+    extra = (np.concatenate([ground[0], r]), np.concatenate([ground[1], c]), np.concatenate([ground[2], v]))
+    return extra, None, f.ravel()
 
 
 ACTIONS = ["baseline", "resection", "osteotomy", "interposition", "fusion", "load0.5", "load1.5", "load2"]

@@ -50,7 +50,8 @@ def compress_materials(lam, mu, active, max_labels=255):
 
 class VoxelSystem:
     def __init__(self, grid, *, sri, active, lam=None, mu=None, labels=None, lam_tab=None, mu_tab=None,
-                 copy_orig=None, irr_cells=None, irr_nodes=None, extra=None, used=None, min_dofs=6000, max_levels=6,
+                 copy_orig=None, irr_cells=None, irr_nodes=None, extra=None, rank1=None, used=None, min_dofs=6000,
+                 max_levels=6,
                  levels=None, smoother="chebyshev", degree=3, omega=0.6, npre=2, npost=2, materials="auto",
                  device=None):
         """grid: Grid; active: (ncell,) bool in reference cell order; materials either per cell
@@ -75,6 +76,11 @@ class VoxelSystem:
         self.materials = materials
         try:
             self._describe(labels, lam_tab, mu_tab, lam, mu, copy_orig, irr_cells, irr_nodes, extra)
+            if rank1 is not None:
+                rp, rd, rw, rk = (np.ascontiguousarray(a, t) for a, t in
+                                  zip(rank1, (np.int64, np.int64, np.float64, np.float64)))
+                check(self.lib.cm_sys_set_rank1(self.h, len(rk), ptr(rp, ctypes.c_longlong), ptr(rd, ctypes.c_longlong),
+                                                ptr(rw, ctypes.c_double), ptr(rk, ctypes.c_double)))
             if used is not None:
                 um = np.ascontiguousarray(used, np.uint8)
                 check(self.lib.cm_sys_set_used(self.h, len(um), ptr(um, ctypes.c_uint8)))
@@ -133,6 +139,13 @@ class VoxelSystem:
         return [dict(nx=int(r[0]), ny=int(r[1]), nz=int(r[2]), nodes=int(r[3]), ndof=int(r[4]), extra_rows=int(r[5]))
                 for r in d]
 
+    def timings(self):
+        """Build stage times (ms)."""
+        t = np.zeros(7)
+        check(self.lib.cm_sys_timings(self.h, ptr(t, ctypes.c_double), 7))
+        keys = ("prepare", "structure", "upload", "moments", "diag", "coarse_inverse", "total")
+        return dict(zip(keys, (round(float(v), 3) for v in t)))
+
     def zeros(self, dtype=torch.float64, level=0):
         nn = self.nn if level == 0 else self.levels()[level]["nodes"]
         return torch.zeros(3 * nn, dtype=dtype, device=self.device)
@@ -159,7 +172,7 @@ class VoxelSystem:
         check(self.lib.cm_sys_vcycle(self.h, prec, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr())))
         return x
 
-    def solve(self, b, x0=None, tol=1e-8, maxiter=800, mode="fp64", check_every=5, inner_tol=1e-4):
+    def solve(self, b, x0=None, tol=1e-8, maxiter=800, mode="fp64", check_every=5, inner_tol=1e-3):
         """PCG on A x = b (FP64 device vectors, level-0 layout).  Returns (x, info)."""
         n = 3 * self.nn
         self._ck(b, torch.float64, n)

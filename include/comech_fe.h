@@ -72,6 +72,13 @@ int cm_sys_set_split(cm_sys* sys, int64_t ncopy, const int64_t* copy_orig, int64
  * (rb/replica3.py:68-86) and interface springs.  Duplicates are summed. */
 int cm_sys_set_extra(cm_sys* sys, int64_t nnz, const int64_t* rows, const int64_t* cols, const double* vals);
 
+/* Extra stiffness as rank-one terms sum_t k[t] w_t w_t^T, w_t = (dofs, w) entries ptr[t] ..
+ * ptr[t+1]-1 over level-0 node DOFs: node-pair springs (interface: one term per component, w = e_a
+ * - e_b), directional ligaments (w = +d at a, -d at b; rb/replica3.py:68-86 patches), penalties
+ * (fe/solver.py:222-232).  Coarsened exactly as w -> P^T w. */
+int cm_sys_set_rank1(cm_sys* sys, int64_t nterm, const int64_t* ptr, const int64_t* dofs, const double* w,
+                     const double* k);
+
 /* Mark nodes as unknowns in addition to the corners of active cells (nn = grid nodes + copies).
  * Used when the stiffness comes only from cm_sys_set_extra, as for the CSR input of
  * fe/gmg.py:120 pcg_gmg(A, b, used_nodes, ...). */
@@ -103,9 +110,11 @@ int cm_sys_vcycle(cm_sys* sys, int prec, const void* b, void* x);
 /* PCG to ||b - A x|| <= tol ||b|| (rb/gpusolve.py:206-262, fe/gmg.py:120-137).  b, x: FP64
  * device, level-0 layout; x holds the initial guess (x0) on entry.
  * mode 0: FP64 PCG, FP64 V-cycle.  mode 1: FP64 PCG, FP32 V-cycle (mg_dtype="float32" of
- * rb/gpusolve.py:265).  mode 2: FP64 iterative refinement around FP32 PCG with FP64 reductions,
- * inner relative tolerance inner_tol.  The residual is checked at iterations < check_every and
- * at multiples of check_every (rb/gpusolve.py:244). */
+ * rb/gpusolve.py:265).  mode 2: iterative refinement -- FP64 true residual b - A64 x, inner FP32
+ * PCG (FP32 operator, V-cycle and vectors, FP64 alpha/beta reductions, FP64 coarse solve) to
+ * relative tolerance inner_tol, x += d in FP64; convergence is decided on the FP64 residual.
+ * The residual is checked at iterations < check_every and at multiples of check_every
+ * (rb/gpusolve.py:244).  info->outer counts refinement steps. */
 int cm_sys_solve(cm_sys* sys, int mode, const double* b, double* x, double tol, int maxiter, int check_every,
                  double inner_tol, cm_solve_info* info);
 
@@ -131,6 +140,10 @@ int cm_sys_time_apply(cm_sys* sys, int level, int prec, const void* u, void* y, 
 /* Tile autotuner: for every level, time the structured stencil kernel in precision `prec` for
  * z extents 1..32 (nrep launches each, HIP events) and keep the fastest. */
 int cm_sys_autotune(cm_sys* sys, int prec, int nrep);
+
+/* Build stage times in ms: level-0 preparation, hierarchy structure, uploads, coarse moments,
+ * diagonal blocks, coarsest inverse, total (cap >= 7). */
+int cm_sys_timings(cm_sys* sys, double* out, int cap);
 
 /* Override the z extent of a structured block (0 = automatic). */
 int cm_sys_set_kz(cm_sys* sys, int kz);

@@ -23,13 +23,13 @@ def _sys1(n=10, **kw):
                           mu_tab=c["mu_tab"], extra=extra, device=DEV, **kw)
 
 
-def _sys2(dims=(40, 24, 24), h=2.0, **kw):
+def _sys2(dims=(40, 24, 24), h=2.0, as_terms=True, **kw):
     c = C.cfg2(dims, h)
-    extra, f = synth.solve_inputs_config2(c)
+    extra, terms, f = synth.solve_inputs_config2(c, as_terms=as_terms)
     irr_c, irr_n = __import__("comech_amd.engine", fromlist=["x"]).split_cells(c["grid"], c["active"], c["cn"])
     s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
                     mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c, irr_nodes=irr_n,
-                    extra=extra, device=DEV, **kw)
+                    extra=extra, rank1=terms, device=DEV, **kw)
     return c, s
 
 
@@ -53,8 +53,10 @@ def test_apply_config1_full_integration():
     _check_apply(s, C.oracle_system1(10), c["grid"].nnode)
 
 
-def test_apply_config2_split_sri_springs():
-    c, s = _sys2()
+@pytest.mark.parametrize("as_terms", [True, False])
+def test_apply_config2_split_sri_springs(as_terms):
+    """Springs as rank-one terms or as generic COO (diagonal -> rank-one, off-diagonal -> CSR)."""
+    c, s = _sys2(as_terms=as_terms)
     assert s.ncopy > 0 and len(s.irr_cells) > 0
     _check_apply(s, C.oracle_system2(), len(c["orig"]))
 
@@ -82,23 +84,47 @@ def _oracle_hierarchy(K, used, grid, levels):
     return mg
 
 
-@pytest.mark.parametrize("which", ["cfg1", "cfg2"])
+def _patch_case():
+    """fe/test_patch.py grid: stretched edges, odd node counts (a last coarse cell one fine cell
+    wide), springs spanning 1e-6 .. 1e9 per area; 6 levels (axes of one cell keep their size)."""
+    xe, ye, ze = np.r_[0, 1, 3, 4, 7, 10], np.r_[0, 2, 5, 10], np.r_[0, 1, 2, 4, 6, 10]
+    g, og = Grid(xe, ye, ze), O.Grid(xe, ye, ze)
+    lam, mu = O.lame(np.full(g.ncell, 1000.0), np.full(g.ncell, 0.3))
+    bot, top = np.flatnonzero(og.cK == 0), np.flatnonzero(og.cK == og.nz - 1)
+    nb, ab = O.face_node_areas(og, bot, "z-")
+    nt, at = O.face_node_areas(og, top, "z+")
+    springs = [(nb, ab, np.array([1e-6, 1e-6, 1e9]))]
+    _, info = O.solve(og, lam, mu, np.ones(g.ncell, bool), springs, [(nt, at, np.array([0, 0, -1.0]))],
+                      method="none")
+    kd = np.zeros((g.nnode, 3))
+    np.add.at(kd, nb, ab[:, None] * springs[0][2][None, :])
+    rows = np.arange(3 * g.nnode)[kd.ravel() != 0]
+    s = VoxelSystem(g, sri=False, active=np.ones(g.ncell, bool), lam=lam, mu=mu, extra=(rows, rows, kd.ravel()[rows]),
+                    device=DEV, min_dofs=10)
+    return dict(grid=g), s, info, info["used"]
+
+
+@pytest.mark.parametrize("which", ["cfg1", "cfg2", "cfg2coo", "patch"])
 def test_coarse_levels_equal_galerkin(which):
     """Level-l GPU operator (moment cells + Galerkin springs) == P^T A P of the oracle hierarchy
-    (fe/gmg.py:91-99) on the kept coarse DOFs."""
+    (fe/gmg.py:91-99) on the kept coarse DOFs, every level."""
+    nlev = 3
     if which == "cfg1":
         c, s = _sys1(10, levels=3)
         info = C.oracle_system1(10)
         used_nodes = info["used"]
+    elif which == "patch":
+        c, s, info, used_nodes = _patch_case()
+        nlev = 6
     else:
-        c, s = _sys2(levels=3)
+        c, s = _sys2(levels=3, as_terms=(which == "cfg2"))
         info = C.oracle_system2()
         used_nodes = c["orig"][info["used"]]
     g = c["grid"]
-    mg = _oracle_hierarchy(info["K"], used_nodes, g, 3)
+    mg = _oracle_hierarchy(info["K"], used_nodes, g, nlev)
     lv = s.levels()
-    assert len(lv) == 3
-    for l in (1, 2):
+    assert len(lv) == nlev
+    for l in range(1, nlev):
         Ac = mg.levels[l].A
         keep = mg.keeps[l - 1]
         nn = lv[l]["nodes"]
@@ -110,6 +136,14 @@ def test_coarse_levels_equal_galerkin(which):
         err = np.abs(y[keep] - ref).max() / np.abs(ref).max()
         assert err < 1e-11, (l, err)
         assert lv[l]["ndof"] == len(keep)
+        # a rigid x translation: elasticity gives nothing, only the (possibly weak) springs respond,
+        # so a dropped or mis-coarsened spring term shows up here however small it is
+        u = np.zeros(3 * nn)
+        u[keep[keep % 3 == 0]] = 1.0
+        y = s.apply(torch.tensor(u, dtype=torch.float64, device=DEV), level=l).cpu().numpy()
+        ref = Ac @ u[keep]
+        err = np.abs(y[keep] - ref).max() / np.abs(ref).max()
+        assert err < 1e-6, (l, "rigid", err)
 
 
 def test_level_rule_matches_reference():

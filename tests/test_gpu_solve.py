@@ -100,8 +100,14 @@ def test_patch_test_known_answer():
     bot, top = np.flatnonzero(g.cK == 0), np.flatnonzero(g.cK == g.nz - 1)
     nb, ab = FS.face_node_areas(g, bot, "z-")
     nt, at = FS.face_node_areas(g, top, "z+")
+    # Springs from 1e-6 to 1e9 per area: the FP64 residual floor eps*|A||x|/|b| is about 1e-10, so
+    # the iterative solve stops at 1e-9 (the reference solves this case directly).  The 1e-6 lateral
+    # springs leave x/y rigid motions almost free: the rigid-mode correction of fe/gmg.py:127-135
+    # (one group: all nodes) handles them, as the engine does for every bone.
     U, info = FS.solve(g, lam, mu, active, [(nb, ab, np.array([1e-6, 1e-6, 1e9]))],
-                       [(nt, at, np.array([0, 0, -1.0]))], method="direct", device=DEV, min_dofs=10)
+                       [(nt, at, np.array([0, 0, -1.0]))], method="gmg", tol=1e-9, device=DEV, min_dofs=10,
+                       node_groups=[np.arange(g.nnode)], maxiter=2000)
+    assert info["flag"] == 0, info
     sig, vm, eps = FS.cell_stress(g, lam, mu, U, np.arange(g.ncell), device=DEV)
     top_nodes = np.flatnonzero(g.node_xyz[:, 2] == 10)
     assert np.allclose(U[top_nodes, 2], -10 / E, rtol=1e-6, atol=1e-9)

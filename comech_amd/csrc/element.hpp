@@ -61,6 +61,58 @@ CM_INLINE void wht_bwd(T* v) {
   }
 }
 
+// Transforms of one cell face (4 nodes, corner a + 2b), used by the z-marching kernels: the 3-D
+// transform of a cell = xy transforms of its bottom (lo) and top (hi) faces, then
+// C[m] = hi[m] + lo[m], C[m + 4] = hi[m] - lo[m]; the forces split back as lo = F[m] - F[m+4],
+// hi = F[m] + F[m+4], then an xy back-transform per face.
+//   face_fwd: v[corner][comp] -> f[4 comp + (mx + 2 my)]
+template <typename T>
+CM_INLINE void face_fwd(const T (&v)[4][3], T* __restrict__ f) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T s0 = v[1][c] + v[0][c], s1 = v[1][c] - v[0][c];
+    const T s2 = v[3][c] + v[2][c], s3 = v[3][c] - v[2][c];
+    f[4 * c + 0] = s2 + s0;
+    f[4 * c + 1] = s3 + s1;
+    f[4 * c + 2] = s2 - s0;
+    f[4 * c + 3] = s3 - s1;
+  }
+}
+//   face_bwd: F[4 comp + m] -> cr[corner][comp]
+template <typename T>
+CM_INLINE void face_bwd(const T* __restrict__ F, T (&cr)[4][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const T t0 = F[4 * c] - F[4 * c + 1], t1 = F[4 * c] + F[4 * c + 1];
+    const T t2 = F[4 * c + 2] - F[4 * c + 3], t3 = F[4 * c + 2] + F[4 * c + 3];
+    cr[0][c] = t0 - t2;
+    cr[2][c] = t0 + t2;
+    cr[1][c] = t1 - t3;
+    cr[3][c] = t1 + t3;
+  }
+}
+//   z stage: faces -> 3-D transform c3[comp][m]; forces F3 -> faces (lo, hi)
+template <typename T>
+CM_INLINE void faces_to_cell(const T* __restrict__ lo, const T* __restrict__ hi, T (&c3)[3][8]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      c3[c][m] = hi[4 * c + m] + lo[4 * c + m];
+      c3[c][m + 4] = hi[4 * c + m] - lo[4 * c + m];
+    }
+}
+template <typename T>
+CM_INLINE void cell_to_faces(const T (&F3)[3][8], T* __restrict__ lo, T* __restrict__ hi) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      lo[4 * c + m] = F3[c][m] - F3[c][m + 4];
+      hi[4 * c + m] = F3[c][m] + F3[c][m + 4];
+    }
+}
+
 // Per-cell constants of the constant-material operator.
 //   g_j = (2/h_j)/8 converts transform coefficients into physical gradients.
 //   wl = lam * V, wm = mu * V.
@@ -69,19 +121,19 @@ struct CellW {
   T gx, gy, gz, wl, wm;
 };
 
-// f = (lam KL + mu KM) u for one cell with constant material.  u, f: 24 values, DOF 3*l+comp.
-// SRI = true integrates the lam term at the cell centre only (rb/splitfe.py:58-69).
-template <typename T, bool SRI>
-CM_INLINE void elem_apply_const(const T* __restrict__ u, T* __restrict__ f, const CellW<T>& w) {
-  T cx[8], cy[8], cz[8];
-#pragma unroll
-  for (int l = 0; l < 8; ++l) { cx[l] = u[3 * l]; cy[l] = u[3 * l + 1]; cz[l] = u[3 * l + 2]; }
-  wht_fwd(cx); wht_fwd(cy); wht_fwd(cz);
-  const T gx = w.gx, gy = w.gy, gz = w.gz;
+// Transform-space core of the constant-material operator: (cx, cy, cz) = forward transforms of
+// the three displacement components, (Fx, Fy, Fz) = the transform-space forces (the element
+// forces are wht_bwd of them).  ISO = true is the cubic cell (h_x = h_y = h_z = h): every term
+// carries exactly two gradient factors, so the caller passes wl = lam V g^2, wm = mu V g^2
+// (V g^2 = h / 16) and the gradient factors are compile-time ones.
+template <typename T, bool SRI, bool ISO>
+CM_INLINE void elem_mid(const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+                        T* __restrict__ Fx, T* __restrict__ Fy, T* __restrict__ Fz, const CellW<T>& w) {
+  // ISO: cubic cell, the common factor g^2 is folded into wl and wm by the caller
+  const T gx = ISO ? T(1) : w.gx, gy = ISO ? T(1) : w.gy, gz = ISO ? T(1) : w.gz;
   const T third = T(1) / T(3), ninth = T(1) / T(9);
   // monomial weights V * 3^-|beta| folded with the material factor
   const T m0 = w.wm, m1 = w.wm * third, m2 = w.wm * ninth;
-  T Fx[8], Fy[8], Fz[8];
 #pragma unroll
   for (int m = 0; m < 8; ++m) { Fx[m] = T(0); Fy[m] = T(0); Fz[m] = T(0); }
 
@@ -147,6 +199,18 @@ CM_INLINE void elem_apply_const(const T* __restrict__ u, T* __restrict__ f, cons
       Fx[7] += gx * gx * l2 * cx[7];                                             // beta 011
     }
   }
+}
+
+// f = (lam KL + mu KM) u for one cell with constant material.  u, f: 24 values, DOF 3*l+comp.
+// SRI = true integrates the lam term at the cell centre only (rb/splitfe.py:58-69).
+template <typename T, bool SRI>
+CM_INLINE void elem_apply_const(const T* __restrict__ u, T* __restrict__ f, const CellW<T>& w) {
+  T cx[8], cy[8], cz[8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) { cx[l] = u[3 * l]; cy[l] = u[3 * l + 1]; cz[l] = u[3 * l + 2]; }
+  wht_fwd(cx); wht_fwd(cy); wht_fwd(cz);
+  T Fx[8], Fy[8], Fz[8];
+  elem_mid<T, SRI, false>(cx, cy, cz, Fx, Fy, Fz, w);
   wht_bwd(Fx); wht_bwd(Fy); wht_bwd(Fz);
 #pragma unroll
   for (int l = 0; l < 8; ++l) { f[3 * l] = Fx[l]; f[3 * l + 1] = Fy[l]; f[3 * l + 2] = Fz[l]; }
@@ -174,15 +238,12 @@ CM_INLINE void mom_mul(const T* __restrict__ mom, const int (&B)[N], const T* a,
   }
 }
 
+// Transform-space core of the moment operator: c = forward transforms of the three displacement
+// components, F = transform-space forces (element forces = wht_bwd of them).
 template <typename T>
-CM_INLINE void elem_apply_mom(const T* __restrict__ u, T* __restrict__ f, const T* __restrict__ momL,
-                              const T* __restrict__ momM, T gx, T gy, T gz) {
-  T c[3][8];
-#pragma unroll
-  for (int l = 0; l < 8; ++l) { c[0][l] = u[3 * l]; c[1][l] = u[3 * l + 1]; c[2][l] = u[3 * l + 2]; }
-  wht_fwd(c[0]); wht_fwd(c[1]); wht_fwd(c[2]);
+CM_INLINE void elem_mid_mom(const T (&c)[3][8], T (&F)[3][8], const T* __restrict__ momL, const T* __restrict__ momM,
+                            T gx, T gy, T gz) {
   const T g[3] = {gx, gy, gz};
-  T F[3][8];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -264,6 +325,17 @@ CM_INLINE void elem_apply_mom(const T* __restrict__ u, T* __restrict__ f, const 
       if (!((B[k] >> i) & 1)) F[j][B[k] | (1 << i)] += g[i] * q[k];
     }
   }
+}
+
+template <typename T>
+CM_INLINE void elem_apply_mom(const T* __restrict__ u, T* __restrict__ f, const T* __restrict__ momL,
+                              const T* __restrict__ momM, T gx, T gy, T gz) {
+  T c[3][8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) { c[0][l] = u[3 * l]; c[1][l] = u[3 * l + 1]; c[2][l] = u[3 * l + 2]; }
+  wht_fwd(c[0]); wht_fwd(c[1]); wht_fwd(c[2]);
+  T F[3][8];
+  elem_mid_mom<T>(c, F, momL, momM, gx, gy, gz);
   wht_bwd(F[0]); wht_bwd(F[1]); wht_bwd(F[2]);
 #pragma unroll
   for (int l = 0; l < 8; ++l) { f[3 * l] = F[0][l]; f[3 * l + 1] = F[1][l]; f[3 * l + 2] = F[2][l]; }

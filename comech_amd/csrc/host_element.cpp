@@ -52,4 +52,50 @@ void cm_host_cell_stress(const double* u, const double* dims, double lam, double
   cell_centre_stress<double>(u, 0.25 / dims[0], 0.25 / dims[1], 0.25 / dims[2], lam, mu, eps, sig, vm);
 }
 
+// CPU emulation of the structured march on one cell column (1 x 1 x nz cells; node (a, b, k) at
+// index 4k + a + 2b, 3 values each): the same face transforms, z stage, transform-space cell
+// operator, force carry and back-transform as the GPU kernel.  iso != 0 uses the cubic-cell form
+// (requires hxy[0] = hxy[1] = hz[k] for all k).
+void cm_host_march_column(int nz, const double* hxy, const double* hz, const double* lam, const double* mu, int sri,
+                          int iso, const double* u, double* y) {
+  const double gx = 0.25 / hxy[0], gy = 0.25 / hxy[1];
+  double ulo[12], uhi[12], car[12] = {0};
+  auto face = [&](int kk, double* f) {
+    double v[4][3];
+    for (int ab = 0; ab < 4; ++ab)
+      for (int c = 0; c < 3; ++c) v[ab][c] = u[(4 * kk + ab) * 3 + c];
+    face_fwd(v, f);
+  };
+  face(0, ulo);
+  for (int k = 0; k <= nz; ++k) {
+    double Flo[12] = {0}, Fhi[12] = {0};
+    if (k < nz) {
+      face(k + 1, uhi);
+      double c3[3][8], F3[3][8];
+      faces_to_cell(ulo, uhi, c3);
+      if (iso) {
+        const double f = hxy[0] / 16.0;
+        CellW<double> w{1.0, 1.0, 1.0, lam[k] * f, mu[k] * f};
+        if (sri) elem_mid<double, true, true>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+        else elem_mid<double, false, true>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+      } else {
+        const double V = hxy[0] * hxy[1] * hz[k];
+        CellW<double> w{gx, gy, 0.25 / hz[k], lam[k] * V, mu[k] * V};
+        if (sri) elem_mid<double, true, false>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+        else elem_mid<double, false, false>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+      }
+      cell_to_faces(F3, Flo, Fhi);
+    }
+    for (int q = 0; q < 12; ++q) {
+      Flo[q] += car[q];
+      car[q] = Fhi[q];
+    }
+    double cr[4][3];
+    face_bwd(Flo, cr);
+    for (int ab = 0; ab < 4; ++ab)
+      for (int c = 0; c < 3; ++c) y[(4 * k + ab) * 3 + c] = cr[ab][c];
+    for (int q = 0; q < 12; ++q) ulo[q] = uhi[q];
+  }
+}
+
 }  // extern "C"

@@ -7,14 +7,13 @@
 //   cells         x-fastest internal order i + nx (j + ny k); the boundary converts from the
 //                 reference's z-fastest cell ids (fe/solver.py:104-105,128)
 //   level 0       one uint8 label per cell (255 = no stiffness) + a (lam, mu) table, or two
-//                 per-cell values; irregular cells (touching split copies) are applied by a
-//                 separate list kernel
+//                 per-cell values; irregular cells (touching split copies) go to the sparse pass
 //   level >= 1    27 lam-moments + 27 mu-moments per coarse cell (structure of arrays)
 //
-// The structured apply marches along z (2.5-D blocking): a block of 16x16 threads owns a 15x15
-// column of output nodes; every z step stages one node layer of u (17x17 nodes) in LDS, computes
-// the 16x16 cells of one cell layer once, carries the cells' top-corner forces to the next step
-// in registers and gathers the four in-plane neighbours' corner forces through LDS.
+// One operator application = sparse pass (irregular cells, spring rows, rank-one terms into a
+// compact vector over the few nodes they touch) + structured march over the grid (every cell
+// computed once, epilogue fused: plain product, residual, or a whole smoothing step) + fix-up of
+// the touched nodes.
 #pragma once
 #include "common.hpp"
 #include "element.hpp"
@@ -28,32 +27,34 @@ constexpr int BLOCK = TILE * TILE;
 constexpr uint8_t LAB_NONE = 255;
 
 // -------------------------------------------------------------------------------------------
-// Cell material policies
-template <typename T, bool SRI>
-struct FineLabelCells {
-  const uint8_t* __restrict__ lab;   // x-fastest
+// Cell material policies of the structured march
+//   kind 0: level 0, one uint8 label per cell + (lam, mu) table
+//   kind 1: level 0, per-cell (lam, mu) values
+//   kind 2: coarse levels, 27 lam + 27 mu moments per cell
+// ISO = the level-0 grid is made of identical cubes (h_x = h_y = h_z = h everywhere); the common
+// factor V g^2 = h / 16 is then folded into the material weights (GridArgs::iso).
+template <typename T, bool SRI_, bool ISO_>
+struct LabelMat {
+  const uint8_t* __restrict__ lab;   // x-fastest, LAB_NONE = no stiffness (or irregular)
   const T* __restrict__ tabL;        // lam per label
   const T* __restrict__ tabM;        // mu per label
-  static constexpr bool kMoments = false;
+  static constexpr int kind = 0;
+  static constexpr bool sri = SRI_, iso = ISO_;
 };
 
-template <typename T, bool SRI>
-struct FineValueCells {
+template <typename T, bool SRI_, bool ISO_>
+struct ValueMat {
   const T* __restrict__ cl;          // lam per cell (x-fastest), 0 where no stiffness
   const T* __restrict__ cm;          // mu per cell
-  static constexpr bool kMoments = false;
+  static constexpr int kind = 1;
+  static constexpr bool sri = SRI_, iso = ISO_;
 };
 
 template <typename T>
-struct MomentCells {
+struct MomentMat {
   const T* __restrict__ mom;         // [54][ncell]: 27 lam moments then 27 mu moments
-  static constexpr bool kMoments = true;
-};
-
-// Epilogue modes of the structured apply
-enum ApplyMode : int {
-  MODE_APPLY = 0,   // y = A u
-  MODE_RESID = 1,   // y = b - A u
+  static constexpr int kind = 2;
+  static constexpr bool sri = false, iso = false;
 };
 
 template <typename T>
@@ -62,293 +63,419 @@ struct GridArgs {
   const T* __restrict__ hx;          // cell sizes per axis
   const T* __restrict__ hy;
   const T* __restrict__ hz;
+  T iso;                             // h / 16 when every cell is the same cube, else 0
 };
 
-// Load the material of cell (i,j,k) into per-cell weights.  Returns false when the cell has no
-// stiffness (outside, inactive, or irregular for the structured pass).
-template <typename T, bool SRI>
-__device__ __forceinline__ bool load_cell(const FineLabelCells<T, SRI>& c, const T* sL, const T* sM, long long cid,
-                                          T& lam, T& mu) {
-  const uint8_t l = c.lab[cid];
-  if (l == LAB_NONE) return false;
-  lam = sL[l];
-  mu = sM[l];
-  return true;
-}
-
-template <typename T, bool SRI>
-__device__ __forceinline__ bool load_cell(const FineValueCells<T, SRI>& c, const T*, const T*, long long cid, T& lam,
-                                          T& mu) {
-  lam = c.cl[cid];
-  mu = c.cm[cid];
-  return (lam != T(0)) || (mu != T(0));
-}
-
-template <class C>
-struct cell_traits;
-template <typename T, bool SRI>
-struct cell_traits<FineLabelCells<T, SRI>> {
-  static constexpr bool sri = SRI;
-  static constexpr bool labels = true;
-};
-template <typename T, bool SRI>
-struct cell_traits<FineValueCells<T, SRI>> {
-  static constexpr bool sri = SRI;
-  static constexpr bool labels = false;
+// Block order of the structured march.  Workgroups are dispatched round-robin over the 8 XCDs
+// (block b runs on XCD b % 8); tile ids are handed out so that every XCD gets one contiguous run
+// of tiles (x fastest, then y, then z), which keeps the tiles sharing halo nodes in one L2.
+struct TileMap {
+  int ntx, nty, ntz;
+  int q, rem;                        // nblocks / 8, nblocks % 8
+  __device__ __forceinline__ void decode(int b, int& bx, int& by, int& bz) const {
+    const int xcd = b & 7, pos = b >> 3;
+    const int id = xcd * q + min(xcd, rem) + pos;
+    bx = id % ntx;
+    const int r = id / ntx;
+    by = r % nty;
+    bz = r / nty;
+  }
 };
 
 // -------------------------------------------------------------------------------------------
-// Structured apply, constant-material (level 0) cells.
-template <typename T, class Cells, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_apply_fine(GridArgs<T> g, Cells cells, const T* __restrict__ u,
-                                                       T* y, const T* b, int kz) {
-  constexpr bool SRI = cell_traits<Cells>::sri;
-  __shared__ T s_u[2][TN * TN * 3];
-  __shared__ T s_g[BLOCK * 9];
-  __shared__ T s_tL[256], s_tM[256];
+// Epilogues.  au = (A s) at one node from the structured part, sn = the node's value of s.
+enum EpiMode : int {
+  EPI_APPLY = 0,   // y = A s
+  EPI_RESID = 1,   // y = rin - A s
+  EPI_CHEB = 2,    // one smoothing step: r = rin - A s;  d = c1 sn + c2 Dinv r;  x += xa sn + d
+};
+// Names used by the entry points that speak of the plain operator.
+constexpr int MODE_APPLY = EPI_APPLY;
+constexpr int MODE_RESID = EPI_RESID;
+
+template <typename T>
+struct Epi {
+  const T* rin;       // RESID / CHEB: right-hand side or running residual
+  T* y;               // APPLY / RESID: output;  CHEB: updated residual r (nullptr: not kept)
+  const T* Dinv;      // CHEB: symmetric 3x3 block inverses, 6 per node (00 01 02 11 12 22)
+  T* dout;            // CHEB: new direction d (nullptr: not kept)
+  T* x;               // CHEB: iterate, updated in place (nullptr: not updated)
+  T c1, c2, xa;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ void sym3_mul(const T* __restrict__ m, const T r[3], T z[3]) {
+  const T a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5];
+  z[0] = a * r[0] + b * r[1] + c * r[2];
+  z[1] = b * r[0] + d * r[1] + e * r[2];
+  z[2] = c * r[0] + e * r[1] + f * r[2];
+}
+
+template <typename T, int EM>
+__device__ __forceinline__ void epilogue(const Epi<T>& ep, long long n, const T au[3], const T sn[3]) {
+  if (EM == EPI_APPLY) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ep.y[3 * n + c] = au[c];
+  } else if (EM == EPI_RESID) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ep.y[3 * n + c] = ep.rin[3 * n + c] - au[c];
+  } else {
+    T r[3], z[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = ep.rin[3 * n + c] - au[c];
+    if (ep.y) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.y[3 * n + c] = r[c];
+    }
+    sym3_mul(ep.Dinv + 6 * n, r, z);
+    T dn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dn[c] = ep.c1 * sn[c] + ep.c2 * z[c];
+    if (ep.dout) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.dout[3 * n + c] = dn[c];
+    }
+    if (ep.x) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.x[3 * n + c] += ep.xa * sn[c] + dn[c];
+    }
+  }
+}
+
+// -------------------------------------------------------------------------------------------
+// Structured march (all levels).  A block of 16x16 threads owns the 15x15 output node columns of
+// one xy tile and marches a z-range of kz node layers.  Thread (tx, ty) owns cell column
+// (I0-1+tx, J0-1+ty).  Per z step:
+//   * the next node layer of s (17x17 nodes, padded to 4 values) is written to LDS from registers
+//     loaded one step earlier (the global loads of layer k+2 are in flight during step k);
+//   * the thread transforms its cell's top face in xy (the bottom face's transform is the previous
+//     step's top face, kept in registers), finishes the z transform, applies the cell operator in
+//     transform space and splits the forces into bottom/top faces (still in xy transform space);
+//   * the top-face forces are carried to the next step, the bottom-face forces plus the carry
+//     are transformed back to the 4 bottom corners; three corners go to LDS for the in-plane
+//     neighbours, the fourth is the thread's own output node.
+template <typename T, class M, int EM>
+__global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* __restrict__ s, Epi<T> ep, int kz,
+                                                 TileMap tm) {
+  constexpr int NST = TN * TN * 3;                 // scalars per staged node layer
+  constexpr int NSTP = (NST + BLOCK - 1) / BLOCK;  // per thread
+  __shared__ __align__(16) T s_u[2][TN * TN * 4];
+  __shared__ __align__(16) T s_g[BLOCK * 12];
+  __shared__ T s_tL[M::kind == 0 ? 256 : 1], s_tM[M::kind == 0 ? 256 : 1];
   const int tid = threadIdx.x;
   const int tx = tid % TILE, ty = tid / TILE;
-  const int I0 = blockIdx.x * TOUT, J0 = blockIdx.y * TOUT, K0 = blockIdx.z * kz;
+  int bx, by, bz;
+  tm.decode(blockIdx.x, bx, by, bz);
+  const int I0 = bx * TOUT, J0 = by * TOUT, K0 = bz * kz;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const long long nxn = nx + 1, nyn = ny + 1;
-  if constexpr (cell_traits<Cells>::labels) {
-    s_tL[tid] = cells.tabL[tid];
-    s_tM[tid] = cells.tabM[tid];
+  const long long nxn = nx + 1, nyn = ny + 1, lay = nxn * nyn;
+  const long long ncl = (long long)nx * ny;
+  if constexpr (M::kind == 0) {
+    const T sc = M::iso ? g.iso : T(1);
+    s_tL[tid] = mat.tabL[tid] * sc;
+    s_tM[tid] = mat.tabM[tid] * sc;
   }
   const int ci = I0 - 1 + tx, cj = J0 - 1 + ty;
   const bool cin = (ci >= 0 && ci < nx && cj >= 0 && cj < ny);
+  const long long cxy = cin ? ci + (long long)nx * cj : 0;
   T gx = T(0), gy = T(0), axy = T(0);
-  if (cin) {
+  if (!M::iso && cin) {
     const T hx = g.hx[ci], hy = g.hy[cj];
     gx = T(0.25) / hx;
     gy = T(0.25) / hy;
     axy = hx * hy;
   }
-  auto stage = [&](int kk) {
-    T* dst = s_u[kk & 1];
-    const bool kin = (kk >= 0 && kk <= nz);
-    for (int e = tid; e < TN * TN * 3; e += BLOCK) {
-      const int r = e / (TN * 3);
-      const int q = e - r * (TN * 3);
-      const int gj = J0 - 1 + r;
-      const int gi = I0 - 1 + q / 3;
-      T v = T(0);
-      if (kin && gj >= 0 && gj <= ny && gi >= 0 && gi <= nx)
-        v = u[((kk * nyn + gj) * nxn + (I0 - 1)) * 3 + q];
-      dst[e] = v;
-    }
-  };
-  T carry[12];
+  // staging plan (fixed for all layers): scalar e = tid + m*BLOCK of a layer is node row r,
+  // entry q of the row's 3*TN contiguous scalars
+  int goff[NSTP], loff[NSTP];
+  bool sv[NSTP];
 #pragma unroll
-  for (int q = 0; q < 12; ++q) carry[q] = T(0);
-  const int kend = min(K0 + kz, nz + 1);   // exclusive end of output node layers
-  stage(K0 - 1);
-  for (int k = K0 - 1; k < kend; ++k) {
-    stage(k + 1);
-    __syncthreads();
-    T f[24];
-    bool act = false;
-    if (cin && k >= 0 && k < nz) {
-      T lam, mu;
-      act = load_cell(cells, s_tL, s_tM, (long long)ci + nx * ((long long)cj + (long long)ny * k), lam, mu);
-      if (act) {
-        T ue[24];
-        const T* lo = s_u[k & 1];
-        const T* hi = s_u[(k + 1) & 1];
-#pragma unroll
-        for (int l = 0; l < 8; ++l) {
-          const int a = l & 1, bb = (l >> 1) & 1, c = l >> 2;
-          const T* src = (c ? hi : lo) + ((ty + bb) * TN + tx + a) * 3;
-          ue[3 * l] = src[0]; ue[3 * l + 1] = src[1]; ue[3 * l + 2] = src[2];
-        }
-        const T hz = g.hz[k];
-        const T V = axy * hz;
-        CellW<T> w{gx, gy, T(0.25) / hz, lam * V, mu * V};
-        elem_apply_const<T, SRI>(ue, f, w);
-      }
-    }
-    if (!act) {
-#pragma unroll
-      for (int q = 0; q < 24; ++q) f[q] = T(0);
-    }
-    T g0[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) g0[c] = f[c] + carry[c];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) s_g[tid * 9 + q] = f[3 + q] + carry[3 + q];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) carry[q] = f[12 + q];
-    __syncthreads();
-    if (tx >= 1 && ty >= 1 && k >= K0 && ci <= nx && cj <= ny) {
-      const T* p1 = s_g + (tid - 1) * 9;        // cell (tx-1, ty): corner l=1
-      const T* p2 = s_g + (tid - TILE) * 9;     // cell (tx, ty-1): corner l=2
-      const T* p3 = s_g + (tid - TILE - 1) * 9; // cell (tx-1, ty-1): corner l=3
-      const long long n = ((long long)k * nyn + cj) * nxn + ci;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const T val = g0[c] + p1[c] + p2[3 + c] + p3[6 + c];
-        if (MODE == MODE_APPLY) y[3 * n + c] = val;
-        else y[3 * n + c] = b[3 * n + c] - val;
-      }
-    }
+  for (int m = 0; m < NSTP; ++m) {
+    const int e = tid + m * BLOCK;
+    const int r = e / (TN * 3), q = e - r * (TN * 3);
+    const int gj = J0 - 1 + r, gi = I0 - 1 + q / 3;
+    sv[m] = e < NST && gj >= 0 && gj <= ny && gi >= 0 && gi <= nx;
+    goff[m] = sv[m] ? (int)((gj * nxn + gi) * 3 + q % 3) : 0;
+    loff[m] = e < NST ? (r * TN + q / 3) * 4 + q % 3 : -1;
   }
-}
-
-// Structured apply, coarse levels (moment cells).
-template <typename T, int MODE>
-__global__ __launch_bounds__(BLOCK) void k_apply_mom(GridArgs<T> g, MomentCells<T> cells, const T* __restrict__ u,
-                                                      T* y, const T* b, int kz) {
-  __shared__ T s_u[2][TN * TN * 3];
-  __shared__ T s_g[BLOCK * 9];
-  const int tid = threadIdx.x;
-  const int tx = tid % TILE, ty = tid / TILE;
-  const int I0 = blockIdx.x * TOUT, J0 = blockIdx.y * TOUT, K0 = blockIdx.z * kz;
-  const int nx = g.nx, ny = g.ny, nz = g.nz;
-  const long long nxn = nx + 1, nyn = ny + 1;
-  const long long ncell = (long long)nx * ny * nz;
-  const int ci = I0 - 1 + tx, cj = J0 - 1 + ty;
-  const bool cin = (ci >= 0 && ci < nx && cj >= 0 && cj < ny);
-  T gx = T(0), gy = T(0);
-  if (cin) { gx = T(0.25) / g.hx[ci]; gy = T(0.25) / g.hy[cj]; }
-  auto stage = [&](int kk) {
-    T* dst = s_u[kk & 1];
+  T pre[NSTP];
+  auto gload = [&](int kk) {
     const bool kin = (kk >= 0 && kk <= nz);
-    for (int e = tid; e < TN * TN * 3; e += BLOCK) {
-      const int r = e / (TN * 3);
-      const int q = e - r * (TN * 3);
-      const int gj = J0 - 1 + r;
-      const int gi = I0 - 1 + q / 3;
-      T v = T(0);
-      if (kin && gj >= 0 && gj <= ny && gi >= 0 && gi <= nx)
-        v = u[((kk * nyn + gj) * nxn + (I0 - 1)) * 3 + q];
-      dst[e] = v;
+    const long long base = kin ? (long long)kk * lay * 3 : 0;
+#pragma unroll
+    for (int m = 0; m < NSTP; ++m) pre[m] = (kin && sv[m]) ? s[base + goff[m]] : T(0);
+  };
+  auto lstore = [&](int kk) {
+    T* dst = s_u[kk & 1];
+#pragma unroll
+    for (int m = 0; m < NSTP; ++m)
+      if (loff[m] >= 0) dst[loff[m]] = pre[m];
+  };
+  // xy transform of the cell's face on node layer kk: f[4*comp + (mx + 2 my)]
+  auto face = [&](int kk, T* f) {
+    const T* src = s_u[kk & 1];
+    T v[4][3];
+#pragma unroll
+    for (int ab = 0; ab < 4; ++ab) {
+      const int p = ((ty + (ab >> 1)) * TN + tx + (ab & 1)) * 4;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[ab][c] = src[p + c];
+    }
+    face_fwd(v, f);
+  };
+  // material of cell layer kk, loaded one step ahead (kinds 0 and 1)
+  uint8_t labn = LAB_NONE;
+  T lamn = T(0), mun = T(0);
+  auto mload = [&](int kk) {
+    labn = LAB_NONE;
+    lamn = T(0);
+    mun = T(0);
+    if (!(cin && kk >= 0 && kk < nz)) return;
+    const long long cid = cxy + ncl * kk;
+    if constexpr (M::kind == 0) labn = mat.lab[cid];
+    if constexpr (M::kind == 1) {
+      lamn = mat.cl[cid];
+      mun = mat.cm[cid];
     }
   };
-  T carry[12];
+
+  const int kend = min(K0 + kz, nz + 1);   // exclusive end of the output node layers
+  gload(K0 - 1);
+  lstore(K0 - 1);
+  gload(K0);
+  mload(K0 - 1);
+  __syncthreads();
+  T ulo[12], car[12];
+  face(K0 - 1, ulo);
 #pragma unroll
-  for (int q = 0; q < 12; ++q) carry[q] = T(0);
-  const int kend = min(K0 + kz, nz + 1);
-  stage(K0 - 1);
+  for (int q = 0; q < 12; ++q) car[q] = T(0);
+
   for (int k = K0 - 1; k < kend; ++k) {
-    stage(k + 1);
-    __syncthreads();
-    T f[24];
-#pragma unroll
-    for (int q = 0; q < 24; ++q) f[q] = T(0);
-    if (cin && k >= 0 && k < nz) {
-      const long long cid = (long long)ci + nx * ((long long)cj + (long long)ny * k);
-      T mL[27], mM[27];
-      bool nz_ = false;
-#pragma unroll
-      for (int q = 0; q < 27; ++q) {
-        mL[q] = cells.mom[q * ncell + cid];
-        mM[q] = cells.mom[(27 + q) * ncell + cid];
+    lstore(k + 1);
+    if (k + 2 <= kend) gload(k + 2);
+    // this step's cell material
+    bool act = false;
+    CellW<T> w{gx, gy, T(0), T(0), T(0)};
+    if constexpr (M::kind == 0) {
+      const uint8_t lb = labn;
+      act = (lb != LAB_NONE);
+      if (act) {
+        w.wl = s_tL[lb];
+        w.wm = s_tM[lb];
       }
-      nz_ = (mL[0] != T(0)) || (mM[0] != T(0));
-      if (nz_) {
-        T ue[24];
-        const T* lo = s_u[k & 1];
-        const T* hi = s_u[(k + 1) & 1];
-#pragma unroll
-        for (int l = 0; l < 8; ++l) {
-          const int a = l & 1, bb = (l >> 1) & 1, c = l >> 2;
-          const T* src = (c ? hi : lo) + ((ty + bb) * TN + tx + a) * 3;
-          ue[3 * l] = src[0]; ue[3 * l + 1] = src[1]; ue[3 * l + 2] = src[2];
-        }
-        elem_apply_mom<T>(ue, f, mL, mM, gx, gy, T(0.25) / g.hz[k]);
-      }
+    } else if constexpr (M::kind == 1) {
+      act = (lamn != T(0)) || (mun != T(0));
+      const T sc = M::iso ? g.iso : T(1);
+      w.wl = lamn * sc;
+      w.wm = mun * sc;
+    } else {
+      act = cin && k >= 0 && k < nz;
     }
-    T g0[3];
+    if constexpr (M::kind != 2) mload(k + 1);
+    __syncthreads();
+    T uhi[12];
+    face(k + 1, uhi);
+    T sn[3] = {T(0), T(0), T(0)};
+    if (EM == EPI_CHEB) {
+      const T* src = s_u[k & 1] + (ty * TN + tx) * 4;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) g0[c] = f[c] + carry[c];
+      for (int c = 0; c < 3; ++c) sn[c] = src[c];
+    }
+    T Flo[12], Fhi[12];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) s_g[tid * 9 + q] = f[3 + q] + carry[3 + q];
+    for (int q = 0; q < 12; ++q) {
+      Flo[q] = T(0);
+      Fhi[q] = T(0);
+    }
+    if (act) {
+      T c3[3][8];
+      faces_to_cell(ulo, uhi, c3);
+      T F3[3][8];
+      bool done = true;
+      if constexpr (M::kind == 2) {
+        const long long ncell = ncl * nz;
+        const long long cid = cxy + ncl * k;
+        T mL[27], mM[27];
 #pragma unroll
-    for (int q = 0; q < 12; ++q) carry[q] = f[12 + q];
+        for (int q = 0; q < 27; ++q) {
+          mL[q] = mat.mom[q * ncell + cid];
+          mM[q] = mat.mom[(27 + q) * ncell + cid];
+        }
+        done = (mL[0] != T(0)) || (mM[0] != T(0));
+        if (done) elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
+      } else {
+        if (!M::iso) {
+          const T hz = g.hz[k];
+          const T V = axy * hz;
+          w.gz = T(0.25) / hz;
+          w.wl *= V;
+          w.wm *= V;
+        }
+        elem_mid<T, M::sri, M::iso>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+      }
+      if (done) cell_to_faces(F3, Flo, Fhi);
+    }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      Flo[q] += car[q];
+      car[q] = Fhi[q];
+    }
+    // back to the 4 bottom corners (corner a + 2b)
+    T cr[4][3];
+    face_bwd(Flo, cr);
+    T* gq = s_g + tid * 12;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      gq[c] = cr[1][c];
+      gq[4 + c] = cr[2][c];
+      gq[8 + c] = cr[3][c];
+    }
     __syncthreads();
     if (tx >= 1 && ty >= 1 && k >= K0 && ci <= nx && cj <= ny) {
-      const T* p1 = s_g + (tid - 1) * 9;
-      const T* p2 = s_g + (tid - TILE) * 9;
-      const T* p3 = s_g + (tid - TILE - 1) * 9;
-      const long long n = ((long long)k * nyn + cj) * nxn + ci;
+      const T* p1 = s_g + (tid - 1) * 12;            // cell (tx-1, ty): its corner 1
+      const T* p2 = s_g + (tid - TILE) * 12 + 4;     // cell (tx, ty-1): its corner 2
+      const T* p3 = s_g + (tid - TILE - 1) * 12 + 8; // cell (tx-1, ty-1): its corner 3
+      T au[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const T val = g0[c] + p1[c] + p2[3 + c] + p3[6 + c];
-        if (MODE == MODE_APPLY) y[3 * n + c] = val;
-        else y[3 * n + c] = b[3 * n + c] - val;
-      }
+      for (int c = 0; c < 3; ++c) au[c] = cr[0][c] + p1[c] + p2[c] + p3[c];
+      epilogue<T, EM>(ep, (long long)k * lay + (long long)cj * nxn + ci, au, sn);
     }
+#pragma unroll
+    for (int q = 0; q < 12; ++q) ulo[q] = uhi[q];
   }
 }
 
 // -------------------------------------------------------------------------------------------
-// Irregular (split-copy) cells: explicit 8 node ids, scatter with atomics.  sign = +1 adds A u,
-// -1 subtracts (residual mode).
-template <typename T, bool SRI>
-__global__ void k_apply_irregular(int nirr, const int* __restrict__ nodes, const int* __restrict__ ijk,
-                                  const T* __restrict__ lam, const T* __restrict__ mu, GridArgs<T> g,
-                                  const T* __restrict__ u, T* __restrict__ y, T sign) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nirr) return;
-  const int i = ijk[3 * e], j = ijk[3 * e + 1], k = ijk[3 * e + 2];
-  const T hx = g.hx[i], hy = g.hy[j], hz = g.hz[k];
-  const T V = hx * hy * hz;
-  CellW<T> w{T(0.25) / hx, T(0.25) / hy, T(0.25) / hz, lam[e] * V, mu[e] * V};
-  T ue[24], f[24];
-  int nd[8];
-#pragma unroll
-  for (int l = 0; l < 8; ++l) {
-    nd[l] = nodes[8 * e + l];
-    ue[3 * l] = u[3LL * nd[l]];
-    ue[3 * l + 1] = u[3LL * nd[l] + 1];
-    ue[3 * l + 2] = u[3LL * nd[l] + 2];
-  }
-  elem_apply_const<T, SRI>(ue, f, w);
-#pragma unroll
-  for (int l = 0; l < 8; ++l) {
-    atomicAdd(&y[3LL * nd[l]], sign * f[3 * l]);
-    atomicAdd(&y[3LL * nd[l] + 1], sign * f[3 * l + 1]);
-    atomicAdd(&y[3LL * nd[l] + 2], sign * f[3 * l + 2]);
-  }
-}
-
-// Sparse extra stiffness (springs, penalties, ligaments; Galerkin-coarsened on coarse levels):
-// y[row] += sign * sum_k val[k] u[col[k]] over the listed rows (rows unique).  W lanes share a row
-// (strided over its entries, shuffle-reduced), W chosen per level from the mean row length.
-template <typename T, int W>
-__global__ void k_apply_extra(int nrow, const int* __restrict__ rows, const int* __restrict__ ptr,
-                              const int* __restrict__ cols, const T* __restrict__ vals, const T* __restrict__ u,
-                              T* y, T sign) {
-  const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = (int)(gid / W), lane = (int)(gid % W);
-  if (r >= nrow) return;   // the W lanes of a row leave together
-  T s = T(0);
-  for (int q = ptr[r] + lane; q < ptr[r + 1]; q += W) s += vals[q] * u[cols[q]];
-  if (W > 1) {
-#pragma unroll
-    for (int off = W / 2; off > 0; off >>= 1) s += __shfl_down(s, off, W);
-  }
-  if (lane == 0) y[rows[r]] += sign * s;
-}
-
-// Rank-one extra stiffness sum_t k_t w_t w_t^T (springs, penalties, ligaments and their exact
-// Galerkin coarsening): y += sign * k_t (w_t . u) w_t.  One wavefront per term: lanes stride over
-// the term's entries, the dot product is shuffle-reduced, each lane scatters its entries (atomics).
+// Sparse pass: everything the structured march leaves out, accumulated into a compact vector tc
+// over the level's "touched" nodes (3 values per listed node):
+//   * irregular cells (level 0: cells touching split copies), 8 explicit node ids;
+//   * extra stiffness rows (springs, penalties, ligaments, their Galerkin products), CSR;
+//   * rank-one terms k w w^T with a support wider than one node neighbourhood (coarse levels).
+// One launch, three block ranges.  Atomics: rows and cells overlap on the touched nodes.
 template <typename T>
-__global__ __launch_bounds__(256) void k_apply_rank1(int nterm, const int* __restrict__ ptr,
-                                                     const int* __restrict__ dof, const T* __restrict__ w,
-                                                     const T* __restrict__ k, const T* __restrict__ u, T* y,
-                                                     T sign) {
-  const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (t >= nterm) return;     // the 64 lanes of a term leave together
-  const int q0 = ptr[t], q1 = ptr[t + 1];
-  T s = T(0);
-  for (int q = q0 + lane; q < q1; q += 64) s += w[q] * u[dof[q]];
+struct SparseArgs {
+  int nirr, nrow, nterm;
+  int nb_irr, nb_row;                // blocks of the first two ranges
+  const int* __restrict__ irr_nodes; // [nirr][8] node ids
+  const int* __restrict__ irr_ci;    // [nirr][8] compact node index
+  const int* __restrict__ irr_ijk;   // [nirr][3]
+  const T* __restrict__ irrL;
+  const T* __restrict__ irrM;
+  const int* __restrict__ xrc;       // compact DOF of each CSR row
+  const int* __restrict__ xptr;
+  const int* __restrict__ xcol;      // full DOF
+  const T* __restrict__ xval;
+  const int* __restrict__ r1ptr;
+  const int* __restrict__ r1dof;     // full DOF
+  const int* __restrict__ r1c;       // compact DOF
+  const T* __restrict__ r1w;
+  const T* __restrict__ r1k;
+};
+
+template <typename T, bool SRI, int W>
+__global__ __launch_bounds__(256) void k_sparse(SparseArgs<T> a, GridArgs<T> g, const T* __restrict__ s,
+                                                T* __restrict__ tc) {
+  int b = blockIdx.x;
+  if (b < a.nb_irr) {
+    const int e = b * 256 + threadIdx.x;
+    if (e >= a.nirr) return;
+    const int i = a.irr_ijk[3 * e], j = a.irr_ijk[3 * e + 1], k = a.irr_ijk[3 * e + 2];
+    const T hx = g.hx[i], hy = g.hy[j], hz = g.hz[k];
+    const T V = hx * hy * hz;
+    CellW<T> w{T(0.25) / hx, T(0.25) / hy, T(0.25) / hz, a.irrL[e] * V, a.irrM[e] * V};
+    T ue[24], f[24];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-  s *= sign * k[t];
-  for (int q = q0 + lane; q < q1; q += 64) atomicAdd(&y[dof[q]], s * w[q]);
+    for (int l = 0; l < 8; ++l) {
+      const long long nd = a.irr_nodes[8 * e + l];
+      ue[3 * l] = s[3 * nd];
+      ue[3 * l + 1] = s[3 * nd + 1];
+      ue[3 * l + 2] = s[3 * nd + 2];
+    }
+    elem_apply_const<T, SRI>(ue, f, w);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const int q = 3 * a.irr_ci[8 * e + l];
+      atomicAdd(&tc[q], f[3 * l]);
+      atomicAdd(&tc[q + 1], f[3 * l + 1]);
+      atomicAdd(&tc[q + 2], f[3 * l + 2]);
+    }
+    return;
+  }
+  b -= a.nb_irr;
+  if (b < a.nb_row) {
+    const long long gid = (long long)b * 256 + threadIdx.x;
+    const int r = (int)(gid / W), lane = (int)(gid % W);
+    if (r >= a.nrow) return;   // the W lanes of a row leave together
+    T acc = T(0);
+    for (int q = a.xptr[r] + lane; q < a.xptr[r + 1]; q += W) acc += a.xval[q] * s[a.xcol[q]];
+    if (W > 1) {
+#pragma unroll
+      for (int off = W / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, W);
+    }
+    if (lane == 0) atomicAdd(&tc[a.xrc[r]], acc);
+    return;
+  }
+  b -= a.nb_row;
+  const int t = (b * 256 + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (t >= a.nterm) return;   // whole wavefronts leave together
+  const int q0 = a.r1ptr[t], q1 = a.r1ptr[t + 1];
+  T acc = T(0);
+  for (int q = q0 + lane; q < q1; q += 64) acc += a.r1w[q] * s[a.r1dof[q]];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  acc *= a.r1k[t];
+  for (int q = q0 + lane; q < q1; q += 64) atomicAdd(&tc[a.r1c[q]], acc * a.r1w[q]);
+}
+
+// Fix-up after the structured march, one thread per touched node: adds the sparse part tc (and
+// clears it for the next pass).  Grid nodes got the structured epilogue without tc; the epilogue
+// is affine in A s, so the correction is -tc (RESID), +tc (APPLY), or for a smoothing step
+// r -= t, d -= c2 Dinv t, x -= c2 Dinv t.  Split copies have no structured part: their epilogue
+// runs here in full with A s = tc.
+template <typename T, int EM>
+__global__ void k_fixup(int ntouch, const int* __restrict__ tnode, long long nng, T* __restrict__ tc,
+                        const T* __restrict__ s, Epi<T> ep) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntouch) return;
+  const long long n = tnode[i];
+  T t[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    t[c] = tc[3 * i + c];
+    tc[3 * i + c] = T(0);
+  }
+  if (n >= nng) {
+    T sn[3] = {T(0), T(0), T(0)};
+    if (EM == EPI_CHEB) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sn[c] = s[3 * n + c];
+    }
+    epilogue<T, EM>(ep, n, t, sn);
+    return;
+  }
+  if (EM == EPI_APPLY) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ep.y[3 * n + c] += t[c];
+  } else if (EM == EPI_RESID) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ep.y[3 * n + c] -= t[c];
+  } else {
+    if (ep.y) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.y[3 * n + c] -= t[c];
+    }
+    T z[3];
+    sym3_mul(ep.Dinv + 6 * n, t, z);
+    if (ep.dout) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.dout[3 * n + c] -= ep.c2 * z[c];
+    }
+    if (ep.x) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.x[3 * n + c] -= ep.c2 * z[c];
+    }
+  }
 }
 
 template <typename T>

@@ -176,54 +176,65 @@ __global__ void k_diag_extra(int nrow, const int* __restrict__ rows, const int* 
   }
 }
 
-// Diagonal blocks of rank-one terms: D[node] += k w_i w_j for entry pairs on the same node.
-__global__ void k_diag_rank1(int nterm, const int* __restrict__ ptr, const int* __restrict__ dof,
-                             const double* __restrict__ w, const double* __restrict__ k, double* __restrict__ D) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+// 3x3 node blocks of rank-one terms k w w^T: one wavefront per term.  The DOFs of a term are
+// sorted and unique, so the DOFs of one node are at most two positions apart.
+__global__ __launch_bounds__(256) void k_diag_rank1(int nterm, const int* __restrict__ ptr, const int* __restrict__ dof,
+                                                    const double* __restrict__ w, const double* __restrict__ k,
+                                                    double* __restrict__ D) {
+  const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   if (t >= nterm) return;
-  for (int a = ptr[t]; a < ptr[t + 1]; ++a)
-    for (int b = ptr[t]; b < ptr[t + 1]; ++b) {
-      const int na = dof[a] / 3, nb = dof[b] / 3;
-      if (na != nb) continue;
-      atomicAdd(&D[9LL * na + 3 * (dof[a] % 3) + dof[b] % 3], k[t] * w[a] * w[b]);
+  const int q0 = ptr[t], q1 = ptr[t + 1];
+  const double kt = k[t];
+  for (int a = q0 + lane; a < q1; a += 64) {
+    const int da = dof[a], na = da / 3;
+    const double ka = kt * w[a];
+    for (int b = max(q0, a - 2); b < min(q1, a + 3); ++b) {
+      const int db = dof[b];
+      if (db / 3 == na) atomicAdd(&D[9LL * na + 3 * (da % 3) + db % 3], ka * w[b]);
     }
+  }
 }
 
-// Rank-one terms into the dense coarsest matrix.
-__global__ void k_dense_add_rank1(int nterm, const int* __restrict__ ptr, const int* __restrict__ dof,
-                                  const double* __restrict__ w, const double* __restrict__ k, int n,
-                                  double* __restrict__ A) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nterm) return;
-  for (int a = ptr[t]; a < ptr[t + 1]; ++a)
-    for (int b = ptr[t]; b < ptr[t + 1]; ++b) atomicAdd(&A[(long long)dof[a] * n + dof[b]], k[t] * w[a] * w[b]);
+// Rank-one terms into the dense coarsest matrix: one block per term, threads over the (a, b) pairs.
+__global__ __launch_bounds__(256) void k_dense_add_rank1(const int* __restrict__ ptr, const int* __restrict__ dof,
+                                                         const double* __restrict__ w, const double* __restrict__ k,
+                                                         int n, double* __restrict__ A) {
+  const int t = blockIdx.x;
+  const int q0 = ptr[t], s = ptr[t + 1] - q0;
+  const double kt = k[t];
+  for (long long p = threadIdx.x; p < (long long)s * s; p += blockDim.x) {
+    const int a = q0 + (int)(p / s), b = q0 + (int)(p % s);
+    atomicAdd(&A[(long long)dof[a] * n + dof[b]], kt * w[a] * w[b]);
+  }
 }
 
-// Inverse of the 3x3 blocks; blocks with |det| < 1e-30 become the identity (fe/gmg.py:39-41).
+// Inverse of the 3x3 blocks (symmetric), stored as 6 values per node: 00 01 02 11 12 22 (the
+// off-diagonal pairs averaged).  Blocks with |det| < 1e-30 become the identity (fe/gmg.py:39-41).
 template <typename T>
 __global__ void k_block_inverse(long long nn, const double* __restrict__ D, T* __restrict__ Dinv) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
   const double* m = D + 9 * n;
-  T* o = Dinv + 9 * n;
+  T* o = Dinv + 6 * n;
   const double c0 = m[4] * m[8] - m[5] * m[7];
   const double c1 = m[5] * m[6] - m[3] * m[8];
   const double c2 = m[3] * m[7] - m[4] * m[6];
   const double det = m[0] * c0 + m[1] * c1 + m[2] * c2;
   if (fabs(det) < 1e-30) {
-    for (int q = 0; q < 9; ++q) o[q] = (q % 4 == 0) ? T(1) : T(0);
+    o[0] = T(1); o[1] = T(0); o[2] = T(0); o[3] = T(1); o[4] = T(0); o[5] = T(1);
     return;
   }
   const double id = 1.0 / det;
+  const double i01 = (m[2] * m[7] - m[1] * m[8]) * id, i10 = c1 * id;
+  const double i02 = (m[1] * m[5] - m[2] * m[4]) * id, i20 = c2 * id;
+  const double i12 = (m[2] * m[3] - m[0] * m[5]) * id, i21 = (m[1] * m[6] - m[0] * m[7]) * id;
   o[0] = T(c0 * id);
-  o[1] = T((m[2] * m[7] - m[1] * m[8]) * id);
-  o[2] = T((m[1] * m[5] - m[2] * m[4]) * id);
-  o[3] = T(c1 * id);
-  o[4] = T((m[0] * m[8] - m[2] * m[6]) * id);
-  o[5] = T((m[2] * m[3] - m[0] * m[5]) * id);
-  o[6] = T(c2 * id);
-  o[7] = T((m[1] * m[6] - m[0] * m[7]) * id);
-  o[8] = T((m[0] * m[4] - m[1] * m[3]) * id);
+  o[1] = T(0.5 * (i01 + i10));
+  o[2] = T(0.5 * (i02 + i20));
+  o[3] = T((m[0] * m[8] - m[2] * m[6]) * id);
+  o[4] = T(0.5 * (i12 + i21));
+  o[5] = T((m[0] * m[4] - m[1] * m[3]) * id);
 }
 
 // Dense coarsest operator: one block of 24 threads per coarse cell, thread = element column.
@@ -272,114 +283,156 @@ __global__ void k_dense_fix_empty(int n, double* __restrict__ A) {
 
 // In-place blocked Gauss-Jordan inversion of an SPD matrix (no pivoting), pivot blocks of GJB
 // rows.  For the pivot block K (all other rows/columns I, J):
-//   Dk = A_KK^-1;  R_KJ = Dk A_KJ;  A_IJ -= A_IK R_KJ;  A_IK = -A_IK Dk;  A_KJ = R_KJ;  A_KK = Dk
-// which is GJB consecutive steps of scalar Gauss-Jordan.
+//   Dk = A_KK^-1;  A_KJ <- Dk A_KJ;  A_IJ -= A_IK A_KJ;  A_IK <- -A_IK Dk;  A_KK <- Dk
+// which is GJB consecutive steps of scalar Gauss-Jordan.  Pivot blocks start at multiples of
+// GJB, so the K rows/columns are exactly one 64-wide tile of every kernel below.
 constexpr int GJB = 64;
 
-// (1) Dk = inverse of the nb x nb diagonal block at k0 (one block of 256 threads, LDS).
+// (1) Dk = inverse of the nb x nb diagonal block at k0, padded with the identity to 64 x 64
+// (padding rows/columns stay the identity through the elimination).  256 threads: thread
+// (g = t / 64, j = t % 64) keeps M[g + 4r][j], r = 0..15, in registers; each of the 64 steps
+// publishes the pivot row and column through LDS (double-buffered, one barrier per step).
 __global__ __launch_bounds__(256) void k_gjb_diag(int n, int k0, int nb, const double* __restrict__ A,
                                                   double* __restrict__ Dk) {
-  __shared__ double M[GJB][GJB + 1];
-  __shared__ double col[GJB];
-  for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
-    const int i = t / GJB, j = t % GJB;
-    M[i][j] = (i < nb && j < nb) ? A[(long long)(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
+  __shared__ double rowk[2][GJB], colk[2][GJB];
+  const int j = threadIdx.x & 63, g = threadIdx.x >> 6;
+  double m[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int i = g + 4 * r;
+    m[r] = (i < nb && j < nb) ? A[(long long)(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
   }
-  __syncthreads();
-  for (int k = 0; k < nb; ++k) {
-    if (threadIdx.x < GJB) col[threadIdx.x] = M[threadIdx.x][k];
-    __syncthreads();
-    const double ip = 1.0 / col[k];
-    for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
-      const int i = t / GJB, j = t % GJB;
-      if (i == k) {
-        M[i][j] = (j == k) ? ip : M[i][j] * ip;
-      }
+#pragma unroll
+  for (int k = 0; k < GJB; ++k) {
+    const int buf = k & 1;
+    if (g == (k & 3)) rowk[buf][j] = m[k >> 2];
+    if (j == k) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) colk[buf][g + 4 * r] = m[r];
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
-      const int i = t / GJB, j = t % GJB;
-      if (i != k) M[i][j] = (j == k) ? -col[i] * ip : M[i][j] - col[i] * M[k][j];
+    const double ip = 1.0 / rowk[buf][k];
+    const double rk = rowk[buf][j] * ip;       // new pivot row entry (j != k)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int i = g + 4 * r;
+      if (i == k) m[r] = (j == k) ? ip : rk;
+      else m[r] = (j == k) ? -colk[buf][i] * ip : m[r] - colk[buf][i] * rk;
     }
-    __syncthreads();
   }
-  for (int t = threadIdx.x; t < nb * nb; t += blockDim.x) {
-    const int i = t / nb, j = t % nb;
-    Dk[i * GJB + j] = M[i][j];
-  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) Dk[(g + 4 * r) * GJB + j] = m[r];
 }
 
-// (2) R[:, j] = Dk A[K, j] for every column j (R is GJB x n; columns inside K are skipped later).
-__global__ __launch_bounds__(256) void k_gjb_rowpanel(int n, int k0, int nb, const double* __restrict__ A,
-                                                      const double* __restrict__ Dk, double* __restrict__ R) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  double a[GJB];
-  for (int q = 0; q < nb; ++q) a[q] = A[(long long)(k0 + q) * n + j];
-  for (int i = 0; i < nb; ++i) {
-    double s = 0.0;
-    for (int q = 0; q < nb; ++q) s += Dk[i * GJB + q] * a[q];
-    R[(long long)i * n + j] = s;
-  }
-}
-
-// (3) A_IJ -= A_IK R_KJ for I, J outside K: 64x64 output tiles, K-panel staged in LDS.
-__global__ __launch_bounds__(256) void k_gjb_update(int n, int k0, int nb, double* __restrict__ A,
-                                                    const double* __restrict__ R) {
-  __shared__ double sa[64][GJB + 1];   // A[I-tile, K]
-  __shared__ double sr[GJB][64 + 1];   // R[K, J-tile]
-  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
-  for (int t = threadIdx.x; t < 64 * GJB; t += blockDim.x) {
-    const int r = t / GJB, q = t % GJB;
-    const int i = i0 + r;
-    sa[r][q] = (i < n && q < nb) ? A[(long long)i * n + k0 + q] : 0.0;
-    const int q2 = t / 64, c = t % 64;
-    const int j = j0 + c;
-    sr[q2][c] = (j < n && q2 < nb) ? R[(long long)q2 * n + j] : 0.0;
-  }
-  __syncthreads();
-  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;   // 16x16 threads, 4x4 outputs each
-  double acc[4][4];
+// 64x64x64 product of two LDS tiles, 16x16 threads with 4x4 outputs each
+// (rows tr + 16a, columns tc + 16b).
+__device__ __forceinline__ void gjb_tile_mul(const double (*X)[GJB + 1], const double (*Y)[GJB + 1], int tr, int tc,
+                                             double acc[4][4]) {
+#pragma unroll
   for (int a = 0; a < 4; ++a)
+#pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = 0.0;
-  for (int q = 0; q < nb; ++q) {
+#pragma unroll 4
+  for (int q = 0; q < GJB; ++q) {
     double x[4], y[4];
-    for (int a = 0; a < 4; ++a) x[a] = sa[tr + 16 * a][q];
-    for (int b = 0; b < 4; ++b) y[b] = sr[q][tc + 16 * b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) x[a] = X[tr + 16 * a][q];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) y[b] = Y[q][tc + 16 * b];
+#pragma unroll
     for (int a = 0; a < 4; ++a)
+#pragma unroll
       for (int b = 0; b < 4; ++b) acc[a][b] += x[a] * y[b];
   }
+}
+
+// (2) A_KJ <- Dk A_KJ for every 64-column tile J (in place; the K tile gets Dk itself).
+__global__ __launch_bounds__(256) void k_gjb_rowpanel(int n, int k0, int nb, double* __restrict__ A,
+                                                      const double* __restrict__ Dk) {
+  __shared__ double sd[GJB][GJB + 1], sr[GJB][GJB + 1];
+  const int j0 = blockIdx.x * 64;
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  if (j0 == k0) {
+    for (int t = threadIdx.x; t < nb * GJB; t += blockDim.x) {
+      const int q = t >> 6, c = t & 63;
+      if (c < nb) A[(long long)(k0 + q) * n + k0 + c] = Dk[q * GJB + c];
+    }
+    return;
+  }
+  for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
+    const int q = t >> 6, c = t & 63;
+    sd[q][c] = (q < nb && c < nb) ? Dk[q * GJB + c] : 0.0;
+    sr[q][c] = (q < nb && j0 + c < n) ? A[(long long)(k0 + q) * n + j0 + c] : 0.0;
+  }
+  __syncthreads();
+  double acc[4][4];
+  gjb_tile_mul(sd, sr, tr, tc, acc);
+#pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int i = i0 + tr + 16 * a;
-    if (i >= n || (i >= k0 && i < k0 + nb)) continue;
+    const int q = tr + 16 * a;
+    if (q >= nb) continue;
+#pragma unroll
     for (int b = 0; b < 4; ++b) {
       const int j = j0 + tc + 16 * b;
-      if (j >= n || (j >= k0 && j < k0 + nb)) continue;
-      A[(long long)i * n + j] -= acc[a][b];
+      if (j < n) A[(long long)(k0 + q) * n + j] = acc[a][b];
     }
   }
 }
 
-// (4) A_IK = -A_IK Dk for rows outside K; (5) A_KJ = R_KJ for columns outside K, A_KK = Dk.
-__global__ __launch_bounds__(256) void k_gjb_colpanel(int n, int k0, int nb, double* __restrict__ A,
-                                                      const double* __restrict__ Dk) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || (i >= k0 && i < k0 + nb)) return;
-  double a[GJB];
-  for (int q = 0; q < nb; ++q) a[q] = A[(long long)i * n + k0 + q];
-  for (int j = 0; j < nb; ++j) {
-    double s = 0.0;
-    for (int q = 0; q < nb; ++q) s += a[q] * Dk[q * GJB + j];
-    A[(long long)i * n + k0 + j] = -s;
+// (3) A_IJ -= A_IK A_KJ for I, J outside K (A_KJ already holds Dk A_KJ): 64x64 output tiles.
+__global__ __launch_bounds__(256) void k_gjb_update(int n, int k0, int nb, double* __restrict__ A) {
+  __shared__ double sa[64][GJB + 1];   // A[I-tile, K]
+  __shared__ double sr[GJB][64 + 1];   // A[K, J-tile]
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  if (i0 == k0 || j0 == k0) return;
+  for (int t = threadIdx.x; t < 64 * GJB; t += blockDim.x) {
+    const int r = t >> 6, q = t & 63;
+    const int i = i0 + r;
+    sa[r][q] = (i < n && q < nb) ? A[(long long)i * n + k0 + q] : 0.0;
+    const int j = j0 + q;
+    sr[r][q] = (j < n && r < nb) ? A[(long long)(k0 + r) * n + j] : 0.0;
+  }
+  __syncthreads();
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4];
+  gjb_tile_mul(sa, sr, tr, tc, acc);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + tr + 16 * a;
+    if (i >= n) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int j = j0 + tc + 16 * b;
+      if (j < n) A[(long long)i * n + j] -= acc[a][b];
+    }
   }
 }
-__global__ __launch_bounds__(256) void k_gjb_finish(int n, int k0, int nb, double* __restrict__ A,
-                                                    const double* __restrict__ R, const double* __restrict__ Dk) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const bool inK = (j >= k0 && j < k0 + nb);
-  for (int q = 0; q < nb; ++q)
-    A[(long long)(k0 + q) * n + j] = inK ? Dk[q * GJB + (j - k0)] : R[(long long)q * n + j];
+
+// (4) A_IK <- -A_IK Dk for every 64-row tile I outside K.
+__global__ __launch_bounds__(256) void k_gjb_colpanel(int n, int k0, int nb, double* __restrict__ A,
+                                                      const double* __restrict__ Dk) {
+  __shared__ double sa[GJB][GJB + 1], sd[GJB][GJB + 1];
+  const int i0 = blockIdx.x * 64;
+  if (i0 == k0) return;
+  for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
+    const int r = t >> 6, c = t & 63;
+    sa[r][c] = (i0 + r < n && c < nb) ? A[(long long)(i0 + r) * n + k0 + c] : 0.0;
+    sd[r][c] = (r < nb && c < nb) ? Dk[r * GJB + c] : 0.0;
+  }
+  __syncthreads();
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
+  double acc[4][4];
+  gjb_tile_mul(sa, sd, tr, tc, acc);
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const int i = i0 + tr + 16 * a;
+    if (i >= n) continue;
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int c = tc + 16 * b;
+      if (c < nb) A[(long long)i * n + k0 + c] = -acc[a][b];
+    }
+  }
 }
 
 // Coarse solve x = Ainv b (dense FP64), one wavefront per row.

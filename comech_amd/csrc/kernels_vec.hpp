@@ -31,38 +31,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_finalize(int nb, const double* 
 }
 
 // ---- smoothers -------------------------------------------------------------------------
-// x = x + omega Dinv (b - Ax)   (first: x = omega Dinv b, i.e. x was zero)
+// Dinv holds the symmetric 3x3 block inverses as 6 values per node (00 01 02 11 12 22).
+// First step of a smoother from x = 0: x = c * Dinv b.
 template <typename T>
-__global__ void k_jacobi(long long nn, T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ Ax,
-                         const T* __restrict__ Dinv, T omega, int first) {
+__global__ void k_cheb0(long long nn, T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ Dinv, T c) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
-  T r[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) r[c] = first ? b[3 * n + c] : b[3 * n + c] - Ax[3 * n + c];
-  const T* d = Dinv + 9 * n;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T z = d[3 * c] * r[0] + d[3 * c + 1] * r[1] + d[3 * c + 2] * r[2];
-    x[3 * n + c] = first ? omega * z : x[3 * n + c] + omega * z;
-  }
-}
-
-// Chebyshev (fe/gmg.py:61-76): d = c1 * d + c2 * Dinv r; x += d
-template <typename T>
-__global__ void k_cheb(long long nn, T* __restrict__ x, T* __restrict__ d, const T* __restrict__ r,
-                       const T* __restrict__ Dinv, T c1, T c2, int first) {
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
-  const T* m = Dinv + 9 * n;
-  const T r0 = r[3 * n], r1 = r[3 * n + 1], r2 = r[3 * n + 2];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const T z = m[3 * c] * r0 + m[3 * c + 1] * r1 + m[3 * c + 2] * r2;
-    const T dn = first ? c2 * z : c1 * d[3 * n + c] + c2 * z;
-    d[3 * n + c] = dn;
-    x[3 * n + c] = first ? dn : x[3 * n + c] + dn;
-  }
+  const T* m = Dinv + 6 * n;
+  const T r0 = b[3 * n], r1 = b[3 * n + 1], r2 = b[3 * n + 2];
+  x[3 * n] = c * (m[0] * r0 + m[1] * r1 + m[2] * r2);
+  x[3 * n + 1] = c * (m[1] * r0 + m[3] * r1 + m[4] * r2);
+  x[3 * n + 2] = c * (m[2] * r0 + m[4] * r1 + m[5] * r2);
 }
 
 // r -= t
@@ -77,10 +56,11 @@ template <typename T>
 __global__ void k_block_apply(long long nn, const T* __restrict__ Dinv, const T* __restrict__ x, T* __restrict__ y) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
-  const T* m = Dinv + 9 * n;
+  const T* m = Dinv + 6 * n;
   const T x0 = x[3 * n], x1 = x[3 * n + 1], x2 = x[3 * n + 2];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) y[3 * n + c] = m[3 * c] * x0 + m[3 * c + 1] * x1 + m[3 * c + 2] * x2;
+  y[3 * n] = m[0] * x0 + m[1] * x1 + m[2] * x2;
+  y[3 * n + 1] = m[1] * x0 + m[3] * x1 + m[4] * x2;
+  y[3 * n + 2] = m[2] * x0 + m[4] * x1 + m[5] * x2;
 }
 
 template <typename T>

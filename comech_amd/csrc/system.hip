@@ -30,7 +30,8 @@ struct LevelT {
   DevBuf<T> cellL, cellM;   // level 0, value mode (x-fastest, irregular cells zeroed)
   DevBuf<T> irrL, irrM;     // level 0 irregular cells
   DevBuf<T> mom;            // coarse levels: [54][ncell]
-  DevBuf<T> Dinv;           // [nn][9]
+  DevBuf<T> Dinv;           // [nn][6] symmetric 3x3 block inverses
+  DevBuf<T> tc;             // sparse-pass accumulator, 3 per touched node (kept zero between uses)
   DevBuf<T> pw[3];          // prolongation weights, this level as the fine side
   DevBuf<T> rw[3];          // restriction weights of the next level's coarse nodes
   DevBuf<T> xval;           // extra stiffness values
@@ -72,6 +73,12 @@ struct Level {
   DevBuf<int> r1ptr, r1dof;
   DevBuf<double> r1w64, r1k64;
   int kz[2] = {0, 0};         // z extent of a structured block per precision (0 = heuristic)
+  // nodes the sparse pass writes (irregular cells, extra rows, rank-one terms, all split copies),
+  // sorted; the sparse pass addresses them through compact indices
+  int ntouch = 0;
+  std::vector<int> tnode_h;
+  DevBuf<int> tnode, irr_ci, xrc, r1c;
+  double iso_fac = 0.0;       // level 0: h / 16 when all cells are the same cube, else 0
   // level 0 only
   bool label_mode = false;
   DevBuf<uint8_t> lab;                      // apply labels (x-fastest, irregular -> 255)
@@ -81,6 +88,7 @@ struct Level {
   int ncopy = 0, nirr = 0;
   std::vector<int> copy_orig_h;
   DevBuf<int> copy_orig, irr_nodes, irr_ijk;
+  std::vector<int> irr_nodes_h;
   std::vector<double> irrL_h, irrM_h;
   // coarse-level moments (FP64 master)
   DevBuf<double> mom64;
@@ -223,6 +231,7 @@ struct System {
     if (v.label_mode) v.lab.upload(labx, stream);
     v.irr_nodes.upload(nodes, stream);
     v.irr_ijk.upload(ijk, stream);
+    v.irr_nodes_h.swap(nodes);
     v.copy_orig.upload(v.copy_orig_h, stream);
     // extra stiffness COO -> unique-row CSR (duplicates summed)
     const long long nd3 = 3 * v.nn;
@@ -573,6 +582,45 @@ struct System {
   }
 
   // ------------------------------------------------------------------ build
+  // h / 16 when every cell of the level is the same cube (edge lengths equal to 1e-13 relative),
+  // else 0.  The structured march then folds the gradient factors into the material weights.
+  static double iso_factor(const Level& v) {
+    const double h = v.e[0][1] - v.e[0][0];
+    if (!(h > 0.0)) return 0.0;
+    for (int ax = 0; ax < 3; ++ax)
+      for (size_t i = 0; i + 1 < v.e[ax].size(); ++i)
+        if (std::fabs((v.e[ax][i + 1] - v.e[ax][i]) - h) > 1e-13 * h) return 0.0;
+    return h / 16.0;
+  }
+
+  // Compact node list of the sparse pass: every node an irregular cell, an extra row or a
+  // rank-one term writes, plus every split copy (copies have no structured part).  Uploads the
+  // node list and the compact indices of the sparse pass's outputs.
+  static int compact_of(const std::vector<int>& sorted, long long node) {
+    const auto it = std::lower_bound(sorted.begin(), sorted.end(), (int)node);
+    return (int)(it - sorted.begin());
+  }
+  void build_touch(Level& v) {
+    std::vector<int> nodes;
+    nodes.reserve(v.irr_nodes_h.size() + v.xrow_h.size() + v.r1dof_h.size() + (size_t)(v.nn - v.nng));
+    for (int nd : v.irr_nodes_h) nodes.push_back(nd);
+    for (int r : v.xrow_h) nodes.push_back(r / 3);
+    for (int q : v.r1dof_h) nodes.push_back(q / 3);
+    for (long long n = v.nng; n < v.nn; ++n) nodes.push_back((int)n);
+    std::sort(nodes.begin(), nodes.end());
+    nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+    v.tnode_h.swap(nodes);
+    v.ntouch = (int)v.tnode_h.size();
+    std::vector<int> ci(v.irr_nodes_h.size()), xr(v.xrow_h.size()), rc(v.r1dof_h.size());
+    for (size_t q = 0; q < ci.size(); ++q) ci[q] = compact_of(v.tnode_h, v.irr_nodes_h[q]);
+    for (size_t q = 0; q < xr.size(); ++q) xr[q] = 3 * compact_of(v.tnode_h, v.xrow_h[q] / 3) + v.xrow_h[q] % 3;
+    for (size_t q = 0; q < rc.size(); ++q) rc[q] = 3 * compact_of(v.tnode_h, v.r1dof_h[q] / 3) + v.r1dof_h[q] % 3;
+    v.tnode.upload(v.tnode_h, stream);
+    v.irr_ci.upload(ci, stream);
+    v.xrc.upload(xr, stream);
+    v.r1c.upload(rc, stream);
+  }
+
   void build(int min_dofs, int max_levels, int fixed_levels, const Options& o) {
     CM_CHECK(!L.empty(), "system has no fine level");
     opt = o;
@@ -593,6 +641,7 @@ struct System {
     lap(0);
     Level& f0 = lv(0);
     f0.used.upload(f0.used_h, stream);
+    f0.iso_fac = iso_factor(f0);
     // level structure (fe/gmg.py:86-101, rb/gpusolve.py:44-46)
     const int nl_max = fixed_levels > 0 ? fixed_levels : max_levels;
     while (true) {
@@ -635,6 +684,7 @@ struct System {
       v.r1dof.upload(v.r1dof_h, stream);
       v.r1w64.upload(v.r1w_h, stream);
       v.r1k64.upload(v.r1k_h, stream);
+      build_touch(v);
     }
     prepare_prec<double>(0, false);
     lap(2);
@@ -716,7 +766,7 @@ struct System {
     }
     const int nterm = (int)v.r1k_h.size();
     if (nterm) {
-      hipLaunchKernelGGL(k_diag_rank1, grid1(nterm), dim3(256), 0, stream, nterm, v.r1ptr.p, v.r1dof.p, v.r1w64.p,
+      hipLaunchKernelGGL(k_diag_rank1, grid1(64LL * nterm), dim3(256), 0, stream, nterm, v.r1ptr.p, v.r1dof.p, v.r1w64.p,
                          v.r1k64.p, D.p);
       CM_LAUNCH_CHECK();
     }
@@ -757,22 +807,21 @@ struct System {
       }
       const int nterm = (int)c.r1k_h.size();
       if (nterm) {
-        hipLaunchKernelGGL(k_dense_add_rank1, grid1(nterm), dim3(256), 0, stream, nterm, c.r1ptr.p, c.r1dof.p,
+        hipLaunchKernelGGL(k_dense_add_rank1, dim3((unsigned)nterm), dim3(256), 0, stream, c.r1ptr.p, c.r1dof.p,
                            c.r1w64.p, c.r1k64.p, (int)n, Ainv.p);
         CM_LAUNCH_CHECK();
       }
     }
     hipLaunchKernelGGL(k_dense_fix_empty, dim3((unsigned)n), dim3(256), 0, stream, (int)n, Ainv.p);
     CM_LAUNCH_CHECK();
-    DevBuf<double> Dk(GJB * GJB), R((size_t)GJB * n);
-    const dim3 tiles(div_up(n, 64), div_up(n, 64));
+    DevBuf<double> Dk(GJB * GJB);
+    const unsigned nt = (unsigned)div_up(n, 64);
     for (int k0 = 0; k0 < n; k0 += GJB) {
       const int nb = (int)std::min<long long>(GJB, n - k0);
       hipLaunchKernelGGL(k_gjb_diag, dim3(1), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
-      hipLaunchKernelGGL(k_gjb_rowpanel, grid1(n), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p, R.p);
-      hipLaunchKernelGGL(k_gjb_update, tiles, dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, R.p);
-      hipLaunchKernelGGL(k_gjb_colpanel, grid1(n), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
-      hipLaunchKernelGGL(k_gjb_finish, grid1(n), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, R.p, Dk.p);
+      hipLaunchKernelGGL(k_gjb_rowpanel, dim3(nt), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
+      hipLaunchKernelGGL(k_gjb_update, dim3(nt, nt), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p);
+      hipLaunchKernelGGL(k_gjb_colpanel, dim3(nt), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
     }
     CM_LAUNCH_CHECK();
     CM_HIP(hipStreamSynchronize(stream));
@@ -814,10 +863,15 @@ struct System {
       if (!v.r1k_h.empty()) { upload_cast(d.r1w, v.r1w_h, stream); upload_cast(d.r1k, v.r1k_h, stream); }
       const size_t nv = 3 * (size_t)v.nn;
       d.x.alloc(nv); d.b.alloc(nv); d.r.alloc(nv); d.t.alloc(nv); d.d.alloc(nv);
+      if (v.ntouch) {
+        d.tc.alloc(3 * (size_t)v.ntouch);
+        hipLaunchKernelGGL(k_fill<T>, grid1(3LL * v.ntouch), dim3(256), 0, stream, d.tc.p, 3LL * v.ntouch, T(0));
+        CM_LAUNCH_CHECK();
+      }
       d.ready = true;
     }
     if (with_dinv && !d.Dinv.n && l + 1 < nlev()) {
-      d.Dinv.alloc(9 * (size_t)v.nn);
+      d.Dinv.alloc(6 * (size_t)v.nn);
       hipLaunchKernelGGL(k_block_inverse<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, D64[l].p, d.Dinv.p);
       CM_LAUNCH_CHECK();
       if (opt.smoother == 0) power_iteration<T>(l);
@@ -831,73 +885,105 @@ struct System {
 
   // ------------------------------------------------------------------ operator
   template <typename T>
-  void apply(int l, const T* u, T* y, int mode, const T* b) {
+  GridArgs<T> grid_args(int l) {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
-    GridArgs<T> g{v.nx, v.ny, v.nz, d.hx.p, d.hy.p, d.hz.p};
-    const int kz = pick_kz<T>(v);
-    dim3 grid(div_up(v.nx + 1, TOUT), div_up(v.ny + 1, TOUT), div_up(v.nz + 1, kz));
-    const T sign = mode == MODE_APPLY ? T(1) : T(-1);
-    if (l == 0) {
-      launch_fine<T>(v, d, g, grid, kz, u, y, mode, b);
-      const long long nc3 = 3 * (v.nn - v.nng);
-      if (nc3) {
-        if (mode == MODE_APPLY)
-          hipLaunchKernelGGL(k_fill<T>, grid1(nc3), dim3(256), 0, stream, y + 3 * v.nng, nc3, T(0));
-        else if (b != y)
-          hipLaunchKernelGGL(k_copy_range<T>, grid1(nc3), dim3(256), 0, stream, y + 3 * v.nng, b + 3 * v.nng, nc3);
-      }
-      if (v.nirr) {
-        if (sri)
-          hipLaunchKernelGGL((k_apply_irregular<T, true>), grid1(v.nirr), dim3(256), 0, stream, v.nirr,
-                             v.irr_nodes.p, v.irr_ijk.p, d.irrL.p, d.irrM.p, g, u, y, sign);
-        else
-          hipLaunchKernelGGL((k_apply_irregular<T, false>), grid1(v.nirr), dim3(256), 0, stream, v.nirr,
-                             v.irr_nodes.p, v.irr_ijk.p, d.irrL.p, d.irrM.p, g, u, y, sign);
-      }
-    } else {
-      MomentCells<T> mc{d.mom.p};
-      if (mode == MODE_APPLY)
-        hipLaunchKernelGGL((k_apply_mom<T, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, mc, u, y, b, kz);
-      else
-        hipLaunchKernelGGL((k_apply_mom<T, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, mc, u, y, b, kz);
+    return GridArgs<T>{v.nx, v.ny, v.nz, d.hx.p, d.hy.p, d.hz.p, T(v.iso_fac)};
+  }
+
+  // y = A u (MODE_APPLY) or y = b - A u (MODE_RESID); b may alias y.
+  template <typename T>
+  void apply(int l, const T* u, T* y, int mode, const T* b) {
+    Epi<T> ep{b, y, nullptr, nullptr, nullptr, T(0), T(0), T(0)};
+    if (mode == MODE_APPLY) apply_ep<T, EPI_APPLY>(l, u, ep);
+    else apply_ep<T, EPI_RESID>(l, u, ep);
+  }
+
+  // One operator application on s with a fused epilogue: sparse pass -> structured march ->
+  // fix-up of the touched nodes (which also clears the sparse accumulator).
+  template <typename T, int EM>
+  void apply_ep(int l, const T* s, const Epi<T>& ep) {
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    const GridArgs<T> g = grid_args<T>(l);
+    if (v.ntouch) launch_sparse<T>(v, d, g, s);
+    launch_march<T, EM>(l, s, ep);
+    if (v.ntouch) {
+      hipLaunchKernelGGL((k_fixup<T, EM>), grid1(v.ntouch), dim3(256), 0, stream, v.ntouch, v.tnode.p, v.nng, d.tc.p, s,
+                         ep);
     }
-    const int nrow = (int)v.xrow_h.size();
-    if (nrow) launch_extra<T>(v, d, u, y, sign);
-    const int nterm = (int)v.r1k_h.size();
-    if (nterm)
-      hipLaunchKernelGGL(k_apply_rank1<T>, grid1(64LL * nterm), dim3(256), 0, stream, nterm, v.r1ptr.p, v.r1dof.p,
-                         d.r1w.p, d.r1k.p, u, y, sign);
     CM_LAUNCH_CHECK();
   }
 
   template <typename T>
-  void launch_extra(Level& v, LevelT<T>& d, const T* u, T* y, T sign) {
-    const int nrow = (int)v.xrow_h.size();
-    const dim3 g = grid1((long long)nrow * v.xw);
+  void launch_sparse(Level& v, LevelT<T>& d, const GridArgs<T>& g, const T* s) {
+    SparseArgs<T> a{};
+    a.nirr = v.nirr;
+    a.nrow = (int)v.xrow_h.size();
+    a.nterm = (int)v.r1k_h.size();
+    a.nb_irr = (int)div_up(a.nirr, 256);
+    a.nb_row = (int)div_up((long long)a.nrow * v.xw, 256);
+    const int nb_r1 = (int)div_up(64LL * a.nterm, 256);
+    a.irr_nodes = v.irr_nodes.p; a.irr_ci = v.irr_ci.p; a.irr_ijk = v.irr_ijk.p; a.irrL = d.irrL.p; a.irrM = d.irrM.p;
+    a.xrc = v.xrc.p; a.xptr = v.xptr.p; a.xcol = v.xcol.p; a.xval = d.xval.p;
+    a.r1ptr = v.r1ptr.p; a.r1dof = v.r1dof.p; a.r1c = v.r1c.p; a.r1w = d.r1w.p; a.r1k = d.r1k.p;
+    const dim3 grid((unsigned)(a.nb_irr + a.nb_row + nb_r1));
+    if (grid.x == 0) return;
     switch (v.xw) {
-#define CM_XW(W) \
-  case W: hipLaunchKernelGGL((k_apply_extra<T, W>), g, dim3(256), 0, stream, nrow, v.xrow.p, v.xptr.p, v.xcol.p, d.xval.p, u, y, sign); break;
+#define CM_XW(W)                                                                                         \
+  case W:                                                                                                \
+    if (sri) hipLaunchKernelGGL((k_sparse<T, true, W>), grid, dim3(256), 0, stream, a, g, s, d.tc.p);  \
+    else hipLaunchKernelGGL((k_sparse<T, false, W>), grid, dim3(256), 0, stream, a, g, s, d.tc.p);     \
+    break;
       CM_XW(1) CM_XW(2) CM_XW(4) CM_XW(8) CM_XW(16) CM_XW(32) CM_XW(64)
 #undef CM_XW
       default: throw Error("bad extra width");
     }
   }
 
-  // structured part only (timing)
-  template <typename T>
-  void apply_structured(int l, const T* u, T* y) {
+  // structured march of level l with epilogue EM
+  template <typename T, int EM>
+  void launch_march(int l, const T* s, const Epi<T>& ep) {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
-    GridArgs<T> g{v.nx, v.ny, v.nz, d.hx.p, d.hy.p, d.hz.p};
+    const GridArgs<T> g = grid_args<T>(l);
     const int kz = pick_kz<T>(v);
-    dim3 grid(div_up(v.nx + 1, TOUT), div_up(v.ny + 1, TOUT), div_up(v.nz + 1, kz));
-    if (l == 0) {
-      launch_fine<T>(v, d, g, grid, kz, u, y, MODE_APPLY, nullptr);
-    } else {
-      MomentCells<T> mc{d.mom.p};
-      hipLaunchKernelGGL((k_apply_mom<T, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, mc, u, y, (const T*)nullptr, kz);
+    const int ntx = (int)div_up(v.nx + 1, TOUT), nty = (int)div_up(v.ny + 1, TOUT), ntz = (int)div_up(v.nz + 1, kz);
+    const long long nb = (long long)ntx * nty * ntz;
+    CM_CHECK(nb < (1LL << 31), "grid too large for one launch");
+    const TileMap tm{ntx, nty, ntz, (int)(nb / 8), (int)(nb % 8)};
+    const dim3 grid((unsigned)nb), blk(BLOCK);
+    if (l > 0) {
+      MomentMat<T> m{d.mom.p};
+      hipLaunchKernelGGL((k_march<T, MomentMat<T>, EM>), grid, blk, 0, stream, g, m, s, ep, kz, tm);
+      return;
     }
+    const bool iso = v.iso_fac > 0.0;
+#define CM_FINE(MAT, SRI_, ISO_, ...)                                                                      \
+  if (sri == SRI_ && iso == ISO_) {                                                                        \
+    MAT<T, SRI_, ISO_> m{__VA_ARGS__};                                                                     \
+    hipLaunchKernelGGL((k_march<T, MAT<T, SRI_, ISO_>, EM>), grid, blk, 0, stream, g, m, s, ep, kz, tm); \
+    return;                                                                                                \
+  }
+    if (v.label_mode) {
+      CM_FINE(LabelMat, true, true, v.lab.p, d.tabL.p, d.tabM.p)
+      CM_FINE(LabelMat, true, false, v.lab.p, d.tabL.p, d.tabM.p)
+      CM_FINE(LabelMat, false, true, v.lab.p, d.tabL.p, d.tabM.p)
+      CM_FINE(LabelMat, false, false, v.lab.p, d.tabL.p, d.tabM.p)
+    } else {
+      CM_FINE(ValueMat, true, true, d.cellL.p, d.cellM.p)
+      CM_FINE(ValueMat, true, false, d.cellL.p, d.cellM.p)
+      CM_FINE(ValueMat, false, true, d.cellL.p, d.cellM.p)
+      CM_FINE(ValueMat, false, false, d.cellL.p, d.cellM.p)
+    }
+#undef CM_FINE
+  }
+
+  // structured part only, plain product (timing, tuning)
+  template <typename T>
+  void apply_structured(int l, const T* u, T* y) {
+    Epi<T> ep{nullptr, y, nullptr, nullptr, nullptr, T(0), T(0), T(0)};
+    launch_march<T, EPI_APPLY>(l, u, ep);
     CM_LAUNCH_CHECK();
   }
 
@@ -944,31 +1030,6 @@ struct System {
     CM_HIP(hipEventDestroy(e0)); CM_HIP(hipEventDestroy(e1));
   }
 
-  template <typename T>
-  void launch_fine(Level& v, LevelT<T>& d, GridArgs<T> g, dim3 grid, int kz, const T* u, T* y, int mode, const T* b) {
-    FineLabelCells<T, true> cells_lab_t{v.lab.p, d.tabL.p, d.tabM.p};
-    FineLabelCells<T, false> cells_lab_f{v.lab.p, d.tabL.p, d.tabM.p};
-    FineValueCells<T, true> cells_val_t{d.cellL.p, d.cellM.p};
-    FineValueCells<T, false> cells_val_f{d.cellL.p, d.cellM.p};
-    if (v.label_mode) {
-      if (sri) {
-        if (mode == MODE_APPLY) hipLaunchKernelGGL((k_apply_fine<T, FineLabelCells<T, true>, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, cells_lab_t, u, y, b, kz);
-        else hipLaunchKernelGGL((k_apply_fine<T, FineLabelCells<T, true>, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, cells_lab_t, u, y, b, kz);
-      } else {
-        if (mode == MODE_APPLY) hipLaunchKernelGGL((k_apply_fine<T, FineLabelCells<T, false>, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, cells_lab_f, u, y, b, kz);
-        else hipLaunchKernelGGL((k_apply_fine<T, FineLabelCells<T, false>, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, cells_lab_f, u, y, b, kz);
-      }
-    } else {
-      if (sri) {
-        if (mode == MODE_APPLY) hipLaunchKernelGGL((k_apply_fine<T, FineValueCells<T, true>, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, cells_val_t, u, y, b, kz);
-        else hipLaunchKernelGGL((k_apply_fine<T, FineValueCells<T, true>, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, cells_val_t, u, y, b, kz);
-      } else {
-        if (mode == MODE_APPLY) hipLaunchKernelGGL((k_apply_fine<T, FineValueCells<T, false>, MODE_APPLY>), grid, dim3(BLOCK), 0, stream, g, cells_val_f, u, y, b, kz);
-        else hipLaunchKernelGGL((k_apply_fine<T, FineValueCells<T, false>, MODE_RESID>), grid, dim3(BLOCK), 0, stream, g, cells_val_f, u, y, b, kz);
-      }
-    }
-  }
-
   // ------------------------------------------------------------------ smoothers, V-cycle
   // lambda_max of M A by 12 power steps from a fixed pseudo-normal start (fe/gmg.py:51-59):
   // lam = |y| / |x| of the last step, then lmax = 1.1 lam, lmin = lmax / 30.  The iterate is
@@ -1009,43 +1070,65 @@ struct System {
     LevelT<T>& d = v.dat<T>();
     const long long nn = v.nn;
     if (opt.smoother == 1) {
+      // damped block Jacobi: x += omega Dinv (b - A x); the step lands in d, then x += d
       int s0 = 0;
       if (zero_start) {
-        hipLaunchKernelGGL(k_jacobi<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, b, d.Dinv.p, T(opt.omega), 1);
+        hipLaunchKernelGGL(k_cheb0<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(opt.omega));
         s0 = 1;
       }
       for (int s = s0; s < nsweep; ++s) {
-        apply<T>(l, x, d.t.p, MODE_APPLY, nullptr);
-        hipLaunchKernelGGL(k_jacobi<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.t.p, d.Dinv.p, T(opt.omega), 0);
+        const Epi<T> ep{b, nullptr, d.Dinv.p, d.d.p, nullptr, T(0), T(opt.omega), T(0)};
+        apply_ep<T, EPI_CHEB>(l, x, ep);
+        hipLaunchKernelGGL((k_add<T, T>), grid1(3 * nn), dim3(256), 0, stream, 3 * nn, x, d.d.p);
       }
       CM_LAUNCH_CHECK();
       return;
     }
-    // Chebyshev, degree opt.degree (fe/gmg.py:61-76)
+    // Chebyshev of degree opt.degree (fe/gmg.py:61-76):
+    //   r = b - A x;  d = M r / theta;  x += d;
+    //   repeat: r -= A d;  rho' = 1 / (2 sigma - rho);  d = rho' rho d + 2 rho' / delta M r;  x += d
+    // Every "r -= A d; d = ...; x += ..." is one fused operator application with d as its input.
+    // Directions alternate between two buffers; x is updated in place whenever it is not the
+    // operator's input (a direction that could not be added yet is added by the next step: xa).
     const double theta = 0.5 * (d.lmax + d.lmin), delta = 0.5 * (d.lmax - d.lmin);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;
     T* r = d.r.p;
-    T* dd = d.d.p;
-    const long long n3 = 3 * nn;
-    if (zero_start)
-      hipLaunchKernelGGL(k_copy_range<T>, grid1(n3), dim3(256), 0, stream, r, b, n3);
-    else
-      apply<T>(l, x, r, MODE_RESID, b);
-    // d = M r / theta ; x += d
-    if (zero_start)
-      hipLaunchKernelGGL(k_cheb<T>, grid1(nn), dim3(256), 0, stream, nn, x, dd, r, d.Dinv.p, T(0), T(1.0 / theta), 1);
-    else {
-      hipLaunchKernelGGL(k_cheb<T>, grid1(nn), dim3(256), 0, stream, nn, d.t.p, dd, r, d.Dinv.p, T(0), T(1.0 / theta), 1);
-      hipLaunchKernelGGL((k_add<T, T>), grid1(n3), dim3(256), 0, stream, n3, x, dd);
+    T* dirs[2] = {d.d.p, d.t.p};
+    const T* s;            // input of the next application
+    const T* rin = b;
+    bool pending;          // s is a direction not yet added to x
+    int nd = 0;            // next direction buffer
+    if (zero_start) {
+      hipLaunchKernelGGL(k_cheb0<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(1.0 / theta));
+      s = x;
+      pending = false;
+    } else {
+      const Epi<T> ep{b, r, d.Dinv.p, dirs[nd], nullptr, T(0), T(1.0 / theta), T(0)};
+      apply_ep<T, EPI_CHEB>(l, x, ep);
+      s = dirs[nd];
+      nd ^= 1;
+      rin = r;
+      pending = true;
     }
     for (int k = 1; k < opt.degree; ++k) {
-      apply<T>(l, dd, r, MODE_RESID, r);   // r -= A d
       const double rho_new = 1.0 / (2.0 * sigma - rho);
-      hipLaunchKernelGGL(k_cheb<T>, grid1(nn), dim3(256), 0, stream, nn, x, dd, r, d.Dinv.p, T(rho_new * rho),
-                         T(2.0 * rho_new / delta), 0);
+      const bool last = (k == opt.degree - 1);
+      const bool upd = (s != x);
+      T* dout = (!last || !upd) ? dirs[nd] : nullptr;
+      const Epi<T> ep{rin, last ? nullptr : r, d.Dinv.p, dout, upd ? x : nullptr, T(rho_new * rho),
+                      T(2.0 * rho_new / delta), T(pending ? 1 : 0)};
+      apply_ep<T, EPI_CHEB>(l, s, ep);
+      pending = !upd;
+      if (dout) {
+        s = dout;
+        nd ^= 1;
+      }
+      rin = r;
       rho = rho_new;
     }
+    if (pending && s != x)
+      hipLaunchKernelGGL((k_add<T, T>), grid1(3 * nn), dim3(256), 0, stream, 3 * nn, x, s);
     CM_LAUNCH_CHECK();
   }
 

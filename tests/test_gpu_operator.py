@@ -176,3 +176,21 @@ def test_vcycle_is_spd_and_contracts(smoother):
     K = info["K"]
     en = lambda v: np.sqrt(v.reshape(-1, 3)[info["used"]].ravel() @ (K @ v.reshape(-1, 3)[info["used"]].ravel()))
     assert en(e1) < 0.7 * en(e)
+
+
+def test_single_level_vcycle_is_dense_solve():
+    """One level: the V-cycle is the dense coarsest solve (blocked Gauss-Jordan inverse of the
+    3195-DOF split-node system, 50 pivot blocks, the last one partial) and equals K^-1 r of the
+    oracle's assembled matrix (cond(K) ~ 1e6, so 1e-8 leaves room for FP64 rounding)."""
+    import scipy.sparse.linalg as sla
+    c, s = _sys2(dims=(12, 8, 8), h=3.0, levels=1)
+    info = C.oracle_system2((12, 8, 8), 3.0)
+    nn = len(c["orig"])
+    used = np.zeros(nn, bool)
+    used[info["used"]] = True
+    r = C.random_full(nn, used, 7)
+    z = s.vcycle(torch.tensor(r, dtype=torch.float64, device=DEV)).cpu().numpy().reshape(-1, 3)
+    ref = sla.spsolve(info["K"].tocsc(), r.reshape(-1, 3)[info["used"]].ravel())
+    got = z[info["used"]].ravel()
+    assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max()
+    assert not np.any(z[~used])

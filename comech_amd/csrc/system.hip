@@ -1164,6 +1164,57 @@ struct System {
     smooth<T>(l, b, x, false, opt.npost);
   }
 
+  // ------------------------------------------------------------------ V-cycle graphs
+  // The top-level V-cycle (all levels, ~100 launches) is captured once per precision into a HIP
+  // graph and replayed while its right-hand side and result buffers stay the same.  Capture runs
+  // on the system's own stream (non-blocking, default priority), entered through OwnStream.
+  bool graphs = true;
+  hipStream_t own = nullptr;
+  struct VGraph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t e = nullptr;
+    const void* b = nullptr;
+    void* x = nullptr;
+  };
+  VGraph vg[2];
+  void drop_graph(VGraph& G) {
+    if (G.e) (void)hipGraphExecDestroy(G.e);
+    if (G.g) (void)hipGraphDestroy(G.g);
+    G = VGraph{};
+  }
+  void drop_graphs() {
+    for (auto& G : vg) drop_graph(G);
+  }
+  template <typename T>
+  void vcycle_top(const T* b, T* x) {
+    if (!graphs || own == nullptr || stream != own) {
+      vcycle<T>(0, b, x);
+      return;
+    }
+    VGraph& G = vg[std::is_same<T, float>::value ? 1 : 0];
+    if (!G.e || G.b != (const void*)b || G.x != (void*)x) {
+      drop_graph(G);
+      CM_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+      try {
+        vcycle<T>(0, b, x);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      CM_HIP(hipStreamEndCapture(stream, &G.g));
+      CM_HIP(hipGraphInstantiate(&G.e, G.g, nullptr, nullptr, 0));
+      G.b = b;
+      G.x = x;
+    }
+    CM_HIP(hipGraphLaunch(G.e, stream));
+  }
+  ~System() {
+    drop_graphs();
+    if (own) (void)hipStreamDestroy(own);
+  }
+
   // ------------------------------------------------------------------ PCG
   template <typename T>
   void finalize_to(int slot) {
@@ -1196,10 +1247,10 @@ struct System {
     Level& v0 = lv(0);
     const long long n = 3 * v0.nn;
     if (vprec == 0) {
-      vcycle<double>(0, r, z);
+      vcycle_top<double>(r, z);
     } else {
       hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, r32.p, r);
-      vcycle<float>(0, r32.p, z32.p);
+      vcycle_top<float>(r32.p, z32.p);
       hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z, z32.p);
     }
     rigid_correct(r, z);
@@ -1265,7 +1316,7 @@ struct System {
   // z32 = prec(r32) in FP32 (+ the FP64 rigid-mode correction when present)
   void precondition32(const float* r, float* z) {
     const long long n = 3 * lv(0).nn;
-    vcycle<float>(0, r, z);
+    vcycle_top<float>(r, z);
     if (zcol) {
       hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z64.p, z);
       hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, Ap64.p, r);
@@ -1375,6 +1426,26 @@ struct cm_sys {
 };
 
 namespace {
+// Runs an entry point on the system's own stream, ordered after the work already queued on the
+// caller's stream and before whatever the caller queues next (events, no host wait).
+struct OwnStream {
+  System& s;
+  hipStream_t user;
+  hipEvent_t ev = nullptr;
+  explicit OwnStream(System& sys) : s(sys), user(sys.stream) {
+    if (!s.own) CM_HIP(hipStreamCreateWithFlags(&s.own, hipStreamNonBlocking));
+    CM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    CM_HIP(hipEventRecord(ev, user));
+    CM_HIP(hipStreamWaitEvent(s.own, ev, 0));
+    s.stream = s.own;
+  }
+  ~OwnStream() {
+    (void)hipEventRecord(ev, s.own);
+    (void)hipStreamWaitEvent(user, ev, 0);
+    (void)hipEventDestroy(ev);
+    s.stream = user;
+  }
+};
 template <typename T>
 const T* cp(const void* p) { return static_cast<const T*>(p); }
 template <typename T>
@@ -1586,13 +1657,11 @@ int cm_sys_vcycle(cm_sys* h, int prec, const void* b, void* x) {
   CM_API_BEGIN
   need_built(h);
   System& s = h->s;
-  if (prec == 0) {
-    s.ensure_prec<double>();
-    s.vcycle<double>(0, cp<double>(b), mp<double>(x));
-  } else {
-    s.ensure_prec<float>();
-    s.vcycle<float>(0, cp<float>(b), mp<float>(x));
-  }
+  if (prec == 0) s.ensure_prec<double>();
+  else s.ensure_prec<float>();
+  OwnStream os(s);
+  if (prec == 0) s.vcycle_top<double>(cp<double>(b), mp<double>(x));
+  else s.vcycle_top<float>(cp<float>(b), mp<float>(x));
   CM_API_END
 }
 
@@ -1604,6 +1673,7 @@ int cm_sys_solve(cm_sys* h, int mode, const double* b, double* x, double tol, in
   CM_CHECK(info != nullptr, "info is null");
   System& s = h->s;
   if (check_every < 1) check_every = 5;
+  OwnStream os(s);
   hipEvent_t e0, e1;
   CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
   CM_HIP(hipEventRecord(e0, s.stream));
@@ -1695,6 +1765,7 @@ int cm_sys_autotune(cm_sys* h, int prec, int nrep) {
     s.ensure_prec<float>();
     s.autotune_kz<float>(nrep);
   }
+  s.drop_graphs();
   CM_API_END
 }
 
@@ -1710,6 +1781,15 @@ int cm_sys_set_kz(cm_sys* h, int kz) {
   CM_CHECK(h != nullptr, "null system");
   CM_CHECK(kz >= 0 && kz <= 4096, "kz out of range");
   h->s.kz_override = kz;
+  h->s.drop_graphs();
+  CM_API_END
+}
+
+int cm_sys_set_graphs(cm_sys* h, int on) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  h->s.graphs = on != 0;
+  h->s.drop_graphs();
   CM_API_END
 }
 

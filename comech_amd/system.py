@@ -200,6 +200,10 @@ class VoxelSystem:
     def set_kz(self, kz):
         check(self.lib.cm_sys_set_kz(self.h, int(kz)))
 
+    def set_graphs(self, on=True):
+        """Replay the V-cycle through a captured HIP graph (default) or launch it eagerly."""
+        check(self.lib.cm_sys_set_graphs(self.h, int(bool(on))))
+
     def set_rigid(self, rows, cols, vals, ncol):
         """Rigid-mode correction Z Einv Z^T (fe/gmg.py:127-135): Z as COO over level-0 DOFs.
         E = sym(Z^T A Z) is formed with the GPU operator, Einv = pinv(E) (ncol x ncol, host)."""

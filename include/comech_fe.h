@@ -148,6 +148,12 @@ int cm_sys_timings(cm_sys* sys, double* out, int cap);
 /* Override the z extent of a structured block (0 = automatic). */
 int cm_sys_set_kz(cm_sys* sys, int kz);
 
+/* V-cycle replay through HIP graphs (on by default): cm_sys_solve and cm_sys_vcycle run on the
+ * system's own stream (ordered after the caller's stream by events) and replay the captured
+ * V-cycle while its buffers stay the same.  on = 0 launches every kernel eagerly.  No reference
+ * counterpart (the reference launches eagerly from Python, rb/gpusolve.py:163-171). */
+int cm_sys_set_graphs(cm_sys* sys, int on);
+
 int cm_sys_sync(cm_sys* sys);
 
 /* Analysis-grid resampling of rb/rbsim.py:110-140 (sample_fe).  Per axis sample tables (FE cell

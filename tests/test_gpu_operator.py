@@ -194,3 +194,28 @@ def test_single_level_vcycle_is_dense_solve():
     got = z[info["used"]].ravel()
     assert np.abs(got - ref).max() <= 1e-8 * np.abs(ref).max()
     assert not np.any(z[~used])
+
+
+@pytest.mark.parametrize("dt,tol", [(torch.float64, 1e-12), (torch.float32, 1e-5)])
+def test_vcycle_graph_replays_match_eager(dt, tol):
+    """The V-cycle replayed from its captured HIP graph equals the eagerly launched V-cycle on
+    every replay, with other work (operator applications, torch kernels, host synchronisation)
+    between the replays.  Tolerance: rounding level (the sparse pass accumulates with atomics)."""
+    c, s = _sys2(dims=(24, 16, 16), h=3.0)
+    nn = len(c["orig"])
+    rng = np.random.default_rng(3)
+    r = torch.tensor(rng.standard_normal(3 * nn), dtype=dt, device=DEV)
+    s.set_graphs(False)
+    ref = s.vcycle(r).clone()
+    s.set_graphs(True)
+    scale = ref.abs().max().item()
+    z = torch.empty_like(r)                       # same buffers on every call: one capture, then replays
+    for rep in range(4):
+        z.fill_(float("nan"))
+        s.vcycle(r, z)
+        err = (z - ref).abs().max().item() / scale
+        assert err <= tol, (rep, err)
+        # eager work between replays
+        y = s.apply(r)
+        _ = (y * 2.0 + r).sum().item()
+        torch.cuda.synchronize()

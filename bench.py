@@ -42,6 +42,7 @@ def make_system(c, extra, irr_c, irr_n, kz_cache=None):
     s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
                     mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c, irr_nodes=irr_n,
                     extra=extra[0], rank1=extra[1], device="cuda:0", smoother="chebyshev")
+    s.autotune()          # structured-block z extents: timed once per level shape, then cached
     return s
 
 
@@ -95,7 +96,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--mode", default="mixed", choices=["mixed", "ref", "fp64"])
+    ap.add_argument("--mode", default="ref", choices=["mixed", "ref", "fp64"])
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-only", action="store_true",

@@ -4,7 +4,9 @@
 
 #include <cstdint>
 #include <stdexcept>
+#include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 namespace cm {
@@ -28,30 +30,109 @@ struct Error : std::runtime_error {
 
 #define CM_LAUNCH_CHECK() CM_HIP(hipGetLastError())
 
-// Owning device buffer (hipMalloc / hipFree).
+// Process-wide cache of device blocks.  Blocks come back here only from a destroyed system,
+// after its stream was synchronised (nothing in flight can touch them), and are handed to later
+// allocations of the same size class on the same device: repeated solves (one system per solve,
+// as the engine does per action) skip hipMalloc/hipFree.  Size classes: 8 per octave (at most
+// 12.5 % padding).  At most 16 GiB stay cached per process; a failed hipMalloc empties the cache
+// and retries.
+struct BlockCache {
+  std::mutex mu;
+  std::unordered_map<unsigned long long, std::vector<void*>> free;
+  size_t cached = 0;
+  static constexpr size_t kCap = size_t(16) << 30;
+  static size_t size_class(size_t b) {
+    if (b <= 4096) return 4096;
+    size_t p = 4096;
+    while (p < b) p <<= 1;
+    const size_t q = p >> 4;      // p/2 < b <= p: steps of p/16 = 8 per octave
+    return (b + q - 1) / q * q;
+  }
+  static unsigned long long key(int dev, size_t cls) { return ((unsigned long long)dev << 48) | cls; }
+  void* take(int dev, size_t cls) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = free.find(key(dev, cls));
+    if (it == free.end() || it->second.empty()) return nullptr;
+    void* q = it->second.back();
+    it->second.pop_back();
+    cached -= cls;
+    return q;
+  }
+  bool give(int dev, size_t cls, void* q) {
+    std::lock_guard<std::mutex> g(mu);
+    if (cached + cls > kCap) return false;
+    free[key(dev, cls)].push_back(q);
+    cached += cls;
+    return true;
+  }
+  void flush() {
+    std::lock_guard<std::mutex> g(mu);
+    for (auto& kv : free) {
+      const int dev = (int)(kv.first >> 48);
+      int cur = 0;
+      (void)hipGetDevice(&cur);
+      (void)hipSetDevice(dev);
+      for (void* q : kv.second) (void)hipFree(q);
+      (void)hipSetDevice(cur);
+    }
+    free.clear();
+    cached = 0;
+  }
+};
+inline BlockCache& block_cache() {
+  static BlockCache* c = new BlockCache;   // never destroyed: outlives the HIP runtime teardown
+  return *c;
+}
+// Set while a synchronised system is being destroyed: its buffers go to the cache.
+inline bool& release_to_cache() {
+  static thread_local bool on = false;
+  return on;
+}
+
+// Owning device buffer.
 template <typename T>
 struct DevBuf {
   T* p = nullptr;
   size_t n = 0;
+  size_t cls = 0;   // allocated bytes (size class)
+  int dev = 0;
   DevBuf() = default;
   explicit DevBuf(size_t count) { alloc(count); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), cls(o.cls), dev(o.dev) { o.p = nullptr; o.n = 0; o.cls = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
-    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n; cls = o.cls; dev = o.dev;
+      o.p = nullptr; o.n = 0; o.cls = 0;
+    }
     return *this;
   }
   ~DevBuf() { release(); }
   void alloc(size_t count) {
     release();
     n = count;
-    if (count) CM_HIP(hipMalloc(&p, count * sizeof(T)));
+    if (!count) return;
+    CM_HIP(hipGetDevice(&dev));
+    cls = BlockCache::size_class(count * sizeof(T));
+    void* q = block_cache().take(dev, cls);
+    if (!q) {
+      if (hipMalloc(&q, cls) != hipSuccess) {
+        (void)hipGetLastError();
+        block_cache().flush();
+        CM_HIP(hipMalloc(&q, cls));
+      }
+    }
+    p = static_cast<T*>(q);
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (!(release_to_cache() && block_cache().give(dev, cls, p))) (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
+    cls = 0;
   }
   void upload(const T* host, size_t count, hipStream_t s) {
     alloc(count);

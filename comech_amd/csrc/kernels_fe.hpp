@@ -50,9 +50,21 @@ struct ValueMat {
   static constexpr bool sri = SRI_, iso = ISO_;
 };
 
+// Coarse levels also carry their extra stiffness into the march epilogue:
+//   block rows: node n adds sum_e B_e s[bcol_e] over e in [bptr[n], bptr[n+1]) (3x3 blocks B_e,
+//               row-major) -- the Galerkin-coarsened springs;
+//   rank-one terms k w w^T: node n adds sum_e rnw_e * dots[rnterm_e] (rnw_e = the term's 3
+//               weights at n, dots[t] = k_t w_t . s from k_rank1_dots launched just before).
 template <typename T>
 struct MomentMat {
   const T* __restrict__ mom;         // [54][ncell]: 27 lam moments then 27 mu moments
+  const int* __restrict__ bptr;      // nullptr: no block rows
+  const int* __restrict__ bcol;
+  const T* __restrict__ bval;
+  const int* __restrict__ rnptr;     // nullptr: no rank-one terms
+  const int* __restrict__ rnterm;
+  const T* __restrict__ rnw;
+  const T* __restrict__ dots;
   static constexpr int kind = 2;
   static constexpr bool sri = false, iso = false;
 };
@@ -338,7 +350,26 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
       T au[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) au[c] = cr[0][c] + p1[c] + p2[c] + p3[c];
-      epilogue<T, EM>(ep, (long long)k * lay + (long long)cj * nxn + ci, au, sn);
+      const long long n = (long long)k * lay + (long long)cj * nxn + ci;
+      if constexpr (M::kind == 2) {
+        if (mat.bptr) {
+          for (int e = mat.bptr[n]; e < mat.bptr[n + 1]; ++e) {
+            const long long m = mat.bcol[e];
+            const T* B = mat.bval + 9LL * e;
+            const T s0 = s[3 * m], s1 = s[3 * m + 1], s2 = s[3 * m + 2];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) au[c] += B[3 * c] * s0 + B[3 * c + 1] * s1 + B[3 * c + 2] * s2;
+          }
+        }
+        if (mat.rnptr) {
+          for (int e = mat.rnptr[n]; e < mat.rnptr[n + 1]; ++e) {
+            const T dt = mat.dots[mat.rnterm[e]];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) au[c] += mat.rnw[3LL * e + c] * dt;
+          }
+        }
+      }
+      epilogue<T, EM>(ep, n, au, sn);
     }
 #pragma unroll
     for (int q = 0; q < 12; ++q) ulo[q] = uhi[q];
@@ -426,6 +457,21 @@ __global__ __launch_bounds__(256) void k_sparse(SparseArgs<T> a, GridArgs<T> g, 
   for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
   acc *= a.r1k[t];
   for (int q = q0 + lane; q < q1; q += 64) atomicAdd(&tc[a.r1c[q]], acc * a.r1w[q]);
+}
+
+// dots[t] = k_t (w_t . s) for the rank-one terms of a coarse level: one wavefront per term.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rank1_dots(int nterm, const int* __restrict__ ptr, const int* __restrict__ dof,
+                                                    const T* __restrict__ w, const T* __restrict__ k,
+                                                    const T* __restrict__ s, T* __restrict__ dots) {
+  const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (t >= nterm) return;   // whole wavefronts leave together
+  T acc = T(0);
+  for (int q = ptr[t] + lane; q < ptr[t + 1]; q += 64) acc += w[q] * s[dof[q]];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if (lane == 0) dots[t] = k[t] * acc;
 }
 
 // Fix-up after the structured march, one thread per touched node: adds the sparse part tc (and

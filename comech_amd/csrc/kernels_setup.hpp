@@ -209,6 +209,74 @@ __global__ __launch_bounds__(256) void k_dense_add_rank1(const int* __restrict__
   }
 }
 
+// ---- level-0 cell arrays from the reference-order description ------------------------------
+// Reference cell id c = (i ny + j) nz + k (fe/solver.py:128) -> internal x-fastest id
+// x = i + nx (j + ny k).  Label mode (lab_ref != nullptr): labels 255 = no stiffness, materials
+// from the tables; value mode: per-cell (lam, mu) and activity.  Writes the real labels, FP64
+// materials and activity in x-fastest order.
+__global__ void k_prep_cells(int nx, int ny, int nz, const uint8_t* __restrict__ lab_ref,
+                             const double* __restrict__ tabL, const double* __restrict__ tabM,
+                             const double* __restrict__ lam_ref, const double* __restrict__ mu_ref,
+                             const uint8_t* __restrict__ act_ref, uint8_t* __restrict__ lab_x,
+                             double* __restrict__ matL, double* __restrict__ matM, uint8_t* __restrict__ act_x) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ncell = (long long)nx * ny * nz;
+  if (c >= ncell) return;
+  const long long k = c % nz, j = (c / nz) % ny, i = c / ((long long)ny * nz);
+  const long long x = i + (long long)nx * (j + (long long)ny * k);
+  bool act;
+  double l = 0.0, m = 0.0;
+  if (lab_ref) {
+    const uint8_t lb = lab_ref[c];
+    act = lb != 255;
+    if (act) { l = tabL[lb]; m = tabM[lb]; }
+    lab_x[x] = lb;
+  } else {
+    act = act_ref[c] != 0;
+    if (act) { l = lam_ref[c]; m = mu_ref[c]; }
+  }
+  matL[x] = l;
+  matM[x] = m;
+  act_x[x] = act ? 1 : 0;
+}
+
+// Irregular cells (x ids) leave the structured operator: no label, zero apply materials.
+__global__ void k_prep_irregular(int nirr, const int* __restrict__ irr_x, uint8_t* __restrict__ lab_apply,
+                                 double* __restrict__ applyL, double* __restrict__ applyM, uint8_t* __restrict__ irr) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nirr) return;
+  const long long x = irr_x[e];
+  if (lab_apply) lab_apply[x] = 255;
+  applyL[x] = 0.0;
+  applyM[x] = 0.0;
+  irr[x] = 1;
+}
+
+// Used grid nodes (fe/solver.py:141, rb/splitfe.py:73): corners of active, non-irregular cells.
+__global__ void k_used_grid(int nx, int ny, int nz, const uint8_t* __restrict__ act_x, const uint8_t* __restrict__ irr,
+                            uint8_t* __restrict__ used) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nxn = nx + 1, nyn = ny + 1;
+  if (n >= nxn * nyn * (nz + 1)) return;
+  const int i = (int)(n % nxn), j = (int)((n / nxn) % nyn), k = (int)(n / (nxn * nyn));
+  uint8_t u = 0;
+  for (int c = 0; c < 2; ++c)
+    for (int b = 0; b < 2; ++b)
+      for (int a = 0; a < 2; ++a) {
+        const int ci = i - a, cj = j - b, ck = k - c;
+        if (ci < 0 || ci >= nx || cj < 0 || cj >= ny || ck < 0 || ck >= nz) continue;
+        const long long x = ci + (long long)nx * (cj + (long long)ny * ck);
+        if (act_x[x] && !irr[x]) u = 1;
+      }
+  used[n] = u;
+}
+
+// used[nodes[q]] = 1 (the 8 nodes of every irregular cell; the same value from every writer)
+__global__ void k_mark_nodes(long long n, const int* __restrict__ nodes, uint8_t* __restrict__ used) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) used[nodes[q]] = 1;
+}
+
 // Inverse of the 3x3 blocks (symmetric), stored as 6 values per node: 00 01 02 11 12 22 (the
 // off-diagonal pairs averaged).  Blocks with |det| < 1e-30 become the identity (fe/gmg.py:39-41).
 template <typename T>

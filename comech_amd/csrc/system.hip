@@ -2,10 +2,13 @@
 // Galerkin moment cells on coarse levels, dense FP64 coarsest inverse), V-cycle, PCG drivers and
 // the extern "C" boundary declared in include/comech_fe.h.
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 #include <tuple>
@@ -32,6 +35,7 @@ struct LevelT {
   DevBuf<T> mom;            // coarse levels: [54][ncell]
   DevBuf<T> Dinv;           // [nn][6] symmetric 3x3 block inverses
   DevBuf<T> tc;             // sparse-pass accumulator, 3 per touched node (kept zero between uses)
+  DevBuf<T> bval, rnw, dots;  // coarse levels: block-row values, rank-one node weights, k_t w_t.s
   DevBuf<T> pw[3];          // prolongation weights, this level as the fine side
   DevBuf<T> rw[3];          // restriction weights of the next level's coarse nodes
   DevBuf<T> xval;           // extra stiffness values
@@ -79,6 +83,10 @@ struct Level {
   std::vector<int> tnode_h;
   DevBuf<int> tnode, irr_ci, xrc, r1c;
   double iso_fac = 0.0;       // level 0: h / 16 when all cells are the same cube, else 0
+  // coarse levels: extra stiffness applied in the march epilogue (per-node 3x3 block rows and
+  // per-node rank-one weights)
+  DevBuf<int> bptr, bcol, rnptr, rnterm;
+  std::vector<double> bval_h, rnw_h;
   // level 0 only
   bool label_mode = false;
   DevBuf<uint8_t> lab;                      // apply labels (x-fastest, irregular -> 255)
@@ -156,31 +164,19 @@ struct System {
     const long long ncell = v.ncell;
     CM_CHECK(st_nmat >= 0 || !st_lam.empty(), "materials not set (cm_sys_set_labels or cm_sys_set_cellmat)");
     v.label_mode = st_nmat >= 0;
-    std::vector<double> mL(ncell, 0.0), mM(ncell, 0.0);
-    std::vector<uint8_t> act(ncell, 0), labx;
     if (v.label_mode) {
-      labx.assign(ncell, LAB_NONE);
       v.tabL_h.assign(256, 0.0);
       v.tabM_h.assign(256, 0.0);
       for (int m = 0; m < st_nmat; ++m) { v.tabL_h[m] = st_tabL[m]; v.tabM_h[m] = st_tabM[m]; }
+      for (long long c = 0; c < ncell; ++c)
+        CM_CHECK(st_lab[c] == LAB_NONE || (int)st_lab[c] < st_nmat, "label outside the material table");
     }
     auto xf = [&](long long c) {
       const long long k = c % nz, j = (c / nz) % ny, i = c / ((long long)ny * nz);
       return i + (long long)nx * (j + (long long)ny * k);
     };
-    for (long long c = 0; c < ncell; ++c) {
-      const long long x = xf(c);
-      if (v.label_mode) {
-        const uint8_t l = st_lab[c];
-        if (l == LAB_NONE) continue;
-        CM_CHECK((int)l < st_nmat, "label outside the material table");
-        act[x] = 1; labx[x] = l; mL[x] = st_tabL[l]; mM[x] = st_tabM[l];
-      } else {
-        if (!st_active[c]) continue;
-        act[x] = 1; mL[x] = st_lam[c]; mM[x] = st_mu[c];
-      }
-    }
-    // split copies and irregular cells
+    auto active_ref = [&](long long c) { return v.label_mode ? st_lab[c] != LAB_NONE : st_active[c] != 0; };
+    // split copies and irregular cells (host lists)
     v.ncopy = (int)st_copy_orig.size();
     v.nn = v.nng + v.ncopy;
     v.copy_orig_h.resize(v.ncopy);
@@ -189,47 +185,70 @@ struct System {
       v.copy_orig_h[q] = (int)st_copy_orig[q];
     }
     v.nirr = (int)st_irr_cell.size();
-    std::vector<int> ijk(3 * (size_t)v.nirr), nodes(8 * (size_t)v.nirr);
+    std::vector<int> ijk(3 * (size_t)v.nirr), nodes(8 * (size_t)v.nirr), irr_x(v.nirr);
     v.irrL_h.resize(v.nirr); v.irrM_h.resize(v.nirr);
-    std::vector<uint8_t> is_irr(ncell, 0);
     for (int e = 0; e < v.nirr; ++e) {
       const long long c = st_irr_cell[e];
       CM_CHECK(c >= 0 && c < ncell, "irregular cell id out of range");
-      const long long x = xf(c);
-      CM_CHECK(act[x], "irregular cell is not active");
-      is_irr[x] = 1;
+      CM_CHECK(active_ref(c), "irregular cell is not active");
+      irr_x[e] = (int)xf(c);
       ijk[3 * e] = (int)(c / ((long long)ny * nz)); ijk[3 * e + 1] = (int)((c / nz) % ny); ijk[3 * e + 2] = (int)(c % nz);
-      v.irrL_h[e] = mL[x]; v.irrM_h[e] = mM[x];
+      v.irrL_h[e] = v.label_mode ? st_tabL[st_lab[c]] : st_lam[c];
+      v.irrM_h[e] = v.label_mode ? st_tabM[st_lab[c]] : st_mu[c];
       for (int l = 0; l < 8; ++l) {
         const long long nd = st_irr_nodes[8 * (size_t)e + l];
         CM_CHECK(nd >= 0 && nd < v.nn, "irregular node id out of range");
         nodes[8 * (size_t)e + l] = (int)nd;
       }
     }
-    // used nodes (fe/solver.py:141, rb/splitfe.py:73): corners of active cells
-    v.used_h.assign(v.nn, 0);
-    const long long nxn = nx + 1, nyn = ny + 1;
-    for (int k = 0; k < nz; ++k)
-      for (int j = 0; j < ny; ++j)
-        for (int i = 0; i < nx; ++i) {
-          const long long x = i + (long long)nx * (j + (long long)ny * k);
-          if (!act[x] || is_irr[x]) continue;
-          for (int l = 0; l < 8; ++l)
-            v.used_h[(i + (l & 1)) + nxn * ((j + ((l >> 1) & 1)) + nyn * (k + (l >> 2)))] = 1;
-        }
-    for (size_t q = 0; q < nodes.size(); ++q) v.used_h[nodes[q]] = 1;
+    // cell arrays on the GPU: x-fastest real labels/materials, apply arrays without the
+    // irregular cells, used nodes (corners of active regular cells + irregular cells' nodes)
+    {
+      DevBuf<uint8_t> lab_ref, act_ref, actx, irr;
+      DevBuf<double> tl, tm, lam_ref, mu_ref;
+      v.matL.alloc(ncell); v.matM.alloc(ncell);
+      actx.alloc(ncell);
+      if (v.label_mode) {
+        lab_ref.upload(st_lab, stream);
+        tl.upload(v.tabL_h, stream); tm.upload(v.tabM_h, stream);
+        v.lab.alloc(ncell);
+      } else {
+        lam_ref.upload(st_lam, stream); mu_ref.upload(st_mu, stream); act_ref.upload(st_active, stream);
+      }
+      hipLaunchKernelGGL(k_prep_cells, grid1(ncell), dim3(256), 0, stream, nx, ny, nz, lab_ref.p, tl.p, tm.p,
+                         lam_ref.p, mu_ref.p, act_ref.p, v.lab.p, v.matL.p, v.matM.p, actx.p);
+      CM_LAUNCH_CHECK();
+      v.applyL.alloc(ncell); v.applyM.alloc(ncell);
+      CM_HIP(hipMemcpyAsync(v.applyL.p, v.matL.p, ncell * sizeof(double), hipMemcpyDeviceToDevice, stream));
+      CM_HIP(hipMemcpyAsync(v.applyM.p, v.matM.p, ncell * sizeof(double), hipMemcpyDeviceToDevice, stream));
+      irr.alloc(ncell);
+      hipLaunchKernelGGL(k_fill<uint8_t>, grid1(ncell), dim3(256), 0, stream, irr.p, ncell, uint8_t(0));
+      if (v.nirr) {
+        DevBuf<int> ix;
+        ix.upload(irr_x, stream);
+        hipLaunchKernelGGL(k_prep_irregular, grid1(v.nirr), dim3(256), 0, stream, v.nirr, ix.p,
+                           v.label_mode ? v.lab.p : nullptr, v.applyL.p, v.applyM.p, irr.p);
+        CM_LAUNCH_CHECK();
+        CM_HIP(hipStreamSynchronize(stream));
+      }
+      DevBuf<uint8_t> used(v.nn);
+      if (v.nn > v.nng)
+        hipLaunchKernelGGL(k_fill<uint8_t>, grid1(v.nn - v.nng), dim3(256), 0, stream, used.p + v.nng, v.nn - v.nng,
+                           uint8_t(0));
+      hipLaunchKernelGGL(k_used_grid, grid1(v.nng), dim3(256), 0, stream, nx, ny, nz, actx.p, irr.p, used.p);
+      v.irr_nodes.upload(nodes, stream);
+      if (!nodes.empty())
+        hipLaunchKernelGGL(k_mark_nodes, grid1((long long)nodes.size()), dim3(256), 0, stream, (long long)nodes.size(),
+                           v.irr_nodes.p, used.p);
+      CM_LAUNCH_CHECK();
+      v.used_h.resize(v.nn);
+      CM_HIP(hipMemcpyAsync(v.used_h.data(), used.p, v.nn, hipMemcpyDeviceToHost, stream));
+      CM_HIP(hipStreamSynchronize(stream));
+    }
     if (!st_used_extra.empty()) {
       CM_CHECK((long long)st_used_extra.size() == v.nn, "used mask length must equal grid nodes + copies");
       for (long long q = 0; q < v.nn; ++q) v.used_h[q] |= st_used_extra[q];
     }
-    // apply arrays: irregular cells are handled by their own kernel
-    std::vector<double> aL(mL), aM(mM);
-    for (long long x = 0; x < ncell; ++x)
-      if (is_irr[x]) { aL[x] = 0.0; aM[x] = 0.0; if (v.label_mode) labx[x] = LAB_NONE; }
-    v.matL.upload(mL, stream); v.matM.upload(mM, stream);
-    v.applyL.upload(aL, stream); v.applyM.upload(aM, stream);
-    if (v.label_mode) v.lab.upload(labx, stream);
-    v.irr_nodes.upload(nodes, stream);
     v.irr_ijk.upload(ijk, stream);
     v.irr_nodes_h.swap(nodes);
     v.copy_orig.upload(v.copy_orig_h, stream);
@@ -600,6 +619,57 @@ struct System {
     const auto it = std::lower_bound(sorted.begin(), sorted.end(), (int)node);
     return (int)(it - sorted.begin());
   }
+  // Coarse levels: the extra-stiffness CSR (rows sorted and unique) regrouped as per-node 3x3
+  // block rows, and the rank-one terms as per-node weight lists; both are applied by the march
+  // epilogue of the node's own thread, so coarse levels need no sparse pass.
+  void build_coarse_rows(Level& v) {
+    const long long nn = v.nn;
+    v.ntouch = 0;
+    if (!v.xrow_h.empty()) {
+      std::vector<int> bptr(nn + 1, 0), bcol;
+      v.bval_h.clear();
+      std::map<int, std::array<double, 9>> blocks;
+      const size_t nrow = v.xrow_h.size();
+      for (size_t r = 0; r < nrow;) {
+        const int node = v.xrow_h[r] / 3;
+        blocks.clear();
+        for (; r < nrow && v.xrow_h[r] / 3 == node; ++r) {
+          const int comp = v.xrow_h[r] % 3;
+          for (int q = v.xptr_h[r]; q < v.xptr_h[r + 1]; ++q) blocks[v.xcol_h[q] / 3][3 * comp + v.xcol_h[q] % 3] += v.xval_h[q];
+        }
+        bptr[node + 1] = (int)blocks.size();
+        for (auto& kv : blocks) {
+          bcol.push_back(kv.first);
+          v.bval_h.insert(v.bval_h.end(), kv.second.begin(), kv.second.end());
+        }
+      }
+      for (long long n = 0; n < nn; ++n) bptr[n + 1] += bptr[n];
+      v.bptr.upload(bptr, stream);
+      v.bcol.upload(bcol, stream);
+    }
+    if (!v.r1k_h.empty()) {
+      struct Ent { int node, term; double w[3]; };
+      std::vector<Ent> ent;
+      for (size_t t = 0; t + 1 < v.r1ptr_h.size(); ++t)
+        for (int q = v.r1ptr_h[t]; q < v.r1ptr_h[t + 1]; ++q) {
+          const int node = v.r1dof_h[q] / 3;
+          if (ent.empty() || ent.back().node != node || ent.back().term != (int)t) ent.push_back(Ent{node, (int)t, {0, 0, 0}});
+          ent.back().w[v.r1dof_h[q] % 3] += v.r1w_h[q];
+        }
+      std::stable_sort(ent.begin(), ent.end(), [](const Ent& x, const Ent& y) { return x.node < y.node; });
+      std::vector<int> rnptr(nn + 1, 0), rnterm(ent.size());
+      v.rnw_h.resize(3 * ent.size());
+      for (size_t e = 0; e < ent.size(); ++e) {
+        ++rnptr[ent[e].node + 1];
+        rnterm[e] = ent[e].term;
+        for (int c = 0; c < 3; ++c) v.rnw_h[3 * e + c] = ent[e].w[c];
+      }
+      for (long long n = 0; n < nn; ++n) rnptr[n + 1] += rnptr[n];
+      v.rnptr.upload(rnptr, stream);
+      v.rnterm.upload(rnterm, stream);
+    }
+  }
+
   void build_touch(Level& v) {
     std::vector<int> nodes;
     nodes.reserve(v.irr_nodes_h.size() + v.xrow_h.size() + v.r1dof_h.size() + (size_t)(v.nn - v.nng));
@@ -684,7 +754,8 @@ struct System {
       v.r1dof.upload(v.r1dof_h, stream);
       v.r1w64.upload(v.r1w_h, stream);
       v.r1k64.upload(v.r1k_h, stream);
-      build_touch(v);
+      if (l == 0) build_touch(v);
+      else build_coarse_rows(v);
     }
     prepare_prec<double>(0, false);
     lap(2);
@@ -861,6 +932,11 @@ struct System {
       }
       if (!v.xval_h.empty()) upload_cast(d.xval, v.xval_h, stream);
       if (!v.r1k_h.empty()) { upload_cast(d.r1w, v.r1w_h, stream); upload_cast(d.r1k, v.r1k_h, stream); }
+      if (!v.bval_h.empty()) upload_cast(d.bval, v.bval_h, stream);
+      if (!v.rnw_h.empty()) {
+        upload_cast(d.rnw, v.rnw_h, stream);
+        d.dots.alloc(v.r1k_h.size());
+      }
       const size_t nv = 3 * (size_t)v.nn;
       d.x.alloc(nv); d.b.alloc(nv); d.r.alloc(nv); d.t.alloc(nv); d.d.alloc(nv);
       if (v.ntouch) {
@@ -907,6 +983,11 @@ struct System {
     LevelT<T>& d = v.dat<T>();
     const GridArgs<T> g = grid_args<T>(l);
     if (v.ntouch) launch_sparse<T>(v, d, g, s);
+    if (l > 0 && v.rnptr.p) {
+      const int nterm = (int)v.r1k_h.size();
+      hipLaunchKernelGGL(k_rank1_dots<T>, grid1(64LL * nterm), dim3(256), 0, stream, nterm, v.r1ptr.p, v.r1dof.p,
+                         d.r1w.p, d.r1k.p, s, d.dots.p);
+    }
     launch_march<T, EM>(l, s, ep);
     if (v.ntouch) {
       hipLaunchKernelGGL((k_fixup<T, EM>), grid1(v.ntouch), dim3(256), 0, stream, v.ntouch, v.tnode.p, v.nng, d.tc.p, s,
@@ -954,7 +1035,7 @@ struct System {
     const TileMap tm{ntx, nty, ntz, (int)(nb / 8), (int)(nb % 8)};
     const dim3 grid((unsigned)nb), blk(BLOCK);
     if (l > 0) {
-      MomentMat<T> m{d.mom.p};
+      MomentMat<T> m{d.mom.p, v.bptr.p, v.bcol.p, d.bval.p, v.rnptr.p, v.rnterm.p, d.rnw.p, d.dots.p};
       hipLaunchKernelGGL((k_march<T, MomentMat<T>, EM>), grid, blk, 0, stream, g, m, s, ep, kz, tm);
       return;
     }
@@ -994,23 +1075,55 @@ struct System {
     const int tuned = v.kz[std::is_same<T, float>::value ? 1 : 0];
     if (tuned > 0) return tuned;
     const long long xy = (long long)div_up(v.nx + 1, TOUT) * div_up(v.ny + 1, TOUT);
-    // enough blocks to fill 256 CUs several times over; short z chains on small levels
+    // the longest z chain that still gives >= 600 blocks (about 2.3 blocks per CU); measured on
+    // MI355X this is the fastest or within 5 % of it on every level of the hindfoot raster
     for (int kz : {32, 16, 8, 4, 2}) {
-      if (xy * div_up(v.nz + 1, kz) >= 1024) return kz;
+      if (xy * div_up(v.nz + 1, kz) >= 600) return kz;
     }
     return 1;
   }
 
   // Tile autotuner: time the structured kernel of every level for each z extent and keep the
-  // fastest (HIP events, on this system's stream).
+  // fastest (HIP events, on this system's stream).  Results are kept process-wide per level shape
+  // and kernel variant, so later systems of the same shape skip the timing.
+  struct KzKey {
+    int nx, ny, nz, variant;
+    bool operator==(const KzKey& o) const { return nx == o.nx && ny == o.ny && nz == o.nz && variant == o.variant; }
+  };
+  struct KzHash {
+    size_t operator()(const KzKey& k) const {
+      return ((size_t)k.nx * 73856093u) ^ ((size_t)k.ny * 19349663u) ^ ((size_t)k.nz * 83492791u) ^ (size_t)k.variant;
+    }
+  };
+  static std::mutex& kz_mu() { static std::mutex m; return m; }
+  static std::unordered_map<KzKey, int, KzHash>& kz_cache() {
+    static auto* c = new std::unordered_map<KzKey, int, KzHash>;
+    return *c;
+  }
+  template <typename T>
+  int kz_variant(int l) const {
+    const Level& v = *L[l];
+    const int prec = std::is_same<T, float>::value ? 1 : 0;
+    const int kind = l > 0 ? 3 : (v.label_mode ? 1 : 2);
+    return prec + 2 * (kind + 4 * ((l == 0 && sri) + 2 * (l == 0 && v.iso_fac > 0.0)));
+  }
   template <typename T>
   void autotune_kz(int nrep) {
     hipEvent_t e0, e1;
     CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
+    const int slot = std::is_same<T, float>::value ? 1 : 0;
     for (int l = 0; l < nlev() - 1; ++l) {
       Level& v = lv(l);
       LevelT<T>& d = v.dat<T>();
-      const int slot = std::is_same<T, float>::value ? 1 : 0;
+      const KzKey key{v.nx, v.ny, v.nz, kz_variant<T>(l)};
+      {
+        std::lock_guard<std::mutex> g(kz_mu());
+        auto it = kz_cache().find(key);
+        if (it != kz_cache().end()) {
+          v.kz[slot] = it->second;
+          continue;
+        }
+      }
       float best = 1e30f;
       int best_kz = 0;
       for (int kz : {1, 2, 4, 8, 16, 32}) {
@@ -1026,6 +1139,8 @@ struct System {
         if (ms < best) { best = ms; best_kz = kz; }
       }
       v.kz[slot] = best_kz;
+      std::lock_guard<std::mutex> g(kz_mu());
+      kz_cache()[key] = best_kz;
     }
     CM_HIP(hipEventDestroy(e0)); CM_HIP(hipEventDestroy(e1));
   }
@@ -1492,8 +1607,12 @@ int cm_sys_create(cm_sys** out, void* stream, int nx, int ny, int nz, const doub
 int cm_sys_destroy(cm_sys* h) {
   CM_API_BEGIN
   if (h) {
-    (void)hipStreamSynchronize(h->s.stream);
+    // nothing of this system is in flight after these waits, so its buffers may be reused
+    const bool idle = hipStreamSynchronize(h->s.stream) == hipSuccess &&
+                      (h->s.own == nullptr || hipStreamSynchronize(h->s.own) == hipSuccess);
+    cm::release_to_cache() = idle;
     delete h;
+    cm::release_to_cache() = false;
   }
   CM_API_END
 }

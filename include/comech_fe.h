@@ -138,7 +138,9 @@ int cm_sys_time_apply(cm_sys* sys, int level, int prec, const void* u, void* y, 
                       double* avg_ms);
 
 /* Tile autotuner: for every level, time the structured stencil kernel in precision `prec` for
- * z extents 1..32 (nrep launches each, HIP events) and keep the fastest. */
+ * z extents 1..32 (nrep launches each, HIP events) and keep the fastest.  Results are cached
+ * process-wide per level shape and kernel variant; a cached level is not timed again.  No
+ * reference counterpart (project autotuner). */
 int cm_sys_autotune(cm_sys* sys, int prec, int nrep);
 
 /* Build stage times in ms: level-0 preparation, hierarchy structure, uploads, coarse moments,

@@ -2,14 +2,18 @@
 """Benchmark of the MI355X voxel FE solve path (one GPU).
 
 Workload: the synthetic hindfoot raster (160x96x96 voxels at 0.5 mm, ~2.38M active DOFs, split
-nodes with interface springs, 2000 ground springs, 200 ligament springs, 700 N load), solved to
-relative residual 1e-8 in mixed precision (FP32 operator, V-cycle and Krylov vectors; FP64 true
-residual, alpha/beta reductions and coarse solve).  One step = one complete solve: multigrid setup
-on the GPU from the level-0 description, PCG, and an FP64 true-residual check of the answer.
+nodes with interface springs, 2000 ground springs, 200 ligament springs, 700 N load) under the 8
+surgical actions (baseline, resection, osteotomy, interposition, fusion, load x0.5/x1.5/x2), each
+solved to relative residual 1e-8 with mixed-precision GMG-PCG: FP32 operator, smoothers, transfers
+and V-cycle; FP64 Krylov vectors, alpha/beta reductions and dense coarsest solve; every answer
+checked by an FP64 true residual.  One step = one complete solve (GPU multigrid setup from the
+level-0 description, PCG, check); ``--concurrency`` solves are in flight at once (batched).
 
-Prints ONE JSON line.  ``value`` = solves/s; ``roofline`` = the FP32 fine-level stencil kernel
-(the matvec) against the 8 TB/s HBM peak; ``cpu_baseline`` = the FP64 CPU oracle (assembled CSR,
-Chebyshev GMG-PCG, SuperLU coarse solve -- the reference algorithm) on a bounded sample.
+Prints ONE JSON line.  ``value`` = solves/s (batched); ``sequential`` = the same solves one at a
+time; ``plain_fp64`` = all-FP64 GMG-PCG batched the same way; ``roofline`` = the FP32 fine-level
+stencil kernel (the matvec) against the 8 TB/s HBM peak; ``cpu_baseline`` = the FP64 CPU oracle
+(assembled CSR, Chebyshev GMG-PCG, SuperLU coarse solve -- the reference algorithm) on a bounded
+sample.
 """
 import argparse
 import glob
@@ -79,7 +83,7 @@ def cpu_baseline(sample_dims=(80, 48, 48)):
                         f"{info['iters']} iterations, extrapolated x{scale:.0f} in cell count"))
 
 
-def pmc_traffic(kernel="k_apply_fine"):
+def pmc_traffic(kernel="k_march"):
     """HBM bytes per launch of the fine stencil from the committed rocprofv3 --pmc summary."""
     for p in sorted(glob.glob(os.path.join(ROOT, "profiles", "pmc_*.json"))):
         try:
@@ -91,11 +95,25 @@ def pmc_traffic(kernel="k_apply_fine"):
     return None
 
 
+def action_inputs(actions):
+    """Inputs of the hindfoot raster under each surgical action (same anatomy seed)."""
+    from comech_amd import synth
+    from comech_amd.engine import split_cells
+    out = []
+    for a in actions:
+        c = synth.config2(dims=(160, 96, 96), h=0.5, action=a)
+        extra, terms, f = synth.solve_inputs_config2(c)
+        irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
+        out.append((c, (extra, terms), f, irr_c, irr_n))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8, help="solves in the timed region (one per surgical action, cycled)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes over the action set")
+    ap.add_argument("--concurrency", type=int, default=4, help="solves in flight on the GPU")
     ap.add_argument("--mode", default="ref", choices=["mixed", "ref", "fp64"])
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -109,14 +127,13 @@ def main():
         sys.exit(2)
     import numpy as np
     import torch
+    from comech_amd.batch import run_batch
+    from comech_amd.synth import ACTIONS
     torch.cuda.init()
     dev = torch.device("cuda:0")
     t0 = time.time()
-    c, extra, f, irr_c, irr_n = build_inputs()
-    f_dev = torch.from_numpy(f).to(dev)
-    log(f"inputs ready in {time.time() - t0:.1f} s: ncell {c['grid'].ncell}, active {int(c['active'].sum())}")
-
     if args.kernel_only:
+        c, extra, f, irr_c, irr_n = build_inputs()
         s = make_system(c, extra, irr_c, irr_n)
         u = torch.randn(3 * s.nn, dtype=torch.float32, device=dev)
         y = torch.empty_like(u)
@@ -131,28 +148,41 @@ def main():
         log("kernel-only probe done")
         return
 
+    inputs = action_inputs(ACTIONS)
+    f_devs = [torch.from_numpy(i[2]).to(dev) for i in inputs]
+    c0 = inputs[0][0]
+    log(f"inputs ready in {time.time() - t0:.1f} s: {len(inputs)} actions, ncell {c0['grid'].ncell}, "
+        f"active {int(c0['active'].sum())}")
+
+    def job(i, mode):
+        c, extra, f, irr_c, irr_n = inputs[i % len(inputs)]
+        return lambda: one_solve(c, extra, f_devs[i % len(inputs)], irr_c, irr_n, mode, args.tol)
+
+    def timed(mode, n, conc):
+        res, wall = run_batch([job(i, mode) for i in range(n)], concurrency=conc, device=dev)
+        return res, wall
+
     for _ in range(args.warmup):
-        info = one_solve(c, extra, f_dev, irr_c, irr_n, args.mode, args.tol)
-        log("warmup", info)
-    torch.cuda.synchronize()
-    t0 = time.time()
-    infos = [one_solve(c, extra, f_dev, irr_c, irr_n, args.mode, args.tol) for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    dt = time.time() - t0
+        res, _ = timed(args.mode, len(inputs), args.concurrency)
+        log("warmup", [round(r["t_solve_ms"], 1) for r in res])
+    infos, dt = timed(args.mode, args.steps, args.concurrency)
     for i in infos:
-        log("step", i)
+        log("step", {k: i[k] for k in ("iters", "flag", "outer", "levels", "relres", "true_relres", "t_setup_ms",
+                                       "t_solve_ms", "mode")})
     worst = max(i["true_relres"] for i in infos)
     if worst > args.tol * 1.0001 or any(i["flag"] for i in infos):
         log(f"bench.py: a solve missed rtol {args.tol}: {worst}")
         sys.exit(1)
-
-    # plain FP64 GMG-PCG on the same GPU (the reference method), same workload
-    t1 = time.time()
-    plain = [one_solve(c, extra, f_dev, irr_c, irr_n, "fp64", args.tol) for _ in range(max(1, args.steps // 2))]
-    torch.cuda.synchronize()
-    plain_sps = len(plain) / (time.time() - t1)
+    # the same jobs one at a time, and plain FP64 GMG-PCG (the reference method) batched the same way
+    seq, dt_seq = timed(args.mode, len(inputs), 1)
+    plain, dt_plain = timed("fp64", args.steps, args.concurrency)
+    sps = args.steps / dt
+    plain_sps = args.steps / dt_plain
+    iters = [i["iters"] for i in infos]
+    dof_it = sum(i["ndof"] * i["iters"] for i in infos) / dt
 
     # roofline of the dominant kernel: FP32 fine-level stencil (HIP events on the solver's stream)
+    c, extra, f, irr_c, irr_n = inputs[0]
     s = make_system(c, extra, irr_c, irr_n)
     g = c["grid"]
     u = torch.randn(3 * s.nn, dtype=torch.float32, device=dev)
@@ -165,7 +195,7 @@ def main():
     achieved = alg_bytes / (ms * 1e-3) / 1e9
     line = {
         "metric": "solves/s to rtol 1e-8 (batched) + matvec HBM GB/s; learned/mixed vs plain",
-        "value": round(args.steps / dt, 4),
+        "value": round(sps, 4),
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -176,17 +206,23 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "hindfoot raster 160x96x96 @0.5mm, mixed-precision GMG-PCG to rtol 1e-8 "
-                               "(configs[1])",
-                   "dofs": infos[0]["ndof"], "mode": args.mode,
-                   "iters": infos[-1]["iters"], "levels": infos[-1]["levels"],
-                   "t_setup_ms": round(infos[-1]["t_setup_ms"], 2), "t_solve_ms": round(infos[-1]["t_solve_ms"], 2),
-                   "true_relres_max": worst},
-        "plain_fp64": {"value": round(plain_sps, 4), "unit": "solves/s", "iters": plain[-1]["iters"],
-                       "ratio_mixed_over_plain": round((args.steps / dt) / plain_sps, 3)},
+        "config": {"workload": "hindfoot raster 160x96x96 @0.5mm under the 8 surgical actions, mixed-precision "
+                               "GMG-PCG to rtol 1e-8 checked by an FP64 true residual; one step = one solve "
+                               "(GPU multigrid setup + PCG + check)",
+                   "dofs": infos[0]["ndof"], "mode": args.mode, "concurrency": args.concurrency,
+                   "iters_mean": round(float(np.mean(iters)), 1), "levels": infos[-1]["levels"],
+                   "t_setup_ms": round(float(np.mean([i["t_setup_ms"] for i in infos])), 2),
+                   "t_solve_ms": round(float(np.mean([i["t_solve_ms"] for i in infos])), 2),
+                   "true_relres_max": worst,
+                   "dof_iterations_per_s": round(dof_it, 1)},
+        "sequential": {"value": round(len(inputs) / dt_seq, 4), "unit": "solves/s", "concurrency": 1},
+        "plain_fp64": {"value": round(plain_sps, 4), "unit": "solves/s", "concurrency": args.concurrency,
+                       "iters_mean": round(float(np.mean([i["iters"] for i in plain])), 1),
+                       "ratio_mixed_over_plain": round(sps / plain_sps, 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(),
-                     "kernel": "k_apply_fine<float,label,SRI> (fine-level K*u)", "avg_launch_us": round(ms * 1e3, 2),
+                     "kernel": "k_march<float, LabelMat<float,SRI,ISO>, APPLY> (fine-level K*u)",
+                     "avg_launch_us": round(ms * 1e3, 2),
                      "alg_bytes_per_launch": alg_bytes, "fp64_avg_launch_us": round(ms64 * 1e3, 2)},
     }
     if not args.no_cpu_baseline:

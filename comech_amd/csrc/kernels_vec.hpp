@@ -111,10 +111,11 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_xr(long long n, TX* __restr
 // beta = S[rz_new] / S[rz_old];  p = z + beta p
 template <typename T>
 __global__ void k_pcg_p(long long n, T* __restrict__ p, const T* __restrict__ z, const double* __restrict__ S,
-                        int s_new, int s_old) {
+                        int s_new, int s_old, int s_sub = -1) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double beta = S[s_new] / S[s_old];
+  // beta = (r.z) / (r_old.z_old); flexible (Polak-Ribiere) form subtracts r.z_old (slot s_sub)
+  const double beta = (S[s_new] - (s_sub >= 0 ? S[s_sub] : 0.0)) / S[s_old];
   p[i] = T((double)z[i] + beta * (double)p[i]);
 }
 

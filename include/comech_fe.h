@@ -33,7 +33,7 @@ typedef struct cm_sys cm_sys;
 typedef struct cm_solve_info {
   int iters;        /* PCG iterations (all inner iterations in mixed mode) */
   int flag;         /* 0 converged, else maxiter (rb/gpusolve.py:252-254) */
-  int outer;        /* refinement steps (mixed mode), 1 otherwise */
+  int outer;        /* refinement steps (mixed mode), recurrence starts otherwise */
   int levels;       /* multigrid levels */
   double relres;    /* ||b - A x|| / ||b|| (FP64) at exit */
   double t_solve_ms;
@@ -114,7 +114,13 @@ int cm_sys_vcycle(cm_sys* sys, int prec, const void* b, void* x);
  * PCG (FP32 operator, V-cycle and vectors, FP64 alpha/beta reductions, FP64 coarse solve) to
  * relative tolerance inner_tol, x += d in FP64; convergence is decided on the FP64 residual.
  * The residual is checked at iterations < check_every and at multiples of check_every
- * (rb/gpusolve.py:244).  info->outer counts refinement steps. */
+ * (rb/gpusolve.py:244).  Modes 0 and 1 confirm a converged recurrence with the FP64 true
+ * residual and, if that is still above tol (stiff systems), restart the recurrence from it
+ * (residual replacement) in flexible (Polak-Ribiere) form; the reference stops on the recurrence
+ * alone.  info->outer counts
+ * refinement steps (mode 2) or recurrence starts (modes 0, 1); info->relres is the true
+ * residual.  A restart that twice fails to halve the true residual ends the solve with
+ * flag = iterations done (not converged). */
 int cm_sys_solve(cm_sys* sys, int mode, const double* b, double* x, double tol, int maxiter, int check_every,
                  double inner_tol, cm_solve_info* info);
 

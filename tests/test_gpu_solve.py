@@ -167,3 +167,42 @@ def test_x0_warm_start(ref1):
     U1, info1 = FS.solve(c["grid"], c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="gmg",
                          tol=1e-10, device=DEV)
     assert rel(U, Uref) <= U_TOL and info["iters"] < info1["iters"]
+
+
+def test_batched_concurrent_solves_match_sequential():
+    """Several systems (hindfoot raster, 4 surgical actions, uneven grids) solved with 3 in flight
+    on one GPU equal the same solves run one after another (relative L2 1e-6, the displacement
+    tolerance; the sparse pass accumulates with atomics, so runs agree to solver tolerance)."""
+    import torch
+    from comech_amd import synth
+    from comech_amd.batch import run_batch
+    from comech_amd.engine import split_cells
+    from comech_amd.system import VoxelSystem
+
+    cases = []
+    for i, (dims, act) in enumerate([((24, 16, 16), "baseline"), ((28, 16, 18), "resection"),
+                                     ((24, 18, 16), "fusion"), ((26, 16, 16), "load2")]):
+        c = synth.config2(dims=dims, h=3.0, seed=2100 + i, action=act)
+        extra, terms, f = synth.solve_inputs_config2(c)
+        irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
+        cases.append((c, extra, terms, f, irr_c, irr_n))
+
+    def job(case):
+        c, extra, terms, f, irr_c, irr_n = case
+
+        def run():
+            s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
+                            mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c,
+                            irr_nodes=irr_n, extra=extra, rank1=terms, device=DEV)
+            b = torch.from_numpy(f).to(DEV)
+            x, info = s.solve(b, tol=1e-8, maxiter=2000, mode="ref")
+            out = x.cpu().numpy()
+            s.close()
+            return out, info
+        return run
+
+    seq = [job(cs)() for cs in cases]
+    par, wall = run_batch([job(cs) for cs in cases], concurrency=3, device=DEV)
+    for (xs, infs), (xp, infp) in zip(seq, par):
+        assert infs["flag"] == 0 and infp["flag"] == 0
+        assert np.linalg.norm(xp - xs) <= 1e-6 * np.linalg.norm(xs)

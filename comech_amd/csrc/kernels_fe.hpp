@@ -198,7 +198,10 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     axy = hx * hy;
   }
   // staging plan (fixed for all layers): scalar e = tid + m*BLOCK of a layer is node row r,
-  // entry q of the row's 3*TN contiguous scalars
+  // entry q of the row's 3*TN contiguous scalars.  Branch-free: a slot outside the grid loads an
+  // in-bounds address and keeps 0; a slot past the layer (e >= NST, only in the last round)
+  // writes the unused 4th value of a node's LDS entry.
+  static_assert((NSTP - 1) * BLOCK < NST && NST <= NSTP * BLOCK, "staging rounds");
   int goff[NSTP], loff[NSTP];
   bool sv[NSTP];
 #pragma unroll
@@ -208,20 +211,23 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     const int gj = J0 - 1 + r, gi = I0 - 1 + q / 3;
     sv[m] = e < NST && gj >= 0 && gj <= ny && gi >= 0 && gi <= nx;
     goff[m] = sv[m] ? (int)((gj * nxn + gi) * 3 + q % 3) : 0;
-    loff[m] = e < NST ? (r * TN + q / 3) * 4 + q % 3 : -1;
+    loff[m] = e < NST ? (r * TN + q / 3) * 4 + q % 3 : (tid % (TN * TN)) * 4 + 3;
   }
   T pre[NSTP];
   auto gload = [&](int kk) {
     const bool kin = (kk >= 0 && kk <= nz);
     const long long base = kin ? (long long)kk * lay * 3 : 0;
 #pragma unroll
-    for (int m = 0; m < NSTP; ++m) pre[m] = (kin && sv[m]) ? s[base + goff[m]] : T(0);
+    for (int m = 0; m < NSTP; ++m) {
+      const bool ok = kin && sv[m];
+      const T v = s[ok ? base + goff[m] : 0];
+      pre[m] = ok ? v : T(0);
+    }
   };
   auto lstore = [&](int kk) {
     T* dst = s_u[kk & 1];
 #pragma unroll
-    for (int m = 0; m < NSTP; ++m)
-      if (loff[m] >= 0) dst[loff[m]] = pre[m];
+    for (int m = 0; m < NSTP; ++m) dst[loff[m]] = pre[m];
   };
   // xy transform of the cell's face on node layer kk: f[4*comp + (mx + 2 my)]
   auto face = [&](int kk, T* f) {
@@ -239,15 +245,16 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   uint8_t labn = LAB_NONE;
   T lamn = T(0), mun = T(0);
   auto mload = [&](int kk) {
-    labn = LAB_NONE;
-    lamn = T(0);
-    mun = T(0);
-    if (!(cin && kk >= 0 && kk < nz)) return;
-    const long long cid = cxy + ncl * kk;
-    if constexpr (M::kind == 0) labn = mat.lab[cid];
+    const bool ok = cin && kk >= 0 && kk < nz;
+    const long long cid = ok ? cxy + ncl * kk : 0;
+    if constexpr (M::kind == 0) {
+      const uint8_t lb = mat.lab[cid];
+      labn = ok ? lb : LAB_NONE;
+    }
     if constexpr (M::kind == 1) {
-      lamn = mat.cl[cid];
-      mun = mat.cm[cid];
+      const T l = mat.cl[cid], m = mat.cm[cid];
+      lamn = ok ? l : T(0);
+      mun = ok ? m : T(0);
     }
   };
 

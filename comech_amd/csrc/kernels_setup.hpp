@@ -18,7 +18,7 @@ struct AxisNest {
 
 // Level-1 moments from level-0 cells.  Fine cell materials come as (lam, mu) per fine cell in
 // x-fastest order (inactive = 0).  sri: lam integrated at the fine cell centre.
-__global__ void k_moments_from_fine(int ncx, int ncy, int ncz, int nfx, int nfy, AxisNest ax, AxisNest ay, AxisNest az,
+__global__ __launch_bounds__(256) void k_moments_from_fine(int ncx, int ncy, int ncz, int nfx, int nfy, AxisNest ax, AxisNest ay, AxisNest az,
                                     const double* __restrict__ hx, const double* __restrict__ hy,
                                     const double* __restrict__ hz, const double* __restrict__ lam,
                                     const double* __restrict__ mu, int sri, double* __restrict__ mom) {
@@ -55,7 +55,7 @@ __global__ void k_moments_from_fine(int ncx, int ncy, int ncz, int nfx, int nfy,
 }
 
 // Level l+1 moments from level-l moments (shift of the child moments into the parent).
-__global__ void k_moments_from_moments(int ncx, int ncy, int ncz, int nfx, int nfy, int nfz, AxisNest ax, AxisNest ay,
+__global__ __launch_bounds__(256) void k_moments_from_moments(int ncx, int ncy, int ncz, int nfx, int nfy, int nfz, AxisNest ax, AxisNest ay,
                                        AxisNest az, const double* __restrict__ momf, double* __restrict__ mom) {
   const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nc = (long long)ncx * ncy * ncz;
@@ -90,8 +90,9 @@ __global__ void k_moments_from_moments(int ncx, int ncy, int ncz, int nfx, int n
 
 // 3x3 diagonal block of corner l of one cell's element matrix: D[r][c] = (K e_{l,c})[3l + r].
 template <bool MOM, bool SRI>
-__device__ void corner_block(int l, const double* mL, const double* mM, double lam, double mu, double hx, double hy,
-                             double hz, double* D) {
+__device__ __forceinline__ void corner_block(int l, const double* mL, const double* mM, double lam, double mu,
+                                             double hx, double hy, double hz, double* D) {
+#pragma unroll
   for (int c = 0; c < 3; ++c) {
     double u[24], f[24];
     for (int q = 0; q < 24; ++q) u[q] = 0.0;
@@ -111,7 +112,7 @@ __device__ void corner_block(int l, const double* mL, const double* mM, double l
 // Level 0: constant-material cells with (lam, mu) per cell (x-fastest, irregular cells zeroed).
 // Coarse levels: moments.
 template <bool MOM, bool SRI>
-__global__ void k_diag_structured(int nx, int ny, int nz, const double* __restrict__ hx, const double* __restrict__ hy,
+__global__ __launch_bounds__(256) void k_diag_structured(int nx, int ny, int nz, const double* __restrict__ hx, const double* __restrict__ hy,
                                   const double* __restrict__ hz, const double* __restrict__ lam,
                                   const double* __restrict__ mu, const double* __restrict__ mom,
                                   double* __restrict__ D) {
@@ -122,8 +123,13 @@ __global__ void k_diag_structured(int nx, int ny, int nz, const double* __restri
   const int i = id % (nx + 1), j = (id / (nx + 1)) % (ny + 1), k = id / ((long long)(nx + 1) * (ny + 1));
   double acc[9];
   for (int q = 0; q < 9; ++q) acc[q] = 0.0;
+  // fully unrolled: the corner index l is a compile-time constant in every copy of the body,
+  // so the element vectors stay in registers
+#pragma unroll
   for (int c = 0; c < 2; ++c)
+#pragma unroll
     for (int b = 0; b < 2; ++b)
+#pragma unroll
       for (int a = 0; a < 2; ++a) {
         // cell whose corner (a,b,c) is this node
         const int ci = i - a, cj = j - b, ck = k - c;
@@ -148,13 +154,14 @@ __global__ void k_diag_structured(int nx, int ny, int nz, const double* __restri
 
 // Diagonal blocks of irregular cells (atomics into D).
 template <bool SRI>
-__global__ void k_diag_irregular(int nirr, const int* __restrict__ nodes, const int* __restrict__ ijk,
+__global__ __launch_bounds__(256) void k_diag_irregular(int nirr, const int* __restrict__ nodes, const int* __restrict__ ijk,
                                  const double* __restrict__ lam, const double* __restrict__ mu,
                                  const double* __restrict__ hx, const double* __restrict__ hy,
                                  const double* __restrict__ hz, double* __restrict__ D) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nirr) return;
   const int i = ijk[3 * e], j = ijk[3 * e + 1], k = ijk[3 * e + 2];
+#pragma unroll
   for (int l = 0; l < 8; ++l) {
     double blk[9];
     corner_block<false, SRI>(l, nullptr, nullptr, lam[e], mu[e], hx[i], hy[j], hz[k], blk);
@@ -306,7 +313,7 @@ __global__ void k_block_inverse(long long nn, const double* __restrict__ D, T* _
 }
 
 // Dense coarsest operator: one block of 24 threads per coarse cell, thread = element column.
-__global__ void k_dense_assemble(int nx, int ny, int nz, const double* __restrict__ hx, const double* __restrict__ hy,
+__global__ __launch_bounds__(64) void k_dense_assemble(int nx, int ny, int nz, const double* __restrict__ hx, const double* __restrict__ hy,
                                  const double* __restrict__ hz, const double* __restrict__ mom, int n,
                                  double* __restrict__ A) {
   const long long cid = blockIdx.x;
@@ -318,8 +325,8 @@ __global__ void k_dense_assemble(int nx, int ny, int nz, const double* __restric
   double mL[27], mM[27];
   for (int q = 0; q < 27; ++q) { mL[q] = mom[q * ncell + cid]; mM[q] = mom[(27 + q) * ncell + cid]; }
   double u[24], f[24];
-  for (int q = 0; q < 24; ++q) u[q] = 0.0;
-  u[col] = 1.0;
+#pragma unroll
+  for (int q = 0; q < 24; ++q) u[q] = (q == col) ? 1.0 : 0.0;   // no runtime index: stays in registers
   elem_apply_mom<double>(u, f, mL, mM, 0.25 / hx[ci], 0.25 / hy[cj], 0.25 / hz[ck]);
   auto gdof = [&](int q) {
     const int l = q / 3, c = q % 3;
@@ -359,7 +366,9 @@ constexpr int GJB = 64;
 // (1) Dk = inverse of the nb x nb diagonal block at k0, padded with the identity to 64 x 64
 // (padding rows/columns stay the identity through the elimination).  256 threads: thread
 // (g = t / 64, j = t % 64) keeps M[g + 4r][j], r = 0..15, in registers; each of the 64 steps
-// publishes the pivot row and column through LDS (double-buffered, one barrier per step).
+// publishes the pivot row and column through LDS (double-buffered, one barrier per step).  The
+// step loop stays rolled (a one-pass kernel runs its code once, so a fully unrolled body is
+// fetched cold); the pivot row's register is picked with a select chain.
 __global__ __launch_bounds__(256) void k_gjb_diag(int n, int k0, int nb, const double* __restrict__ A,
                                                   double* __restrict__ Dk) {
   __shared__ double rowk[2][GJB], colk[2][GJB];
@@ -370,10 +379,15 @@ __global__ __launch_bounds__(256) void k_gjb_diag(int n, int k0, int nb, const d
     const int i = g + 4 * r;
     m[r] = (i < nb && j < nb) ? A[(long long)(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
   }
-#pragma unroll
+#pragma unroll 1
   for (int k = 0; k < GJB; ++k) {
-    const int buf = k & 1;
-    if (g == (k & 3)) rowk[buf][j] = m[k >> 2];
+    const int buf = k & 1, kr = k >> 2;
+    if (g == (k & 3)) {
+      double v = m[0];
+#pragma unroll
+      for (int r = 1; r < 16; ++r) v = (r == kr) ? m[r] : v;
+      rowk[buf][j] = v;
+    }
     if (j == k) {
 #pragma unroll
       for (int r = 0; r < 16; ++r) colk[buf][g + 4 * r] = m[r];

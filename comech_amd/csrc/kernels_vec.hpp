@@ -109,8 +109,8 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_xr(long long n, TX* __restr
 }
 
 // beta = S[rz_new] / S[rz_old];  p = z + beta p
-template <typename T>
-__global__ void k_pcg_p(long long n, T* __restrict__ p, const T* __restrict__ z, const double* __restrict__ S,
+template <typename T, typename TZ = T>
+__global__ void k_pcg_p(long long n, T* __restrict__ p, const TZ* __restrict__ z, const double* __restrict__ S,
                         int s_new, int s_old, int s_sub = -1) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;

@@ -145,7 +145,7 @@ struct System {
   double t_setup_ms = 0.0;
   // build stage times (ms): level-0 prep, hierarchy structure, uploads, moments, diagonals,
   // coarsest inverse, total
-  double t_stage[7] = {0, 0, 0, 0, 0, 0, 0};
+  double t_stage[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // + structure split: used masks, extra coarsening
   // staged level-0 input (converted in prepare_level0)
   int st_nmat = -1;
   std::vector<double> st_tabL, st_tabM, st_lam, st_mu;
@@ -743,8 +743,12 @@ struct System {
       c->nx = (int)c->e[0].size() - 1; c->ny = (int)c->e[1].size() - 1; c->nz = (int)c->e[2].size() - 1;
       c->nng = c->nn = (long long)(c->nx + 1) * (c->ny + 1) * (c->nz + 1);
       c->ncell = (long long)c->nx * c->ny * c->nz;
+      auto ta = clk::now();
       coarse_used(f, *c);
+      auto tb = clk::now();
       coarse_extra(f, *c);
+      t_stage[7] += std::chrono::duration<double, std::milli>(tb - ta).count();
+      t_stage[8] += std::chrono::duration<double, std::milli>(clk::now() - tb).count();
       L.push_back(std::move(c));
     }
     lap(1);
@@ -1425,8 +1429,11 @@ struct System {
     while (flag && it < maxiter) {
       apply<double>(0, p64.p, Ap64.p, MODE_APPLY, nullptr);
       dot(n, p64.p, Ap64.p, S_PAP);
+      // FP32 V-cycle without rigid-mode correction: the residual's FP32 copy comes out of the
+      // update kernel and z stays FP32 (the dots and the direction update read it directly)
+      const bool z32_path = vprec == 1 && zcol == 0;
       hipLaunchKernelGGL((k_pcg_xr<double, double, float>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, x,
-                         r64.p, p64.p, Ap64.p, S.p, cur, S_PAP, partial.p, (float*)nullptr);
+                         r64.p, p64.p, Ap64.p, S.p, cur, S_PAP, partial.p, z32_path ? r32.p : (float*)nullptr);
       finalize_to<double>(S_RR);
       ++it;
       if (it < check_every || it % check_every == 0) {
@@ -1454,6 +1461,16 @@ struct System {
       }
       // after a restart the preconditioner's inexactness (FP32 V-cycle, atomics) has shown up:
       // flexible CG from then on, beta = r.(z - z_old) / (r_old.z_old)
+      if (z32_path) {
+        if (flexible) dot(n, r64.p, z32.p, S_RZP);   // z32 still holds z_old here
+        vcycle_top<float>(r32.p, z32.p);
+        dot(n, r64.p, z32.p, nxt);
+        hipLaunchKernelGGL((k_pcg_p<double, float>), grid1(n), dim3(256), 0, stream, n, p64.p, z32.p, S.p, nxt, cur,
+                           flexible ? (int)S_RZP : -1);
+        CM_LAUNCH_CHECK();
+        std::swap(cur, nxt);
+        continue;
+      }
       if (flexible) dot(n, r64.p, z64.p, S_RZP);     // z64 still holds z_old here
       precondition64(vprec, r64.p, z64.p);
       dot(n, r64.p, z64.p, nxt);
@@ -1937,7 +1954,7 @@ int cm_sys_autotune(cm_sys* h, int prec, int nrep) {
 int cm_sys_timings(cm_sys* h, double* out, int cap) {
   CM_API_BEGIN
   need_built(h);
-  for (int k = 0; k < 7 && k < cap; ++k) out[k] = h->s.t_stage[k];
+  for (int k = 0; k < 9 && k < cap; ++k) out[k] = h->s.t_stage[k];
   CM_API_END
 }
 

@@ -141,9 +141,10 @@ class VoxelSystem:
 
     def timings(self):
         """Build stage times (ms)."""
-        t = np.zeros(7)
-        check(self.lib.cm_sys_timings(self.h, ptr(t, ctypes.c_double), 7))
-        keys = ("prepare", "structure", "upload", "moments", "diag", "coarse_inverse", "total")
+        t = np.zeros(9)
+        check(self.lib.cm_sys_timings(self.h, ptr(t, ctypes.c_double), 9))
+        keys = ("prepare", "structure", "upload", "moments", "diag", "coarse_inverse", "total", "structure_used",
+                "structure_extra")
         return dict(zip(keys, (round(float(v), 3) for v in t)))
 
     def zeros(self, dtype=torch.float64, level=0):

@@ -150,7 +150,8 @@ int cm_sys_time_apply(cm_sys* sys, int level, int prec, const void* u, void* y, 
 int cm_sys_autotune(cm_sys* sys, int prec, int nrep);
 
 /* Build stage times in ms: level-0 preparation, hierarchy structure, uploads, coarse moments,
- * diagonal blocks, coarsest inverse, total (cap >= 7). */
+ * diagonal blocks, coarsest inverse, total, then the structure stage's used-mask and
+ * extra-stiffness coarsening parts (up to cap values, at most 9). */
 int cm_sys_timings(cm_sys* sys, double* out, int cap);
 
 /* Override the z extent of a structured block (0 = automatic). */

@@ -46,6 +46,10 @@ SIGNATURES = {
     "cm_sys_timings": [c_void_p, P_double, c_int],
     "cm_sys_set_kz": [c_void_p, c_int],
     "cm_sys_set_graphs": [c_void_p, c_int],
+    "cm_lattice_materials": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "cm_split_connectivity": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
+                              c_ll, P_ll, P_ll],
     "cm_sys_sync": [c_void_p],
     "cm_sample_fe": [c_void_p] + [c_int] * 7 + [c_void_p] * 9 + [c_void_p] * 5 + [c_void_p] * 4,
 }

@@ -17,6 +17,7 @@
 #include "../../include/comech_fe.h"
 #include "kernels_fe.hpp"
 #include "kernels_post.hpp"
+#include "kernels_prep.hpp"
 #include "kernels_setup.hpp"
 #include "kernels_vec.hpp"
 
@@ -1979,6 +1980,70 @@ int cm_sys_sync(cm_sys* h) {
   CM_API_BEGIN
   CM_CHECK(h != nullptr, "null system");
   CM_HIP(hipStreamSynchronize(h->s.stream));
+  CM_API_END
+}
+
+int cm_lattice_materials(void* stream, int nx, int ny, int nz, int ns3, const int16_t* L, int nlab,
+                         const int8_t* lut_cls, const int8_t* lut_grp, int ngrp, const double* lam_k,
+                         const double* mu_k, double* E, double* nu, uint8_t* active, double* F, uint8_t* ser, float* S,
+                         int32_t* part) {
+  CM_API_BEGIN
+  CM_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "empty lattice");
+  CM_CHECK(ns3 >= 1 && ns3 <= cm::LAT_MAX_NS3, "1..64 sub-samples per cell");
+  CM_CHECK(nlab >= 1, "empty label table");
+  CM_CHECK(ngrp >= 1 && ngrp <= 64, "at most 63 bone groups");
+  hipStream_t st = (hipStream_t)stream;
+  const long long ncell = (long long)nx * ny * nz;
+  cm::DevBuf<double> mix(7 * (size_t)ncell);
+  cm::DevBuf<unsigned long long> pres((size_t)ncell);
+  cm::DevBuf<uint8_t> cart((size_t)ncell);
+  cm::LatticeArgs a{nx, ny, nz, ns3, L, lut_cls, lut_grp, nlab, ngrp, lam_k, mu_k, F, S, mix.p, pres.p, cart.p};
+  hipLaunchKernelGGL(cm::k_lattice_cells, cm::grid1(ncell), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(cm::k_lattice_finish, cm::grid1(ncell), dim3(256), 0, st, a, E, nu, active, ser, part);
+  CM_LAUNCH_CHECK();
+  CM_HIP(hipStreamSynchronize(st));
+  CM_API_END
+}
+
+int cm_split_connectivity(void* stream, int nx, int ny, int nz, const uint8_t* active, const int32_t* part,
+                          const float* share, int nb, int64_t* cn, int64_t* copy_orig, int64_t cap, int64_t* ncopy,
+                          int64_t* nsplit) {
+  CM_API_BEGIN
+  CM_CHECK(nx >= 1 && ny >= 1 && nz >= 1, "empty grid");
+  CM_CHECK(nb >= 2 && nb <= 4096, "bone count out of range");
+  CM_CHECK(ncopy != nullptr && nsplit != nullptr, "null output");
+  hipStream_t st = (hipStream_t)stream;
+  const long long ncell = (long long)nx * ny * nz;
+  const long long nn = (long long)(nx + 1) * (ny + 1) * (nz + 1);
+  CM_CHECK(nn * nb < (1LL << 40), "grid x bones too large");
+  cm::DevBuf<int> cnt((size_t)(nn * nb)), nc((size_t)nn);
+  cm::DevBuf<long long> base((size_t)nn), btot((size_t)cm::div_up(nn, cm::SCAN_ITEMS)), tot(1);
+  cm::DevBuf<unsigned long long> ns(1);
+  hipLaunchKernelGGL(cm::k_fill<int>, cm::grid1(nn * nb), dim3(256), 0, st, cnt.p, nn * nb, 0);
+  hipLaunchKernelGGL(cm::k_fill<unsigned long long>, dim3(1), dim3(256), 0, st, ns.p, 1LL, 0ull);
+  hipLaunchKernelGGL(cm::k_split_count, cm::grid1(ncell), dim3(256), 0, st, nx, ny, nz, active, (const int*)part, nb,
+                     cnt.p);
+  hipLaunchKernelGGL(cm::k_split_nodes, cm::grid1(nn), dim3(256), 0, st, nn, nb, cnt.p, nc.p, ns.p);
+  const long long nblk = cm::div_up(nn, cm::SCAN_ITEMS);
+  hipLaunchKernelGGL(cm::k_scan_blocks, dim3((unsigned)nblk), dim3(256), 0, st, nn, nc.p, base.p, btot.p);
+  hipLaunchKernelGGL(cm::k_scan_totals, dim3(1), dim3(256), 0, st, nblk, btot.p, tot.p);
+  hipLaunchKernelGGL(cm::k_scan_add, cm::grid1(nn), dim3(256), 0, st, nn, base.p, btot.p);
+  CM_LAUNCH_CHECK();
+  long long total = 0;
+  unsigned long long nsp = 0;
+  CM_HIP(hipMemcpyAsync(&total, tot.p, sizeof(total), hipMemcpyDeviceToHost, st));
+  CM_HIP(hipMemcpyAsync(&nsp, ns.p, sizeof(nsp), hipMemcpyDeviceToHost, st));
+  CM_HIP(hipStreamSynchronize(st));
+  *ncopy = total;
+  *nsplit = (int64_t)nsp;
+  CM_CHECK(total <= cap, "copy_orig capacity too small: need " + std::to_string(total));
+  if (total > 0)
+    hipLaunchKernelGGL(cm::k_split_orig, cm::grid1(nn), dim3(256), 0, st, nn, nb, cnt.p, nc.p, base.p,
+                       (long long*)copy_orig);
+  hipLaunchKernelGGL(cm::k_split_cells, cm::grid1(ncell), dim3(256), 0, st, nx, ny, nz, active, (const int*)part,
+                     share, nb, cnt.p, nc.p, base.p, (long long*)cn);
+  CM_LAUNCH_CHECK();
+  CM_HIP(hipStreamSynchronize(st));
   CM_API_END
 }
 

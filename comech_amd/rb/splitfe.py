@@ -1,6 +1,7 @@
 """Drop-in for the reference's ``rb/splitfe.py`` (voxel FE with split nodes between bones), GPU path.
 
-Names and arguments as ``rb/splitfe.py``: ``split_connectivity_fast`` (and ``split_connectivity``),
+Names and arguments as ``rb/splitfe.py``: ``split_connectivity_fast`` (and ``split_connectivity``; host,
+as the reference, and ``split_connectivity_gpu``, the same result from the GPU kernels),
 ``KL_family``, ``face_nodes_cn``, ``rigid_modes_cn``, ``solve_cn``, ``cell_stress_cn``,
 ``element_forces_cn``.  The volumetric term uses selective reduced integration (SRI = True,
 rb/splitfe.py:58).  ``solve_cn`` runs matrix-free GMG-PCG on the GPU; by default it mirrors the
@@ -8,6 +9,8 @@ reference GPU solver (FP64 PCG with an FP32 V-cycle, rb/gpusolve.py:265) with th
 at the reference cadence.  Extra keyword options as ``comech_amd.fe.solver.solve``.  The laminate
 ``aniso`` option is not supported on the GPU path and raises ``NotImplementedError``.
 """
+import ctypes
+
 import numpy as np
 import scipy.sparse as sp
 import torch
@@ -76,6 +79,42 @@ def split_connectivity_fast(g, active, part, bone_share):
 
 
 split_connectivity = split_connectivity_fast
+
+
+def split_connectivity_gpu(g, active, part, bone_share, device=None):
+    """split_connectivity_fast (rb/splitfe.py:261-305) on the GPU (cm_split_connectivity): same
+    (cn, orig, number of split nodes), bit-exact.  ``active``, ``part``, ``bone_share`` may be numpy
+    arrays or device tensors (reference cell order)."""
+    dev = require_gpu(device)
+    lib = _lib.load()
+    nx, ny, nz = g.nx, g.ny, g.nz
+    ncell, nnode = nx * ny * nz, (nx + 1) * (ny + 1) * (nz + 1)
+    t = lambda a, dt: (a.to(device=dev, dtype=dt).contiguous() if isinstance(a, torch.Tensor)
+                       else torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev))
+    act = t(active, torch.uint8)
+    pt = t(part, torch.int32)
+    sh = t(bone_share, torch.float32)
+    nb = int(sh.shape[1])
+    if int(pt.max().item()) >= nb or int(pt.min().item()) < 0:
+        raise ValueError("part must index a column of bone_share")
+    cn = torch.empty((ncell, 8), dtype=torch.int64, device=dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    vp = lambda x: ctypes.c_void_p(x.data_ptr())
+    cap = max(1024, nnode // 8)
+    for _ in range(2):
+        orig = torch.empty(cap, dtype=torch.int64, device=dev)
+        ncopy, nsplit = ctypes.c_longlong(), ctypes.c_longlong()
+        rc = lib.cm_split_connectivity(st, nx, ny, nz, vp(act), vp(pt), vp(sh), nb, vp(cn), vp(orig), cap,
+                                       ctypes.byref(ncopy), ctypes.byref(nsplit))
+        if rc == 0:
+            break
+        if ncopy.value <= cap:
+            _lib.check(rc)
+        cap = int(ncopy.value)
+    else:
+        _lib.check(rc)
+    origs = np.concatenate([np.arange(nnode, dtype=np.int64), orig[: ncopy.value].cpu().numpy()])
+    return cn.cpu().numpy(), origs, int(nsplit.value)
 
 
 def KL_family(g):
@@ -185,5 +224,5 @@ def element_forces_cn(g, lam, mu, U, cells, cn, aniso=None, device=None):
     return _post("forces", g, lam, mu, U, cells, cn, device)
 
 
-__all__ = ["split_connectivity_fast", "split_connectivity", "KL_family", "face_nodes_cn", "rigid_modes_cn",
+__all__ = ["split_connectivity_fast", "split_connectivity", "split_connectivity_gpu", "KL_family", "face_nodes_cn", "rigid_modes_cn",
            "solve_cn", "cell_stress_cn", "element_forces_cn", "SRI", "LOC"]

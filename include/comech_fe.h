@@ -154,6 +154,28 @@ int cm_sys_autotune(cm_sys* sys, int prec, int nrep);
  * extra-stiffness coarsening parts (up to cap values, at most 9). */
 int cm_sys_timings(cm_sys* sys, double* out, int cap);
 
+/* Engine preparation on the GPU (all arrays device memory, reference cell order
+ * c = (i ny + j) nz + k).
+ *
+ * Lattice materials: rb/rbfe.py:75-134 (materials: class fractions, mixing rule, joint cells,
+ * activity), rb/replica3.py:17-29 (bone_share) and rb/rbfe.py:140-141 (part) in one pass.
+ * L: [ncell][ns3] body labels (int16); lut_cls / lut_grp: [nlab] class and bone group of every
+ * label (rb/rbgeom.py:45 classes, rb/replica2.py:34-46 groups); lam_k / mu_k: Lame parameters of
+ * the 10 classes.  Outputs E, nu [ncell] (FP64), active, ser [ncell] (0/1), F [ncell][10] class
+ * fractions, S [ncell][ngrp] bone shares (float), part [ncell]. */
+int cm_lattice_materials(void* stream, int nx, int ny, int nz, int ns3, const int16_t* L, int nlab,
+                         const int8_t* lut_cls, const int8_t* lut_grp, int ngrp, const double* lam_k,
+                         const double* mu_k, double* E, double* nu, uint8_t* active, double* F, uint8_t* ser, float* S,
+                         int32_t* part);
+
+/* Split connectivity: rb/splitfe.py:261-305 (split_connectivity_fast).  active [ncell] (0/1),
+ * part [ncell] (bone per cell, 0 = none), share [ncell][nb] (float).  Outputs cn [ncell][8]
+ * (node ids, copies at nnode + rank), copy_orig [*ncopy] (original node of every copy; fails
+ * with the needed size when it exceeds cap), *ncopy, *nsplit (host). */
+int cm_split_connectivity(void* stream, int nx, int ny, int nz, const uint8_t* active, const int32_t* part,
+                          const float* share, int nb, int64_t* cn, int64_t* copy_orig, int64_t cap, int64_t* ncopy,
+                          int64_t* nsplit);
+
 /* Override the z extent of a structured block (0 = automatic). */
 int cm_sys_set_kz(cm_sys* sys, int kz);
 

@@ -23,6 +23,12 @@ patch test                  fe/test_patch.py:4-14 closed forms uz_top=-10/E, ux=
 cell_stress(_cn)            uniaxial / hydrostatic / pure shear closed forms for von Mises
 spring_matrix_cn            k w w^T closed form on hand-built springs
 sample_fe                   constant and linear fields reproduced exactly
+lattice_materials           single-phase cells return that phase's (E, nu); a hard phase in
+                            series with one layer is the Reuss average of constrained and shear
+                            moduli; two phases in a non-joint cell the Voigt average; joint cells
+                            against a brute-force 3x3x3 neighbourhood scan; activity thresholds
+bone_groups / bone_share    hand-built label maps; shares sum to the bone fraction
+lattice_part                closed form on hand-built shares
 ==========================  =====================================================================
 """
 import numpy as np
@@ -728,3 +734,121 @@ def rel_l2(a, b):
     """Relative L2 error ||a-b|| / ||b|| (ml/evaluate.py:93,96)."""
     a = np.asarray(a, float).ravel(); b = np.asarray(b, float).ravel()
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+# ---------------------------------------------------------------------------------------------
+# Lattice materials (the engine's voxel preparation).  A lattice V holds, per cell in reference
+# order, ns^3 sub-sample body labels L (ncell, ns^3) and a body -> class map cls.
+# rb/rbgeom.py:45-49: tissue classes and their (E, nu); cartilage and soft tissue are overridden
+# by materials()'s arguments.
+LATTICE_CLASSES = ("void", "cortical", "cancellous", "cartilage", "fascia", "soft", "plate", "fibrous", "fat", "gap")
+LATTICE_MAT = {"cortical": (7300.0, 0.3), "cancellous": (100.0, 0.3), "cartilage": (10.0, 0.4),
+               "fascia": (350.0, 0.4), "soft": (1.15, 0.49), "plate": (17000.0, 0.1), "fibrous": (60.0, 0.4),
+               "fat": (1.15, 0.45), "gap": (0.01, 0.3)}
+
+
+def bone_groups(cls, roles):
+    """rb/replica2.py:34-46: group ids 1.. for the named bones (in roles order), then every other
+    bone body (class cortical or cancellous, not bonded_extra) on its own, in label order."""
+    grp, names = {}, {}
+    gid = 1
+    for name, bodies in roles["bones"].items():
+        for b in bodies:
+            grp[b] = gid
+        names[gid] = name
+        gid += 1
+    bonded = set(roles.get("bonded_extra", ()))
+    for b in sorted(cls):
+        if cls[b] in (1, 2) and b not in grp and b not in bonded:
+            grp[b] = gid
+            names[gid] = f"body{b}"
+            gid += 1
+    return grp, names
+
+
+def bone_share(L, cls, roles):
+    """rb/replica3.py:17-29: fraction of each cell's sub-samples in every bone group (float32,
+    column 0 unused)."""
+    grp, names = bone_groups(cls, roles)
+    nlab = max(max(cls), int(L.max())) + 1
+    lutg = np.zeros(nlab, np.int64)
+    for b, g in grp.items():
+        lutg[b] = g
+    Gb = lutg[L]
+    S = np.zeros((L.shape[0], max(names) + 1), np.float32)
+    for g in range(1, max(names) + 1):
+        S[:, g] = (Gb == g).mean(1)
+    return S, names
+
+
+def lattice_materials(L, cls, roles, dims, cart_E=10.0, nu_soft=0.49):
+    """rb/rbfe.py:75-134 (mixing rule iface3 of replica2 with fibrous layers, fat and gaps).
+    dims = (nx, ny, nz); rows of L in reference cell order.  Returns E, nu, active, F, ser."""
+    from scipy import ndimage
+    nlab = max(max(cls), int(L.max())) + 1
+    lut = np.zeros(nlab, np.int64)
+    for b, c in cls.items():
+        lut[b] = c
+    C = lut[L]
+    F = np.stack([(C == k).mean(1) for k in range(len(LATTICE_CLASSES))], 1)
+    mat = dict(LATTICE_MAT)
+    mat["cartilage"] = (cart_E, 0.4)
+    mat["soft"] = (1.15, nu_soft)
+    E_k = np.array([1.0] + [mat[c][0] for c in LATTICE_CLASSES[1:]])
+    nu_k = np.array([0.3] + [mat[c][1] for c in LATTICE_CLASSES[1:]])
+    lam_k, mu_k = lame(E_k, nu_k)
+    M_k = lam_k + 2 * mu_k
+    H, LAY, SOFT, GAPC = [1, 2, 4], [3, 7], [5, 8], [9]
+    PH = H + LAY + SOFT
+    occ = F[:, PH].sum(1)
+    hard, lay, fgap = F[:, H].sum(1), F[:, LAY].sum(1), F[:, GAPC].sum(1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        # Voigt average over the occupied phases
+        lamV = (F[:, PH] * lam_k[PH]).sum(1) / occ
+        muV = (F[:, PH] * mu_k[PH]).sum(1) / occ
+        # hard phases in series with joint layers / soft tissue (Reuss on constrained and shear moduli)
+        lam_h = (F[:, H] * lam_k[H]).sum(1) / hard
+        mu_h = (F[:, H] * mu_k[H]).sum(1) / hard
+        hM = np.where(hard > 0, hard / (lam_h + 2 * mu_h), 0.0)
+        hG = np.where(hard > 0, hard / mu_h, 0.0)
+        gapM = (F[:, GAPC] / M_k[GAPC]).sum(1)
+        gapG = (F[:, GAPC] / mu_k[GAPC]).sum(1)
+        layM = (F[:, LAY] / M_k[LAY]).sum(1) + gapM
+        layG = (F[:, LAY] / mu_k[LAY]).sum(1) + gapG
+        fch = lay + hard + fgap
+        occ_g = occ + fgap
+        M_ch, G_ch = fch / (layM + hM), fch / (layG + hG)
+        lam_soft, mu_soft = (F[:, SOFT] * lam_k[SOFT]).sum(1), (F[:, SOFT] * mu_k[SOFT]).sum(1)
+        lam_c = (fch * (M_ch - 2 * G_ch) + lam_soft) / occ_g
+        mu_c = (fch * G_ch + mu_soft) / occ_g
+        softM, softG = (F[:, SOFT] / M_k[SOFT]).sum(1), (F[:, SOFT] / mu_k[SOFT]).sum(1)
+        M_s = occ_g / (softM + hM + gapM)
+        G_s = occ_g / (softG + hG + gapG)
+        lam_s, mu_s = M_s - 2 * G_s, G_s
+    # joint cells: within one cell (3x3x3) of two different bone groups
+    grp, _ = bone_groups(cls, roles)
+    lutg = np.zeros(nlab, np.int64)
+    for b, g in grp.items():
+        lutg[b] = g
+    Gb = lutg[L]
+    count = np.zeros(L.shape[0], np.int64)
+    for g in np.unique(Gb[Gb > 0]):
+        pres = (Gb == g).any(1).reshape(dims)
+        count += ndimage.binary_dilation(pres, np.ones((3, 3, 3), bool)).ravel()
+    cart = lay > 0
+    ser = cart | (count >= 2)
+    lamX = np.where(cart, lam_c, np.where(ser, lam_s, lamV))
+    muX = np.where(cart, mu_c, np.where(ser, mu_s, muV))
+    with np.errstate(divide="ignore", invalid="ignore"):
+        E = muX * (3 * lamX + 2 * muX) / (lamX + muX)
+        nu = lamX / (2 * (lamX + muX))
+    active = np.where(ser, occ > 0.15, occ >= 0.5)
+    E = np.where(active, E, 1.0)
+    nu = np.where(active, np.clip(nu, 0.0, 0.4995), 0.3)
+    return np.nan_to_num(E, nan=1.0), np.nan_to_num(nu, nan=0.3), active, F, ser
+
+
+def lattice_part(S, F):
+    """rb/rbfe.py:140-141: bone group of a cell holding >= 50 % of one bone and no joint layer."""
+    cart = (F[:, 3] + F[:, 7]) > 0
+    return np.where((S.max(1) >= 0.5) & ~cart, S.argmax(1), 0)

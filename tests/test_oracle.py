@@ -244,3 +244,122 @@ def test_sample_fe_constant_and_linear():
     assert np.allclose(vm, 3.0) and np.allclose(cov, 1.0) and np.allclose(sig, np.arange(6))
     centres = gm.cell_xyz * np.array([1, 2, 3.0])
     assert np.allclose(u, centres, atol=1e-6)
+
+
+# ---- lattice materials (rb/rbfe.py:75-134) ---------------------------------------------------
+LAT_CLS = {1: 1, 2: 2, 3: 1, 4: 2, 5: 3, 6: 4, 7: 5, 8: 6, 9: 7, 10: 8, 11: 9, 12: 1}
+LAT_ROLES = {"bones": {"calc": (1, 2), "talus": (3, 4)}, "bonded_extra": [12]}
+
+
+def _lat_E_nu(name, cart_E=10.0, nu_soft=0.49):
+    if name == "cartilage":
+        return cart_E, 0.4
+    if name == "soft":
+        return 1.15, nu_soft
+    return O.LATTICE_MAT[name]
+
+
+def _lattice(dims, fill):
+    nc = int(np.prod(dims))
+    return np.full((nc, 27), fill, np.int64)
+
+
+def test_lattice_single_phase_cells_return_the_phase():
+    """A cell of one class gets that class's (E, nu); void, gap and plate cells (plate is not one of
+    the occupied phases of rb/rbfe.py:96-99) are inactive (E=1, nu=0.3)."""
+    dims = (3, 1, 1)
+    for lab, cname in [(1, "cortical"), (2, "cancellous"), (5, "cartilage"), (6, "fascia"), (7, "soft"),
+                       (9, "fibrous"), (10, "fat")]:
+        L = _lattice(dims, lab)
+        E, nu, act, F, ser = O.lattice_materials(L, LAT_CLS, LAT_ROLES, dims, cart_E=12.5, nu_soft=0.47)
+        Ek, nk = _lat_E_nu(cname, 12.5, 0.47)
+        assert act.all(), cname
+        assert np.allclose(E, Ek, rtol=1e-12) and np.allclose(nu, nk, rtol=1e-12), (cname, E, nu)
+    for lab in (0, 11, 8):                                # void, gap, plate
+        L = _lattice(dims, lab)
+        E, nu, act, F, ser = O.lattice_materials(L, LAT_CLS, LAT_ROLES, dims)
+        assert not act.any() and np.all(E == 1.0) and np.all(nu == 0.3)
+
+
+def test_lattice_hard_in_series_with_cartilage_is_reuss():
+    """Cortical fraction h and cartilage 1-h in one cell: constrained modulus M and shear G are the
+    series (Reuss) averages of the two phases."""
+    dims = (1, 1, 1)
+    L = _lattice(dims, 1)
+    L[0, 18:] = 5                                         # 9 of 27 samples cartilage
+    E, nu, act, F, ser = O.lattice_materials(L, LAT_CLS, LAT_ROLES, dims, cart_E=10.0)
+    h = 18 / 27
+    lam = lambda E_, n_: E_ * n_ / ((1 + n_) * (1 - 2 * n_))
+    mu = lambda E_, n_: E_ / (2 * (1 + n_))
+    Ec, nc = O.LATTICE_MAT["cortical"]
+    Mh, Gh = lam(Ec, nc) + 2 * mu(Ec, nc), mu(Ec, nc)
+    Ml, Gl = lam(10.0, 0.4) + 2 * mu(10.0, 0.4), mu(10.0, 0.4)
+    M = 1 / (h / Mh + (1 - h) / Ml)
+    G = 1 / (h / Gh + (1 - h) / Gl)
+    l_, m_ = M - 2 * G, G
+    assert ser[0] and act[0]
+    assert np.isclose(E[0], m_ * (3 * l_ + 2 * m_) / (l_ + m_), rtol=1e-12)
+    assert np.isclose(nu[0], l_ / (2 * (l_ + m_)), rtol=1e-12)
+
+
+def test_lattice_two_phases_in_a_plain_cell_is_voigt():
+    """Cortical and soft tissue in a cell that is no joint: Lame parameters are the volume (Voigt)
+    averages."""
+    dims = (1, 1, 1)
+    L = _lattice(dims, 1)
+    L[0, 20:] = 7
+    E, nu, act, F, ser = O.lattice_materials(L, LAT_CLS, LAT_ROLES, dims, nu_soft=0.49)
+    h = 20 / 27
+    lam = lambda E_, n_: E_ * n_ / ((1 + n_) * (1 - 2 * n_))
+    mu = lambda E_, n_: E_ / (2 * (1 + n_))
+    l_ = h * lam(*O.LATTICE_MAT["cortical"]) + (1 - h) * lam(1.15, 0.49)
+    m_ = h * mu(*O.LATTICE_MAT["cortical"]) + (1 - h) * mu(1.15, 0.49)
+    assert not ser[0] and act[0]
+    assert np.isclose(E[0], m_ * (3 * l_ + 2 * m_) / (l_ + m_), rtol=1e-12)
+    assert np.isclose(nu[0], l_ / (2 * (l_ + m_)), rtol=1e-12)
+
+
+def test_lattice_joint_cells_and_activity_against_brute_force():
+    """ser (joint) = cartilage present, or >= 2 bone groups within the 3x3x3 neighbourhood (brute
+    force); active = occupancy > 0.15 in joint cells, >= 0.5 elsewhere."""
+    rng = np.random.default_rng(4)
+    dims = (7, 6, 5)
+    nc = int(np.prod(dims))
+    L = rng.choice([0, 0, 0, 1, 2, 3, 4, 7, 10, 12, 13], size=(nc, 27))
+    cls = dict(LAT_CLS)
+    cls[13] = 2                                            # an unnamed bone body: its own group
+    E, nu, act, F, ser = O.lattice_materials(L, cls, LAT_ROLES, dims)
+    grp, names = O.bone_groups(cls, LAT_ROLES)
+    assert grp == {1: 1, 2: 1, 3: 2, 4: 2, 13: 3} and names[3] == "body13"
+    gl = np.zeros(14, np.int64)
+    for b, g in grp.items():
+        gl[b] = g
+    pres = [set(gl[L[c]][gl[L[c]] > 0]) for c in range(nc)]
+    nx, ny, nz = dims
+    for i in range(nx):
+        for j in range(ny):
+            for k in range(nz):
+                c = (i * ny + j) * nz + k
+                near = set()
+                for di in (-1, 0, 1):
+                    for dj in (-1, 0, 1):
+                        for dk in (-1, 0, 1):
+                            a, b, d = i + di, j + dj, k + dk
+                            if 0 <= a < nx and 0 <= b < ny and 0 <= d < nz:
+                                near |= pres[(a * ny + b) * nz + d]
+                cart = (L[c] == 5).any() or (L[c] == 9).any()
+                assert ser[c] == (cart or len(near) >= 2), (i, j, k)
+                occ = np.isin(L[c], [1, 2, 3, 4, 6, 5, 9, 7, 10, 12, 13]).sum() / 27
+                assert act[c] == ((occ > 0.15) if ser[c] else (occ >= 0.5)), (i, j, k)
+
+
+def test_lattice_bone_share_and_part():
+    """Shares are the per-group sample fractions; part = the group holding >= 50 % of a cell without
+    joint layers (rb/replica3.py:17-29, rb/rbfe.py:140-141)."""
+    L = np.array([[1] * 14 + [3] * 13, [3] * 20 + [5] * 7, [7] * 27, [1] * 10 + [3] * 10 + [7] * 7])
+    S, names = O.bone_share(L, LAT_CLS, LAT_ROLES)
+    assert S.dtype == np.float32 and S.shape == (4, 3)
+    assert np.allclose(S[0], [0, 14 / 27, 13 / 27]) and np.allclose(S[3], [0, 10 / 27, 10 / 27])
+    F = np.stack([(np.vectorize(lambda b: LAT_CLS.get(b, 0))(L) == k).mean(1) for k in range(10)], 1)
+    part = O.lattice_part(S, F)
+    assert list(part) == [1, 0, 0, 0]                     # row 1: cartilage present; rows 2-3: < 50 %

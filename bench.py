@@ -164,6 +164,71 @@ def large_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def prep_workload(args):
+    """Engine preparation at the hindfoot size: a 27-sample lattice built from the raster (bone
+    bodies of the engine's label table, coalition bar, soft tissue; 30 % of cells mix 9 samples
+    with their +x neighbour), then lattice materials + bone shares + parts and split connectivity.
+    GPU (cm_lattice_materials, cm_split_connectivity, results copied back) against the oracle's
+    restatement of the same reference algorithm on the host (one thread).  One JSON line."""
+    import numpy as np
+    import torch
+    from comech_amd import synth
+    from comech_amd.rb import rbfe as RF
+    from comech_amd.rb import splitfe as SF
+    os.environ["OMP_NUM_THREADS"] = "1"
+    dev = torch.device("cuda:0")
+    c = synth.config2(dims=(160, 96, 96), h=0.5)
+    g = c["grid"]
+    lab = c["labels"]
+    bodies = [("calcaneus", 0), ("talus", 0), ("navicular", 0)]
+    body_of = np.zeros(256, np.int64)
+    body_of[0], body_of[1], body_of[255] = 61, RF.COAL, 0
+    for b, (name, _) in enumerate(bodies):
+        body_of[2 + 2 * b] = RF.BONES[name][0]
+        body_of[3 + 2 * b] = RF.BONES[name][1]
+    cls = {61: 5, RF.COAL: 1}
+    for name, _ in bodies:
+        cls[RF.BONES[name][0]], cls[RF.BONES[name][1]] = 1, 2
+    base = body_of[lab]
+    rng = np.random.default_rng(7)
+    L = np.repeat(base[:, None], 27, 1).astype(np.int16)
+    nbr = np.roll(base.reshape(g.nx, g.ny, g.nz), -1, 0).ravel()
+    mix = rng.uniform(size=g.ncell) < 0.3
+    L[mix, :9] = nbr[mix, None]
+    V = dict(L=L, cls=cls, xe=g.xe, ye=g.ye, ze=g.ze)
+    for _ in range(2):                                     # warm up (library load, caches)
+        o = RF.lattice_prep(V, device=dev, host=False)
+        SF.split_connectivity_gpu(g, o["active"], o["part"], o["S"], device=dev)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    nrep = 5
+    for _ in range(nrep):
+        o = RF.lattice_prep(V, device=dev, host=False)
+        cn, orig, ns = SF.split_connectivity_gpu(g, o["active"], o["part"], o["S"], device=dev)
+        E = o["E"].cpu().numpy()
+    torch.cuda.synchronize()
+    t_gpu = (time.time() - t0) / nrep
+    line = {"workload": "engine preparation, hindfoot raster 160x96x96 as a 27-sample lattice",
+            "ncell": g.ncell, "samples": int(L.size), "split_nodes": ns, "copies": int(len(orig) - g.nnode),
+            "gpu_ms": round(t_gpu * 1e3, 2)}
+    if not args.no_cpu_baseline:
+        from oracle import fe_oracle as O
+        t1 = time.time()
+        Eo, nuo, acto, Fo, sero = O.lattice_materials(L, cls, RF.ROLES, (g.nx, g.ny, g.nz))
+        So, _ = O.bone_share(L, cls, RF.ROLES)
+        po = O.lattice_part(So, Fo)
+        og = O.Grid(g.xe, g.ye, g.ze)
+        cno, origo, nso = O.split_connectivity_fast(og, acto, po, So)
+        t_cpu = time.time() - t1
+        line["cpu_baseline"] = {"seconds": round(t_cpu, 2), "cores": 1, "kind": "port",
+                                "sample": "the same lattice, oracle restatement of rb/rbfe.py:75-141 and "
+                                          "rb/splitfe.py:261-305 on the host"}
+        line["speedup"] = round(t_cpu / t_gpu, 1)
+        line["parity"] = bool(np.array_equal(cn, cno) and np.array_equal(orig, origo) and ns == nso
+                              and np.allclose(E, Eo, rtol=1e-12, atol=0))
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,8 +238,9 @@ def main():
     ap.add_argument("--mode", default="ref", choices=["mixed", "ref", "fp64"])
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="hindfoot", choices=["hindfoot", "large"],
-                    help="large: the 384x256x256 single solve (separate JSON line, not the headline metric)")
+    ap.add_argument("--workload", default="hindfoot", choices=["hindfoot", "large", "prep"],
+                    help="large: the 384x256x256 single solve; prep: lattice materials + split connectivity "
+                         "(separate JSON lines, not the headline metric)")
     ap.add_argument("--kernel-only", action="store_true",
                     help="for rocprofv3 --pmc runs: 10 launches of the FP32 fine stencil on the workload, then "
                          "5 plain device copies of 256 MiB (counter calibration)")
@@ -192,6 +258,9 @@ def main():
     t0 = time.time()
     if args.workload == "large":
         large_workload(args)
+        return
+    if args.workload == "prep":
+        prep_workload(args)
         return
     if args.kernel_only:
         c, extra, f, irr_c, irr_n = build_inputs()

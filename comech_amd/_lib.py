@@ -48,6 +48,8 @@ SIGNATURES = {
     "cm_sys_set_graphs": [c_void_p, c_int],
     "cm_lattice_materials": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "cm_bank_pack": [c_void_p, c_ll, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                     c_void_p, c_void_p],
     "cm_split_connectivity": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p,
                               c_ll, P_ll, P_ll],
     "cm_sys_sync": [c_void_p],

@@ -1,6 +1,8 @@
 // Post-processing kernels: cell-centre strain, stress and von Mises (fe/solver.py:257-272,
 // rb/splitfe.py:161-175) and the analysis-grid resampling of rb/rbsim.py:110-140.
 #pragma once
+#include <hip/hip_fp16.h>
+
 #include "common.hpp"
 #include "element.hpp"
 
@@ -114,6 +116,37 @@ __global__ void k_sample_fe(int mx, int my, int mz, int sub, int nx, int ny, int
   for (int q = 0; q < 3; ++q) u_out[3 * id + q] = float(us[q] / den);
   for (int q = 0; q < 6; ++q) sig_out[6 * id + q] = float(ss[q] / den);
   cover[id] = okc / (double)(sub * sub * sub);
+}
+
+// Bank record of one action (exp/bank.py:117-121): von Mises and the cell-mean relative
+// displacement (mean over the analysis cell's 8 nodes, exp/bank.py:116) in FP16, labels as int8,
+// tissue fractions as round(255 f) in uint8.  One thread per analysis cell; u_nodes: [nnode][3],
+// cell_nodes: [ncell][8], frac: [ncell][ncls].
+__global__ __launch_bounds__(256) void k_bank_pack(long long ncell, int ncls, const float* __restrict__ vm,
+                                                   const float* __restrict__ u_nodes,
+                                                   const long long* __restrict__ cell_nodes,
+                                                   const int* __restrict__ lab, const float* __restrict__ frac,
+                                                   __half* __restrict__ vm16, __half* __restrict__ uc16,
+                                                   int8_t* __restrict__ lab8, uint8_t* __restrict__ frac8) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  vm16[c] = __float2half(vm[c]);
+  // FP32 mean of the 8 node values, summed pairwise ((0+1)+(2+3))+((4+5)+(6+7)) as numpy sums
+  // eight float32 values
+  float v[8][3];
+  for (int l = 0; l < 8; ++l) {
+    const long long n = cell_nodes[8 * c + l];
+    for (int q = 0; q < 3; ++q) v[l][q] = u_nodes[3 * n + q];
+  }
+  for (int q = 0; q < 3; ++q) {
+    const float s = ((v[0][q] + v[1][q]) + (v[2][q] + v[3][q])) + ((v[4][q] + v[5][q]) + (v[6][q] + v[7][q]));
+    uc16[3 * c + q] = __float2half(s / 8.0f);
+  }
+  lab8[c] = (int8_t)lab[c];
+  for (int k = 0; k < ncls; ++k) {
+    // FP32 product, round half to even (numpy's round of a float32 array)
+    frac8[c * ncls + k] = (uint8_t)rintf(255.0f * frac[c * ncls + k]);
+  }
 }
 
 }  // namespace cm

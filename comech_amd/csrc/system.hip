@@ -2047,6 +2047,18 @@ int cm_split_connectivity(void* stream, int nx, int ny, int nz, const uint8_t* a
   CM_API_END
 }
 
+int cm_bank_pack(void* stream, int64_t ncell, int ncls, const float* vm, const float* u_nodes, const int64_t* cell_nodes,
+                 const int32_t* lab, const float* frac, void* vm16, void* uc16, int8_t* lab8, uint8_t* frac8) {
+  CM_API_BEGIN
+  CM_CHECK(ncell >= 0 && ncls >= 0, "negative sizes");
+  if (ncell == 0) return 0;
+  hipLaunchKernelGGL(cm::k_bank_pack, cm::grid1(ncell), dim3(256), 0, (hipStream_t)stream, (long long)ncell, ncls, vm,
+                     u_nodes, (const long long*)cell_nodes, (const int*)lab, frac, (__half*)vm16, (__half*)uc16, lab8,
+                     frac8);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
 int cm_sample_fe(void* stream, int mx, int my, int mz, int sub, int nx, int ny, int nz, const int* si, const double* st,
                  const uint8_t* sin_, const int* sj, const double* su, const uint8_t* sjn, const int* sk,
                  const double* sv, const uint8_t* skn, const uint8_t* active, const int64_t* cn, const float* VM,

@@ -176,6 +176,13 @@ int cm_split_connectivity(void* stream, int nx, int ny, int nz, const uint8_t* a
                           const float* share, int nb, int64_t* cn, int64_t* copy_orig, int64_t cap, int64_t* ncopy,
                           int64_t* nsplit);
 
+/* Bank record of one action (exp/bank.py:115-121): vm16 = FP16 von Mises [ncell], uc16 = FP16
+ * mean relative displacement of every analysis cell over its 8 nodes [ncell][3] (u_nodes
+ * [nnode][3] FP32, cell_nodes [ncell][8]), lab8 = int8 labels, frac8 = round(255 frac) as uint8
+ * [ncell][ncls].  All device memory. */
+int cm_bank_pack(void* stream, int64_t ncell, int ncls, const float* vm, const float* u_nodes, const int64_t* cell_nodes,
+                 const int32_t* lab, const float* frac, void* vm16, void* uc16, int8_t* lab8, uint8_t* frac8);
+
 /* Override the z extent of a structured block (0 = automatic). */
 int cm_sys_set_kz(cm_sys* sys, int kz);
 

@@ -42,3 +42,33 @@ def test_solve_cn_with_reference_shaped_grid():
                           extra=c["extra_cb"], tol=1e-12, maxiter=2000)
     err = np.linalg.norm(np.nan_to_num(U - Uo)) / np.linalg.norm(np.nan_to_num(Uo))
     assert info["flag"] == 0 and err <= 1e-6
+
+
+def test_bank_pack_and_files(tmp_path):
+    """Bank arrays packed on the GPU equal the reference's numpy expressions (exp/bank.py:116-118:
+    vm and the cell-mean displacement as float16, lab as int8, round(255 frac) as uint8), and the
+    written foot files load back with allow_pickle=False."""
+    import json
+    from comech_amd import bank
+    from comech_amd.grid import Grid
+    rng = np.random.default_rng(5)
+    g = Grid(np.arange(9.0), np.arange(7.0), np.arange(6.0))
+    Urel = rng.standard_normal((g.nnode, 3)).astype(np.float32) * 3
+    vm = rng.uniform(0, 40, g.ncell).astype(np.float32)
+    lab = rng.integers(-1, 9, g.ncell)
+    frac = rng.dirichlet(np.ones(10), g.ncell).astype(np.float32)
+    frac[0, :2] = [0.5 / 255, 1.5 / 255]                  # rounding ties
+    p = bank.pack_action(vm, Urel, g.cell_nodes, lab, frac, device=DEV)
+    assert np.array_equal(p["vm"], vm.astype(np.float16))
+    uc = Urel[g.cell_nodes].mean(1).astype(np.float16)
+    assert np.array_equal(p["ucell"], uc)
+    assert np.array_equal(p["lab"], lab.astype(np.int8))
+    assert np.array_equal(p["frac"], np.round(frac * 255).astype(np.uint8))
+    rec = [dict(k=k, meta="baseline", act={"load": 1.0}, eff={"heel_share": 0.5}, iters=80, time=0.1, fe={})
+           for k in range(2)]
+    npz, js = bank.write_foot(str(tmp_path), "train", 7, {"ctype": 1}, rec, [p, p])
+    z = np.load(npz)
+    assert z["vm"].shape == (2, g.ncell) and z["vm"].dtype == np.float16 and z["ucell"].shape == (2, g.ncell, 3)
+    assert z["lab"].dtype == np.int8 and z["frac"].dtype == np.uint8
+    m = json.load(open(js))
+    assert m["fid"] == 7 and m["split"] == "train" and len(m["actions"]) == 2

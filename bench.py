@@ -292,6 +292,9 @@ def main():
         res, wall = run_batch([job(i, mode) for i in range(n)], concurrency=conc, device=dev)
         return res, wall
 
+    # the first solve runs alone: the z-extent autotuner (cached per level shape) must time its
+    # kernels on an otherwise idle GPU, not between the kernels of concurrent solves
+    timed(args.mode, 1, 1)
     for _ in range(args.warmup):
         res, _ = timed(args.mode, len(inputs), args.concurrency)
         log("warmup", [round(r["t_solve_ms"], 1) for r in res])

@@ -34,6 +34,7 @@ SIGNATURES = {
     "cm_sys_set_rigid": [c_void_p, c_int, c_ll, P_ll, P_ll, P_double, P_double],
     "cm_sys_build": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_double, c_int, c_int],
     "cm_sys_levels": [c_void_p, P_int, P_ll, c_int],
+    "cm_sys_level_stats": [c_void_p, c_int, P_ll, c_int],
     "cm_sys_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p],
     "cm_sys_vcycle": [c_void_p, c_int, c_void_p, c_void_p],
     "cm_sys_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_int, c_double, ctypes.POINTER(SolveInfo)],
@@ -45,6 +46,7 @@ SIGNATURES = {
     "cm_sys_autotune": [c_void_p, c_int, c_int],
     "cm_sys_timings": [c_void_p, P_double, c_int],
     "cm_sys_set_kz": [c_void_p, c_int],
+    "cm_sys_set_block_skip": [c_void_p, c_int],
     "cm_sys_set_graphs": [c_void_p, c_int],
     "cm_lattice_materials": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,
                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -31,6 +31,8 @@ constexpr uint8_t LAB_NONE = 255;
 //   kind 0: level 0, one uint8 label per cell + (lam, mu) table
 //   kind 1: level 0, per-cell (lam, mu) values
 //   kind 2: coarse levels, 27 lam + 27 mu moments per cell
+//   kind 3: level 1, the 8 lam and 8 mu weights of its level-0 child cells (constant materials),
+//           expanded in registers into the 27 + 27 moments
 // ISO = the level-0 grid is made of identical cubes (h_x = h_y = h_z = h everywhere); the common
 // factor V g^2 = h / 16 is then folded into the material weights (GridArgs::iso).
 template <typename T, bool SRI_, bool ISO_>
@@ -50,24 +52,66 @@ struct ValueMat {
   static constexpr bool sri = SRI_, iso = ISO_;
 };
 
-// Coarse levels also carry their extra stiffness into the march epilogue:
-//   block rows: node n adds sum_e B_e s[bcol_e] over e in [bptr[n], bptr[n+1]) (3x3 blocks B_e,
-//               row-major) -- the Galerkin-coarsened springs;
-//   rank-one terms k w w^T: node n adds sum_e rnw_e * dots[rnterm_e] (rnw_e = the term's 3
-//               weights at n, dots[t] = k_t w_t . s from k_rank1_dots launched just before).
+// Coarse levels also carry extra stiffness (the Galerkin-coarsened springs as 3x3 block rows,
+// rank-one terms k w w^T with a wide support): k_coarse_extra, launched just before the march,
+// accumulates its product with s into ex (3 per node), and the march epilogue of every node adds
+// ex and sets it back to zero (ex is all zero between applications).
 template <typename T>
 struct MomentMat {
   const T* __restrict__ mom;         // [54][ncell]: 27 lam moments then 27 mu moments
-  const int* __restrict__ bptr;      // nullptr: no block rows
-  const int* __restrict__ bcol;
-  const T* __restrict__ bval;
-  const int* __restrict__ rnptr;     // nullptr: no rank-one terms
-  const int* __restrict__ rnterm;
-  const T* __restrict__ rnw;
-  const T* __restrict__ dots;
+  T* ex;                             // nullptr: no extra stiffness
   static constexpr int kind = 2;
   static constexpr bool sri = false, iso = false;
 };
+
+// Level-1 cells are unions of (up to) 2x2x2 level-0 cells of constant material.  Along every
+// axis a child occupies xi in its parent's coordinates as xi = a xi_c + b; the parent's moments
+// int xi^g m dV are then sums over the children of w_c prod_axes phi(g), phi = (1, b, a^2/3 + b^2)
+// (exact integration) or (1, b, b^2) (lam at the child's centre, SRI), w_c = m_c V_c.
+template <typename T, bool SRI_>
+struct ChildMat {
+  const T* __restrict__ cw;          // [16][ncell]: lam weights of children (dx + 2dy + 4dz), then mu
+  const T* __restrict__ phx;         // per coarse index: b0, q0, s0, b1, q1, s1 (q = a^2/3 + b^2, s = b^2)
+  const T* __restrict__ phy;
+  const T* __restrict__ phz;
+  T* ex;
+  static constexpr int kind = 3;
+  static constexpr bool sri = SRI_, iso = false;
+};
+
+// Expands child weights w[8] into the 27 moments (index gx + 3 gy + 9 gz) with per-axis factor
+// tables f?[d][g] (g = 1, 2; g = 0 is 1).
+template <typename T>
+__device__ __forceinline__ void child_moments(const T w[8], const T fx[2][2], const T fy[2][2], const T fz[2][2],
+                                              T m[27]) {
+  T t[3][4];                  // [gx][dy + 2 dz]
+#pragma unroll
+  for (int p = 0; p < 4; ++p) {
+    const T w0 = w[2 * p], w1 = w[2 * p + 1];
+    t[0][p] = w0 + w1;
+    t[1][p] = fx[0][0] * w0 + fx[1][0] * w1;
+    t[2][p] = fx[0][1] * w0 + fx[1][1] * w1;
+  }
+  T u[3][3][2];               // [gx][gy][dz]
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int dz = 0; dz < 2; ++dz) {
+      const T v0 = t[a][2 * dz], v1 = t[a][2 * dz + 1];
+      u[a][0][dz] = v0 + v1;
+      u[a][1][dz] = fy[0][0] * v0 + fy[1][0] * v1;
+      u[a][2][dz] = fy[0][1] * v0 + fy[1][1] * v1;
+    }
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+      const T v0 = u[a][b][0], v1 = u[a][b][1];
+      m[a + 3 * b] = v0 + v1;
+      m[a + 3 * b + 9] = fz[0][0] * v0 + fz[1][0] * v1;
+      m[a + 3 * b + 18] = fz[0][1] * v0 + fz[1][1] * v1;
+    }
+}
 
 template <typename T>
 struct GridArgs {
@@ -166,22 +210,73 @@ __device__ __forceinline__ void epilogue(const Epi<T>& ep, long long n, const T 
 //   * the top-face forces are carried to the next step, the bottom-face forces plus the carry
 //     are transformed back to the 4 bottom corners; three corners go to LDS for the in-plane
 //     neighbours, the fourth is the thread's own output node.
+// Extra stiffness of a coarse node (kinds 2, 3): au += ex[n], ex[n] = 0.
+template <typename T, class M>
+__device__ __forceinline__ void add_extra(const M& mat, long long n, T au[3]) {
+  if constexpr (M::kind >= 2) {
+    if (mat.ex) {
+      T e3[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) e3[c] = mat.ex[3 * n + c];
+      if (e3[0] != T(0) || e3[1] != T(0) || e3[2] != T(0)) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          au[c] += e3[c];
+          mat.ex[3 * n + c] = T(0);
+        }
+      }
+    }
+  }
+}
+
+// order (nullptr: TileMap order): the tile of every block, ~tile for a tile none of whose cells has
+// stiffness.  Such a block's structured part is zero at all its output nodes; it runs only the
+// epilogue there.  The order puts the tiles with stiffness first, dealt to the XCDs in contiguous
+// runs, so the VALU-heavy blocks spread evenly over all CUs.
 template <typename T, class M, int EM>
 __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* __restrict__ s, Epi<T> ep, int kz,
-                                                 TileMap tm) {
+                                                 TileMap tm, const int* __restrict__ order) {
   constexpr int NST = TN * TN * 3;                 // scalars per staged node layer
   constexpr int NSTP = (NST + BLOCK - 1) / BLOCK;  // per thread
   __shared__ __align__(16) T s_u[2][TN * TN * 4];
   __shared__ __align__(16) T s_g[BLOCK * 12];
   __shared__ T s_tL[M::kind == 0 ? 256 : 1], s_tM[M::kind == 0 ? 256 : 1];
+  __shared__ T s_ph[M::kind == 3 ? 2 * TILE * 6 : 1];
   const int tid = threadIdx.x;
   const int tx = tid % TILE, ty = tid / TILE;
   int bx, by, bz;
-  tm.decode(blockIdx.x, bx, by, bz);
+  bool empty = false;
+  if (order) {
+    int id = order[blockIdx.x];
+    empty = id < 0;
+    id = empty ? ~id : id;
+    bx = id % tm.ntx;
+    by = (id / tm.ntx) % tm.nty;
+    bz = id / (tm.ntx * tm.nty);
+  } else {
+    tm.decode(blockIdx.x, bx, by, bz);
+  }
   const int I0 = bx * TOUT, J0 = by * TOUT, K0 = bz * kz;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const long long nxn = nx + 1, nyn = ny + 1, lay = nxn * nyn;
   const long long ncl = (long long)nx * ny;
+  if (empty) {   // block-uniform
+    const int ci = I0 - 1 + tx, cj = J0 - 1 + ty;
+    if (tx >= 1 && ty >= 1 && ci <= nx && cj <= ny) {
+      const int kend = min(K0 + kz, nz + 1);
+      for (int k = K0; k < kend; ++k) {
+        const long long n = (long long)k * lay + (long long)cj * nxn + ci;
+        T au[3] = {T(0), T(0), T(0)}, sn[3] = {T(0), T(0), T(0)};
+        if (EM == EPI_CHEB) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sn[c] = s[3 * n + c];
+        }
+        add_extra<T, M>(mat, n, au);
+        epilogue<T, EM>(ep, n, au, sn);
+      }
+    }
+    return;
+  }
   if constexpr (M::kind == 0) {
     const T sc = M::iso ? g.iso : T(1);
     s_tL[tid] = mat.tabL[tid] * sc;
@@ -196,6 +291,14 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     gx = T(0.25) / hx;
     gy = T(0.25) / hy;
     axy = hx * hy;
+  }
+  // kind 3: child factor tables of the tile's cell columns and rows (read per step from LDS)
+  if constexpr (M::kind == 3) {
+    if (tid < 2 * TILE * 6) {
+      const int ax = tid / (TILE * 6), r = tid % (TILE * 6), c = (ax ? J0 : I0) - 1 + r / 6;
+      const int nc = ax ? ny : nx;
+      s_ph[tid] = (c >= 0 && c < nc) ? (ax ? mat.phy : mat.phx)[6 * c + r % 6] : T(0);
+    }
   }
   // staging plan (fixed for all layers): scalar e = tid + m*BLOCK of a layer is node row r,
   // entry q of the row's 3*TN contiguous scalars.  Branch-free: a slot outside the grid loads an
@@ -241,12 +344,21 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     }
     face_fwd(v, f);
   };
-  // material of cell layer kk, loaded one step ahead (kinds 0 and 1)
+  // material of cell layer kk, loaded one step ahead (kinds 0, 1 and 3)
   uint8_t labn = LAB_NONE;
   T lamn = T(0), mun = T(0);
+  T cwn[M::kind == 3 ? 16 : 1];
   auto mload = [&](int kk) {
     const bool ok = cin && kk >= 0 && kk < nz;
     const long long cid = ok ? cxy + ncl * kk : 0;
+    if constexpr (M::kind == 3) {
+      const long long ncell = ncl * nz;
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const T v = mat.cw[q * ncell + cid];
+        cwn[q] = ok ? v : T(0);
+      }
+    }
     if constexpr (M::kind == 0) {
       const uint8_t lb = mat.lab[cid];
       labn = ok ? lb : LAB_NONE;
@@ -290,6 +402,11 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     } else {
       act = cin && k >= 0 && k < nz;
     }
+    T cw[M::kind == 3 ? 16 : 1];
+    if constexpr (M::kind == 3) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) cw[q] = cwn[q];
+    }
     if constexpr (M::kind != 2) mload(k + 1);
     __syncthreads();
     T uhi[12];
@@ -322,6 +439,33 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
         }
         done = (mL[0] != T(0)) || (mM[0] != T(0));
         if (done) elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
+      } else if constexpr (M::kind == 3) {
+        T ws = T(0);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) ws += cw[q];
+        done = (ws != T(0));
+        if (done) {
+          T fxL[2][2], fxM[2][2], fyL[2][2], fyM[2][2], fzL[2][2], fzM[2][2];
+          const T* px = s_ph + 6 * tx;
+          const T* py = s_ph + TILE * 6 + 6 * ty;
+          const T* pz = mat.phz + 6 * k;
+#pragma unroll
+          for (int d = 0; d < 2; ++d) {
+            fxM[d][0] = fxL[d][0] = px[3 * d];
+            fxM[d][1] = px[3 * d + 1];
+            fxL[d][1] = M::sri ? px[3 * d + 2] : px[3 * d + 1];
+            fyM[d][0] = fyL[d][0] = py[3 * d];
+            fyM[d][1] = py[3 * d + 1];
+            fyL[d][1] = M::sri ? py[3 * d + 2] : py[3 * d + 1];
+            fzM[d][0] = fzL[d][0] = pz[3 * d];
+            fzM[d][1] = pz[3 * d + 1];
+            fzL[d][1] = M::sri ? pz[3 * d + 2] : pz[3 * d + 1];
+          }
+          T mL[27], mM[27];
+          child_moments<T>(cw, fxL, fyL, fzL, mL);
+          child_moments<T>(cw + 8, fxM, fyM, fzM, mM);
+          elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
+        }
       } else {
         if (!M::iso) {
           const T hz = g.hz[k];
@@ -358,29 +502,40 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
 #pragma unroll
       for (int c = 0; c < 3; ++c) au[c] = cr[0][c] + p1[c] + p2[c] + p3[c];
       const long long n = (long long)k * lay + (long long)cj * nxn + ci;
-      if constexpr (M::kind == 2) {
-        if (mat.bptr) {
-          for (int e = mat.bptr[n]; e < mat.bptr[n + 1]; ++e) {
-            const long long m = mat.bcol[e];
-            const T* B = mat.bval + 9LL * e;
-            const T s0 = s[3 * m], s1 = s[3 * m + 1], s2 = s[3 * m + 2];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) au[c] += B[3 * c] * s0 + B[3 * c + 1] * s1 + B[3 * c + 2] * s2;
-          }
-        }
-        if (mat.rnptr) {
-          for (int e = mat.rnptr[n]; e < mat.rnptr[n + 1]; ++e) {
-            const T dt = mat.dots[mat.rnterm[e]];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) au[c] += mat.rnw[3LL * e + c] * dt;
-          }
-        }
-      }
+      add_extra<T, M>(mat, n, au);
       epilogue<T, EM>(ep, n, au, sn);
     }
 #pragma unroll
     for (int q = 0; q < 12; ++q) ulo[q] = uhi[q];
   }
+}
+
+// Block flags of the structured march for z extent kz: one block per tile id (x fastest, then y,
+// then z), flag = 1 when any cell the march block reads (cell columns I0-1 .. I0+14, rows
+// J0-1 .. J0+14, layers K0-1 .. K0+kz-1) has stiffness.
+template <typename T, class M>
+__global__ __launch_bounds__(BLOCK) void k_block_flags(GridArgs<T> g, M mat, int kz, int ntx, int nty,
+                                                       uint8_t* __restrict__ flag) {
+  const int id = blockIdx.x;
+  const int bx = id % ntx, by = (id / ntx) % nty, bz = id / (ntx * nty);
+  const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
+  const int ci = bx * TOUT - 1 + tx, cj = by * TOUT - 1 + ty;
+  const long long ncl = (long long)g.nx * g.ny, ncell = ncl * g.nz;
+  int any = 0;
+  if (ci >= 0 && ci < g.nx && cj >= 0 && cj < g.ny) {
+    const int k0 = max(bz * kz - 1, 0), k1 = min(bz * kz + kz, g.nz);
+    for (int k = k0; k < k1 && !any; ++k) {
+      const long long cid = ci + (long long)g.nx * cj + ncl * k;
+      if constexpr (M::kind == 0) any = mat.lab[cid] != LAB_NONE;
+      if constexpr (M::kind == 1) any = mat.cl[cid] != T(0) || mat.cm[cid] != T(0);
+      if constexpr (M::kind == 2) any = mat.mom[cid] != T(0) || mat.mom[27 * ncell + cid] != T(0);
+      if constexpr (M::kind == 3) {
+        for (int q = 0; q < 16; ++q) any |= mat.cw[q * ncell + cid] != T(0);
+      }
+    }
+  }
+  any = __syncthreads_or(any);
+  if (threadIdx.x == 0) flag[id] = any ? 1 : 0;
 }
 
 // -------------------------------------------------------------------------------------------
@@ -466,19 +621,54 @@ __global__ __launch_bounds__(256) void k_sparse(SparseArgs<T> a, GridArgs<T> g, 
   for (int q = q0 + lane; q < q1; q += 64) atomicAdd(&tc[a.r1c[q]], acc * a.r1w[q]);
 }
 
-// dots[t] = k_t (w_t . s) for the rank-one terms of a coarse level: one wavefront per term.
+// Extra stiffness of a coarse level, accumulated into ex (all zero on entry; the march that
+// follows adds it to its nodes and clears it).  Two block ranges:
+//   * rank-one terms k_t w_t w_t^T, one wavefront per term: d = k_t (w_t . s) reduced over the
+//     wavefront, then ex[dof] += d w for every entry of the term;
+//   * block entries (row node brow_e, column node bcol_e, 3x3 block B_e row-major), one thread each:
+//     ex[brow_e] += B_e s[bcol_e].
 template <typename T>
-__global__ __launch_bounds__(256) void k_rank1_dots(int nterm, const int* __restrict__ ptr, const int* __restrict__ dof,
-                                                    const T* __restrict__ w, const T* __restrict__ k,
-                                                    const T* __restrict__ s, T* __restrict__ dots) {
-  const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (t >= nterm) return;   // whole wavefronts leave together
-  T acc = T(0);
-  for (int q = ptr[t] + lane; q < ptr[t + 1]; q += 64) acc += w[q] * s[dof[q]];
+struct CoarseExtra {
+  int nterm, nblk;
+  int nb_term;                       // blocks of the first range
+  const int* __restrict__ r1ptr;
+  const int* __restrict__ r1dof;
+  const T* __restrict__ r1w;
+  const T* __restrict__ r1k;
+  const int* __restrict__ brow;
+  const int* __restrict__ bcol;
+  const T* __restrict__ bval;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T* __restrict__ s, T* __restrict__ ex) {
+  if ((int)blockIdx.x < a.nb_term) {
+    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (t >= a.nterm) return;   // whole wavefronts leave together
+    const int q0 = a.r1ptr[t], q1 = a.r1ptr[t + 1];
+    const T kt = a.r1k[t];
+    // the first 64 entries stay in registers between the dot and the scatter
+    const int qf = q0 + lane;
+    const bool has = qf < q1;
+    const int df = has ? a.r1dof[qf] : 0;
+    const T wf = has ? a.r1w[qf] : T(0);
+    T acc = has ? wf * s[df] : T(0);
+    for (int q = qf + 64; q < q1; q += 64) acc += a.r1w[q] * s[a.r1dof[q]];
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-  if (lane == 0) dots[t] = k[t] * acc;
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    const T dk = kt * acc;
+    if (has) atomicAdd(ex + df, dk * wf);
+    for (int q = qf + 64; q < q1; q += 64) atomicAdd(ex + a.r1dof[q], dk * a.r1w[q]);
+    return;
+  }
+  const int e = (blockIdx.x - a.nb_term) * blockDim.x + threadIdx.x;
+  if (e >= a.nblk) return;
+  const long long m = a.bcol[e], n = a.brow[e];
+  const T* B = a.bval + 9LL * e;
+  const T s0 = s[3 * m], s1 = s[3 * m + 1], s2 = s[3 * m + 2];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) atomicAdd(ex + 3 * n + c, B[3 * c] * s0 + B[3 * c + 1] * s1 + B[3 * c + 2] * s2);
 }
 
 // Fix-up after the structured march, one thread per touched node: adds the sparse part tc (and

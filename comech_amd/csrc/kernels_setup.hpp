@@ -54,6 +54,40 @@ __global__ __launch_bounds__(256) void k_moments_from_fine(int ncx, int ncy, int
   }
 }
 
+// Level-1 child weights: lam V and mu V of the (up to) 2x2x2 level-0 children of every level-1
+// cell, slot dx + 2 dy + 4 dz (lam 0..7, mu 8..15; a missing child or inactive cell is 0).  The
+// same fine materials and volumes as k_moments_from_fine; the march expands them into moments.
+__global__ __launch_bounds__(256) void k_child_weights(int ncx, int ncy, int ncz, int nfx, int nfy, AxisNest ax,
+                                                       AxisNest ay, AxisNest az, const double* __restrict__ hx,
+                                                       const double* __restrict__ hy, const double* __restrict__ hz,
+                                                       const double* __restrict__ lam, const double* __restrict__ mu,
+                                                       double* __restrict__ cw) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nc = (long long)ncx * ncy * ncz;
+  if (id >= nc) return;
+  const int I = id % ncx, J = (id / ncx) % ncy, K = id / ((long long)ncx * ncy);
+  double w[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) w[q] = 0.0;
+  for (int dk = 0; dk < az.cnum[K]; ++dk)
+    for (int dj = 0; dj < ay.cnum[J]; ++dj)
+      for (int di = 0; di < ax.cnum[I]; ++di) {
+        const int i = ax.cs[I] + di, j = ay.cs[J] + dj, k = az.cs[K] + dk;
+        const long long f = i + (long long)nfx * (j + (long long)nfy * k);
+        const double V = hx[i] * hy[j] * hz[k];
+        const int slot = di + 2 * dj + 4 * dk;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (q == slot) {   // select chain: constant register indices
+            w[q] = lam[f] * V;
+            w[8 + q] = mu[f] * V;
+          }
+        }
+      }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) cw[q * nc + id] = w[q];
+}
+
 // Level l+1 moments from level-l moments (shift of the child moments into the parent).
 __global__ __launch_bounds__(256) void k_moments_from_moments(int ncx, int ncy, int ncz, int nfx, int nfy, int nfz, AxisNest ax, AxisNest ay,
                                        AxisNest az, const double* __restrict__ momf, double* __restrict__ mom) {

@@ -33,10 +33,12 @@ struct LevelT {
   DevBuf<T> tabL, tabM;     // level 0, label mode (256 entries)
   DevBuf<T> cellL, cellM;   // level 0, value mode (x-fastest, irregular cells zeroed)
   DevBuf<T> irrL, irrM;     // level 0 irregular cells
-  DevBuf<T> mom;            // coarse levels: [54][ncell]
+  DevBuf<T> mom;            // levels >= 2: [54][ncell]
+  DevBuf<T> cw, ph[3];      // level 1: child weights [16][ncell], per-axis child factor tables
   DevBuf<T> Dinv;           // [nn][6] symmetric 3x3 block inverses
   DevBuf<T> tc;             // sparse-pass accumulator, 3 per touched node (kept zero between uses)
-  DevBuf<T> bval, rnw, dots;  // coarse levels: block-row values, rank-one node weights, k_t w_t.s
+  DevBuf<T> bval;           // coarse levels: block values (3x3 row-major per block entry)
+  DevBuf<T> ex;             // coarse levels: extra-stiffness accumulator, 3 per node (kept zero)
   DevBuf<T> pw[3];          // prolongation weights, this level as the fine side
   DevBuf<T> rw[3];          // restriction weights of the next level's coarse nodes
   DevBuf<T> xval;           // extra stiffness values
@@ -84,10 +86,17 @@ struct Level {
   std::vector<int> tnode_h;
   DevBuf<int> tnode, irr_ci, xrc, r1c;
   double iso_fac = 0.0;       // level 0: h / 16 when all cells are the same cube, else 0
-  // coarse levels: extra stiffness applied in the march epilogue (per-node 3x3 block rows and
-  // per-node rank-one weights)
-  DevBuf<int> bptr, bcol, rnptr, rnterm;
-  std::vector<double> bval_h, rnw_h;
+  // coarse levels: the extra-stiffness rows regrouped as 3x3 blocks (row node, column node)
+  DevBuf<int> brow, bcol;
+  std::vector<double> bval_h;
+  // level 1: lam V, mu V of the level-0 children of every cell, child factor tables per axis
+  DevBuf<double> cw64;
+  std::vector<double> ph_h[3];
+  long long nblk = 0, nrn = 0;  // block-row entries and rank-one list entries (totals)
+  DevBuf<uint8_t> bflag[2];     // per precision: stiffness flags of the structured-march tiles
+  DevBuf<int> border[2];        // per precision: block order of the march (~tile = no stiffness)
+  int bflag_kz[2] = {0, 0};     // the z extent flags and order were built for (0 = none)
+  int blk_max = 0, rn_max = 0;  // longest block row and longest rank-one list of one node
   // level 0 only
   bool label_mode = false;
   DevBuf<uint8_t> lab;                      // apply labels (x-fastest, irregular -> 255)
@@ -639,14 +648,14 @@ struct System {
     const auto it = std::lower_bound(sorted.begin(), sorted.end(), (int)node);
     return (int)(it - sorted.begin());
   }
-  // Coarse levels: the extra-stiffness CSR (rows sorted and unique) regrouped as per-node 3x3
-  // block rows, and the rank-one terms as per-node weight lists; both are applied by the march
-  // epilogue of the node's own thread, so coarse levels need no sparse pass.
+  // Coarse levels: the extra-stiffness CSR (rows sorted and unique) regrouped as 3x3 blocks in
+  // row-node order; k_coarse_extra applies them (with the rank-one terms) before the march, so
+  // coarse levels need no sparse pass.  The per-node counts are kept for level_stats.
   void build_coarse_rows(Level& v) {
     const long long nn = v.nn;
     v.ntouch = 0;
     if (!v.xrow_h.empty()) {
-      std::vector<int> bptr(nn + 1, 0), bcol;
+      std::vector<int> bptr(nn + 1, 0), bcol, brow;
       v.bval_h.clear();
       std::map<int, std::array<double, 9>> blocks;
       const size_t nrow = v.xrow_h.size();
@@ -659,34 +668,30 @@ struct System {
         }
         bptr[node + 1] = (int)blocks.size();
         for (auto& kv : blocks) {
+          brow.push_back(node);
           bcol.push_back(kv.first);
           v.bval_h.insert(v.bval_h.end(), kv.second.begin(), kv.second.end());
         }
       }
+      for (long long n = 0; n < nn; ++n) v.blk_max = std::max(v.blk_max, bptr[n + 1]);
       for (long long n = 0; n < nn; ++n) bptr[n + 1] += bptr[n];
-      v.bptr.upload(bptr, stream);
+      v.nblk = bptr[nn];
+      v.brow.upload(brow, stream);
       v.bcol.upload(bcol, stream);
     }
     if (!v.r1k_h.empty()) {
-      struct Ent { int node, term; double w[3]; };
-      std::vector<Ent> ent;
-      for (size_t t = 0; t + 1 < v.r1ptr_h.size(); ++t)
+      // per node: the number of rank-one terms whose support includes it (entries of a term are
+      // sorted by DOF, so a node's entries are adjacent)
+      std::vector<int> cnt(nn, 0);
+      v.nrn = 0;
+      for (size_t t = 0; t + 1 < v.r1ptr_h.size(); ++t) {
+        int last = -1;
         for (int q = v.r1ptr_h[t]; q < v.r1ptr_h[t + 1]; ++q) {
           const int node = v.r1dof_h[q] / 3;
-          if (ent.empty() || ent.back().node != node || ent.back().term != (int)t) ent.push_back(Ent{node, (int)t, {0, 0, 0}});
-          ent.back().w[v.r1dof_h[q] % 3] += v.r1w_h[q];
+          if (node != last) { ++cnt[node]; ++v.nrn; last = node; }
         }
-      std::stable_sort(ent.begin(), ent.end(), [](const Ent& x, const Ent& y) { return x.node < y.node; });
-      std::vector<int> rnptr(nn + 1, 0), rnterm(ent.size());
-      v.rnw_h.resize(3 * ent.size());
-      for (size_t e = 0; e < ent.size(); ++e) {
-        ++rnptr[ent[e].node + 1];
-        rnterm[e] = ent[e].term;
-        for (int c = 0; c < 3; ++c) v.rnw_h[3 * e + c] = ent[e].w[c];
       }
-      for (long long n = 0; n < nn; ++n) rnptr[n + 1] += rnptr[n];
-      v.rnptr.upload(rnptr, stream);
-      v.rnterm.upload(rnterm, stream);
+      for (long long n = 0; n < nn; ++n) v.rn_max = std::max(v.rn_max, cnt[n]);
     }
   }
 
@@ -794,6 +799,22 @@ struct System {
       if (l == 1) {
         hipLaunchKernelGGL(k_moments_from_fine, grid1(c.ncell), dim3(256), 0, stream, c.nx, c.ny, c.nz, f.nx, f.ny,
                            ax, ay, az, f.d64.hx.p, f.d64.hy.p, f.d64.hz.p, f.matL.p, f.matM.p, sri, c.mom64.p);
+        c.cw64.alloc(16 * (size_t)c.ncell);
+        hipLaunchKernelGGL(k_child_weights, grid1(c.ncell), dim3(256), 0, stream, c.nx, c.ny, c.nz, f.nx, f.ny, ax,
+                           ay, az, f.d64.hx.p, f.d64.hy.p, f.d64.hz.p, f.matL.p, f.matM.p, c.cw64.p);
+        // child factor tables: per coarse index, per child d: b, a^2/3 + b^2, b^2 (xi_p = a xi_c + b)
+        const int ncoarse[3] = {c.nx, c.ny, c.nz};
+        for (int a2 = 0; a2 < 3; ++a2) {
+          c.ph_h[a2].assign(6 * (size_t)ncoarse[a2], 0.0);
+          for (int C = 0; C < ncoarse[a2]; ++C)
+            for (int d = 0; d < c.ncn_h[a2][C]; ++d) {
+              const int i = c.ncs_h[a2][C] + d;
+              const double A = c.na_h[a2][i], B = c.nb_h[a2][i];
+              c.ph_h[a2][6 * C + 3 * d] = B;
+              c.ph_h[a2][6 * C + 3 * d + 1] = A * A / 3.0 + B * B;
+              c.ph_h[a2][6 * C + 3 * d + 2] = B * B;
+            }
+        }
       } else {
         hipLaunchKernelGGL(k_moments_from_moments, grid1(c.ncell), dim3(256), 0, stream, c.nx, c.ny, c.nz, f.nx, f.ny,
                            f.nz, ax, ay, az, f.mom64.p, c.mom64.p);
@@ -945,6 +966,12 @@ struct System {
           CM_LAUNCH_CHECK();
         }
         if (v.nirr) { upload_cast(d.irrL, v.irrL_h, stream); upload_cast(d.irrM, v.irrM_h, stream); }
+      } else if (l == 1) {
+        d.cw.alloc(v.cw64.n);
+        hipLaunchKernelGGL((k_cast<T, double>), grid1((long long)v.cw64.n), dim3(256), 0, stream,
+                           (long long)v.cw64.n, d.cw.p, v.cw64.p);
+        CM_LAUNCH_CHECK();
+        for (int ax = 0; ax < 3; ++ax) upload_cast(d.ph[ax], v.ph_h[ax], stream);
       } else if (v.mom64.n) {
         d.mom.alloc(v.mom64.n);
         hipLaunchKernelGGL((k_cast<T, double>), grid1((long long)v.mom64.n), dim3(256), 0, stream,
@@ -957,9 +984,10 @@ struct System {
       if (!v.xval_h.empty()) upload_cast(d.xval, v.xval_h, stream);
       if (!v.r1k_h.empty()) { upload_cast(d.r1w, v.r1w_h, stream); upload_cast(d.r1k, v.r1k_h, stream); }
       if (!v.bval_h.empty()) upload_cast(d.bval, v.bval_h, stream);
-      if (!v.rnw_h.empty()) {
-        upload_cast(d.rnw, v.rnw_h, stream);
-        d.dots.alloc(v.r1k_h.size());
+      if (l > 0 && (v.nblk || !v.r1k_h.empty())) {
+        d.ex.alloc(3 * (size_t)v.nn);
+        hipLaunchKernelGGL(k_fill<T>, grid1(3LL * v.nn), dim3(256), 0, stream, d.ex.p, 3LL * v.nn, T(0));
+        CM_LAUNCH_CHECK();
       }
       const size_t nv = 3 * (size_t)v.nn;
       d.x.alloc(nv); d.b.alloc(nv); d.r.alloc(nv); d.t.alloc(nv); d.d.alloc(nv);
@@ -976,6 +1004,7 @@ struct System {
       CM_LAUNCH_CHECK();
       if (opt.smoother == 0) power_iteration<T>(l);
     }
+    refresh_flags<T>(l);
   }
 
   template <typename T>
@@ -1007,10 +1036,15 @@ struct System {
     LevelT<T>& d = v.dat<T>();
     const GridArgs<T> g = grid_args<T>(l);
     if (v.ntouch) launch_sparse<T>(v, d, g, s);
-    if (l > 0 && v.rnptr.p) {
-      const int nterm = (int)v.r1k_h.size();
-      hipLaunchKernelGGL(k_rank1_dots<T>, grid1(64LL * nterm), dim3(256), 0, stream, nterm, v.r1ptr.p, v.r1dof.p,
-                         d.r1w.p, d.r1k.p, s, d.dots.p);
+    if (l > 0 && d.ex.p) {
+      CoarseExtra<T> a{};
+      a.nterm = (int)v.r1k_h.size();
+      a.nblk = (int)v.nblk;
+      a.nb_term = (int)div_up(64LL * a.nterm, 256);
+      a.r1ptr = v.r1ptr.p; a.r1dof = v.r1dof.p; a.r1w = d.r1w.p; a.r1k = d.r1k.p;
+      a.brow = v.brow.p; a.bcol = v.bcol.p; a.bval = d.bval.p;
+      const long long nb = a.nb_term + div_up(a.nblk, 256);
+      if (nb > 0) hipLaunchKernelGGL(k_coarse_extra<T>, dim3((unsigned)nb), dim3(256), 0, stream, a, s, d.ex.p);
     }
     launch_march<T, EM>(l, s, ep);
     if (v.ntouch) {
@@ -1046,29 +1080,25 @@ struct System {
     }
   }
 
-  // structured march of level l with epilogue EM
-  template <typename T, int EM>
-  void launch_march(int l, const T* s, const Epi<T>& ep) {
+  // Calls f with the cell-material policy of level l's structured march.
+  template <typename T, class F>
+  void with_mat(int l, F&& f) {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
-    const GridArgs<T> g = grid_args<T>(l);
-    const int kz = pick_kz<T>(v);
-    const int ntx = (int)div_up(v.nx + 1, TOUT), nty = (int)div_up(v.ny + 1, TOUT), ntz = (int)div_up(v.nz + 1, kz);
-    const long long nb = (long long)ntx * nty * ntz;
-    CM_CHECK(nb < (1LL << 31), "grid too large for one launch");
-    const TileMap tm{ntx, nty, ntz, (int)(nb / 8), (int)(nb % 8)};
-    const dim3 grid((unsigned)nb), blk(BLOCK);
-    if (l > 0) {
-      MomentMat<T> m{d.mom.p, v.bptr.p, v.bcol.p, d.bval.p, v.rnptr.p, v.rnterm.p, d.rnw.p, d.dots.p};
-      hipLaunchKernelGGL((k_march<T, MomentMat<T>, EM>), grid, blk, 0, stream, g, m, s, ep, kz, tm);
+    if (l == 1) {
+      if (sri) f(ChildMat<T, true>{d.cw.p, d.ph[0].p, d.ph[1].p, d.ph[2].p, d.ex.p});
+      else f(ChildMat<T, false>{d.cw.p, d.ph[0].p, d.ph[1].p, d.ph[2].p, d.ex.p});
+      return;
+    }
+    if (l > 1) {
+      f(MomentMat<T>{d.mom.p, d.ex.p});
       return;
     }
     const bool iso = v.iso_fac > 0.0;
-#define CM_FINE(MAT, SRI_, ISO_, ...)                                                                      \
-  if (sri == SRI_ && iso == ISO_) {                                                                        \
-    MAT<T, SRI_, ISO_> m{__VA_ARGS__};                                                                     \
-    hipLaunchKernelGGL((k_march<T, MAT<T, SRI_, ISO_>, EM>), grid, blk, 0, stream, g, m, s, ep, kz, tm); \
-    return;                                                                                                \
+#define CM_FINE(MAT, SRI_, ISO_, ...)      \
+  if (sri == SRI_ && iso == ISO_) {        \
+    f(MAT<T, SRI_, ISO_>{__VA_ARGS__});    \
+    return;                                \
   }
     if (v.label_mode) {
       CM_FINE(LabelMat, true, true, v.lab.p, d.tabL.p, d.tabM.p)
@@ -1084,6 +1114,88 @@ struct System {
 #undef CM_FINE
   }
 
+  struct MarchShape {
+    int kz, ntx, nty, ntz;
+    long long nb;
+  };
+  template <typename T>
+  MarchShape march_shape(int l, int kz) {
+    Level& v = lv(l);
+    MarchShape m{kz, (int)div_up(v.nx + 1, TOUT), (int)div_up(v.ny + 1, TOUT), (int)div_up(v.nz + 1, kz), 0};
+    m.nb = (long long)m.ntx * m.nty * m.ntz;
+    CM_CHECK(m.nb < (1LL << 31), "grid too large for one launch");
+    return m;
+  }
+
+  // Empty-block flags of level l for the z extent the march uses now (rebuilt when it changed).
+  // Runs outside graph capture: entry points refresh before they launch or capture.
+  template <typename T>
+  void refresh_flags(int l) {
+    Level& v = lv(l);
+    if (!v.dat<T>().ready || l == nlev() - 1) return;
+    const int slot = std::is_same<T, float>::value ? 1 : 0;
+    const int kz = pick_kz<T>(v);
+    if (v.bflag_kz[slot] == kz) return;
+    const MarchShape ms = march_shape<T>(l, kz);
+    if (v.bflag[slot].n < (size_t)ms.nb) v.bflag[slot].alloc((size_t)march_shape<T>(l, 1).nb);
+    const GridArgs<T> g = grid_args<T>(l);
+    with_mat<T>(l, [&](auto m) {
+      using M = decltype(m);
+      hipLaunchKernelGGL((k_block_flags<T, M>), dim3((unsigned)ms.nb), dim3(BLOCK), 0, stream, g, m, kz, ms.ntx,
+                         ms.nty, v.bflag[slot].p);
+    });
+    CM_LAUNCH_CHECK();
+    const int nb = (int)ms.nb;
+    std::vector<uint8_t> fl(nb);
+    CM_HIP(hipMemcpyAsync(fl.data(), v.bflag[slot].p, nb, hipMemcpyDeviceToHost, stream));
+    CM_HIP(hipStreamSynchronize(stream));
+    // blocks 0 .. nf-1 take the tiles with stiffness, the rest the others; block b runs on XCD
+    // b % 8, and every XCD gets one contiguous run of each list (tile order: x, then y, then z)
+    std::vector<int> full, none;
+    for (int id = 0; id < nb; ++id) (fl[id] ? full : none).push_back(id);
+    const int nf = (int)full.size();
+    std::vector<int> ord(nb);
+    auto deal = [&](const std::vector<int>& list, int b0, bool neg) {
+      const int n = (int)list.size();
+      int cnt[8], start[8], first[8];
+      for (int x = 0; x < 8; ++x) {
+        first[x] = b0 + ((x - b0 % 8) + 8) % 8;                   // first block >= b0 on XCD x
+        cnt[x] = first[x] < b0 + n ? (b0 + n - first[x] + 7) / 8 : 0;
+      }
+      start[0] = 0;
+      for (int x = 1; x < 8; ++x) start[x] = start[x - 1] + cnt[x - 1];
+      for (int b = b0; b < b0 + n; ++b) {
+        const int x = b % 8, pos = (b - first[x]) / 8;
+        const int id = list[start[x] + pos];
+        ord[b] = neg ? ~id : id;
+      }
+    };
+    deal(full, 0, false);
+    deal(none, nf, true);
+    v.border[slot].upload(ord, stream);
+    v.bflag_kz[slot] = kz;
+  }
+  template <typename T>
+  void refresh_all_flags() {
+    for (int l = 0; l < nlev(); ++l) refresh_flags<T>(l);
+  }
+
+  // structured march of level l with epilogue EM
+  template <typename T, int EM>
+  void launch_march(int l, const T* s, const Epi<T>& ep) {
+    Level& v = lv(l);
+    const GridArgs<T> g = grid_args<T>(l);
+    const int slot = std::is_same<T, float>::value ? 1 : 0;
+    const MarchShape ms = march_shape<T>(l, pick_kz<T>(v));
+    const TileMap tm{ms.ntx, ms.nty, ms.ntz, (int)(ms.nb / 8), (int)(ms.nb % 8)};
+    const int* fl = (v.bflag_kz[slot] == ms.kz && use_flags) ? v.border[slot].p : nullptr;
+    with_mat<T>(l, [&](auto m) {
+      using M = decltype(m);
+      hipLaunchKernelGGL((k_march<T, M, EM>), dim3((unsigned)ms.nb), dim3(BLOCK), 0, stream, g, m, s, ep, ms.kz, tm,
+                         fl);
+    });
+  }
+
   // structured part only, plain product (timing, tuning)
   template <typename T>
   void apply_structured(int l, const T* u, T* y) {
@@ -1093,6 +1205,7 @@ struct System {
   }
 
   int kz_override = 0;
+  bool use_flags = true;      // empty blocks of the structured march take the epilogue-only path
   template <typename T>
   int pick_kz(const Level& v) const {
     if (kz_override > 0) return kz_override;
@@ -1145,6 +1258,7 @@ struct System {
         auto it = kz_cache().find(key);
         if (it != kz_cache().end()) {
           v.kz[slot] = it->second;
+          refresh_flags<T>(l);
           continue;
         }
       }
@@ -1153,6 +1267,7 @@ struct System {
       for (int kz : {1, 2, 4, 8, 16, 32}) {
         if (kz > v.nz + 1 && kz != 1) continue;
         v.kz[slot] = kz;
+        refresh_flags<T>(l);
         apply_structured<T>(l, d.x.p, d.t.p);
         CM_HIP(hipEventRecord(e0, stream));
         for (int r = 0; r < nrep; ++r) apply_structured<T>(l, d.x.p, d.t.p);
@@ -1163,6 +1278,7 @@ struct System {
         if (ms < best) { best = ms; best_kz = kz; }
       }
       v.kz[slot] = best_kz;
+      refresh_flags<T>(l);
       std::lock_guard<std::mutex> g(kz_mu());
       kz_cache()[key] = best_kz;
     }
@@ -1822,6 +1938,17 @@ int cm_sys_levels(cm_sys* h, int* nlev, int64_t* desc, int cap) {
   CM_API_END
 }
 
+int cm_sys_level_stats(cm_sys* h, int level, int64_t* st, int cap) {
+  CM_API_BEGIN
+  need_level(h, level);
+  cm::Level& v = h->s.lv(level);
+  const int64_t vals[10] = {v.ncell, v.nirr, (int64_t)v.xrow_h.size(), (int64_t)v.xcol_h.size(),
+                            (int64_t)(v.r1ptr_h.empty() ? 0 : v.r1ptr_h.size() - 1), (int64_t)v.r1dof_h.size(),
+                            v.nblk, v.blk_max, v.nrn, v.rn_max};
+  for (int q = 0; q < 10 && q < cap; ++q) st[q] = vals[q];
+  CM_API_END
+}
+
 int cm_sys_apply(cm_sys* h, int level, int prec, const void* u, void* y) {
   CM_API_BEGIN
   need_level(h, level);
@@ -1964,6 +2091,18 @@ int cm_sys_set_kz(cm_sys* h, int kz) {
   CM_CHECK(h != nullptr, "null system");
   CM_CHECK(kz >= 0 && kz <= 4096, "kz out of range");
   h->s.kz_override = kz;
+  h->s.drop_graphs();
+  if (h->s.built) {
+    h->s.refresh_all_flags<double>();
+    h->s.refresh_all_flags<float>();
+  }
+  CM_API_END
+}
+
+int cm_sys_set_block_skip(cm_sys* h, int on) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  h->s.use_flags = on != 0;
   h->s.drop_graphs();
   CM_API_END
 }

@@ -139,6 +139,15 @@ class VoxelSystem:
         return [dict(nx=int(r[0]), ny=int(r[1]), nz=int(r[2]), nodes=int(r[3]), ndof=int(r[4]), extra_rows=int(r[5]))
                 for r in d]
 
+    def level_stats(self, level):
+        """Operator structure of one level: cell, irregular-cell, extra-row and rank-one counts and
+        the coarse per-node list lengths applied inside the structured march."""
+        st = np.zeros(10, np.int64)
+        check(self.lib.cm_sys_level_stats(self.h, int(level), ptr(st, ctypes.c_longlong), 10))
+        keys = ("cells", "irregular", "extra_rows", "extra_nnz", "rank1_terms", "rank1_nnz", "block_entries",
+                "block_row_max", "rank1_node_entries", "rank1_node_max")
+        return dict(zip(keys, (int(v) for v in st)))
+
     def timings(self):
         """Build stage times (ms)."""
         t = np.zeros(9)
@@ -200,6 +209,10 @@ class VoxelSystem:
 
     def set_kz(self, kz):
         check(self.lib.cm_sys_set_kz(self.h, int(kz)))
+
+    def set_block_skip(self, on=True):
+        """Structured-march blocks without stiffness run only the epilogue (default on)."""
+        check(self.lib.cm_sys_set_block_skip(self.h, int(bool(on))))
 
     def set_graphs(self, on=True):
         """Replay the V-cycle through a captured HIP graph (default) or launch it eagerly."""

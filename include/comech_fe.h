@@ -99,6 +99,12 @@ int cm_sys_build(cm_sys* sys, int min_dofs, int max_levels, int fixed_levels, in
 /* Level table: 6 int64 per level: nx, ny, nz, nodes (incl. copies), compact DOFs, extra rows. */
 int cm_sys_levels(cm_sys* sys, int* nlev, int64_t* desc, int cap);
 
+/* Operator structure of one level (diagnostics; no reference counterpart): 10 int64 --
+ * cells, irregular cells, extra-stiffness CSR rows and entries, rank-one terms and their DOF
+ * entries, coarse block-row entries and the longest block row, rank-one node-list entries and the
+ * longest list of one node. */
+int cm_sys_level_stats(cm_sys* sys, int level, int64_t* stats, int cap);
+
 /* ---- compute (device pointers) ------------------------------------------------------------- */
 
 /* y = A_level u (A64 @ p of rb/gpusolve.py:239; L.A @ x of :104). */
@@ -185,6 +191,10 @@ int cm_bank_pack(void* stream, int64_t ncell, int ncls, const float* vm, const f
 
 /* Override the z extent of a structured block (0 = automatic). */
 int cm_sys_set_kz(cm_sys* sys, int kz);
+
+/* Structured-march blocks whose cells have no stiffness run only the epilogue (on by default;
+ * 0 = every block runs the full march).  Both give the same results; no reference counterpart. */
+int cm_sys_set_block_skip(cm_sys* sys, int on);
 
 /* V-cycle replay through HIP graphs (on by default): cm_sys_solve and cm_sys_vcycle run on the
  * system's own stream (ordered after the caller's stream by events) and replay the captured

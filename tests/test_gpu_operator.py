@@ -219,3 +219,38 @@ def test_vcycle_graph_replays_match_eager(dt, tol):
         y = s.apply(r)
         _ = (y * 2.0 + r).sum().item()
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("kz", [0, 1, 4])
+def test_block_skip_equals_full_march(kz):
+    """Structured-march blocks whose cells carry no stiffness run only the epilogue; the result
+    equals the full march on every level, in both precisions (operator and V-cycle), and FP32
+    equals FP64 to FP32 rounding (covers the level-1 child-weight and coarser moment cells)."""
+    c, s = _sys2(levels=4)
+    if kz:
+        s.set_kz(kz)
+    lv = s.levels()
+    rng = np.random.default_rng(5)
+    for l in range(len(lv) - 1):
+        u = rng.standard_normal(3 * lv[l]["nodes"])
+        outs = {}
+        for skip in (True, False):
+            s.set_block_skip(skip)
+            for dt in (torch.float64, torch.float32):
+                outs[skip, dt] = s.apply(torch.tensor(u, dtype=dt, device=DEV), level=l).double().cpu().numpy()
+        ref = outs[False, torch.float64]
+        scale = np.abs(ref).max()
+        assert np.abs(outs[True, torch.float64] - ref).max() <= 1e-13 * scale, l
+        assert np.abs(outs[True, torch.float32] - outs[False, torch.float32]).max() <= 1e-6 * scale, l
+        assert np.abs(outs[True, torch.float32] - ref).max() <= 1e-5 * scale, l
+    nn = len(c["orig"])
+    r = rng.standard_normal(3 * nn)
+    zs = {}
+    for skip in (True, False):
+        s.set_block_skip(skip)
+        for dt in (torch.float64, torch.float32):
+            zs[skip, dt] = s.vcycle(torch.tensor(r, dtype=dt, device=DEV)).double().cpu().numpy()
+    scale = np.abs(zs[False, torch.float64]).max()
+    assert np.abs(zs[True, torch.float64] - zs[False, torch.float64]).max() <= 1e-12 * scale
+    assert np.abs(zs[True, torch.float32] - zs[False, torch.float32]).max() <= 1e-5 * scale
+    s.close()

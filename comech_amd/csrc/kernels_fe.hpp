@@ -157,6 +157,7 @@ struct Epi {
   T* dout;            // CHEB: new direction d (nullptr: not kept)
   T* x;               // CHEB: iterate, updated in place (nullptr: not updated)
   T c1, c2, xa;
+  double* part;       // APPLY: per-block partial sums of s . (A s) (nullptr: none)
 };
 
 template <typename T>
@@ -260,6 +261,8 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   const long long nxn = nx + 1, nyn = ny + 1, lay = nxn * nyn;
   const long long ncl = (long long)nx * ny;
+  const bool want_dot = (EM == EPI_APPLY) && ep.part != nullptr;
+  double dsum = 0.0;     // s . (A s) over this block's output nodes (APPLY with part)
   if (empty) {   // block-uniform
     const int ci = I0 - 1 + tx, cj = J0 - 1 + ty;
     if (tx >= 1 && ty >= 1 && ci <= nx && cj <= ny) {
@@ -267,13 +270,18 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
       for (int k = K0; k < kend; ++k) {
         const long long n = (long long)k * lay + (long long)cj * nxn + ci;
         T au[3] = {T(0), T(0), T(0)}, sn[3] = {T(0), T(0), T(0)};
-        if (EM == EPI_CHEB) {
+        if (EM == EPI_CHEB || want_dot) {
 #pragma unroll
           for (int c = 0; c < 3; ++c) sn[c] = s[3 * n + c];
         }
         add_extra<T, M>(mat, n, au);
+        if (want_dot) dsum += (double)sn[0] * au[0] + (double)sn[1] * au[1] + (double)sn[2] * au[2];
         epilogue<T, EM>(ep, n, au, sn);
       }
+    }
+    if (want_dot) {
+      dsum = block_sum<BLOCK>(dsum);
+      if (tid == 0) ep.part[blockIdx.x] = dsum;
     }
     return;
   }
@@ -412,7 +420,7 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     T uhi[12];
     face(k + 1, uhi);
     T sn[3] = {T(0), T(0), T(0)};
-    if (EM == EPI_CHEB) {
+    if (EM == EPI_CHEB || want_dot) {
       const T* src = s_u[k & 1] + (ty * TN + tx) * 4;
 #pragma unroll
       for (int c = 0; c < 3; ++c) sn[c] = src[c];
@@ -503,10 +511,15 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
       for (int c = 0; c < 3; ++c) au[c] = cr[0][c] + p1[c] + p2[c] + p3[c];
       const long long n = (long long)k * lay + (long long)cj * nxn + ci;
       add_extra<T, M>(mat, n, au);
+      if (want_dot) dsum += (double)sn[0] * au[0] + (double)sn[1] * au[1] + (double)sn[2] * au[2];
       epilogue<T, EM>(ep, n, au, sn);
     }
 #pragma unroll
     for (int q = 0; q < 12; ++q) ulo[q] = uhi[q];
+  }
+  if (want_dot) {
+    dsum = block_sum<BLOCK>(dsum);
+    if (tid == 0) ep.part[blockIdx.x] = dsum;
   }
 }
 
@@ -549,6 +562,7 @@ template <typename T>
 struct SparseArgs {
   int nirr, nrow, nterm;
   int nb_irr, nb_row;                // blocks of the first two ranges
+  int rw;                            // lanes per rank-one term (power of two <= 64)
   const int* __restrict__ irr_nodes; // [nirr][8] node ids
   const int* __restrict__ irr_ci;    // [nirr][8] compact node index
   const int* __restrict__ irr_ijk;   // [nirr][3]
@@ -609,16 +623,16 @@ __global__ __launch_bounds__(256) void k_sparse(SparseArgs<T> a, GridArgs<T> g, 
     return;
   }
   b -= a.nb_row;
-  const int t = (b * 256 + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (t >= a.nterm) return;   // whole wavefronts leave together
+  const int rw = a.rw;
+  const long long gid = (long long)b * 256 + threadIdx.x;
+  const int t = (int)(gid / rw), lane = (int)(gid % rw);
+  if (t >= a.nterm) return;   // the rw lanes of a term leave together
   const int q0 = a.r1ptr[t], q1 = a.r1ptr[t + 1];
   T acc = T(0);
-  for (int q = q0 + lane; q < q1; q += 64) acc += a.r1w[q] * s[a.r1dof[q]];
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  for (int q = q0 + lane; q < q1; q += rw) acc += a.r1w[q] * s[a.r1dof[q]];
+  for (int off = rw / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, rw);
   acc *= a.r1k[t];
-  for (int q = q0 + lane; q < q1; q += 64) atomicAdd(&tc[a.r1c[q]], acc * a.r1w[q]);
+  for (int q = q0 + lane; q < q1; q += rw) atomicAdd(&tc[a.r1c[q]], acc * a.r1w[q]);
 }
 
 // Extra stiffness of a coarse level, accumulated into ex (all zero on entry; the march that
@@ -631,6 +645,7 @@ template <typename T>
 struct CoarseExtra {
   int nterm, nblk;
   int nb_term;                       // blocks of the first range
+  int rw;                            // lanes per rank-one term (power of two <= 64)
   const int* __restrict__ r1ptr;
   const int* __restrict__ r1dof;
   const T* __restrict__ r1w;
@@ -643,23 +658,23 @@ struct CoarseExtra {
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T* __restrict__ s, T* __restrict__ ex) {
   if ((int)blockIdx.x < a.nb_term) {
-    const int t = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int lane = threadIdx.x & 63;
-    if (t >= a.nterm) return;   // whole wavefronts leave together
+    const int rw = a.rw;
+    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = (int)(gid / rw), lane = (int)(gid % rw);
+    if (t >= a.nterm) return;   // the rw lanes of a term leave together
     const int q0 = a.r1ptr[t], q1 = a.r1ptr[t + 1];
     const T kt = a.r1k[t];
-    // the first 64 entries stay in registers between the dot and the scatter
+    // the first rw entries stay in registers between the dot and the scatter
     const int qf = q0 + lane;
     const bool has = qf < q1;
     const int df = has ? a.r1dof[qf] : 0;
     const T wf = has ? a.r1w[qf] : T(0);
     T acc = has ? wf * s[df] : T(0);
-    for (int q = qf + 64; q < q1; q += 64) acc += a.r1w[q] * s[a.r1dof[q]];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    for (int q = qf + rw; q < q1; q += rw) acc += a.r1w[q] * s[a.r1dof[q]];
+    for (int off = rw / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, rw);
     const T dk = kt * acc;
     if (has) atomicAdd(ex + df, dk * wf);
-    for (int q = qf + 64; q < q1; q += 64) atomicAdd(ex + a.r1dof[q], dk * a.r1w[q]);
+    for (int q = qf + rw; q < q1; q += rw) atomicAdd(ex + a.r1dof[q], dk * a.r1w[q]);
     return;
   }
   const int e = (blockIdx.x - a.nb_term) * blockDim.x + threadIdx.x;
@@ -677,9 +692,20 @@ __global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T*
 // r -= t, d -= c2 Dinv t, x -= c2 Dinv t.  Split copies have no structured part: their epilogue
 // runs here in full with A s = tc.
 template <typename T, int EM>
-__global__ void k_fixup(int ntouch, const int* __restrict__ tnode, long long nng, T* __restrict__ tc,
-                        const T* __restrict__ s, Epi<T> ep) {
+__global__ __launch_bounds__(256) void k_fixup(int ntouch, const int* __restrict__ tnode, long long nng,
+                                               T* __restrict__ tc, const T* __restrict__ s, Epi<T> ep) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (EM == EPI_APPLY && ep.part) {
+    // s . tc over the touched nodes: A s = structured part (its dot came from the march) + tc
+    double d = 0.0;
+    if (i < ntouch) {
+      const long long n = tnode[i];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) d += (double)s[3 * n + c] * (double)tc[3 * i + c];
+    }
+    d = block_sum<256>(d);
+    if (threadIdx.x == 0) ep.part[blockIdx.x] = d;
+  }
   if (i >= ntouch) return;
   const long long n = tnode[i];
   T t[3];

@@ -335,10 +335,11 @@ struct System {
   int nlev() const { return (int)L.size(); }
 
   // ------------------------------------------------------------------ construction helpers
-  static void axis_transfer(const std::vector<double>& ef, Level& fine, Level& coarse, int ax) {
+  // keep = this axis is not coarsened (every fine node is a coarse node; identity transfer)
+  static void axis_transfer(const std::vector<double>& ef, Level& fine, Level& coarse, int ax, bool keep) {
     const int nfn = (int)ef.size();          // fine nodes
     std::vector<int> cidx;
-    for (int i = 0; i < nfn; i += 2) cidx.push_back(i);
+    for (int i = 0; i < nfn; i += (keep ? 1 : 2)) cidx.push_back(i);
     if (cidx.back() != nfn - 1) cidx.push_back(nfn - 1);
     const int ncn = (int)cidx.size();
     std::vector<double> ec(ncn);
@@ -622,12 +623,13 @@ struct System {
     lv.xw = w;
   }
 
-  long long compact_dofs(int l) {
-    Level& v = lv(l);
+  static long long compact_dofs_of(const Level& v) {
     long long n = 0;
     for (auto u : v.used_h) n += u;
     return 3 * n;
   }
+  long long compact_dofs(int l) { return compact_dofs_of(lv(l)); }
+  int last_axes = 3;   // axes the step to the coarsest level coarsens (7 = all, the reference rule)
 
   // ------------------------------------------------------------------ build
   // h / 16 when every cell of the level is the same cube (edge lengths equal to 1e-13 relative),
@@ -744,17 +746,30 @@ struct System {
       const bool last = fixed_levels > 0 ? (l == fixed_levels - 1) : (compact_dofs(l) <= min_dofs || l == nl_max - 1);
       if (last) break;
       Level& f = lv(l);
-      auto c = std::make_unique<Level>();
-      for (int ax = 0; ax < 3; ++ax) axis_transfer(f.e[ax], f, *c, ax);
-      c->nx = (int)c->e[0].size() - 1; c->ny = (int)c->e[1].size() - 1; c->nz = (int)c->e[2].size() - 1;
-      c->nng = c->nn = (long long)(c->nx + 1) * (c->ny + 1) * (c->nz + 1);
-      c->ncell = (long long)c->nx * c->ny * c->nz;
-      auto ta = clk::now();
-      coarse_used(f, *c);
-      auto tb = clk::now();
-      coarse_extra(f, *c);
-      t_stage[7] += std::chrono::duration<double, std::milli>(tb - ta).count();
-      t_stage[8] += std::chrono::duration<double, std::milli>(clk::now() - tb).count();
+      auto coarsen = [&](int axes) {
+        auto c = std::make_unique<Level>();
+        for (int ax = 0; ax < 3; ++ax) axis_transfer(f.e[ax], f, *c, ax, !((axes >> ax) & 1));
+        c->nx = (int)c->e[0].size() - 1; c->ny = (int)c->e[1].size() - 1; c->nz = (int)c->e[2].size() - 1;
+        c->nng = c->nn = (long long)(c->nx + 1) * (c->ny + 1) * (c->nz + 1);
+        c->ncell = (long long)c->nx * c->ny * c->nz;
+        auto ta = clk::now();
+        coarse_used(f, *c);
+        auto tb = clk::now();
+        coarse_extra(f, *c);
+        t_stage[7] += std::chrono::duration<double, std::milli>(tb - ta).count();
+        t_stage[8] += std::chrono::duration<double, std::milli>(clk::now() - tb).count();
+        return c;
+      };
+      auto c = coarsen(7);
+      // The level just built is the coarsest (dense direct solve): coarsen it along last_axes only
+      // when that level still has <= min_dofs DOFs.  Keeping the vertical axis (the direction of
+      // the ground springs and of the load) keeps the bones stacked along it apart on the coarsest
+      // grid.  (Rebuilding restores the fine level's transfer tables for the variant kept.)
+      if (fixed_levels <= 0 && last_axes != 7 && compact_dofs_of(*c) <= min_dofs) {
+        auto cs = coarsen(last_axes);
+        if (compact_dofs_of(*cs) <= min_dofs) c = std::move(cs);
+        else c = coarsen(7);
+      }
       L.push_back(std::move(c));
     }
     lap(1);
@@ -1036,22 +1051,66 @@ struct System {
     LevelT<T>& d = v.dat<T>();
     const GridArgs<T> g = grid_args<T>(l);
     if (v.ntouch) launch_sparse<T>(v, d, g, s);
-    if (l > 0 && d.ex.p) {
-      CoarseExtra<T> a{};
-      a.nterm = (int)v.r1k_h.size();
-      a.nblk = (int)v.nblk;
-      a.nb_term = (int)div_up(64LL * a.nterm, 256);
-      a.r1ptr = v.r1ptr.p; a.r1dof = v.r1dof.p; a.r1w = d.r1w.p; a.r1k = d.r1k.p;
-      a.brow = v.brow.p; a.bcol = v.bcol.p; a.bval = d.bval.p;
-      const long long nb = a.nb_term + div_up(a.nblk, 256);
-      if (nb > 0) hipLaunchKernelGGL(k_coarse_extra<T>, dim3((unsigned)nb), dim3(256), 0, stream, a, s, d.ex.p);
-    }
+    if (l > 0 && d.ex.p) launch_extra<T>(l, s);
     launch_march<T, EM>(l, s, ep);
     if (v.ntouch) {
       hipLaunchKernelGGL((k_fixup<T, EM>), grid1(v.ntouch), dim3(256), 0, stream, v.ntouch, v.tnode.p, v.nng, d.tc.p, s,
                          ep);
     }
     CM_LAUNCH_CHECK();
+  }
+
+  // extra stiffness of a coarse level into its accumulator ex (consumed by the next march)
+  template <typename T>
+  void launch_extra(int l, const T* s) {
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    CoarseExtra<T> a{};
+    a.nterm = (int)v.r1k_h.size();
+    a.nblk = (int)v.nblk;
+    a.rw = term_lanes(v);
+    a.nb_term = (int)div_up((long long)a.rw * a.nterm, 256);
+    a.r1ptr = v.r1ptr.p; a.r1dof = v.r1dof.p; a.r1w = d.r1w.p; a.r1k = d.r1k.p;
+    a.brow = v.brow.p; a.bcol = v.bcol.p; a.bval = d.bval.p;
+    const long long nb = a.nb_term + div_up(a.nblk, 256);
+    if (nb > 0) hipLaunchKernelGGL(k_coarse_extra<T>, dim3((unsigned)nb), dim3(256), 0, stream, a, s, d.ex.p);
+  }
+
+  // lanes per rank-one term in the sparse passes: the power of two >= the mean support (<= 64)
+  static int term_lanes(const Level& v) {
+    const size_t nt = v.r1ptr_h.empty() ? 0 : v.r1ptr_h.size() - 1;
+    const double avg = nt ? (double)v.r1dof_h.size() / nt : 1.0;
+    int w = 1;
+    while (w < 64 && w < avg) w <<= 1;
+    return w;
+  }
+
+  // Ap = A p with the dot p . Ap into S[slot] (FP64 PCG): the march and the fix-up leave per-block
+  // partials, summed in fixed order (no separate pass over p and Ap).
+  void apply_dot(int l, const double* p, double* Ap, int slot) {
+    Level& v = lv(l);
+    const int nbm = (int)march_shape<double>(l, pick_kz<double>(v)).nb;
+    const int nbf = v.ntouch ? div_up(v.ntouch, 256) : 0;
+    CM_CHECK(partial.n >= (size_t)(nbm + nbf), "partial buffer too small for the fused dot");
+    Epi<double> ep{nullptr, Ap, nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, partial.p};
+    LevelT<double>& d = v.dat<double>();
+    const GridArgs<double> g = grid_args<double>(l);
+    if (v.ntouch) launch_sparse<double>(v, d, g, p);
+    if (l > 0 && d.ex.p) launch_extra<double>(l, p);
+    launch_march<double, EPI_APPLY>(l, p, ep);
+    if (v.ntouch) {
+      Epi<double> ef = ep;
+      ef.part = partial.p + nbm;
+      hipLaunchKernelGGL((k_fixup<double, EPI_APPLY>), grid1(v.ntouch), dim3(256), 0, stream, v.ntouch, v.tnode.p,
+                         v.nng, d.tc.p, p, ef);
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, nbm + nbf, partial.p, S.p, slot);
+    CM_LAUNCH_CHECK();
+  }
+  // partials needed by apply_dot on level l (FP64)
+  size_t dot_partials(int l) {
+    Level& v = lv(l);
+    return (size_t)march_shape<double>(l, pick_kz<double>(v)).nb + (v.ntouch ? div_up(v.ntouch, 256) : 0);
   }
 
   template <typename T>
@@ -1062,7 +1121,8 @@ struct System {
     a.nterm = (int)v.r1k_h.size();
     a.nb_irr = (int)div_up(a.nirr, 256);
     a.nb_row = (int)div_up((long long)a.nrow * v.xw, 256);
-    const int nb_r1 = (int)div_up(64LL * a.nterm, 256);
+    a.rw = term_lanes(v);
+    const int nb_r1 = (int)div_up((long long)a.rw * a.nterm, 256);
     a.irr_nodes = v.irr_nodes.p; a.irr_ci = v.irr_ci.p; a.irr_ijk = v.irr_ijk.p; a.irrL = d.irrL.p; a.irrM = d.irrM.p;
     a.xrc = v.xrc.p; a.xptr = v.xptr.p; a.xcol = v.xcol.p; a.xval = d.xval.p;
     a.r1ptr = v.r1ptr.p; a.r1dof = v.r1dof.p; a.r1c = v.r1c.p; a.r1w = d.r1w.p; a.r1k = d.r1k.p;
@@ -1530,6 +1590,7 @@ struct System {
     alloc_pcg64();
     if (vprec) { alloc_pcg32(); ensure_prec<float>(); }
     ensure_prec<double>();
+    if (partial.n < dot_partials(0)) partial.alloc(std::max((size_t)RED_BLOCKS, dot_partials(0)));
     apply<double>(0, x, r64.p, MODE_RESID, b);
     dot(n, b, b, S_BB);
     const double bn = std::sqrt(read_slot(S_BB));
@@ -1544,8 +1605,7 @@ struct System {
     bool flexible = false;
     double last_true = 1e300;
     while (flag && it < maxiter) {
-      apply<double>(0, p64.p, Ap64.p, MODE_APPLY, nullptr);
-      dot(n, p64.p, Ap64.p, S_PAP);
+      apply_dot(0, p64.p, Ap64.p, S_PAP);
       // FP32 V-cycle without rigid-mode correction: the residual's FP32 copy comes out of the
       // update kernel and z stays FP32 (the dots and the direction update read it directly)
       const bool z32_path = vprec == 1 && zcol == 0;
@@ -2096,6 +2156,15 @@ int cm_sys_set_kz(cm_sys* h, int kz) {
     h->s.refresh_all_flags<double>();
     h->s.refresh_all_flags<float>();
   }
+  CM_API_END
+}
+
+int cm_sys_set_last_coarsening(cm_sys* h, int axes) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  CM_CHECK(!h->s.built, "set the coarsening before cm_sys_build");
+  CM_CHECK(axes >= 1 && axes <= 7, "axes is a bit mask of x (1), y (2), z (4)");
+  h->s.last_axes = axes;
   CM_API_END
 }
 

@@ -16,6 +16,8 @@ assemble / assemble_cn      brute-force element loop summation on tiny grids
 split_connectivity(_fast)   the vectorised form against the loop form (rb/splitfe.py:262 says
                             "same result") on random rasters, plus hand-built 2-bone cases
 p1d                         reproduces linear functions exactly; partition of unity
+prolongation(axes)          all axes kept = identity; x kept = reference prolongation of y, z
+                            kron identity; linear fields reproduced
 Level.smooth (Chebyshev)    textbook Chebyshev error polynomial p(MA) by eigendecomposition
 vcycle                      two-grid error-propagation formula on small dense systems
 pcg_gmg / solve             scipy ``splu`` direct solve (library routine) and true residual
@@ -334,9 +336,16 @@ class Level:
         return x
 
 
-def prolongation(coords, act_dofs):
-    """Level prolongation P[act_dofs][:, keep] and next coords (fe/gmg.py:91-96)."""
-    Px, cx = p1d(coords[0]); Py, cy = p1d(coords[1]); Pz, cz = p1d(coords[2])
+def prolongation(coords, act_dofs, axes=7):
+    """Level prolongation P[act_dofs][:, keep] and next coords (fe/gmg.py:91-96).  ``axes``: bit
+    mask of the coarsened axes (7 = all, the reference); an axis left out keeps its nodes
+    (identity prolongation) -- the semi-coarsening option of the GPU build."""
+    def p1(c, on):
+        if on:
+            return p1d(c)
+        c = np.asarray(c, float)
+        return sp.identity(len(c), format="csr"), c
+    Px, cx = p1(coords[0], axes & 1); Py, cy = p1(coords[1], axes & 2); Pz, cz = p1(coords[2], axes & 4)
     Pn = sp.kron(Pz, sp.kron(Py, Px)).tocsr()
     Pd = sp.kron(Pn, sp.identity(3)).tocsr()[act_dofs]
     keep = np.nonzero(np.asarray(abs(Pd).sum(0)).ravel() > 0)[0]
@@ -348,7 +357,7 @@ class GMG:
     max_levels levels; SuperLU coarse solve."""
 
     def __init__(self, A, used_nodes, xe, ye, ze, min_dofs=6000, max_levels=6, smoother="chebyshev",
-                 omega=0.6, sweeps=2, levels=None):
+                 omega=0.6, sweeps=2, levels=None, last_axes=7):
         self.levels, self.P, self.keeps = [], [], []
         act = (3 * np.asarray(used_nodes)[:, None] + np.arange(3)).ravel()
         coords = tuple(np.asarray(e, float) for e in (xe, ye, ze))
@@ -359,7 +368,14 @@ class GMG:
             last = (lev == nlev - 1) if levels is not None else (Acur.shape[0] <= min_dofs)
             if last:
                 break
-            Pd, keep, coords = prolongation(coords, act)
+            Pd, keep, coords_next = prolongation(coords, act)
+            if levels is None and last_axes != 7 and len(keep) <= min_dofs:
+                # the next level is the coarsest: coarsen only the axes in last_axes if that level
+                # is still within min_dofs (no reference counterpart; the reference uses all axes)
+                Ps, ks, cs = prolongation(coords, act, last_axes)
+                if len(ks) <= min_dofs:
+                    Pd, keep, coords_next = Ps, ks, cs
+            coords = coords_next
             Acur = (Pd.T @ Acur @ Pd).tocsr()
             self.P.append(Pd)
             self.keeps.append(keep)

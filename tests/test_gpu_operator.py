@@ -104,12 +104,23 @@ def _patch_case():
     return dict(grid=g), s, info, info["used"]
 
 
-@pytest.mark.parametrize("which", ["cfg1", "cfg2", "cfg2coo", "patch"])
+@pytest.mark.parametrize("which", ["cfg1", "cfg2", "cfg2coo", "patch", "cfg2semi"])
 def test_coarse_levels_equal_galerkin(which):
     """Level-l GPU operator (moment cells + Galerkin springs) == P^T A P of the oracle hierarchy
-    (fe/gmg.py:91-99) on the kept coarse DOFs, every level."""
+    (fe/gmg.py:91-99) on the kept coarse DOFs, every level.  cfg2semi: automatic level rule with the
+    last step coarsening x and y only (the oracle's prolongation with the same axis mask)."""
     nlev = 3
-    if which == "cfg1":
+    if which == "cfg2semi":
+        c, s = _sys2(last_axes=3)
+        info = C.oracle_system2()
+        used_nodes = c["orig"][info["used"]]
+        g = c["grid"]
+        mg = O.GMG(info["K"], used_nodes, g.xe, g.ye, g.ze, last_axes=3)
+        nlev = len(mg.levels)
+        assert nlev >= 3 and len(s.levels()) == nlev
+        top = s.levels()[-1]
+        assert top["nz"] == s.levels()[-2]["nz"] and top["nx"] < s.levels()[-2]["nx"]
+    elif which == "cfg1":
         c, s = _sys1(10, levels=3)
         info = C.oracle_system1(10)
         used_nodes = info["used"]
@@ -121,7 +132,8 @@ def test_coarse_levels_equal_galerkin(which):
         info = C.oracle_system2()
         used_nodes = c["orig"][info["used"]]
     g = c["grid"]
-    mg = _oracle_hierarchy(info["K"], used_nodes, g, nlev)
+    if which != "cfg2semi":
+        mg = _oracle_hierarchy(info["K"], used_nodes, g, nlev)
     lv = s.levels()
     assert len(lv) == nlev
     for l in range(1, nlev):
@@ -146,14 +158,18 @@ def test_coarse_levels_equal_galerkin(which):
         assert err < 1e-6, (l, "rigid", err)
 
 
-def test_level_rule_matches_reference():
+@pytest.mark.parametrize("last_axes,expect", [(7, [107811, 14739, 2187]), (3, [107811, 14739, 4131])])
+def test_level_rule_matches_reference(last_axes, expect):
     """<= 6000 DOFs or 6 levels (fe/gmg.py:80,87-90): 32^3 gives 3 levels (107,811 -> 14,739 ->
-    2,187 DOFs)."""
+    2,187 DOFs) under the reference rule (every axis coarsened).  The default semi-coarsening of
+    the last step keeps z: 8 x 8 x 16 cells, 9 * 9 * 17 * 3 = 4,131 DOFs (still <= 6000)."""
     c = synth.config1(n=32)
     extra, _ = synth.solve_inputs_config1(c)
     s = VoxelSystem(c["grid"], sri=False, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
-                    mu_tab=c["mu_tab"], extra=extra, device=DEV)
-    assert [l["ndof"] for l in s.levels()] == [107811, 14739, 2187]
+                    mu_tab=c["mu_tab"], extra=extra, device=DEV, last_axes=last_axes)
+    assert [l["ndof"] for l in s.levels()] == expect
+    if last_axes == 3:
+        assert (s.levels()[-1]["nx"], s.levels()[-1]["ny"], s.levels()[-1]["nz"]) == (8, 8, 16)
 
 
 @pytest.mark.parametrize("smoother", ["chebyshev", "jacobi"])

@@ -363,3 +363,28 @@ def test_lattice_bone_share_and_part():
     F = np.stack([(np.vectorize(lambda b: LAT_CLS.get(b, 0))(L) == k).mean(1) for k in range(10)], 1)
     part = O.lattice_part(S, F)
     assert list(part) == [1, 0, 0, 0]                     # row 1: cartilage present; rows 2-3: < 50 %
+
+
+def test_prolongation_axis_mask():
+    """A kept axis (mask bit clear) has the identity prolongation: with every axis kept P is the
+    identity on the active DOFs; with x kept only, P equals the reference prolongation of the
+    other two axes kron the identity in x; linear fields are reproduced exactly either way."""
+    xe, ye, ze = np.r_[0.0, 1, 3, 4, 7], np.r_[0.0, 2, 5], np.r_[0.0, 1, 2, 4]
+    g = O.Grid(xe, ye, ze)
+    act = np.arange(3 * g.nnode)
+    P0, keep0, c0 = O.prolongation((xe, ye, ze), act, axes=0)
+    assert (P0 - sp.identity(3 * g.nnode)).count_nonzero() == 0 and len(keep0) == 3 * g.nnode
+    P6, keep6, c6 = O.prolongation((xe, ye, ze), act, axes=6)
+    Py, _ = O.p1d(ye)
+    Pz, _ = O.p1d(ze)
+    ref = sp.kron(sp.kron(Pz, sp.kron(Py, sp.identity(len(xe)))), sp.identity(3)).tocsr()
+    assert abs(P6 - ref[:, keep6]).max() < 1e-15
+    assert np.array_equal(c6[0], xe) and len(c6[1]) < len(ye)
+    # linear field a + b x + c y + d z in component 1
+    X, Y, Z = np.meshgrid(c6[0], c6[1], c6[2], indexing="ij")
+    fc = (1.0 + 2 * X - 3 * Y + 0.5 * Z).transpose(2, 1, 0).ravel()
+    uc = np.zeros((len(fc), 3)); uc[:, 1] = fc
+    Xf, Yf, Zf = np.meshgrid(xe, ye, ze, indexing="ij")
+    ff = (1.0 + 2 * Xf - 3 * Yf + 0.5 * Zf).transpose(2, 1, 0).ravel()
+    uf = (P6 @ uc.ravel()[keep6]).reshape(-1, 3)
+    assert np.abs(uf[:, 1] - ff).max() < 1e-12 and not uf[:, 0].any()

@@ -402,7 +402,9 @@ constexpr int GJB = 64;
 // (g = t / 64, j = t % 64) keeps M[g + 4r][j], r = 0..15, in registers; each of the 64 steps
 // publishes the pivot row and column through LDS (double-buffered, one barrier per step).  The
 // step loop stays rolled (a one-pass kernel runs its code once, so a fully unrolled body is
-// fetched cold); the pivot row's register is picked with a select chain.
+// fetched cold); the pivot row's register is picked with a select chain, and the update of the
+// 16 entries is branch-free (all three candidate values computed, then selected) so that the
+// column reads are issued together instead of one wait per row.
 __global__ __launch_bounds__(256) void k_gjb_diag(int n, int k0, int nb, const double* __restrict__ A,
                                                   double* __restrict__ Dk) {
   __shared__ double rowk[2][GJB], colk[2][GJB];
@@ -427,13 +429,17 @@ __global__ __launch_bounds__(256) void k_gjb_diag(int n, int k0, int nb, const d
       for (int r = 0; r < 16; ++r) colk[buf][g + 4 * r] = m[r];
     }
     __syncthreads();
+    double cv[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) cv[r] = colk[buf][g + 4 * r];
     const double ip = 1.0 / rowk[buf][k];
     const double rk = rowk[buf][j] * ip;       // new pivot row entry (j != k)
+    const bool jk = (j == k);
+    const double rowv = jk ? ip : rk;
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
-      const int i = g + 4 * r;
-      if (i == k) m[r] = (j == k) ? ip : rk;
-      else m[r] = (j == k) ? -colk[buf][i] * ip : m[r] - colk[buf][i] * rk;
+      const double upd = jk ? -cv[r] * ip : m[r] - cv[r] * rk;
+      m[r] = (g + 4 * r == k) ? rowv : upd;
     }
   }
 #pragma unroll
@@ -495,33 +501,58 @@ __global__ __launch_bounds__(256) void k_gjb_rowpanel(int n, int k0, int nb, dou
   }
 }
 
-// (3) A_IJ -= A_IK A_KJ for I, J outside K (A_KJ already holds Dk A_KJ): 64x64 output tiles.
+// (3) A_IJ -= A_IK A_KJ for every 64 x 64 tile (I, J) outside row and column K, on the FP64
+// matrix cores.  The K-column tile of A_I. is stored transposed in LDS (sa[k][i]) and the row
+// panel as is (sr[k][j]); wave w computes the 32 x 32 quarter (w / 2, w % 2) as 2 x 2 products of
+// v_mfma_f64_16x16x4f64 over the 64-deep K.  Operand lanes: A[l % 16][l / 16], B[l / 16][l % 16];
+// result lane l, register r: D[l / 16 + 4 r][l % 16] (the layout of the builtin on gfx950).
+typedef double gjb_d4 __attribute__((ext_vector_type(4)));
+constexpr int GJP = GJB + 4;   // LDS row pitch (doubles)
 __global__ __launch_bounds__(256) void k_gjb_update(int n, int k0, int nb, double* __restrict__ A) {
-  __shared__ double sa[64][GJB + 1];   // A[I-tile, K]
-  __shared__ double sr[GJB][64 + 1];   // A[K, J-tile]
+  __shared__ double sa[GJB][GJP];   // sa[k][i] = A[i0 + i][k0 + k]
+  __shared__ double sr[GJB][GJP];   // sr[k][j] = A[k0 + k][j0 + j]
   const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
   if (i0 == k0 || j0 == k0) return;
   for (int t = threadIdx.x; t < 64 * GJB; t += blockDim.x) {
     const int r = t >> 6, q = t & 63;
     const int i = i0 + r;
-    sa[r][q] = (i < n && q < nb) ? A[(long long)i * n + k0 + q] : 0.0;
+    sa[q][r] = (i < n && q < nb) ? A[(long long)i * n + k0 + q] : 0.0;
     const int j = j0 + q;
     sr[r][q] = (j < n && r < nb) ? A[(long long)(k0 + r) * n + j] : 0.0;
   }
   __syncthreads();
-  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-  double acc[4][4];
-  gjb_tile_mul(sa, sr, tr, tc, acc);
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
+  const int lr = l & 15, lk = l >> 4;
+  gjb_d4 acc[2][2];
 #pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int i = i0 + tr + 16 * a;
-    if (i >= n) continue;
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int j = j0 + tc + 16 * b;
-      if (j < n) A[(long long)i * n + j] -= acc[a][b];
-    }
+    for (int b = 0; b < 2; ++b) acc[a][b] = gjb_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int k4 = 0; k4 < GJB / 4; ++k4) {
+    const int kk = 4 * k4 + lk;
+    double av[2], bv[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) av[a] = sa[kk][wr + 16 * a + lr];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) bv[b] = sr[kk][wc + 16 * b + lr];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
   }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = j0 + wc + 16 * b + lr;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = i0 + wr + 16 * a + lk + 4 * r;
+        if (i < n && j < n) A[(long long)i * n + j] -= acc[a][b][r];
+      }
+    }
 }
 
 // (4) A_IK <- -A_IK Dk for every 64-row tile I outside K.
@@ -553,15 +584,27 @@ __global__ __launch_bounds__(256) void k_gjb_colpanel(int n, int k0, int nb, dou
 
 // Coarse solve x = Ainv b (dense FP64), one wavefront per row.
 template <typename T>
-__global__ void k_dense_gemv(int n, const double* __restrict__ Ainv, const T* __restrict__ b, T* __restrict__ x) {
+// The inverse is over the level's used DOFs only: map[i] = full DOF of compact DOF i (nullptr: the
+// identity).  x at DOFs outside the map is not written (kept zero).
+__global__ void k_dense_gemv(int n, const double* __restrict__ Ainv, const int* __restrict__ map,
+                             const T* __restrict__ b, T* __restrict__ x) {
   const int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int lane = threadIdx.x & 63;
   if (wid >= n) return;
   const double* row = Ainv + (long long)wid * n;
   double s = 0.0;
-  for (int c = lane; c < n; c += 64) s += row[c] * (double)b[c];
+  for (int c = lane; c < n; c += 64) s += row[c] * (double)b[map ? map[c] : c];
   s = wave_sum(s);
-  if (lane == 0) x[wid] = T(s);
+  if (lane == 0) x[map ? map[wid] : wid] = T(s);
+}
+
+// Ac[i][j] = A[map[i]][map[j]] (m x m from n x n)
+__global__ void k_dense_gather(int m, const int* __restrict__ map, int n, const double* __restrict__ A,
+                               double* __restrict__ Ac) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long long)m * m) return;
+  const int i = (int)(id / m), j = (int)(id % m);
+  Ac[id] = A[(long long)map[i] * n + map[j]];
 }
 
 }  // namespace cm

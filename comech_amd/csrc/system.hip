@@ -150,7 +150,8 @@ struct System {
   DevBuf<int> zcptr, zrows_c, zrow, zptr, zcols;
   DevBuf<double> zvals_c, zvals, Einv, zw, zv;
   // coarsest dense inverse
-  DevBuf<double> Ainv;
+  DevBuf<double> Ainv;        // inverse of the coarsest operator over its used DOFs
+  DevBuf<int> dmap;           // compact DOF -> full DOF of the coarsest level
   int nc_dense = 0;
   double t_setup_ms = 0.0;
   // build stage times (ms): level-0 prep, hierarchy structure, uploads, moments, diagonals,
@@ -908,9 +909,10 @@ struct System {
     Level& c = lv(l);
     const long long n = 3 * c.nn;
     CM_CHECK(n <= 30000, "coarsest level too large for the dense direct solve; use more levels");
-    nc_dense = (int)n;
-    Ainv.alloc((size_t)n * n);
-    hipLaunchKernelGGL(k_fill<double>, grid1(n * n), dim3(256), 0, stream, Ainv.p, n * n, 0.0);
+    // assembled over all DOFs of the level, then restricted to the used ones (the others have
+    // empty rows and columns) and inverted there
+    DevBuf<double> Afull((size_t)n * n);
+    hipLaunchKernelGGL(k_fill<double>, grid1(n * n), dim3(256), 0, stream, Afull.p, n * n, 0.0);
     CM_LAUNCH_CHECK();
     if (l == 0) {
       // single-level hierarchy: the fine operator is assembled column by column
@@ -920,31 +922,49 @@ struct System {
         const double one = 1.0;
         CM_HIP(hipMemcpyAsync(e.p + j, &one, sizeof(double), hipMemcpyHostToDevice, stream));
         apply<double>(0, e.p, y.p, MODE_APPLY, nullptr);
-        CM_HIP(hipMemcpyAsync(Ainv.p + j * n, y.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        CM_HIP(hipMemcpyAsync(Afull.p + j * n, y.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
         CM_HIP(hipStreamSynchronize(stream));
       }
     } else {
       hipLaunchKernelGGL(k_dense_assemble, dim3((unsigned)c.ncell), dim3(64), 0, stream, c.nx, c.ny, c.nz,
-                         c.d64.hx.p, c.d64.hy.p, c.d64.hz.p, c.mom64.p, (int)n, Ainv.p);
+                         c.d64.hx.p, c.d64.hy.p, c.d64.hz.p, c.mom64.p, (int)n, Afull.p);
       CM_LAUNCH_CHECK();
       const int nrow = (int)c.xrow_h.size();
       if (nrow) {
         DevBuf<double> xv;
         xv.upload(c.xval_h, stream);
         hipLaunchKernelGGL(k_dense_add_extra, grid1(nrow), dim3(256), 0, stream, nrow, c.xrow.p, c.xptr.p, c.xcol.p,
-                           xv.p, (int)n, Ainv.p);
+                           xv.p, (int)n, Afull.p);
         CM_LAUNCH_CHECK();
         CM_HIP(hipStreamSynchronize(stream));
       }
       const int nterm = (int)c.r1k_h.size();
       if (nterm) {
         hipLaunchKernelGGL(k_dense_add_rank1, dim3((unsigned)nterm), dim3(256), 0, stream, c.r1ptr.p, c.r1dof.p,
-                           c.r1w64.p, c.r1k64.p, (int)n, Ainv.p);
+                           c.r1w64.p, c.r1k64.p, (int)n, Afull.p);
         CM_LAUNCH_CHECK();
       }
     }
-    hipLaunchKernelGGL(k_dense_fix_empty, dim3((unsigned)n), dim3(256), 0, stream, (int)n, Ainv.p);
+    std::vector<int> map;
+    for (long long nd = 0; nd < c.nn; ++nd)
+      if (c.used_h[nd])
+        for (int q = 0; q < 3; ++q) map.push_back((int)(3 * nd + q));
+    const long long m = (long long)map.size();
+    CM_CHECK(m > 0, "coarsest level has no used DOF");
+    dmap.upload(map, stream);
+    nc_dense = (int)m;
+    Ainv.alloc((size_t)m * m);
+    hipLaunchKernelGGL(k_dense_gather, grid1(m * m), dim3(256), 0, stream, (int)m, dmap.p, (int)n, Afull.p, Ainv.p);
+    hipLaunchKernelGGL(k_dense_fix_empty, dim3((unsigned)m), dim3(256), 0, stream, (int)m, Ainv.p);
     CM_LAUNCH_CHECK();
+    CM_HIP(hipStreamSynchronize(stream));
+    Afull.release();
+    invert_dense((int)m);
+  }
+
+  // In-place blocked Gauss-Jordan inverse of the SPD matrix Ainv (m x m).
+  void invert_dense(int m) {
+    const long long n = m;
     DevBuf<double> Dk(GJB * GJB);
     const unsigned nt = (unsigned)div_up(n, 64);
     for (int k0 = 0; k0 < n; k0 += GJB) {
@@ -1006,6 +1026,10 @@ struct System {
       }
       const size_t nv = 3 * (size_t)v.nn;
       d.x.alloc(nv); d.b.alloc(nv); d.r.alloc(nv); d.t.alloc(nv); d.d.alloc(nv);
+      if (l == nlev() - 1) {   // the coarse solve writes only the used DOFs of x
+        hipLaunchKernelGGL(k_fill<T>, grid1((long long)nv), dim3(256), 0, stream, d.x.p, (long long)nv, T(0));
+        CM_LAUNCH_CHECK();
+      }
       if (v.ntouch) {
         d.tc.alloc(3 * (size_t)v.ntouch);
         hipLaunchKernelGGL(k_fill<T>, grid1(3LL * v.ntouch), dim3(256), 0, stream, d.tc.p, 3LL * v.ntouch, T(0));
@@ -1022,9 +1046,15 @@ struct System {
     refresh_flags<T>(l);
   }
 
+  // every level ready for the V-cycle in precision T (data, block-Jacobi inverses, Chebyshev bounds)
   template <typename T>
   void ensure_prec() {
     for (int l = 0; l < nlev(); ++l) prepare_prec<T>(l, true);
+  }
+  // every level's operator data in precision T (no smoother set-up)
+  template <typename T>
+  void ensure_data() {
+    for (int l = 0; l < nlev(); ++l) prepare_prec<T>(l, false);
   }
 
   // ------------------------------------------------------------------ operator
@@ -1450,7 +1480,11 @@ struct System {
   template <typename T>
   void vcycle(int l, const T* b, T* x) {
     if (l == nlev() - 1) {
-      hipLaunchKernelGGL(k_dense_gemv<T>, grid1(64LL * nc_dense), dim3(256), 0, stream, nc_dense, Ainv.p, b, x);
+      if (l == 0) {   // one level: x is the caller's vector; the solve writes only the used DOFs
+        const long long nv = 3 * lv(0).nn;
+        hipLaunchKernelGGL(k_fill<T>, grid1(nv), dim3(256), 0, stream, x, nv, T(0));
+      }
+      hipLaunchKernelGGL(k_dense_gemv<T>, grid1(64LL * nc_dense), dim3(256), 0, stream, nc_dense, Ainv.p, dmap.p, b, x);
       CM_LAUNCH_CHECK();
       return;
     }
@@ -1588,8 +1622,13 @@ struct System {
     Level& v0 = lv(0);
     const long long n = 3 * v0.nn;
     alloc_pcg64();
-    if (vprec) { alloc_pcg32(); ensure_prec<float>(); }
-    ensure_prec<double>();
+    if (vprec) {   // FP32 V-cycle: FP64 is needed for the operator only
+      alloc_pcg32();
+      ensure_prec<float>();
+      ensure_data<double>();
+    } else {
+      ensure_prec<double>();
+    }
     if (partial.n < dot_partials(0)) partial.alloc(std::max((size_t)RED_BLOCKS, dot_partials(0)));
     apply<double>(0, x, r64.p, MODE_RESID, b);
     dot(n, b, b, S_BB);
@@ -1693,7 +1732,7 @@ struct System {
     alloc_pcg64();
     alloc_pcg32();
     ensure_prec<float>();
-    ensure_prec<double>();
+    ensure_data<double>();
     dot(n, b, b, S_BB);
     const double bn = std::sqrt(read_slot(S_BB));
     const double thr = tol * (bn > 0 ? bn : 1.0);
@@ -2129,10 +2168,10 @@ int cm_sys_autotune(cm_sys* h, int prec, int nrep) {
   CM_CHECK(nrep >= 1, "nrep must be positive");
   System& s = h->s;
   if (prec == 0) {
-    s.ensure_prec<double>();
+    s.ensure_data<double>();
     s.autotune_kz<double>(nrep);
   } else {
-    s.ensure_prec<float>();
+    s.ensure_data<float>();
     s.autotune_kz<float>(nrep);
   }
   s.drop_graphs();

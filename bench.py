@@ -208,6 +208,7 @@ def prep_workload(args):
         E = o["E"].cpu().numpy()
     torch.cuda.synchronize()
     t_gpu = (time.time() - t0) / nrep
+    log(f"prep: lattice + split on the GPU {t_gpu * 1e3:.1f} ms")
     line = {"workload": "engine preparation, hindfoot raster 160x96x96 as a 27-sample lattice",
             "ncell": g.ncell, "samples": int(L.size), "split_nodes": ns, "copies": int(len(orig) - g.nnode),
             "gpu_ms": round(t_gpu * 1e3, 2)}
@@ -220,12 +221,76 @@ def prep_workload(args):
         og = O.Grid(g.xe, g.ye, g.ze)
         cno, origo, nso = O.split_connectivity_fast(og, acto, po, So)
         t_cpu = time.time() - t1
+        log(f"prep: lattice + split on the host {t_cpu:.2f} s")
         line["cpu_baseline"] = {"seconds": round(t_cpu, 2), "cores": 1, "kind": "port",
                                 "sample": "the same lattice, oracle restatement of rb/rbfe.py:75-141 and "
                                           "rb/splitfe.py:261-305 on the host"}
         line["speedup"] = round(t_cpu / t_gpu, 1)
         line["parity"] = bool(np.array_equal(cn, cno) and np.array_equal(orig, origo) and ns == nso
                               and np.allclose(E, Eo, rtol=1e-12, atol=0))
+    # preparation around the solve (rb/rbfe.py:166-178): node sets of every part, the bone of each
+    # end of 40 ligaments (seeded points in the bones' boxes), the ligament patches and matrix
+    cn_h, orig_h = cn, orig
+    act, part, names = o["active"].cpu().numpy(), o["part"].cpu().numpy(), o["names"]
+    keys = sorted(names)
+    rng = np.random.default_rng(11)
+    sets, _ = RF.part_node_sets(cn_h, part, act, len(orig_h), keys, device=dev)
+    bone_nodes = {names[k]: sets[k] for k in keys if len(sets[k])}
+    xyz_of = lambda nodes: np.stack([g.xe[orig_h[nodes] % (g.nx + 1)], g.ye[(orig_h[nodes] // (g.nx + 1)) % (g.ny + 1)],
+                                     g.ze[orig_h[nodes] // ((g.nx + 1) * (g.ny + 1))]], 1)
+    springs = []
+    for _ in range(40):
+        ends = []
+        for nm in rng.choice(list(bone_nodes), 2):
+            P = xyz_of(bone_nodes[nm])
+            ends.append((P.min(0) + rng.uniform(size=3) * (P.max(0) - P.min(0))).tolist())
+        springs.append({"name": "", "k": 100.0, "a": ends[0], "b": ends[1]})
+    used_all = np.ones(len(orig_h), bool)
+
+    def gpu_solve_prep():
+        st, _ = RF.part_node_sets(cn_h, part, act, len(orig_h), keys, device=dev)
+        bn = {names[k]: st[k] for k in keys if len(st[k])}
+        sp_ = [dict(x) for x in springs]
+        for x, (ea, eb) in zip(sp_, RF.spring_end_bones(sp_, bn, g, orig_h, device=dev)):
+            x["A"], x["B"] = ea, eb
+        return bn, sp_, RF.spring_matrix_cn(g, orig_h, sp_, bn, device=dev).rank1(used_all)
+    gpu_solve_prep()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(nrep):
+        bn, sp_, terms = gpu_solve_prep()
+    torch.cuda.synchronize()
+    t_sp = (time.time() - t0) / nrep
+    log(f"prep: solve preparation on the GPU {t_sp * 1e3:.1f} ms")
+    line["solve_prep"] = {"what": "node sets of every bone part, nearest-bone ends of 40 ligaments, ligament patches "
+                                  "and their rank-one weights (rb/rbfe.py:166-178, rb/replica3.py:68-86)",
+                          "gpu_ms": round(t_sp * 1e3, 2), "ligament_weights": int(len(terms[1])),
+                          "patch_nodes_mean": round(len(terms[1]) / 6.0 / len(springs), 1)}
+    if not args.no_cpu_baseline:
+        from scipy.spatial import cKDTree
+        t1 = time.time()
+        cells = np.arange(g.ncell)
+        so = {k: np.unique(cn_h[cells[(part == k) & act]].ravel()) for k in keys}
+        bno = {names[k]: so[k] for k in keys if len(so[k])}
+        xyz = xyz_of(np.arange(len(orig_h)))
+        trees = {nm: cKDTree(xyz[nd]) for nm, nd in bno.items()}
+        spo = [dict(x) for x in springs]
+        for x in spo:
+            for key, p in (("A", x["a"]), ("B", x["b"])):
+                x[key] = min(trees, key=lambda nm: trees[nm].query(p)[0])
+        pat = O.spring_patches(xyz, spo, bno, used_all)
+        t_cpu_sp = time.time() - t1
+        log(f"prep: solve preparation on the host {t_cpu_sp:.2f} s")
+        line["solve_prep"]["cpu_baseline"] = {"seconds": round(t_cpu_sp, 3), "cores": 1, "kind": "port",
+                                              "sample": "the same parts and springs: np.unique per part, one cKDTree "
+                                                        "per bone (the reference's method), the oracle's patch search "
+                                                        "(distances and a sort per end)"}
+        line["solve_prep"]["speedup"] = round(t_cpu_sp / t_sp, 1)
+        pat_g = RF.spring_patches(sp_, bn, g, orig_h, used=used_all, device=dev)
+        line["solve_prep"]["parity"] = bool(all(np.array_equal(bn[nm], bno[nm]) for nm in bno)
+                                            and [(x["A"], x["B"]) for x in sp_] == [(x["A"], x["B"]) for x in spo]
+                                            and all(np.array_equal(a, c) and np.array_equal(b_, d)
+                                                    for (a, b_), (c, d) in zip(pat_g, pat)))
     print(json.dumps(line), flush=True)
 
 

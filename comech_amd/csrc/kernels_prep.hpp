@@ -302,3 +302,135 @@ __global__ __launch_bounds__(256) void k_split_cells(int nx, int ny, int nz, con
 }
 
 }  // namespace cm
+
+namespace cm {
+
+// -------------------------------------------------------------------------------------------
+// Engine preparation around the solve (rb/rbfe.py:146-178): the node sets of the bones, the bone
+// nearest to every ligament end, and the node patch of every end.
+
+// mask[node] |= 1 << p for the 8 nodes of every active cell of part p (1..63); cnt[p] = active
+// cells of part p (block histogram in LDS, one global atomic per block and part).
+__global__ __launch_bounds__(256) void k_part_mask(long long ncell, const long long* __restrict__ cn,
+                                                   const int* __restrict__ part, const uint8_t* __restrict__ active,
+                                                   unsigned long long* __restrict__ mask, int* __restrict__ cnt) {
+  __shared__ int h[64];
+  if (threadIdx.x < 64) h[threadIdx.x] = 0;
+  __syncthreads();
+  const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (c < ncell) {
+    const int p = part[c];
+    if (active[c] && p > 0 && p < 64) {
+      const unsigned long long bit = 1ULL << p;
+#pragma unroll
+      for (int l = 0; l < 8; ++l) atomicOr(mask + cn[8 * c + l], bit);
+      atomicAdd(&h[p], 1);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(cnt + threadIdx.x, h[threadIdx.x]);
+}
+
+// Node coordinates: node n sits where its original orig[n] (identity when orig is null) sits on
+// the grid, (xe[i], ye[j], ze[k]) with orig = i + (nx+1)(j + (ny+1)k).
+struct NodeGeo {
+  int nx, ny;
+  const double* __restrict__ xe;
+  const double* __restrict__ ye;
+  const double* __restrict__ ze;
+  const long long* __restrict__ orig;
+};
+__device__ __forceinline__ double node_dist2(const NodeGeo& g, long long n, double px, double py, double pz) {
+  const long long o = g.orig ? g.orig[n] : n;
+  const long long nxn = g.nx + 1, nyn = g.ny + 1;
+  const double dx = g.xe[o % nxn] - px, dy = g.ye[(o / nxn) % nyn] - py, dz = g.ze[o / (nxn * nyn)] - pz;
+  // (dx^2 + dy^2) + dz^2, each operation rounded once: the order numpy sums a 3-vector in
+  return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// d2min[e * nbone + b] = min over the nodes of bone b of |x - p_e|^2 (inf for a bone without
+// nodes).  One block per (point, bone).
+__global__ __launch_bounds__(256) void k_nearest_bone(int npt, const double* __restrict__ pts, int nbone,
+                                                      const long long* __restrict__ bptr,
+                                                      const long long* __restrict__ bnodes, NodeGeo g,
+                                                      double* __restrict__ d2min) {
+  __shared__ double red[4];
+  const int e = blockIdx.x / nbone, b = blockIdx.x % nbone;
+  const double px = pts[3 * e], py = pts[3 * e + 1], pz = pts[3 * e + 2];
+  double m = HUGE_VAL;
+  for (long long q = bptr[b] + threadIdx.x; q < bptr[b + 1]; q += 256) m = fmin(m, node_dist2(g, bnodes[q], px, py, pz));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) d2min[(long long)e * nbone + b] = fmin(fmin(red[0], red[1]), fmin(red[2], red[3]));
+}
+
+// Sorted (ascending) 8 smallest values seen so far: insert v.
+__device__ __forceinline__ void top8_insert(double (&t)[8], double v) {
+  if (!(v < t[7])) return;
+#pragma unroll
+  for (int q = 7; q > 0; --q) t[q] = (v < t[q - 1]) ? t[q - 1] : ((v < t[q]) ? v : t[q]);
+  t[0] = (v < t[0]) ? v : t[0];
+}
+
+// The patch of each spring end e (rb/replica3.py:68-86): the candidate nodes are bone b = bone[e]'s
+// nodes that are used (used == null: all); with d the distances from the end point, radius =
+// max(4, the min(7, count - 1)-th smallest d) and flag = d <= radius.  One block per end;
+// flags[foff[e] + q] for the bone's q-th node (0 for unused nodes), radius[e], count[e] = used
+// candidates.
+__global__ __launch_bounds__(256) void k_spring_patches(int nend, const double* __restrict__ pts,
+                                                        const int* __restrict__ bone,
+                                                        const long long* __restrict__ bptr,
+                                                        const long long* __restrict__ bnodes,
+                                                        const uint8_t* __restrict__ used, NodeGeo g,
+                                                        const long long* __restrict__ foff,
+                                                        uint8_t* __restrict__ flags, double* __restrict__ radius,
+                                                        int* __restrict__ count) {
+  __shared__ double pool[256 * 8];
+  __shared__ int cnt_s[4];
+  __shared__ double r_s;
+  const int e = blockIdx.x;
+  const int b = bone[e];
+  const double px = pts[3 * e], py = pts[3 * e + 1], pz = pts[3 * e + 2];
+  const long long q0 = bptr[b], q1 = bptr[b + 1];
+  double t[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) t[k] = HUGE_VAL;
+  int n = 0;
+  for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
+    const long long nd = bnodes[q];
+    if (used && !used[nd]) continue;
+    top8_insert(t, __dsqrt_rn(node_dist2(g, nd, px, py, pz)));
+    ++n;
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) pool[threadIdx.x * 8 + k] = t[k];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+  if ((threadIdx.x & 63) == 0) cnt_s[threadIdx.x >> 6] = n;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int tot = cnt_s[0] + cnt_s[1] + cnt_s[2] + cnt_s[3];
+    double s[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] = HUGE_VAL;
+    for (int q = 0; q < 256 * 8; ++q) top8_insert(s, pool[q]);
+    const int kth = tot < 8 ? tot - 1 : 7;
+    double dk = s[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) dk = (k == kth) ? s[k] : dk;
+    r_s = tot > 0 ? fmax(4.0, dk) : -1.0;
+    radius[e] = r_s;
+    count[e] = tot;
+  }
+  __syncthreads();
+  const double r = r_s;
+  for (long long q = q0 + threadIdx.x; q < q1; q += 256) {
+    const long long nd = bnodes[q];
+    const bool ok = !(used && !used[nd]) && __dsqrt_rn(node_dist2(g, nd, px, py, pz)) <= r;
+    flags[foff[e] + (q - q0)] = ok ? 1 : 0;
+  }
+}
+
+}  // namespace cm

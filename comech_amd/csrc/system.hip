@@ -2252,6 +2252,50 @@ int cm_lattice_materials(void* stream, int nx, int ny, int nz, int ns3, const in
   CM_API_END
 }
 
+int cm_part_nodes(void* stream, int64_t ncell, const int64_t* cn, const int32_t* part, const uint8_t* active,
+                  int64_t nnode, uint64_t* node_mask, int32_t* part_cells) {
+  CM_API_BEGIN
+  CM_CHECK(ncell >= 0 && nnode >= 0, "negative size");
+  hipStream_t st = (hipStream_t)stream;
+  if (nnode)
+    hipLaunchKernelGGL(cm::k_fill<unsigned long long>, cm::grid1(nnode), dim3(256), 0, st,
+                       (unsigned long long*)node_mask, (long long)nnode, 0ull);
+  hipLaunchKernelGGL(cm::k_fill<int>, dim3(1), dim3(256), 0, st, (int*)part_cells, 64LL, 0);
+  if (ncell)
+    hipLaunchKernelGGL(cm::k_part_mask, cm::grid1(ncell), dim3(256), 0, st, (long long)ncell, (const long long*)cn,
+                       (const int*)part, active, (unsigned long long*)node_mask, (int*)part_cells);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
+int cm_nearest_bone(void* stream, int npt, const double* pts, int nbone, const int64_t* bptr, const int64_t* bnodes,
+                    int nx, int ny, const double* xe, const double* ye, const double* ze, const int64_t* orig,
+                    double* d2min) {
+  CM_API_BEGIN
+  CM_CHECK(npt >= 0 && nbone >= 0 && (long long)npt * nbone < (1LL << 31), "point x bone count out of range");
+  if (npt == 0 || nbone == 0) return 0;
+  const cm::NodeGeo g{nx, ny, xe, ye, ze, (const long long*)orig};
+  hipLaunchKernelGGL(cm::k_nearest_bone, dim3((unsigned)(npt * nbone)), dim3(256), 0, (hipStream_t)stream, npt, pts,
+                     nbone, (const long long*)bptr, (const long long*)bnodes, g, d2min);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
+int cm_spring_patches(void* stream, int nend, const double* pts, const int32_t* bone, const int64_t* bptr,
+                      const int64_t* bnodes, const uint8_t* used, int nx, int ny, const double* xe, const double* ye,
+                      const double* ze, const int64_t* orig, const int64_t* foff, uint8_t* flags, double* radius,
+                      int32_t* count) {
+  CM_API_BEGIN
+  CM_CHECK(nend >= 0, "negative end count");
+  if (nend == 0) return 0;
+  const cm::NodeGeo g{nx, ny, xe, ye, ze, (const long long*)orig};
+  hipLaunchKernelGGL(cm::k_spring_patches, dim3((unsigned)nend), dim3(256), 0, (hipStream_t)stream, nend, pts,
+                     (const int*)bone, (const long long*)bptr, (const long long*)bnodes, used, g,
+                     (const long long*)foff, flags, radius, (int*)count);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
 int cm_split_connectivity(void* stream, int nx, int ny, int nz, const uint8_t* active, const int32_t* part,
                           const float* share, int nb, int64_t* cn, int64_t* copy_orig, int64_t cap, int64_t* ncopy,
                           int64_t* nsplit) {

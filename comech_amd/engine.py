@@ -115,7 +115,7 @@ def split_opts(kw):
 class Problem:
     """One voxel FE problem on the GPU in reference terms."""
 
-    def __init__(self, grid, lam, mu, active, *, sri, cn=None, orig=None, extra=None, node_groups=None,
+    def __init__(self, grid, lam, mu, active, *, sri, cn=None, orig=None, extra=None, rank1=None, node_groups=None,
                  device=None, **mg):
         self.grid = grid
         self.lam = np.asarray(lam, float)
@@ -138,7 +138,7 @@ class Problem:
         self.ndof = 3 * len(self.nused)
         self.sys = VoxelSystem(grid, sri=sri, active=self.active, lam=self.lam, mu=self.mu,
                                copy_orig=self.copy_orig, irr_cells=self.irr_cells, irr_nodes=self.irr_nodes,
-                               extra=extra, device=device, **mg)
+                               extra=extra, rank1=rank1, device=device, **mg)
         self.device = self.sys.device
         if node_groups:
             xyz = grid.node_xyz if cn is None else grid.node_xyz[np.asarray(orig)]

@@ -1,6 +1,8 @@
 """Drop-in for the engine preparation of ``rb/rbfe.py`` on the GPU: lattice material mixing
-(``materials``, rb/rbfe.py:75-134), bone shares (``bone_share``, rb/replica3.py:17-29) and the
-bone part of every cell (rb/rbfe.py:140-141).
+(``materials``, rb/rbfe.py:75-134), bone shares (``bone_share``, rb/replica3.py:17-29), the
+bone part of every cell (rb/rbfe.py:140-141), and the preparation around the solve
+(``prepare_solve``, rb/rbfe.py:146-178): ground contact, leg loads, the node sets of the bones,
+the bone of every ligament end and the ligament patches.
 
 A lattice ``V`` (as built by the reference's voxeliser) holds ``L`` -- (ncell, ns^3) body labels
 of the sub-samples of every cell in reference cell order (i ny + j) nz + k -- and ``cls`` -- the
@@ -12,6 +14,7 @@ The class list, moduli, bone and cartilage body tables are the engine's definiti
 (rb/rbgeom.py:35-49, rb/rbfe.py:29) and are kept identical.
 """
 import ctypes
+import json
 
 import numpy as np
 import torch
@@ -28,8 +31,9 @@ CLASSES = ["void", "cortical", "cancellous", "cartilage", "fascia", "soft", "pla
 MAT = {"cortical": (7300.0, 0.3), "cancellous": (100.0, 0.3), "cartilage": (10.0, 0.4), "fascia": (350.0, 0.4),
        "soft": (1.15, 0.49), "plate": (17000.0, 0.1), "fibrous": (60.0, 0.4), "fat": (1.15, 0.45),
        "gap": (0.01, 0.3)}
-# rb/rbfe.py:29
+# rb/rbfe.py:29-30
 ROLES = {"bones": dict(BONES), "cart": dict(CART), "bonded_extra": [COAL, FUSE]}
+FORCES = {"tibia": 300.0, "fibula": 60.0}
 
 
 def bone_groups(cls, roles):
@@ -120,5 +124,228 @@ def bone_share(V, roles=ROLES, device=None):
     return o["S"], o["names"]
 
 
-__all__ = ["COAL", "FIB", "INTERP", "FUSE", "GAP", "BONES", "CART", "CLASSES", "MAT", "ROLES", "bone_groups",
-           "lattice_prep", "materials", "bone_share"]
+# ---------------------------------------------------------------------------------------------
+# Preparation around the solve (rb/rbfe.py:146-178)
+def _dev(a, dt, dev):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+
+
+def _vp(x):
+    return ctypes.c_void_p(x.data_ptr())
+
+
+def _geo(g, orig, dev):
+    edges = [_dev(e, torch.float64, dev) for e in (g.xe, g.ye, g.ze)]
+    od = _dev(orig, torch.int64, dev) if orig is not None else None
+    return edges, od
+
+
+def part_node_sets(cn, part, active, nnode, keys, device=None):
+    """rb/rbfe.py:166-169,178 on the GPU: for every part id k in ``keys`` the sorted nodes of the
+    active cells of part k, and the number of those cells.  Returns ({k: nodes}, {k: count})."""
+    dev = require_gpu(device)
+    lib = _lib.load()
+    part = np.asarray(part)
+    if len(keys) and (min(keys) < 1 or max(keys) > 63):
+        raise ValueError("part ids must be 1..63")
+    cnd = _dev(np.asarray(cn, np.int64), torch.int64, dev)
+    pd = _dev(part, torch.int32, dev)
+    ad = _dev(np.asarray(active, np.uint8), torch.uint8, dev)
+    mask = torch.empty(int(nnode), dtype=torch.int64, device=dev)
+    cnt = torch.empty(64, dtype=torch.int32, device=dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(lib.cm_part_nodes(st, len(part), _vp(cnd), _vp(pd), _vp(ad), int(nnode), _vp(mask), _vp(cnt)))
+    m = mask.cpu().numpy().view(np.uint64)
+    c = cnt.cpu().numpy()
+    sets = {k: np.flatnonzero(m & np.uint64(1 << k)).astype(np.int64) for k in keys}
+    return sets, {k: int(c[k]) for k in keys}
+
+
+def _bone_lists(bone_nodes, names, dev):
+    ptr = np.zeros(len(names) + 1, np.int64)
+    for i, nm in enumerate(names):
+        ptr[i + 1] = ptr[i] + len(bone_nodes[nm])
+    flat = np.concatenate([np.asarray(bone_nodes[nm], np.int64) for nm in names]) if names else np.zeros(0, np.int64)
+    return ptr, _dev(ptr, torch.int64, dev), _dev(flat if len(flat) else np.zeros(1, np.int64), torch.int64, dev)
+
+
+def spring_end_bones(springs, bone_nodes, g, orig, device=None):
+    """rb/rbfe.py:173-175 on the GPU: every spring end goes to the bone whose nearest node is
+    nearest (first such bone in ``bone_nodes`` order; bones without nodes are skipped).  Node
+    positions: their original's grid node (``orig``).  Returns [(bone of a, bone of b)]."""
+    dev = require_gpu(device)
+    lib = _lib.load()
+    names = [nm for nm, nd in bone_nodes.items() if len(nd)]
+    if not springs:
+        return []
+    if not names:
+        raise ValueError("no bone has nodes")
+    pts = np.array([p for s in springs for p in (s["a"], s["b"])], float).reshape(-1, 3)
+    _, bptr, bnodes = _bone_lists(bone_nodes, names, dev)
+    (xe, ye, ze), od = _geo(g, orig, dev)
+    pd = _dev(pts, torch.float64, dev)
+    d2 = torch.empty((len(pts), len(names)), dtype=torch.float64, device=dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(lib.cm_nearest_bone(st, len(pts), _vp(pd), len(names), _vp(bptr), _vp(bnodes), g.nx, g.ny, _vp(xe),
+                                   _vp(ye), _vp(ze), _vp(od) if od is not None else None, _vp(d2)))
+    d = np.sqrt(d2.cpu().numpy())
+    best = d.argmin(1)          # first minimum: the first bone in order among equal distances
+    return [(names[best[2 * i]], names[best[2 * i + 1]]) for i in range(len(springs))]
+
+
+def spring_patches(springs, bone_nodes, g, orig, used=None, device=None):
+    """The node patch of every ligament end (rb/replica3.py:74-80) on the GPU: the end's bone's
+    used nodes within max(4 mm, distance of the 8th nearest).  ``springs`` carry their end bones in
+    "A" and "B".  Returns [(patch of a, patch of b)] as sorted node arrays."""
+    dev = require_gpu(device)
+    lib = _lib.load()
+    if not springs:
+        return []
+    names = list(bone_nodes)
+    ptr, bptr, bnodes = _bone_lists(bone_nodes, names, dev)
+    ends = [(p, names.index(s[key])) for s in springs for p, key in ((s["a"], "A"), (s["b"], "B"))]
+    pts = np.array([p for p, _ in ends], float).reshape(-1, 3)
+    bone = np.array([b for _, b in ends], np.int32)
+    sizes = ptr[bone + 1] - ptr[bone]
+    foff = np.r_[0, np.cumsum(sizes)].astype(np.int64)
+    (xe, ye, ze), od = _geo(g, orig, dev)
+    ud = _dev(np.asarray(used, np.uint8), torch.uint8, dev) if used is not None else None
+    flags = torch.empty(max(int(foff[-1]), 1), dtype=torch.uint8, device=dev)
+    rad = torch.empty(len(ends), dtype=torch.float64, device=dev)
+    cnt = torch.empty(len(ends), dtype=torch.int32, device=dev)
+    # device copies held in names until the kernel has run (a temporary's memory can be reused by
+    # the next allocation before the launch)
+    pd, bd, fd = _dev(pts, torch.float64, dev), _dev(bone, torch.int32, dev), _dev(foff, torch.int64, dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(lib.cm_spring_patches(st, len(ends), _vp(pd), _vp(bd), _vp(bptr), _vp(bnodes),
+                                     _vp(ud) if ud is not None else None, g.nx, g.ny, _vp(xe), _vp(ye), _vp(ze),
+                                     _vp(od) if od is not None else None, _vp(fd), _vp(flags), _vp(rad), _vp(cnt)))
+    fl = flags.cpu().numpy().astype(bool)
+    if (cnt.cpu().numpy() == 0).any():
+        raise ValueError("a spring end's bone has no used node")
+    out = []
+    for e, b in enumerate(bone):
+        nodes = np.asarray(bone_nodes[names[b]], np.int64)
+        out.append(nodes[fl[foff[e]:foff[e + 1]]])
+    return [(out[2 * i], out[2 * i + 1]) for i in range(len(springs))]
+
+
+def spring_matrix_cn(g, orig, springs, bone_nodes, device=None):
+    """rb/replica3.py:68-86 with the patches found on the GPU: returns the reference's callback
+    ``build(nmap, n)`` -> CSR k w w^T on compact DOFs, w = +d/n_A on the end-A patch and -d/n_B on
+    the end-B patch (d the unit vector a -> b); candidates are the nodes with nmap >= 0.
+    ``build.rank1(used)`` gives the same springs as rank-one terms (ptr, DOFs 3*node+c, w, k) over
+    the full node layout, which the solver applies as k (w.u) w without forming w w^T (a 4 mm patch
+    of a 0.5 mm grid holds ~2000 nodes: its dense block would have ~10^8 entries)."""
+    import scipy.sparse as sp
+
+    def weights(used):
+        patches = spring_patches(springs, bone_nodes, g, orig, used=used, device=device)
+        out = []
+        for s, (pa, pb) in zip(springs, patches):
+            a, b = np.asarray(s["a"], float), np.asarray(s["b"], float)
+            d = (b - a) / np.linalg.norm(b - a)
+            # per-DOF sums in the reference's order (end A's contribution first), nonzero entries
+            # in DOF order
+            dofs, ws = [], []
+            for sel, sign in ((pa, 1.0), (pb, -1.0)):
+                for c in range(3):
+                    dofs.append(3 * np.asarray(sel, np.int64) + c)
+                    ws.append(np.full(len(sel), sign * d[c] / len(sel)))
+            dofs, ws = np.concatenate(dofs), np.concatenate(ws)
+            ud, inv = np.unique(dofs, return_inverse=True)
+            w = np.zeros(len(ud))
+            np.add.at(w, inv, ws)
+            keep = w != 0
+            out.append((ud[keep], w[keep], float(s["k"])))
+        return out
+
+    def build(nmap, n):
+        nmap = np.asarray(nmap, np.int64)
+        rows, cols, vals = [], [], []
+        for dofs, w, k in weights(nmap >= 0):
+            nz = 3 * nmap[dofs // 3] + dofs % 3          # compact DOFs keep the full DOF order
+            rows.append(np.repeat(nz, len(nz)))
+            cols.append(np.tile(nz, len(nz)))
+            vals.append(k * np.outer(w, w).ravel())
+        if not rows:
+            return sp.csr_matrix((n, n))
+        return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+    def rank1(used):
+        terms = weights(np.asarray(used, bool))
+        ptr = np.zeros(len(terms) + 1, np.int64)
+        ptr[1:] = np.cumsum([len(t[0]) for t in terms])
+        cat = lambda q, dt: np.concatenate([t[q] for t in terms]).astype(dt) if terms else np.zeros(0, dt)
+        return ptr, cat(0, np.int64), cat(1, float), np.array([t[2] for t in terms], float)
+
+    build.rank1 = rank1
+    return build
+
+
+def _face_nodes(g, cn, cells, up):
+    """Four nodes of the z- (up False) or z+ face of every cell, tributary area = face / 4
+    (rb/splitfe.py:94-102); corners l = a + 2b + 4c with c = up."""
+    cells = np.asarray(cells, np.int64)
+    cols = [4, 5, 6, 7] if up else [0, 1, 2, 3]
+    i, j = cells // (g.ny * g.nz), (cells // g.nz) % g.ny
+    area = g.dx[i] * g.dy[j] / 4.0
+    return np.asarray(cn)[cells][:, cols].ravel(), np.repeat(area, 4)
+
+
+def prepare_solve(V, g, cn, orig, active, part, names, roles=ROLES, forces=FORCES, load=1.0, rigid=True,
+                  device=None):
+    """rb/rbfe.py:146-178: everything ``solve_cn`` needs besides the materials.  ``names``: bone
+    group id -> name (``lattice_prep``).  Returns dict(fixed=(nodes, area), loads=[(nodes, area*tz,
+    dir)], bone_nodes={name: nodes}, springs=[...with "A"/"B"], extra=build callback or None,
+    groups=[nodes per part with more than 3 active cells] or None).  The node sets, the spring ends
+    and the ligament patches run on the GPU; the ground and top faces (one cell layer each) on the
+    host."""
+    active = np.asarray(active, bool)
+    part = np.asarray(part)
+    orig = np.asarray(orig, np.int64)
+    cn = np.asarray(cn, np.int64)
+    ncell = g.ncell
+    kz = np.arange(ncell) % g.nz
+    # ground contact: bottom-layer active cells under contact (fraction per (i, j) column)
+    cf = np.asarray(V["contact"]).ravel()[np.arange(ncell) // g.nz]
+    fc = np.flatnonzero((kz == 0) & active & (cf > 0))
+    fn, fa = _face_nodes(g, cn, fc, up=False)
+    fixed = (fn, fa * np.repeat(cf[fc], 4))
+    # leg loads: top-layer active cells with more than half of the bone's sub-samples
+    top = np.flatnonzero((kz == g.nz - 1) & active)
+    Ltop = np.asarray(V["L"])[top]
+    loads = []
+    for name, Fz in forces.items():
+        F = Fz * load
+        frac = np.isin(Ltop, list(roles["bones"][name])).mean(1)
+        n_, a_ = _face_nodes(g, cn, top[frac > 0.5], up=True)
+        o = orig[n_]
+        X = np.stack([g.xe[o % (g.nx + 1)], g.ye[(o // (g.nx + 1)) % (g.ny + 1)]], 1)
+        W = a_.sum()
+        c0 = (a_[:, None] * X).sum(0) / W
+        dX = X - c0
+        I = np.einsum("n,ni,nj->ij", a_, dX, dX)
+        ab = np.linalg.solve(I, W * (np.asarray(V["load_line"][name], float) - c0))
+        loads.append((n_, a_ * (-F / W * (1.0 + dX @ ab)), np.array([0.0, 0.0, 1.0])))
+    # node sets of the bones and of every part (rigid groups)
+    keys = sorted(names)
+    sets, counts = part_node_sets(cn, part, active, len(orig), keys, device=device)
+    bone_nodes = {}
+    for bname in roles["bones"]:
+        k = next(i for i, nm in names.items() if nm == bname)
+        bone_nodes[bname] = sets[k]
+    # ligament springs below the cut, each end on its nearest bone
+    springs = [dict(s) for s in json.loads(V["springs_json"])]
+    hc = float(np.diff(V["ze"]).max())
+    springs = [s for s in springs if max(s["a"][2], s["b"][2]) < V["z_cut"] - 2 * hc]
+    for s, (ea, eb) in zip(springs, spring_end_bones(springs, bone_nodes, g, orig, device=device)):
+        s["A"], s["B"] = ea, eb
+    extra = spring_matrix_cn(g, orig, springs, bone_nodes, device=device) if springs else None
+    groups = [sets[k] for k in names if counts[k] > 3] if rigid else None
+    return dict(fixed=fixed, loads=loads, bone_nodes=bone_nodes, springs=springs, extra=extra, groups=groups)
+
+
+__all__ = ["COAL", "FIB", "INTERP", "FUSE", "GAP", "BONES", "CART", "CLASSES", "MAT", "ROLES", "FORCES", "bone_groups",
+           "lattice_prep", "materials", "bone_share", "part_node_sets", "spring_end_bones", "spring_patches",
+           "spring_matrix_cn", "prepare_solve"]

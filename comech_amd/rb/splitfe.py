@@ -142,7 +142,7 @@ def solve_cn(g, lam, mu, active, cn, orig, fixed, loads, extra=None, node_groups
              aniso=None, device=None, **opts):
     """rb/splitfe.py:123-158 on the GPU.  fixed: (node_ids, area) ground faces with k = 1e6 N/mm^3
     per area on 3 components; loads: [(node_ids, area, t[3])]; extra(nmap, n): sparse stiffness on
-    compact DOFs.  Returns U (nnode+ncopy, 3) with NaN for unused nodes, info, reactions R."""
+    compact DOFs (or, when it has ``rank1(used)``, rank-one terms over node DOFs).  Returns U (nnode+ncopy, 3) with NaN for unused nodes, info, reactions R."""
     if aniso is not None:
         raise NotImplementedError("laminate (aniso) cells are not supported by the GPU solver")
     mg, so = E.split_opts(dict(opts))
@@ -162,16 +162,20 @@ def solve_cn(g, lam, mu, active, cn, orig, fixed, loads, extra=None, node_groups
     ab_ = np.asarray(ab_, float)
     okf = used[nb_]
     parts = [E.diag_coo(nb_[okf], np.repeat((ab_[okf] * 1e6)[:, None], 3, 1))]
+    rank1 = None
     if extra is not None:
-        parts.append(E.compact_to_full_coo(extra(nmap, n), nused))
+        if hasattr(extra, "rank1"):          # springs given as rank-one terms (k w w^T never formed)
+            rank1 = extra.rank1(used)
+        else:
+            parts.append(E.compact_to_full_coo(extra(nmap, n), nused))
     f = np.zeros(n)
     for nodes, area, t in loads:
         ln = nmap[np.asarray(nodes, np.int64)]
         ok = ln >= 0
         for c in range(3):
             np.add.at(f, 3 * ln[ok] + c, np.asarray(area, float)[ok] * t[c])
-    P = E.Problem(g, lam, mu, active, sri=SRI, cn=cn, orig=orig, extra=E.cat_coo(parts), node_groups=node_groups,
-                  device=device, **mg)
+    P = E.Problem(g, lam, mu, active, sri=SRI, cn=cn, orig=orig, extra=E.cat_coo(parts), rank1=rank1,
+                  node_groups=node_groups, device=device, **mg)
     U, info, _ = P.solve(P.full_rhs(f), None, tol=tol, maxiter=maxiter, **so)
     R = np.zeros((nn, 3))
     np.add.at(R, nb_, -(ab_[:, None] * 1e6) * U[nb_])

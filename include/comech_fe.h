@@ -182,6 +182,30 @@ int cm_lattice_materials(void* stream, int nx, int ny, int nz, int ns3, const in
                          const double* mu_k, double* E, double* nu, uint8_t* active, double* F, uint8_t* ser, float* S,
                          int32_t* part);
 
+/* Node sets of the bone parts (rb/rbfe.py:166-169,178: np.unique of the nodes of the active cells
+ * of each part): node_mask[n] gets bit p for every active cell of part p (1..63) that uses node n;
+ * part_cells[p] = active cells of part p (64 entries).  cn: (ncell, 8) int64 in reference cell
+ * order.  Device memory; the call clears both outputs first. */
+int cm_part_nodes(void* stream, int64_t ncell, const int64_t* cn, const int32_t* part, const uint8_t* active,
+                  int64_t nnode, uint64_t* node_mask, int32_t* part_cells);
+
+/* Squared distance from each point to the nearest node of each bone (rb/rbfe.py:173-175, one KD
+ * tree per bone there): d2min[e * nbone + b]; bone b's nodes are bnodes[bptr[b] .. bptr[b+1]),
+ * placed at their original's grid position (orig, null = identity) on edges xe, ye, ze.  The
+ * squares are summed as (dx^2 + dy^2) + dz^2 with each operation rounded once. */
+int cm_nearest_bone(void* stream, int npt, const double* pts, int nbone, const int64_t* bptr, const int64_t* bnodes,
+                    int nx, int ny, const double* xe, const double* ye, const double* ze, const int64_t* orig,
+                    double* d2min);
+
+/* Ligament patches (rb/replica3.py:68-86): for spring end e at pts[e] on bone bone[e], the bone's
+ * used nodes (used null: all) within radius[e] = max(4, distance of the min(7, count - 1)-th nearest)
+ * get flags[foff[e] + q] = 1 (q = the node's position in the bone's list); count[e] = used
+ * candidates (0: radius -1, no flag set). */
+int cm_spring_patches(void* stream, int nend, const double* pts, const int32_t* bone, const int64_t* bptr,
+                      const int64_t* bnodes, const uint8_t* used, int nx, int ny, const double* xe, const double* ye,
+                      const double* ze, const int64_t* orig, const int64_t* foff, uint8_t* flags, double* radius,
+                      int32_t* count);
+
 /* Split connectivity: rb/splitfe.py:261-305 (split_connectivity_fast).  active [ncell] (0/1),
  * part [ncell] (bone per cell, 0 = none), share [ncell][nb] (float).  Outputs cn [ncell][8]
  * (node ids, copies at nnode + rank), copy_orig [*ncopy] (original node of every copy; fails

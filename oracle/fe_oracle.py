@@ -31,6 +31,11 @@ lattice_materials           single-phase cells return that phase's (E, nu); a ha
                             against a brute-force 3x3x3 neighbourhood scan; activity thresholds
 bone_groups / bone_share    hand-built label maps; shares sum to the bone fraction
 lattice_part                closed form on hand-built shares
+bone_node_sets              hand-built part maps (node sets by enumeration)
+ground_contact / leg_loads  contact: areas sum to the contact-weighted bottom face area; loads:
+                            the resultant is -Fz along z and its moment puts it on the load line
+                            (closed form), uniform when the line passes through the centroid
+spring_end_bones            scipy cKDTree nearest-neighbour distances (library routine)
 ==========================  =====================================================================
 """
 import numpy as np
@@ -679,20 +684,32 @@ def element_forces_cn(grid, lam, mu, U, cells, cn, sri=True):
 
 # ---------------------------------------------------------------------------------------------
 # rb/replica3.py:68-86
+def spring_patches(xyz, springs, bone_nodes, used):
+    """rb/replica3.py:74-80: the patch of each spring end: the end's bone's used nodes within
+    max(4 mm, distance of the 8th nearest).  Returns [(patch of a, patch of b)]."""
+    out = []
+    for s in springs:
+        pp = []
+        for p, key in ((s["a"], "A"), (s["b"], "B")):
+            cand = np.asarray(bone_nodes[s[key]])
+            cand = cand[used[cand]]
+            dist = np.linalg.norm(xyz[cand] - np.asarray(p, float), axis=1)
+            pp.append(cand[dist <= max(4.0, np.sort(dist)[min(7, len(dist) - 1)])])
+        out.append(tuple(pp))
+    return out
+
+
 def spring_matrix_cn(xyz, springs, bone_nodes):
     """Ligament springs: K += k w w^T with w = +d/n_A on the end-A patch and -d/n_B on the end-B
     patch (patch = bone nodes within max(4 mm, 8th-nearest distance))."""
     def build(nmap, n):
         rows, cols, vals = [], [], []
-        for s in springs:
+        patches = spring_patches(xyz, springs, bone_nodes, nmap >= 0)
+        for s, (pa, pb) in zip(springs, patches):
             a, b = np.asarray(s["a"], float), np.asarray(s["b"], float)
             d = (b - a) / np.linalg.norm(b - a)
             w = np.zeros(n)
-            for p, sign, key in ((a, 1.0, "A"), (b, -1.0, "B")):
-                cand = np.asarray(bone_nodes[s[key]])
-                cand = cand[nmap[cand] >= 0]
-                dist = np.linalg.norm(xyz[cand] - p, axis=1)
-                sel = cand[dist <= max(4.0, np.sort(dist)[min(7, len(dist) - 1)])]
+            for sel, sign in ((pa, 1.0), (pb, -1.0)):
                 ln = nmap[sel]
                 for c in range(3):
                     np.add.at(w, 3 * ln + c, sign * d[c] / len(ln))
@@ -868,3 +885,71 @@ def lattice_part(S, F):
     """rb/rbfe.py:140-141: bone group of a cell holding >= 50 % of one bone and no joint layer."""
     cart = (F[:, 3] + F[:, 7]) > 0
     return np.where((S.max(1) >= 0.5) & ~cart, S.argmax(1), 0)
+
+
+# ---------------------------------------------------------------------------------------------
+# rb/rbfe.py:146-178  (ground contact, leg loads, bone node sets, spring ends, rigid groups)
+def bone_node_sets(cn, part, active, keys):
+    """rb/rbfe.py:166-169,178: for every part id k in ``keys``: the sorted unique nodes of the active
+    cells of part k.  Returns {k: nodes}."""
+    cells = np.arange(len(part))
+    return {k: np.unique(cn[cells[(part == k) & active]].ravel()) for k in keys}
+
+
+def part_cell_counts(part, active, keys):
+    """Active cells per part id (the rigid-group threshold of rb/rbfe.py:178 is > 3)."""
+    return {k: int(((part == k) & active).sum()) for k in keys}
+
+
+def ground_contact(g, cn, active, contact):
+    """rb/rbfe.py:146-151: bottom-layer active cells with contact fraction cf > 0 (cf per (i, j)
+    column, ``contact`` shaped (nx, ny)); their z- face nodes, areas face/4 times cf."""
+    cells = np.arange(g.ncell)
+    cf = np.asarray(contact).ravel()[g.cI * g.ny + g.cJ]
+    fc = cells[(g.cK == 0) & active & (cf > 0)]
+    nodes, area = face_node_areas(g, fc, "z-", cn)
+    return nodes, area * np.repeat(cf[fc], 4)
+
+
+def leg_loads(g, cn, orig, active, top_fraction, forces, load, load_line):
+    """rb/rbfe.py:152-163: for every loaded bone, the z+ faces of the top-layer active cells with
+    more than half of the bone's sub-samples (``top_fraction[name]``, per cell) carry a linear
+    pressure tz = -Fz/sum(w) (1 + dX ab) whose resultant is -Fz along z through the load line:
+    ab = I^-1 sum(w) (line - centroid), I the area second moment of the loaded nodes."""
+    cells = np.arange(g.ncell)
+    top = cells[(g.cK == g.nz - 1) & active]
+    out = []
+    for name, Fz in forces.items():
+        Fz = Fz * load
+        c = top[top_fraction[name][top] > 0.5]
+        n_, a_ = face_node_areas(g, c, "z+", cn)
+        X = g.node_xyz[orig[n_]][:, :2]
+        w = a_
+        xb = (w[:, None] * X).sum(0) / w.sum()
+        dX = X - xb
+        I = np.einsum("n,ni,nj->ij", w, dX, dX)
+        e = np.array(load_line[name]) - xb
+        ab = np.linalg.solve(I, w.sum() * e)
+        tz = -Fz / w.sum() * (1 + dX @ ab)
+        out.append((n_, a_ * tz, np.array([0.0, 0.0, 1.0])))
+    return out
+
+
+def spring_end_bones(springs, bone_nodes, xyz):
+    """rb/rbfe.py:173-175: each spring end goes to the bone whose nearest node is nearest (the
+    first such bone in ``bone_nodes`` order; bones without nodes are skipped).  Brute force: the
+    reference queries one KD-tree per bone, whose nearest-neighbour distance is the same minimum."""
+    names = [nm for nm, nd in bone_nodes.items() if len(nd)]
+    out = []
+    for s in springs:
+        ends = []
+        for p in (s["a"], s["b"]):
+            p = np.asarray(p, float)
+            best, bd = None, None
+            for nm in names:
+                d = np.sqrt(((xyz[bone_nodes[nm]] - p) ** 2).sum(1)).min()
+                if bd is None or d < bd:
+                    best, bd = nm, d
+            ends.append(best)
+        out.append(tuple(ends))
+    return out

@@ -90,3 +90,124 @@ def test_lattice_then_split_chain():
     S, _ = O.bone_share(V["L"], CLS, RF.ROLES)
     cno, origo, nso = O.split_connectivity(og, act, O.lattice_part(S, F), S)
     assert ns == nso and np.array_equal(orig, origo) and np.array_equal(cn, cno)
+
+
+# ---------------------------------------------------------------------------------------------
+# Preparation around the solve (rb/rbfe.py:146-178)
+CLS_E = dict(CLS)
+CLS_E.update({4: 1, 13: 2})
+
+
+def _engine_V(dims, seed):
+    """A lattice with calcaneus/talus below, tibia and fibula in the top layers, voids, a ground
+    contact map, load lines off the loaded areas' centroids and ligament springs."""
+    rng = np.random.default_rng(seed)
+    nx, ny, nz = dims
+    nc = nx * ny * nz
+    i, j, k = np.unravel_index(np.arange(nc), dims)
+    L = rng.choice([0, 61, 60, 5], size=(nc, 27))
+    for lab, sel in ((1, k < nz // 3), (52, (k >= nz // 3) & (k < 2 * nz // 3)), (4, (k >= 2 * nz // 3) & (i < nx // 2)),
+                     (3, (k >= 2 * nz // 3) & (i >= nx // 2))):
+        L[sel & (rng.uniform(size=nc) < 0.9)] = lab
+    L[rng.uniform(size=nc) < 0.05] = 0
+    xe, ye, ze = np.arange(nx + 1.0), np.arange(ny + 1.0), np.arange(nz + 1.0)
+    contact = (rng.uniform(size=(nx, ny)) * (rng.uniform(size=(nx, ny)) > 0.3)).astype(np.float32)
+    springs = [{"name": "", "k": float(rng.uniform(50, 150)), "a": [float(rng.uniform(0, nx)), float(rng.uniform(0, ny)),
+                                                                   float(rng.uniform(0, nz - 3))],
+                "b": [float(rng.uniform(0, nx)), float(rng.uniform(0, ny)), float(rng.uniform(0, nz - 3))]}
+               for _ in range(12)]
+    springs.append({"name": "", "k": 1.0, "a": [1.0, 1.0, nz - 0.5], "b": [2.0, 2.0, nz - 0.2]})   # above the cut
+    import json
+    return dict(L=L, cls=CLS_E, xe=xe, ye=ye, ze=ze, contact=contact,
+                load_line={"tibia": [nx / 4 + 0.3, ny / 2 - 0.2], "fibula": [3 * nx / 4 - 0.4, ny / 2 + 0.1]},
+                springs_json=json.dumps(springs), z_cut=float(nz))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_prepare_solve_matches_oracle(seed):
+    dims = (14, 9, 12)
+    V = _engine_V(dims, seed)
+    o = RF.lattice_prep(V, device=DEV)
+    g, og = Grid(V["xe"], V["ye"], V["ze"]), O.Grid(V["xe"], V["ye"], V["ze"])
+    cn, orig, ns = SF.split_connectivity_gpu(g, o["active"], o["part"], o["S"], device=DEV)
+    P = RF.prepare_solve(V, g, cn, orig, o["active"], o["part"], o["names"], device=DEV)
+    act, part, names = o["active"], o["part"], o["names"]
+    # ground contact and loads
+    fn, fa = O.ground_contact(og, cn, act, V["contact"])
+    assert np.array_equal(P["fixed"][0], fn) and np.allclose(P["fixed"][1], fa, rtol=1e-15, atol=0)
+    frac = {nm: np.isin(V["L"], list(RF.ROLES["bones"][nm])).mean(1) for nm in RF.FORCES}
+    loads = O.leg_loads(og, cn, orig, act, frac, RF.FORCES, 1.0, V["load_line"])
+    assert len(loads) == len(P["loads"]) == 2
+    for (n1, f1, d1), (n2, f2, d2) in zip(P["loads"], loads):
+        assert np.array_equal(n1, n2) and np.array_equal(d1, d2) and len(n1) > 0
+        assert np.allclose(f1, f2, rtol=1e-12, atol=1e-12 * np.abs(f2).max())
+    # node sets and rigid groups
+    keys = sorted(names)
+    sets = O.bone_node_sets(cn, part, act, keys)
+    counts = O.part_cell_counts(part, act, keys)
+    for bname, nodes in P["bone_nodes"].items():
+        k = next(i for i, nm in names.items() if nm == bname)
+        assert np.array_equal(nodes, sets[k]), bname
+    ref_groups = [sets[k] for k in names if counts[k] > 3]
+    assert len(P["groups"]) == len(ref_groups) >= 4
+    assert all(np.array_equal(a, b) for a, b in zip(P["groups"], ref_groups))
+    # spring ends (one spring is above the cut) and the ligament stiffness
+    xyz = og.node_xyz[orig]
+    assert len(P["springs"]) == 12
+    ends = O.spring_end_bones(P["springs"], P["bone_nodes"], xyz)
+    assert [(s["A"], s["B"]) for s in P["springs"]] == ends
+    rng = np.random.default_rng(seed)
+    used = rng.uniform(size=len(orig)) > 0.15                 # a used map that drops some bone nodes
+    nmap = -np.ones(len(orig), np.int64)
+    nmap[used] = np.arange(used.sum())
+    n = 3 * int(used.sum())
+    Kg = P["extra"](nmap, n).tocsr()
+    Ko = O.spring_matrix_cn(xyz, P["springs"], P["bone_nodes"])(nmap, n).tocsr()
+    assert Kg.nnz == Ko.nnz and np.array_equal(Kg.indices, Ko.indices) and np.array_equal(Kg.indptr, Ko.indptr)
+    assert np.abs(Kg.data - Ko.data).max() <= 1e-13 * np.abs(Ko.data).max()
+
+
+def test_spring_patches_radius_rule():
+    """The patch is the 8 nearest used nodes when they lie beyond 4 mm, every used node within 4 mm
+    otherwise, all candidates when fewer than 8; checked against a sort of the distances."""
+    rng = np.random.default_rng(11)
+    g = Grid(*(np.arange(n + 1.0) * 0.9 for n in (30, 20, 18)))
+    og = O.Grid(g.xe, g.ye, g.ze)
+    nodes = np.sort(rng.choice(g.nnode, 5000, replace=False))
+    bone_nodes = {"x": nodes, "few": nodes[:5]}
+    used = rng.uniform(size=g.nnode) > 0.1
+    springs = []
+    for p in ([1.0, 1.0, 1.0], [13.0, 9.0, 8.0], [40.0, 40.0, 40.0], [5.5, 3.3, 2.2]):
+        springs.append({"a": p, "b": [0.0, 0.0, 0.0], "A": "x", "B": "few", "k": 1.0})
+    patches = RF.spring_patches(springs, bone_nodes, g, None, used=used, device=DEV)
+    for s, (pa, pb) in zip(springs, patches):
+        for p, key, pp in ((s["a"], "A", pa), (s["b"], "B", pb)):
+            cand = bone_nodes[s[key]][used[bone_nodes[s[key]]]]
+            d = np.linalg.norm(og.node_xyz[cand] - np.asarray(p), axis=1)
+            r = max(4.0, np.sort(d)[min(7, len(d) - 1)])
+            assert np.array_equal(pp, cand[d <= r])
+            assert len(pp) >= min(8, len(cand))
+
+
+def test_ligament_rank1_terms_equal_matrix():
+    """build.rank1(used) -- the springs as rank-one terms over node DOFs -- gives the same k w w^T as
+    the CSR callback on compact DOFs."""
+    V = _engine_V((12, 9, 10), 5)
+    o = RF.lattice_prep(V, device=DEV)
+    g = Grid(V["xe"], V["ye"], V["ze"])
+    cn, orig, ns = SF.split_connectivity_gpu(g, o["active"], o["part"], o["S"], device=DEV)
+    P = RF.prepare_solve(V, g, cn, orig, o["active"], o["part"], o["names"], device=DEV)
+    used = np.random.default_rng(2).uniform(size=len(orig)) > 0.1
+    nmap = -np.ones(len(orig), np.int64)
+    nmap[used] = np.arange(used.sum())
+    n = 3 * int(used.sum())
+    K = P["extra"](nmap, n).toarray()
+    ptr, dofs, w, k = P["extra"].rank1(used)
+    assert len(k) == len(P["springs"]) and ptr[-1] == len(dofs) == len(w)
+    R = np.zeros((n, n))
+    for t in range(len(k)):
+        d = dofs[ptr[t]:ptr[t + 1]]
+        cdof = 3 * nmap[d // 3] + d % 3
+        assert (nmap[d // 3] >= 0).all()
+        R[np.ix_(cdof, cdof)] += k[t] * np.outer(w[ptr[t]:ptr[t + 1]], w[ptr[t]:ptr[t + 1]])
+    assert np.abs(R - K).max() <= 1e-13 * np.abs(K).max()

@@ -388,3 +388,81 @@ def test_prolongation_axis_mask():
     ff = (1.0 + 2 * Xf - 3 * Yf + 0.5 * Zf).transpose(2, 1, 0).ravel()
     uf = (P6 @ uc.ravel()[keep6]).reshape(-1, 3)
     assert np.abs(uf[:, 1] - ff).max() < 1e-12 and not uf[:, 0].any()
+
+
+def _engine_grid(seed=0, shape=(7, 6, 5)):
+    rng = np.random.default_rng(seed)
+    xe, ye, ze = (np.cumsum(np.r_[0.0, rng.uniform(0.5, 1.5, n)]) for n in shape)
+    return O.Grid(xe, ye, ze), rng
+
+
+def test_bone_node_sets_by_enumeration():
+    g, rng = _engine_grid(1)
+    part = rng.integers(0, 4, g.ncell)
+    active = rng.uniform(size=g.ncell) > 0.2
+    sets = O.bone_node_sets(g.cell_nodes, part, active, [1, 2, 3])
+    counts = O.part_cell_counts(part, active, [1, 2, 3])
+    for k in (1, 2, 3):
+        ref = set()
+        for c in range(g.ncell):
+            if part[c] == k and active[c]:
+                ref.update(int(n) for n in g.cell_nodes[c])
+        assert list(sets[k]) == sorted(ref)
+        assert counts[k] == sum(1 for c in range(g.ncell) if part[c] == k and active[c])
+
+
+def test_ground_contact_area_closed_form():
+    g, rng = _engine_grid(2)
+    active = rng.uniform(size=g.ncell) > 0.3
+    contact = rng.uniform(size=(g.nx, g.ny)) * (rng.uniform(size=(g.nx, g.ny)) > 0.4)
+    nodes, area = O.ground_contact(g, g.cell_nodes, active, contact)
+    ref = sum(g.dx[i] * g.dy[j] * contact[i, j] for i in range(g.nx) for j in range(g.ny)
+              if contact[i, j] > 0 and active[g.cid(i, j, 0)])
+    assert abs(area.sum() - ref) < 1e-12 * max(ref, 1)
+    assert np.all(g.node_xyz[nodes][:, 2] == g.ze[0])
+
+
+@pytest.mark.parametrize("through_centroid", [False, True])
+def test_leg_loads_resultant_on_load_line(through_centroid):
+    """The pressure's resultant is -Fz (times load) and passes through the load line; when the line
+    passes through the loaded area's centroid the pressure is uniform."""
+    g, rng = _engine_grid(3)
+    active = np.ones(g.ncell, bool)
+    frac = {"tibia": rng.uniform(size=g.ncell), "fibula": rng.uniform(size=g.ncell)}
+    top = np.flatnonzero(g.cK == g.nz - 1)
+    lines = {}
+    for nm in frac:
+        c = top[frac[nm][top] > 0.5]
+        n_, a_ = O.face_node_areas(g, c, "z+")
+        X = g.node_xyz[n_][:, :2]
+        xb = (a_[:, None] * X).sum(0) / a_.sum()
+        lines[nm] = xb if through_centroid else xb + rng.uniform(-0.3, 0.3, 2)
+    forces = {"tibia": 300.0, "fibula": 60.0}
+    out = O.leg_loads(g, g.cell_nodes, np.arange(g.nnode), active, frac, forces, 1.5, lines)
+    for (n_, f, d), nm in zip(out, forces):
+        assert np.array_equal(d, [0, 0, 1])
+        assert abs(f.sum() + 1.5 * forces[nm]) < 1e-9 * forces[nm]
+        X = g.node_xyz[n_][:, :2]
+        M = (f[:, None] * X).sum(0)
+        assert np.abs(M + 1.5 * forces[nm] * np.asarray(lines[nm])).max() < 1e-9 * forces[nm] * 10
+        if through_centroid:
+            _, a_ = O.face_node_areas(g, top[frac[nm][top] > 0.5], "z+")
+            assert np.allclose(f / a_, f[0] / a_[0], rtol=1e-9)
+
+
+def test_spring_end_bones_match_kdtree():
+    from scipy.spatial import cKDTree
+    rng = np.random.default_rng(4)
+    xyz = rng.uniform(0, 50, (4000, 3))
+    bone_nodes = {"a": np.sort(rng.choice(4000, 700, replace=False)), "b": np.sort(rng.choice(4000, 900, replace=False)),
+                  "empty": np.zeros(0, np.int64), "c": np.sort(rng.choice(4000, 300, replace=False))}
+    springs = [{"a": rng.uniform(0, 50, 3).tolist(), "b": rng.uniform(0, 50, 3).tolist(), "k": 1.0} for _ in range(40)]
+    # an end exactly on a node shared by two bones: the first bone in order wins the tie
+    shared = np.intersect1d(bone_nodes["a"], bone_nodes["b"])[0]
+    springs.append({"a": xyz[shared].tolist(), "b": xyz[bone_nodes["c"][0]].tolist(), "k": 1.0})
+    got = O.spring_end_bones(springs, bone_nodes, xyz)
+    trees = {nm: cKDTree(xyz[nd]) for nm, nd in bone_nodes.items() if len(nd)}
+    for s, (ea, eb) in zip(springs, got):
+        assert ea == min(trees, key=lambda nm: trees[nm].query(s["a"])[0])
+        assert eb == min(trees, key=lambda nm: trees[nm].query(s["b"])[0])
+    assert got[-1] == ("a", "c")

@@ -168,6 +168,62 @@ __host__ __device__ __forceinline__ void sym3_mul(const T* __restrict__ m, const
   z[2] = c * r[0] + e * r[1] + f * r[2];
 }
 
+// Epilogue inputs of one node (the residual or right-hand side, the block inverse, the iterate).
+// Level 0 loads them one z step ahead, so that their latency overlaps the next step's work.
+template <typename T>
+struct EpiIn {
+  T rin[3], dinv[6], x[3];
+};
+
+template <typename T, int EM>
+__device__ __forceinline__ void epi_load(const Epi<T>& ep, long long n, EpiIn<T>& q) {
+  if (EM == EPI_RESID || EM == EPI_CHEB) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q.rin[c] = ep.rin[3 * n + c];
+  }
+  if (EM == EPI_CHEB) {
+#pragma unroll
+    for (int m = 0; m < 6; ++m) q.dinv[m] = ep.Dinv[6 * n + m];
+    if (ep.x) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) q.x[c] = ep.x[3 * n + c];
+    }
+  }
+}
+
+template <typename T, int EM>
+__device__ __forceinline__ void epilogue_in(const Epi<T>& ep, long long n, const T au[3], const T sn[3],
+                                            const EpiIn<T>& q) {
+  if (EM == EPI_APPLY) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ep.y[3 * n + c] = au[c];
+  } else if (EM == EPI_RESID) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ep.y[3 * n + c] = q.rin[c] - au[c];
+  } else {
+    T r[3], z[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = q.rin[c] - au[c];
+    if (ep.y) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.y[3 * n + c] = r[c];
+    }
+    sym3_mul(q.dinv, r, z);
+    T dn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) dn[c] = ep.c1 * sn[c] + ep.c2 * z[c];
+    if (ep.dout) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.dout[3 * n + c] = dn[c];
+    }
+    if (ep.x) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.x[3 * n + c] = q.x[c] + (ep.xa * sn[c] + dn[c]);
+    }
+  }
+}
+
+// The same epilogue with its inputs loaded where they are used.
 template <typename T, int EM>
 __device__ __forceinline__ void epilogue(const Epi<T>& ep, long long n, const T au[3], const T sn[3]) {
   if (EM == EPI_APPLY) {
@@ -384,6 +440,14 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   gload(K0);
   mload(K0 - 1);
   __syncthreads();
+  // level 0 (residual and smoothing steps): the epilogue inputs of the thread's output node on
+  // layer K0, then always one layer ahead
+  constexpr bool PF = (M::kind <= 1) && (EM != EPI_APPLY);
+  EpiIn<T> qn;
+  if constexpr (PF) {
+    if (tx >= 1 && ty >= 1 && ci <= nx && cj <= ny && K0 < kend)
+      epi_load<T, EM>(ep, (long long)K0 * lay + (long long)cj * nxn + ci, qn);
+  }
   T ulo[12], car[12];
   face(K0 - 1, ulo);
 #pragma unroll
@@ -512,7 +576,12 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
       const long long n = (long long)k * lay + (long long)cj * nxn + ci;
       add_extra<T, M>(mat, n, au);
       if (want_dot) dsum += (double)sn[0] * au[0] + (double)sn[1] * au[1] + (double)sn[2] * au[2];
-      epilogue<T, EM>(ep, n, au, sn);
+      if constexpr (PF) {
+        epilogue_in<T, EM>(ep, n, au, sn, qn);
+        if (k + 1 < kend) epi_load<T, EM>(ep, n + lay, qn);
+      } else {
+        epilogue<T, EM>(ep, n, au, sn);
+      }
     }
 #pragma unroll
     for (int q = 0; q < 12; ++q) ulo[q] = uhi[q];

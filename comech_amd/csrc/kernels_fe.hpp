@@ -267,6 +267,25 @@ __device__ __forceinline__ void epilogue(const Epi<T>& ep, long long n, const T 
 //   * the top-face forces are carried to the next step, the bottom-face forces plus the carry
 //     are transformed back to the 4 bottom corners; three corners go to LDS for the in-plane
 //     neighbours, the fourth is the thread's own output node.
+// Three values of a 4-value LDS slot (16-byte aligned), read and written as whole slots: a
+// 64-lane access of consecutive slots is then served without bank conflicts (a 3-value access is
+// split into dwords 4 words apart, which meet on a quarter of the banks).
+__device__ __forceinline__ void lds_get3(const float* p, float v[3]) {
+  const float4 q = *reinterpret_cast<const float4*>(p);
+  v[0] = q.x; v[1] = q.y; v[2] = q.z;
+}
+__device__ __forceinline__ void lds_get3(const double* p, double v[3]) {
+  const double2 a = *reinterpret_cast<const double2*>(p), b = *reinterpret_cast<const double2*>(p + 2);
+  v[0] = a.x; v[1] = a.y; v[2] = b.x;
+}
+__device__ __forceinline__ void lds_put3(float* p, const float v[3]) {
+  *reinterpret_cast<float4*>(p) = make_float4(v[0], v[1], v[2], 0.0f);
+}
+__device__ __forceinline__ void lds_put3(double* p, const double v[3]) {
+  *reinterpret_cast<double2*>(p) = make_double2(v[0], v[1]);
+  *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], 0.0);
+}
+
 // Extra stiffness of a coarse node (kinds 2, 3): au += ex[n], ex[n] = 0.
 template <typename T, class M>
 __device__ __forceinline__ void add_extra(const M& mat, long long n, T au[3]) {
@@ -296,7 +315,9 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   constexpr int NST = TN * TN * 3;                 // scalars per staged node layer
   constexpr int NSTP = (NST + BLOCK - 1) / BLOCK;  // per thread
   __shared__ __align__(16) T s_u[2][TN * TN * 4];
-  __shared__ __align__(16) T s_g[BLOCK * 12];
+  // corners 1, 2, 3 of every thread's cell, one array per corner, 4 values (xyz, pad) per thread:
+  // consecutive threads write and read consecutive 16-byte slots (no LDS bank conflicts)
+  __shared__ __align__(16) T s_g[3][BLOCK * 4];
   __shared__ T s_tL[M::kind == 0 ? 256 : 1], s_tM[M::kind == 0 ? 256 : 1];
   __shared__ T s_ph[M::kind == 3 ? 2 * TILE * 6 : 1];
   const int tid = threadIdx.x;
@@ -401,11 +422,7 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     const T* src = s_u[kk & 1];
     T v[4][3];
 #pragma unroll
-    for (int ab = 0; ab < 4; ++ab) {
-      const int p = ((ty + (ab >> 1)) * TN + tx + (ab & 1)) * 4;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) v[ab][c] = src[p + c];
-    }
+    for (int ab = 0; ab < 4; ++ab) lds_get3(src + ((ty + (ab >> 1)) * TN + tx + (ab & 1)) * 4, v[ab]);
     face_fwd(v, f);
   };
   // material of cell layer kk, loaded one step ahead (kinds 0, 1 and 3)
@@ -485,9 +502,7 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     face(k + 1, uhi);
     T sn[3] = {T(0), T(0), T(0)};
     if (EM == EPI_CHEB || want_dot) {
-      const T* src = s_u[k & 1] + (ty * TN + tx) * 4;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) sn[c] = src[c];
+      lds_get3(s_u[k & 1] + (ty * TN + tx) * 4, sn);
     }
     T Flo[12], Fhi[12];
 #pragma unroll
@@ -558,18 +573,15 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     // back to the 4 bottom corners (corner a + 2b)
     T cr[4][3];
     face_bwd(Flo, cr);
-    T* gq = s_g + tid * 12;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      gq[c] = cr[1][c];
-      gq[4 + c] = cr[2][c];
-      gq[8 + c] = cr[3][c];
-    }
+    lds_put3(s_g[0] + tid * 4, cr[1]);
+    lds_put3(s_g[1] + tid * 4, cr[2]);
+    lds_put3(s_g[2] + tid * 4, cr[3]);
     __syncthreads();
     if (tx >= 1 && ty >= 1 && k >= K0 && ci <= nx && cj <= ny) {
-      const T* p1 = s_g + (tid - 1) * 12;            // cell (tx-1, ty): its corner 1
-      const T* p2 = s_g + (tid - TILE) * 12 + 4;     // cell (tx, ty-1): its corner 2
-      const T* p3 = s_g + (tid - TILE - 1) * 12 + 8; // cell (tx-1, ty-1): its corner 3
+      T p1[3], p2[3], p3[3];
+      lds_get3(s_g[0] + (tid - 1) * 4, p1);          // cell (tx-1, ty): its corner 1
+      lds_get3(s_g[1] + (tid - TILE) * 4, p2);       // cell (tx, ty-1): its corner 2
+      lds_get3(s_g[2] + (tid - TILE - 1) * 4, p3);   // cell (tx-1, ty-1): its corner 3
       T au[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) au[c] = cr[0][c] + p1[c] + p2[c] + p3[c];

@@ -291,6 +291,52 @@ def prep_workload(args):
                                             and [(x["A"], x["B"]) for x in sp_] == [(x["A"], x["B"]) for x in spo]
                                             and all(np.array_equal(a, c) and np.array_equal(b_, d)
                                                     for (a, b_), (c, d) in zip(pat_g, pat)))
+    # analysis-grid resampling (rb/rbsim.py:110-161) onto the engine's 60x40x44 grid of 2 mm cells
+    # (rb/rbsim.py:15-17), centred on the raster: a smooth displacement field on the FE nodes (NaN on
+    # unused nodes), random cell stresses; connectivity and displacement resident on the GPU
+    from comech_amd.grid import Grid
+    from comech_amd.rb import rbsim as RS
+    mid = [(e[0] + e[-1]) / 2 for e in (g.xe, g.ye, g.ze)]
+    gm = Grid(*[m - 1.0 * n + 2.0 * np.arange(n + 1) for m, n in zip(mid, (60, 40, 44))])
+    nxyz = xyz_of(np.arange(len(orig_h)))
+    Urel = np.stack([np.sin(nxyz[:, 0] / 9.0), np.cos(nxyz[:, 1] / 7.0), nxyz[:, 2] / 50.0], 1) * 0.1
+    Urel[np.setdiff1d(np.arange(len(orig_h)), np.unique(cn_h[act]))] = np.nan
+    VM = rng.uniform(0, 30, g.ncell).astype(np.float32)
+    SIG = rng.standard_normal((g.ncell, 6)).astype(np.float32)
+    o_dev = dict(g=g, cn=torch.from_numpy(cn_h).to(dev), active=torch.from_numpy(act).to(dev),
+                 VM=torch.from_numpy(VM).to(dev), SIG=torch.from_numpy(SIG).to(dev))
+    U_dev = torch.from_numpy(Urel).to(dev)
+
+    def gpu_resample():
+        vm_c, u_c, cov = RS.sample_fe(o_dev, gm, U_dev, sub=2, device=dev)
+        return vm_c, u_c, cov, RS.sample_nodes(o_dev, gm, U_dev, device=dev)
+    gpu_resample()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(nrep):
+        res_g = gpu_resample()
+    torch.cuda.synchronize()
+    t_rs = (time.time() - t0) / nrep
+    log(f"prep: resampling on the GPU {t_rs * 1e3:.1f} ms")
+    line["resample"] = {"what": "sample_fe (sub 2) and sample_nodes onto the 60x40x44 analysis grid at 2 mm "
+                                "(rb/rbsim.py:110-161); results copied to the host",
+                        "gpu_ms": round(t_rs * 1e3, 2), "analysis_cells": int(gm.ncell),
+                        "analysis_nodes": int(gm.nnode)}
+    if not args.no_cpu_baseline:
+        t1 = time.time()
+        og = O.Grid(g.xe, g.ye, g.ze)
+        vmo, uo, covo, _ = O.sample_fe(og, cn_h, act, VM, SIG, Urel, gm, sub=2)
+        uno = O.sample_nodes(og, cn_h, act, Urel, gm)
+        t_cpu_rs = time.time() - t1
+        log(f"prep: resampling on the host {t_cpu_rs:.2f} s")
+        line["resample"]["cpu_baseline"] = {"seconds": round(t_cpu_rs, 3), "cores": 1, "kind": "port",
+                                            "sample": "the same fields: the oracle's restatement of rb/rbsim.py:110-161 "
+                                                      "(numpy, scipy distance_transform_edt)"}
+        line["resample"]["speedup"] = round(t_cpu_rs / t_rs, 1)
+        vm_g, u_g, cov_g, un_g = res_g
+        line["resample"]["parity"] = bool(np.array_equal(un_g, uno) and np.array_equal(cov_g, covo)
+                                          and np.abs(vm_g - vmo).max() <= 1e-5 * np.abs(vmo).max()
+                                          and np.abs(u_g - uo).max() <= 1e-5 * np.abs(uo).max())
     print(json.dumps(line), flush=True)
 
 

@@ -62,6 +62,8 @@ SIGNATURES = {
                               c_ll, P_ll, P_ll],
     "cm_sys_sync": [c_void_p],
     "cm_sample_fe": [c_void_p] + [c_int] * 7 + [c_void_p] * 9 + [c_void_p] * 5 + [c_void_p] * 4,
+    "cm_sample_nodes": [c_void_p] + [c_int] * 3 + [c_void_p] * 6 + [c_int] + [c_void_p] * 2 + [c_int] * 3 +
+                       [c_void_p] * 9,
 }
 _RET = {"cm_last_error": ctypes.c_char_p, "cm_version": c_int}
 

@@ -1,5 +1,5 @@
 // Post-processing kernels: cell-centre strain, stress and von Mises (fe/solver.py:257-272,
-// rb/splitfe.py:161-175) and the analysis-grid resampling of rb/rbsim.py:110-140.
+// rb/splitfe.py:161-175) and the analysis-grid resampling of rb/rbsim.py:110-161.
 #pragma once
 #include <hip/hip_fp16.h>
 
@@ -147,6 +147,107 @@ __global__ __launch_bounds__(256) void k_bank_pack(long long ncell, int ncls, co
     // FP32 product, round half to even (numpy's round of a float32 array)
     frac8[c * ncls + k] = (uint8_t)rintf(255.0f * frac[c * ncls + k]);
   }
+}
+
+// ---------------------------------------------------------------------------------------------
+// sample_nodes (rb/rbsim.py:143-161): the relative displacement at every node of the analysis grid,
+// trilinear in the FE cell of the nearest active FE cell (the containing cell when it is active).
+// "Nearest" is the Euclidean feature transform of the inactive mask in cell-index units
+// (scipy.ndimage.distance_transform_edt(~active, return_indices=True) at rb/rbsim.py:152); among
+// cells at the same squared distance it returns the one with the smallest (k, j, i) (last axis
+// first), the rule scipy 1.15.3 follows on every tie of random 3-D rasters.
+//
+// Stage 1: for every FE cell column (i, j) and every z layer k the analysis nodes query, the
+// nearest active cell of that column: squared distance (k - k')^2, or -1 for a column without one,
+// and k' (the lower one on a tie).  Layout [query layer q][column i*ny + j].
+__global__ void k_column_nearest(int nx, int ny, int nz, int nq, const int* __restrict__ kq,
+                                 const uint8_t* __restrict__ active, int* __restrict__ dz2, int* __restrict__ kbest) {
+  const long long ncol = (long long)nx * ny;
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= ncol * nq) return;
+  const int q = (int)(id / ncol);
+  const long long col = id - q * ncol;
+  const int k = kq[q];
+  const uint8_t* a = active + col * nz;
+  int best = -1, d2 = -1;
+  const int dmax = max(k, nz - 1 - k);
+  for (int d = 0; d <= dmax; ++d) {
+    if (k - d >= 0 && a[k - d]) { best = k - d; d2 = d * d; break; }
+    if (k + d < nz && a[k + d]) { best = k + d; d2 = d * d; break; }
+  }
+  dz2[id] = d2;
+  kbest[id] = best;
+}
+
+// numpy.nan_to_num of a double: NaN -> 0, +-inf -> +-largest finite value
+__device__ __forceinline__ double nan_to_num(double v) {
+  if (v != v) return 0.0;
+  if (v == __longlong_as_double(0x7ff0000000000000LL)) return 1.7976931348623157e308;
+  if (v == -__longlong_as_double(0x7ff0000000000000LL)) return -1.7976931348623157e308;
+  return v;
+}
+
+// Stage 2: one thread per analysis node n = a + (mx+1)(b + (my+1)c) (fe/solver.py:120-123).  ci..ck:
+// FE cell of every analysis node coordinate per axis (clip(searchsorted(edges, x) - 1)), qz: the
+// node layer c's index into the stage-1 query layers.  The interpolation repeats the reference's
+// FP64 expressions operation by operation (no fused multiply-add), so the float32 result is the
+// reference's bit for bit.
+__global__ void k_sample_nodes(int mx1, int my1, int mz1, const double* __restrict__ mxe, const double* __restrict__ mye,
+                               const double* __restrict__ mze, const int* __restrict__ ci, const int* __restrict__ cj,
+                               const int* __restrict__ ck, const int* __restrict__ qz, int nx, int ny, int nz,
+                               const double* __restrict__ xe, const double* __restrict__ ye,
+                               const double* __restrict__ ze, const uint8_t* __restrict__ active,
+                               const int* __restrict__ dz2, const int* __restrict__ kbest,
+                               const long long* __restrict__ cn, const double* __restrict__ Urel,
+                               float* __restrict__ out) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long mnode = (long long)mx1 * my1 * mz1;
+  if (n >= mnode) return;
+  const int a = (int)(n % mx1), b = (int)((n / mx1) % my1), c = (int)(n / ((long long)mx1 * my1));
+  const int i = ci[a], j = cj[b], k = ck[c];
+  int bi = i, bj = j, bk = k;
+  if (!active[((long long)i * ny + j) * nz + k]) {
+    // smallest (d2, k', j', i') over all columns; a column farther in xy than the best d2 cannot win
+    const long long ncol = (long long)nx * ny;
+    const int* D = dz2 + (long long)qz[c] * ncol;
+    const int* Kb = kbest + (long long)qz[c] * ncol;
+    long long bd2 = 0x7fffffffffffffffLL;
+    bi = bj = bk = -1;
+    for (int jj = 0; jj < ny; ++jj) {
+      const long long dj2 = (long long)(j - jj) * (j - jj);
+      if (dj2 > bd2) continue;
+      for (int ii = 0; ii < nx; ++ii) {
+        const long long col = (long long)ii * ny + jj;
+        const int dz = D[col];
+        if (dz < 0) continue;
+        const long long d2 = (long long)(i - ii) * (i - ii) + dj2 + dz;
+        if (d2 > bd2) continue;
+        const int kk = Kb[col];
+        if (d2 < bd2 || kk < bk || (kk == bk && (jj < bj || (jj == bj && ii < bi)))) {
+          bd2 = d2; bi = ii; bj = jj; bk = kk;
+        }
+      }
+    }
+    if (bi < 0) {   // no active cell at all (the host rejects that case before the launch)
+      for (int q = 0; q < 3; ++q) out[3 * n + q] = 0.0f;
+      return;
+    }
+  }
+  // tx = clip((P - xe[ci]) / dx[ci], 0, 1) with dx = np.diff(xe)
+  const double tx = fmin(fmax(__ddiv_rn(__dsub_rn(mxe[a], xe[bi]), __dsub_rn(xe[bi + 1], xe[bi])), 0.0), 1.0);
+  const double ty = fmin(fmax(__ddiv_rn(__dsub_rn(mye[b], ye[bj]), __dsub_rn(ye[bj + 1], ye[bj])), 0.0), 1.0);
+  const double tz = fmin(fmax(__ddiv_rn(__dsub_rn(mze[c], ze[bk]), __dsub_rn(ze[bk + 1], ze[bk])), 0.0), 1.0);
+  const long long cell = ((long long)bi * ny + bj) * nz + bk;
+  double u[3] = {0.0, 0.0, 0.0};
+  for (int l = 0; l < 8; ++l) {
+    const double wx = (l & 1) ? tx : __dsub_rn(1.0, tx);
+    const double wy = ((l >> 1) & 1) ? ty : __dsub_rn(1.0, ty);
+    const double wz = (l >> 2) ? tz : __dsub_rn(1.0, tz);
+    const double w = __dmul_rn(__dmul_rn(wx, wy), wz);
+    const long long nd = cn[8 * cell + l];
+    for (int q = 0; q < 3; ++q) u[q] = __dadd_rn(u[q], __dmul_rn(w, nan_to_num(Urel[3 * nd + q])));
+  }
+  for (int q = 0; q < 3; ++q) out[3 * n + q] = (float)u[q];
 }
 
 }  // namespace cm

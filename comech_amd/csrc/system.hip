@@ -2364,4 +2364,22 @@ int cm_sample_fe(void* stream, int mx, int my, int mz, int sub, int nx, int ny, 
   CM_API_END
 }
 
+int cm_sample_nodes(void* stream, int mx, int my, int mz, const double* mxe, const double* mye, const double* mze,
+                    const int* ci, const int* cj, const int* ck, int nq, const int* kq, const int* qz, int nx, int ny,
+                    int nz, const double* xe, const double* ye, const double* ze, const uint8_t* active,
+                    const int64_t* cn, const double* Urel, int* dz2, int* kbest, float* out) {
+  CM_API_BEGIN
+  CM_CHECK(mx > 0 && my > 0 && mz > 0 && nx > 0 && ny > 0 && nz > 0, "bad grids");
+  CM_CHECK(nq >= 1 && nq <= mz + 1, "bad query layers");
+  const hipStream_t st = (hipStream_t)stream;
+  const long long ncol = (long long)nx * ny;
+  hipLaunchKernelGGL(cm::k_column_nearest, cm::grid1(ncol * nq), dim3(256), 0, st, nx, ny, nz, nq, kq, active, dz2,
+                     kbest);
+  const long long mnode = (long long)(mx + 1) * (my + 1) * (mz + 1);
+  hipLaunchKernelGGL(cm::k_sample_nodes, cm::grid1(mnode), dim3(256), 0, st, mx + 1, my + 1, mz + 1, mxe, mye, mze, ci,
+                     cj, ck, qz, nx, ny, nz, xe, ye, ze, active, dz2, kbest, (const long long*)cn, Urel, out);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
 }  // extern "C"

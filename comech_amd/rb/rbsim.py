@@ -1,9 +1,14 @@
-"""Drop-in for ``sample_fe`` of the reference's ``rb/rbsim.py:110-140`` on the GPU.
+"""Drop-ins for ``sample_fe`` and ``sample_nodes`` of the reference's ``rb/rbsim.py:110-161`` on the GPU.
 
-Every analysis cell averages von Mises stress, stress and displacement over sub^3 sample points in
-the FE cells; samples outside the FE box or in inactive FE cells are excluded.  The host builds
-the per-axis sample tables (cell index, local coordinate, inside flag); the averaging runs in the
-``k_sample_fe`` kernel.  Returns (vm, u, cover) and sets ``sample_fe.sig`` like the reference.
+sample_fe: every analysis cell averages von Mises stress, stress and displacement over sub^3 sample
+points in the FE cells; samples outside the FE box or in inactive FE cells are excluded.  The host
+builds the per-axis sample tables (cell index, local coordinate, inside flag); the averaging runs in
+the ``k_sample_fe`` kernel.  Returns (vm, u, cover) and sets ``sample_fe.sig`` like the reference.
+
+sample_nodes: the relative displacement at every analysis-grid node, trilinear in the FE cell that
+contains the node, or in the nearest active FE cell when that one is empty (the feature transform
+of scipy.ndimage.distance_transform_edt in cell-index units, rb/rbsim.py:152, computed by the
+``k_column_nearest`` / ``k_sample_nodes`` kernels with scipy's tie rule: smallest (k, j, i)).
 """
 import ctypes
 
@@ -12,6 +17,15 @@ import torch
 
 from .. import _lib
 from ..system import require_gpu
+
+
+def _dev(a, dtype, dev):
+    """numpy array or torch tensor -> contiguous flat device tensor of ``dtype`` (no copy when it
+    already is one: the engine's displacement and connectivity can stay on the GPU)."""
+    if isinstance(a, torch.Tensor):
+        return a.to(device=dev, dtype=dtype).contiguous().reshape(-1)
+    np_dt = torch.empty((), dtype=dtype).numpy().dtype
+    return torch.as_tensor(np.ascontiguousarray(np.asarray(a, np_dt).reshape(-1)), device=dev)
 
 
 def _axis(edges, d, ml_edges, ml_d, sub):
@@ -37,11 +51,11 @@ def sample_fe(o, grid_ml, Urel, sub=2, device=None):
         tabs += [t(idx, torch.int32), t(tt, torch.float64), t(ins, torch.uint8)]
     mx, my, mz = grid_ml.shape
     ncm = mx * my * mz
-    a = t(np.asarray(act, np.uint8), torch.uint8)
-    c = t(np.asarray(cn, np.int64).reshape(-1), torch.int64)
-    vm_in = t(np.asarray(o["VM"], np.float32), torch.float32)
-    sg_in = t(np.asarray(o["SIG"], np.float32).reshape(-1), torch.float32)
-    U = t(np.asarray(Urel, float).reshape(-1), torch.float64)
+    a = _dev(act, torch.uint8, dev)
+    c = _dev(cn, torch.int64, dev)
+    vm_in = _dev(o["VM"], torch.float32, dev)
+    sg_in = _dev(o["SIG"], torch.float32, dev)
+    U = _dev(Urel, torch.float64, dev)
     vm = torch.empty(ncm, dtype=torch.float32, device=dev)
     u = torch.empty(ncm * 3, dtype=torch.float32, device=dev)
     cov = torch.empty(ncm, dtype=torch.float64, device=dev)
@@ -52,3 +66,37 @@ def sample_fe(o, grid_ml, Urel, sub=2, device=None):
                                 vp(cov), vp(sg)))
     sample_fe.sig = sg.view(ncm, 6).cpu().numpy()
     return vm.cpu().numpy(), u.view(ncm, 3).cpu().numpy(), cov.cpu().numpy()
+
+
+def sample_nodes(o, grid_ml, Urel, device=None):
+    """rb/rbsim.py:143-161.  o: dict with the FE grid ``g``, connectivity ``cn`` and ``active``;
+    grid_ml: the analysis grid; Urel: (nodes, 3) relative displacement (NaN for unused nodes).
+    cn, active and Urel may be numpy arrays or device tensors.  Returns float32 (nnode_ml, 3) in
+    the analysis grid's node order."""
+    dev = require_gpu(device)
+    lib = _lib.load()
+    g, cn = o["g"], o["cn"]
+    a = _dev(o["active"], torch.uint8, dev)
+    if a.numel() != g.nx * g.ny * g.nz:
+        raise ValueError("active must hold one flag per FE cell")
+    if not bool(a.any()):
+        raise ValueError("sample_nodes needs at least one active FE cell")
+    cells = []
+    for edges, n, me in ((g.xe, g.nx, grid_ml.xe), (g.ye, g.ny, grid_ml.ye), (g.ze, g.nz, grid_ml.ze)):
+        cells.append(np.clip(np.searchsorted(edges, me) - 1, 0, n - 1).astype(np.int32))
+    kq, qz = np.unique(cells[2], return_inverse=True)
+    t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
+    me = [t(np.asarray(e, np.float64), torch.float64) for e in (grid_ml.xe, grid_ml.ye, grid_ml.ze)]
+    ce = [t(c, torch.int32) for c in cells]
+    kqd, qzd = t(kq.astype(np.int32), torch.int32), t(qz.astype(np.int32), torch.int32)
+    fe = [t(np.asarray(e, np.float64), torch.float64) for e in (g.xe, g.ye, g.ze)]
+    c = _dev(cn, torch.int64, dev)
+    U = _dev(Urel, torch.float64, dev)
+    work = torch.empty((2, len(kq) * g.nx * g.ny), dtype=torch.int32, device=dev)
+    mx, my, mz = len(grid_ml.xe) - 1, len(grid_ml.ye) - 1, len(grid_ml.ze) - 1
+    out = torch.empty(((mx + 1) * (my + 1) * (mz + 1), 3), dtype=torch.float32, device=dev)
+    vp = lambda x: ctypes.c_void_p(x.data_ptr())
+    _lib.check(lib.cm_sample_nodes(ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream), mx, my, mz,
+                                   *[vp(x) for x in me], *[vp(x) for x in ce], len(kq), vp(kqd), vp(qzd), g.nx, g.ny,
+                                   g.nz, *[vp(x) for x in fe], vp(a), vp(c), vp(U), vp(work[0]), vp(work[1]), vp(out)))
+    return out.cpu().numpy()

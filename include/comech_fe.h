@@ -246,6 +246,19 @@ int cm_sample_fe(void* stream, int mx, int my, int mz, int sub, int nx, int ny, 
                  const float* VM, const float* SIG, const double* Urel, float* vm_out, float* u_out, double* cover,
                  float* sig_out);
 
+/* Node resampling of rb/rbsim.py:143-161 (sample_nodes): the relative displacement at the
+ * (mx+1)(my+1)(mz+1) analysis nodes (fe/solver.py:120-123 order), trilinear in the nearest active
+ * FE cell (scipy.ndimage.distance_transform_edt feature transform of the inactive mask in cell-index
+ * units, rb/rbsim.py:152; ties: smallest (k, j, i)).  mxe/mye/mze: analysis node coordinates per
+ * axis; ci/cj/ck: their FE cell per axis (clip(searchsorted(edges, x) - 1, 0, n - 1)); kq[nq]: the
+ * distinct z cells of ck, qz[mz+1]: each z node's index into kq.  FE grid edges xe/ye/ze,
+ * active[ncell], cn[ncell*8], Urel[nodes*3] (FP64).  Work arrays dz2, kbest: nq*nx*ny ints each.
+ * out[(mx+1)(my+1)(mz+1)*3] float32.  At least one FE cell must be active.  All device pointers. */
+int cm_sample_nodes(void* stream, int mx, int my, int mz, const double* mxe, const double* mye, const double* mze,
+                    const int* ci, const int* cj, const int* ck, int nq, const int* kq, const int* qz, int nx, int ny,
+                    int nz, const double* xe, const double* ye, const double* ze, const uint8_t* active,
+                    const int64_t* cn, const double* Urel, int* dz2, int* kbest, float* out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,11 @@ patch test                  fe/test_patch.py:4-14 closed forms uz_top=-10/E, ux=
 cell_stress(_cn)            uniaxial / hydrostatic / pure shear closed forms for von Mises
 spring_matrix_cn            k w w^T closed form on hand-built springs
 sample_fe                   constant and linear fields reproduced exactly
+sample_nodes                nodes in active cells: linear fields reproduced exactly; nodes in
+                            empty cells: the cell used is the brute-force nearest active cell
+                            (smallest squared index distance, ties to the smallest (k, j, i),
+                            the rule scipy's feature transform follows) and the value is that
+                            cell's trilinear field at the clamped local coordinates
 lattice_materials           single-phase cells return that phase's (E, nu); a hard phase in
                             series with one layer is the Reuss average of constrained and shear
                             moduli; two phases in a non-joint cell the Voigt average; joint cells
@@ -761,6 +766,43 @@ def sample_fe(g, cn, active, VM, SIG, Urel, grid_ml, sub=2):
     u_c = blocksum(u * ok[:, None]) / den[:, None]
     sig_c = blocksum(sg) / den[:, None]
     return vm_c.astype(np.float32), u_c.astype(np.float32), okc / sub ** 3, sig_c.astype(np.float32)
+
+
+# rb/rbsim.py:143-161
+def sample_nodes(g, cn, active, Urel, grid_ml):
+    """Relative displacement at every analysis-grid node (grid_ml.node_xyz order): trilinear in the
+    FE cell holding the node (clip(searchsorted - 1)), or, when that cell is inactive, in the
+    nearest active cell of the Euclidean feature transform of the inactive mask in cell-index units
+    (scipy.ndimage.distance_transform_edt, rb/rbsim.py:152), local coordinates clamped to [0, 1].
+    Returns float32 (nnode_ml, 3)."""
+    from scipy import ndimage
+    P = grid_ml.node_xyz
+    held = [np.clip(np.searchsorted(e, P[:, a]) - 1, 0, n - 1)
+            for a, (e, n) in enumerate(((g.xe, g.nx), (g.ye, g.ny), (g.ze, g.nz)))]
+    _, feat = ndimage.distance_transform_edt(~np.asarray(active, bool).reshape(g.shape), return_indices=True)
+    near = [feat[a][held[0], held[1], held[2]] for a in range(3)]
+    c = (near[0] * g.ny + near[1]) * g.nz + near[2]
+    t = [np.clip((P[:, a] - e[near[a]]) / d[near[a]], 0, 1)
+         for a, (e, d) in enumerate(((g.xe, g.dx), (g.ye, g.dy), (g.ze, g.dz)))]
+    u = np.zeros((len(P), 3))
+    for l in range(8):
+        w = np.ones(len(P))
+        for a in range(3):
+            w = w * (t[a] if LOC[l, a] else 1 - t[a])
+        u += w[:, None] * np.nan_to_num(Urel[cn[c, l]])
+    return u.astype(np.float32)
+
+
+def nearest_active_bruteforce(active, shape, cells):
+    """Nearest active cell of every listed cell by enumeration: all active cells at the smallest
+    squared index distance, and the one of them with the smallest (k, j, i)."""
+    A = np.argwhere(np.asarray(active, bool).reshape(shape))
+    Q = np.stack(np.unravel_index(np.asarray(cells), shape), 1)
+    d2 = ((Q[:, None, :] - A[None, :, :]) ** 2).sum(-1)
+    best = d2.min(1)
+    order = np.lexsort((A[:, 0], A[:, 1], A[:, 2]))
+    first = np.argmax(d2[:, order] == best[:, None], axis=1)
+    return A[order][first], best
 
 
 def rel_l2(a, b):

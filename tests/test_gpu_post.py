@@ -1,4 +1,4 @@
-"""GPU post-processing and drop-in interoperability: sample_fe (rb/rbsim.py:110-140) against the
+"""GPU post-processing and drop-in interoperability: sample_fe and sample_nodes (rb/rbsim.py:110-161) against the
 oracle, and solve_cn called with a reference-shaped Grid object (the oracle's restatement of
 fe/solver.Grid, which has no helper methods of the product's Grid)."""
 import numpy as np
@@ -31,6 +31,36 @@ def test_sample_fe_matches_oracle():
     assert np.abs(vm_g - vm_o).max() <= 1e-5 * np.abs(vm_o).max()
     assert np.abs(u_g - u_o).max() <= 1e-5 * np.abs(u_o).max()
     assert np.abs(RS.sample_fe.sig - sig_o).max() <= 1e-5 * np.abs(sig_o).max()
+
+
+@pytest.mark.parametrize("sparse", [False, True])
+def test_sample_nodes_matches_oracle(sparse):
+    """rb/rbsim.py:143-161 on the GPU: bit-exact float32 against the oracle (scipy feature transform),
+    split copies in the connectivity, NaN (unused) entries, an analysis grid reaching outside the FE
+    box; ``sparse`` keeps 3 % of the active cells, so most nodes resolve a nearest cell with ties."""
+    c = C.cfg2((24, 16, 16), 3.0)
+    og = C.oracle_grid(c)
+    rng = np.random.default_rng(12)
+    act = c["active"].copy()
+    if sparse:
+        act &= rng.uniform(size=act.size) < 0.03
+    nn = len(c["orig"])
+    Urel = rng.standard_normal((nn, 3))
+    Urel[rng.uniform(size=nn) < 0.05] = np.nan
+    gm = O.Grid(np.arange(-7.0, 80.0, 6.0), np.arange(-2.0, 53.0, 5.0), np.arange(1.0, 56.0, 7.0))
+    u_o = O.sample_nodes(og, c["cn"], act, Urel, gm)
+    u_g = RS.sample_nodes(dict(g=og, cn=c["cn"], active=act), gm, Urel, device=DEV)
+    assert u_g.shape == u_o.shape and u_g.dtype == np.float32
+    assert np.array_equal(u_g, u_o)
+
+
+def test_sample_nodes_rejects_empty_mask():
+    c = C.cfg2((24, 16, 16), 3.0)
+    og = C.oracle_grid(c)
+    gm = O.Grid(np.arange(0.0, 70.0, 10.0), np.arange(0.0, 50.0, 10.0), np.arange(0.0, 50.0, 10.0))
+    with pytest.raises(ValueError):
+        RS.sample_nodes(dict(g=og, cn=c["cn"], active=np.zeros(og.ncell, bool)), gm,
+                        np.zeros((len(c["orig"]), 3)), device=DEV)
 
 
 def test_solve_cn_with_reference_shaped_grid():

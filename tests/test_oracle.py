@@ -246,6 +246,49 @@ def test_sample_fe_constant_and_linear():
     assert np.allclose(u, centres, atol=1e-6)
 
 
+@pytest.mark.parametrize("density", [0.03, 0.25])
+def test_edt_feature_transform_is_nearest_with_kji_ties(density):
+    """The nearest-active-cell map sample_nodes takes from scipy's feature transform equals
+    enumeration: smallest squared index distance, ties to the smallest (k, j, i)."""
+    from scipy import ndimage
+    rng = np.random.default_rng(int(density * 100))
+    for shape in ((7, 5, 6), (9, 4, 8), (5, 8, 7)):
+        act = rng.uniform(size=shape) < density
+        act[0, 0, 0] = True
+        dist, feat = ndimage.distance_transform_edt(~act, return_indices=True)
+        cells = np.arange(act.size)
+        best, d2 = O.nearest_active_bruteforce(act.ravel(), shape, cells)
+        assert np.array_equal(feat.reshape(3, -1).T, best)
+        assert np.allclose(dist.ravel() ** 2, d2)
+
+
+def test_sample_nodes_nearest_cell_and_linear_field():
+    """Nodes in active cells reproduce a linear field; nodes in empty cells take the field of the
+    brute-force nearest active cell at the clamped local coordinates."""
+    rng = np.random.default_rng(5)
+    g = O.Grid(np.cumsum(np.r_[0, rng.uniform(0.5, 1.5, 8)]), np.cumsum(np.r_[0, rng.uniform(0.5, 1.5, 6)]),
+               np.cumsum(np.r_[0, rng.uniform(0.5, 1.5, 7)]))
+    act = rng.uniform(size=g.ncell) < 0.2
+    act[0] = True
+    B = np.array([[1.0, 0.5, -0.25], [0.3, 2.0, 0.1], [-0.7, 0.2, 3.0]])
+    Ur = g.node_xyz @ B.T + np.array([0.1, -0.2, 0.3])
+    gm = O.Grid(np.linspace(-1.0, g.xe[-1] + 1.0, 6), np.linspace(-0.5, g.ye[-1] + 0.7, 5),
+                np.linspace(0.2, g.ze[-1] + 1.5, 7))
+    u = O.sample_nodes(g, g.cell_nodes, act, Ur, gm)
+    P = gm.node_xyz
+    held = [np.clip(np.searchsorted(e, P[:, a]) - 1, 0, n - 1)
+            for a, (e, n) in enumerate(((g.xe, g.nx), (g.ye, g.ny), (g.ze, g.nz)))]
+    cells = (held[0] * g.ny + held[1]) * g.nz + held[2]
+    near, _ = O.nearest_active_bruteforce(act, g.shape, cells)
+    lo = np.stack([g.xe[near[:, 0]], g.ye[near[:, 1]], g.ze[near[:, 2]]], 1)
+    hi = np.stack([g.xe[near[:, 0] + 1], g.ye[near[:, 1] + 1], g.ze[near[:, 2] + 1]], 1)
+    Q = np.clip(P, lo, hi)                        # clamped point inside the chosen cell
+    expect = Q @ B.T + np.array([0.1, -0.2, 0.3])
+    assert np.abs(u - expect).max() <= 1e-5 * np.abs(expect).max()
+    inside = act[cells] & (P == Q).all(1)
+    assert inside.sum() > 0 and (~act[cells]).sum() > 20
+
+
 # ---- lattice materials (rb/rbfe.py:75-134) ---------------------------------------------------
 LAT_CLS = {1: 1, 2: 2, 3: 1, 4: 2, 5: 3, 6: 4, 7: 5, 8: 6, 9: 7, 10: 8, 11: 9, 12: 1}
 LAT_ROLES = {"bones": {"calc": (1, 2), "talus": (3, 4)}, "bonded_extra": [12]}

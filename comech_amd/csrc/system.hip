@@ -1662,11 +1662,13 @@ struct System {
           dot(n, r64.p, r64.p, S_RR);
           rn = std::sqrt(read_slot(S_RR));
           if (rn <= thr) { flag = 0; break; }
-          // two restarts in a row that do not halve the true residual: the attainable accuracy
-          // of this system is above tol; stop (not converged) instead of iterating to maxiter
+          // three restarts in a row that do not halve the true residual: the attainable accuracy
+          // of this system (the residual of its solution rounded to FP64) is above tol; stop (not
+          // converged) instead of iterating to maxiter.  A system whose floor lies just below tol
+          // gets a third sample of it before giving up.
           stalls = (rn > 0.5 * last_true) ? stalls + 1 : 0;
           last_true = rn;
-          if (stalls >= 2) break;
+          if (stalls >= 3) break;
           ++restarts;
           flexible = true;
           precondition64(vprec, r64.p, z64.p);

@@ -35,8 +35,8 @@ def log(*a):
 def build_inputs(dims=(160, 96, 96), h=0.5, shell=2):
     from comech_amd import synth
     from comech_amd.engine import split_cells
-    c = synth.config2(dims=dims, h=h, shell=shell)
-    extra, terms, f = synth.solve_inputs_config2(c)
+    c = synth.hindfoot_raster(dims=dims, h=h, shell=shell)
+    extra, terms, f = synth.solve_inputs_hindfoot(c)
     irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
     return c, (extra, terms), f, irr_c, irr_n
 
@@ -69,7 +69,7 @@ def cpu_baseline(sample_dims=(80, 48, 48)):
     os.environ["OPENBLAS_NUM_THREADS"] = "1"
     from comech_amd import synth
     from oracle import fe_oracle as O
-    c = synth.config2(dims=sample_dims, h=80.0 / sample_dims[0])
+    c = synth.hindfoot_raster(dims=sample_dims, h=80.0 / sample_dims[0])
     g = O.Grid(c["grid"].xe, c["grid"].ye, c["grid"].ze)
     t0 = time.time()
     U, info, R = O.solve_cn(g, c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"], c["loads"],
@@ -101,15 +101,15 @@ def action_inputs(actions):
     from comech_amd.engine import split_cells
     out = []
     for a in actions:
-        c = synth.config2(dims=(160, 96, 96), h=0.5, action=a)
-        extra, terms, f = synth.solve_inputs_config2(c)
+        c = synth.hindfoot_raster(dims=(160, 96, 96), h=0.5, action=a)
+        extra, terms, f = synth.solve_inputs_hindfoot(c)
         irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
         out.append((c, (extra, terms), f, irr_c, irr_n))
     return out
 
 
 def large_workload(args):
-    """Configuration 5: 384x256x256 voxels at 0.25 mm (the hindfoot anatomy upscaled, 4-voxel cortical
+    """Large single solve: 384x256x256 voxels at 0.25 mm (the hindfoot anatomy upscaled, 4-voxel cortical
     shells), one solve per precision form.  Prints one JSON line (not the headline metric): fine
     matvec time and algorithmic GB/s per precision, V-cycle time, solve time and iterations to rtol
     1e-8 (FP64 true residual) for fp64 / ref / mixed, and the von Mises relative L2 of the mixed
@@ -125,7 +125,7 @@ def large_workload(args):
     f_dev = torch.from_numpy(f).to(dev)
     s = make_system(c, extra, irr_c, irr_n)
     lv = s.levels()
-    out = {"workload": "hindfoot raster 384x256x256 @0.25mm (configs[4] anatomy), rtol 1e-8, FP64 true residual",
+    out = {"workload": "hindfoot raster 384x256x256 @0.25mm (the hindfoot anatomy upscaled, 4-voxel cortical shells), rtol 1e-8, FP64 true residual",
            "ncell": g.ncell, "dofs": lv[0]["ndof"], "levels": [l["ndof"] for l in lv],
            "t_setup_ms": round(s.timings()["total"], 1)}
     for dt, name, sz in ((torch.float32, "f32", 4), (torch.float64, "f64", 8)):
@@ -165,7 +165,7 @@ def large_workload(args):
 
 
 def batch3_workload(args):
-    """Configuration 3 subset: the first ``--feet`` feet of the 64-foot set (grids drawn uniformly
+    """Uneven-grid batch: the first ``--feet`` feet of the 64-foot set (grids drawn uniformly
     between 96x64x64 and 192x112x112 at 0.5 mm, each with its own seed) under the 8 surgical
     actions, solved batched (``--concurrency`` in flight) to rtol 1e-8 in mode ``--mode``, every
     answer checked by the FP64 true residual.  One solve per grid shape runs alone first (the
@@ -180,12 +180,12 @@ def batch3_workload(args):
     from comech_amd.synth import ACTIONS
     dev = torch.device("cuda:0")
     t0 = time.time()
-    sizes = synth.config3_sizes(64)[:args.feet]
+    sizes = synth.batch_sizes(64)[:args.feet]
     cases = []
     for foot in range(args.feet):
         for a in ACTIONS:
-            c = synth.config3_case(foot, a)
-            extra, terms, f = synth.solve_inputs_config2(c)
+            c = synth.batch_case(foot, a)
+            extra, terms, f = synth.solve_inputs_hindfoot(c)
             irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
             cases.append((c, (extra, terms), torch.from_numpy(f).to(dev), irr_c, irr_n))
         log(f"batch3: foot {foot} {sizes[foot]} inputs ready ({time.time() - t0:.1f} s)")
@@ -217,7 +217,7 @@ def batch3_workload(args):
                       "GBps": round(alg / (ms * 1e-3) / 1e9, 1)}
         sy.close()
     iters = [i["iters"] for i in infos]
-    line = {"workload": f"configuration 3 subset: {args.feet} feet (grids 96x64x64 .. 192x112x112 at 0.5 mm) x "
+    line = {"workload": f"uneven-grid batch: {args.feet} feet (grids 96x64x64 .. 192x112x112 at 0.5 mm) x "
                         f"{len(ACTIONS)} surgical actions, rtol {args.tol}, FP64 true residual",
             "mode": args.mode, "concurrency": args.concurrency, "solves": len(infos),
             "grids": [list(z) for z in sizes], "solves_per_s": round(len(infos) / wall, 2),
@@ -246,7 +246,7 @@ def prep_workload(args):
     from comech_amd.rb import splitfe as SF
     os.environ["OMP_NUM_THREADS"] = "1"
     dev = torch.device("cuda:0")
-    c = synth.config2(dims=(160, 96, 96), h=0.5)
+    c = synth.hindfoot_raster(dims=(160, 96, 96), h=0.5)
     g = c["grid"]
     lab = c["labels"]
     bodies = [("calcaneus", 0), ("talus", 0), ("navicular", 0)]

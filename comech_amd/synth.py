@@ -1,20 +1,21 @@
-"""Seeded synthetic voxel rasters for the benchmark configurations (inputs in the reference's API).
+"""Seeded synthetic voxel rasters for the benchmark workloads (inputs in the reference's API).
 
-``config1``  32^3 cube at 1 mm, six random ellipsoidal inclusions (E 17000 in / 700 out, nu 0.3),
-             bottom face held by stiff Winkler springs (k = 1e6 N/mm^3 x tributary area, the
-             reference's "fixed" convention, rb/splitfe.py:143), 1 MPa compression on the top face.
-             fe/solver.py ``solve`` form (2x2x2 integration).
-``config2``  hindfoot-like raster (default 160x96x96 at 0.5 mm): soft-tissue envelope (E 1, nu 0.45)
-             holding three ellipsoid bones (cortical shell 17000 / cancellous core 700, nu 0.3), a
-             coalition cylinder (r = 4 voxels, E 700) bridging two of them, split nodes between
-             bones (rb/splitfe.py:261-305) joined by isotropic springs k = 1e3 N/mm, ~2000 plantar
-             ground springs k = 50 N/mm, 200 ligament springs k = 100 N/mm between bone-surface
-             nodes, 700 N downward over the top 10 cell layers.  rb/splitfe.py ``solve_cn`` form
-             (SRI).  The same generator at other sizes gives the batched and the large
-             configurations (anatomy is relative to the box; shell thickness in voxels).
-``config3``  64 feet x 8 surgical actions with grids between 96x64x64 and 192x112x112.
+``cube_inclusions``  32^3 cube at 1 mm, six random ellipsoidal inclusions (E 17000 in / 700 out,
+    nu 0.3), bottom face held by stiff Winkler springs (k = 1e6 N/mm^3 x tributary area, the
+    reference's "fixed" convention, rb/splitfe.py:143), 1 MPa compression on the top face.
+    fe/solver.py ``solve`` form (2x2x2 integration).
+``hindfoot_raster``  hindfoot-like raster (default 160x96x96 at 0.5 mm): soft-tissue envelope (E 1,
+    nu 0.45) holding three ellipsoid bones (cortical shell 17000 / cancellous core 700, nu 0.3), a
+    coalition cylinder (r = 4 voxels, E 700) bridging two of them, split nodes between bones
+    (rb/splitfe.py:261-305) joined by isotropic springs k = 1e3 N/mm, ~2000 plantar ground springs
+    k = 50 N/mm, 200 ligament springs k = 100 N/mm between bone-surface nodes, 700 N downward over
+    the top 10 cell layers.  rb/splitfe.py ``solve_cn`` form (SRI).  The same generator at other
+    sizes gives the uneven-grid batch and the large single solve (anatomy is relative to the box;
+    shell thickness in voxels).
+``batch_case``  64 feet x 8 surgical actions with grids between 96x64x64 and 192x112x112.
 
-Seeds: ``numpy.random.default_rng(config_id * 1000 + item)``.
+Seeds: ``numpy.random.default_rng(workload_id * 1000 + item)`` (cube 1, hindfoot 2, batch 3,
+large 5).
 """
 import numpy as np
 from scipy import ndimage
@@ -51,7 +52,7 @@ def _cell_centres(g):
 
 
 # --------------------------------------------------------------------------------------------
-def config1(n=32, h=1.0, seed=1000, n_incl=6):
+def cube_inclusions(n=32, h=1.0, seed=1000, n_incl=6):
     rng = np.random.default_rng(seed)
     e = np.arange(n + 1) * h
     g = Grid(e, e, e)
@@ -73,12 +74,12 @@ def config1(n=32, h=1.0, seed=1000, n_incl=6):
     nt, at = face_node_areas(g, top, "z+")
     springs = [(nb, ab, np.array([1e6, 1e6, 1e6]))]
     loads = [(nt, at, np.array([0.0, 0.0, -1.0]))]
-    return dict(name="config1", grid=g, lam=lam, mu=mu, active=active, labels=labels, lam_tab=lam_tab,
+    return dict(name="cube_inclusions", grid=g, lam=lam, mu=mu, active=active, labels=labels, lam_tab=lam_tab,
                 mu_tab=mu_tab, springs=springs, loads=loads, sri=False)
 
 
-def solve_inputs_config1(case):
-    """config1 as level-0 GPU inputs: extra COO (diagonal springs) and nodal force vector."""
+def solve_inputs_cube(case):
+    """cube_inclusions as level-0 GPU inputs: extra COO (diagonal springs) and nodal force vector."""
     g = case["grid"]
     nb, ab, kv = case["springs"][0]
     kd = np.zeros((g.nnode, 3))
@@ -96,10 +97,10 @@ def solve_inputs_config1(case):
 FOOT_E = dict(soft=(1.0, 0.45), coalition=(700.0, 0.3), cortical=(17000.0, 0.3), cancellous=(700.0, 0.3))
 
 
-def config2(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000, n_lig=200, k_iface=1e3, k_ground=50.0,
+def hindfoot_raster(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000, n_lig=200, k_iface=1e3, k_ground=50.0,
             k_lig=100.0, load=700.0, action="baseline", angle_seed=None):
     """Hindfoot-like raster.  action: baseline | resection | osteotomy | interposition | fusion |
-    load0.5 | load1.5 | load2 (config 3)."""
+    load0.5 | load1.5 | load2 (the surgical actions of the batched set)."""
     rng = np.random.default_rng(seed)
     nx, ny, nz = dims
     g = Grid(np.arange(nx + 1) * h, np.arange(ny + 1) * h, np.arange(nz + 1) * h)
@@ -215,14 +216,14 @@ def config2(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000, n_lig=
         cc = 3 * nmap[c[ok] // 3] + c[ok] % 3
         return sp.csr_matrix((v[ok], (rr, cc)), shape=(n, n))
 
-    return dict(name="config2", grid=g, lam=lam, mu=mu, active=active, labels=labels, lam_tab=lam_tab,
+    return dict(name="hindfoot_raster", grid=g, lam=lam, mu=mu, active=active, labels=labels, lam_tab=lam_tab,
                 mu_tab=mu_tab, part=part, bone_share=share, cn=cn, orig=orig, nsplit=nsplit, fixed=fixed,
                 loads=loads, extra=extra, extra_cb=extra_cb, terms=terms, sri=True, action=action, used=used,
                 n_ground=len(gsel), n_lig=len(pairs), n_copies=len(copies))
 
 
-def solve_inputs_config2(case, as_terms=True):
-    """config2 as level-0 GPU inputs: extra COO over node DOFs (ground springs; with
+def solve_inputs_hindfoot(case, as_terms=True):
+    """hindfoot_raster as level-0 GPU inputs: extra COO over node DOFs (ground springs; with
     ``as_terms=False`` also the interface and ligament springs as COO), rank-one terms (interface
     and ligament springs) and the nodal force vector (grid nodes then copies)."""
     nn = len(case["orig"])
@@ -242,17 +243,17 @@ def solve_inputs_config2(case, as_terms=True):
 ACTIONS = ["baseline", "resection", "osteotomy", "interposition", "fusion", "load0.5", "load1.5", "load2"]
 
 
-def config3_sizes(n_feet=64, seed=3000):
+def batch_sizes(n_feet=64, seed=3000):
     """Grid sizes of the batched configuration: uniform between 96x64x64 and 192x112x112 (even)."""
     rng = np.random.default_rng(seed)
     lo, hi = np.array([96, 64, 64]), np.array([192, 112, 112])
     return [tuple(int(2 * round(v / 2)) for v in rng.uniform(lo, hi)) for _ in range(n_feet)]
 
 
-def config3_case(foot, action, n_feet=64, h=0.5):
-    dims = config3_sizes(n_feet)[foot]
-    return config2(dims=dims, h=h, seed=3000 + foot, action=action, angle_seed=3500 + foot)
+def batch_case(foot, action, n_feet=64, h=0.5):
+    dims = batch_sizes(n_feet)[foot]
+    return hindfoot_raster(dims=dims, h=h, seed=3000 + foot, action=action, angle_seed=3500 + foot)
 
 
-def config5(dims=(384, 256, 256), h=0.25, seed=5000):
-    return config2(dims=dims, h=h, seed=seed, shell=4)
+def hindfoot_large(dims=(384, 256, 256), h=0.25, seed=5000):
+    return hindfoot_raster(dims=dims, h=h, seed=seed, shell=4)

@@ -8,13 +8,13 @@ from oracle import fe_oracle as O
 
 
 @functools.lru_cache(maxsize=None)
-def cfg1(n=10):
-    return synth.config1(n=n, h=1.0)
+def cube(n=10):
+    return synth.cube_inclusions(n=n, h=1.0)
 
 
 @functools.lru_cache(maxsize=None)
-def cfg2(dims=(40, 24, 24), h=2.0):
-    return synth.config2(dims=dims, h=h)
+def hindfoot(dims=(40, 24, 24), h=2.0):
+    return synth.hindfoot_raster(dims=dims, h=h)
 
 
 def oracle_grid(case):
@@ -24,14 +24,14 @@ def oracle_grid(case):
 
 @functools.lru_cache(maxsize=None)
 def oracle_system1(n=10):
-    c = cfg1(n)
+    c = cube(n)
     _, info = O.solve(oracle_grid(c), c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="none")
     return info
 
 
 @functools.lru_cache(maxsize=None)
 def oracle_system2(dims=(40, 24, 24), h=2.0):
-    c = cfg2(dims, h)
+    c = hindfoot(dims, h)
     _, info, _ = O.solve_cn(oracle_grid(c), c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"],
                             c["loads"], extra=c["extra_cb"], method="none")
     return info

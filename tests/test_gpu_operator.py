@@ -17,15 +17,15 @@ DEV = "cuda:0"
 
 
 def _sys1(n=10, **kw):
-    c = C.cfg1(n)
-    extra, f = synth.solve_inputs_config1(c)
+    c = C.cube(n)
+    extra, f = synth.solve_inputs_cube(c)
     return c, VoxelSystem(c["grid"], sri=False, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
                           mu_tab=c["mu_tab"], extra=extra, device=DEV, **kw)
 
 
 def _sys2(dims=(40, 24, 24), h=2.0, as_terms=True, **kw):
-    c = C.cfg2(dims, h)
-    extra, terms, f = synth.solve_inputs_config2(c, as_terms=as_terms)
+    c = C.hindfoot(dims, h)
+    extra, terms, f = synth.solve_inputs_hindfoot(c, as_terms=as_terms)
     irr_c, irr_n = __import__("comech_amd.engine", fromlist=["x"]).split_cells(c["grid"], c["active"], c["cn"])
     s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
                     mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c, irr_nodes=irr_n,
@@ -48,13 +48,13 @@ def _check_apply(s, info, nn, tol64=1e-12, tol32=1e-5):
         assert not np.any(y[~used])
 
 
-def test_apply_config1_full_integration():
+def test_apply_cube_inclusions_full_integration():
     c, s = _sys1(10)
     _check_apply(s, C.oracle_system1(10), c["grid"].nnode)
 
 
 @pytest.mark.parametrize("as_terms", [True, False])
-def test_apply_config2_split_sri_springs(as_terms):
+def test_apply_hindfoot_raster_split_sri_springs(as_terms):
     """Springs as rank-one terms or as generic COO (diagonal -> rank-one, off-diagonal -> CSR)."""
     c, s = _sys2(as_terms=as_terms)
     assert s.ncopy > 0 and len(s.irr_cells) > 0
@@ -104,13 +104,13 @@ def _patch_case():
     return dict(grid=g), s, info, info["used"]
 
 
-@pytest.mark.parametrize("which", ["cfg1", "cfg2", "cfg2coo", "patch", "cfg2semi"])
+@pytest.mark.parametrize("which", ["cube", "hindfoot", "hindfoot_coo", "patch", "hindfoot_semi"])
 def test_coarse_levels_equal_galerkin(which):
     """Level-l GPU operator (moment cells + Galerkin springs) == P^T A P of the oracle hierarchy
-    (fe/gmg.py:91-99) on the kept coarse DOFs, every level.  cfg2semi: automatic level rule with the
+    (fe/gmg.py:91-99) on the kept coarse DOFs, every level.  hindfoot_semi: automatic level rule with the
     last step coarsening x and y only (the oracle's prolongation with the same axis mask)."""
     nlev = 3
-    if which == "cfg2semi":
+    if which == "hindfoot_semi":
         c, s = _sys2(last_axes=3)
         info = C.oracle_system2()
         used_nodes = c["orig"][info["used"]]
@@ -120,7 +120,7 @@ def test_coarse_levels_equal_galerkin(which):
         assert nlev >= 3 and len(s.levels()) == nlev
         top = s.levels()[-1]
         assert top["nz"] == s.levels()[-2]["nz"] and top["nx"] < s.levels()[-2]["nx"]
-    elif which == "cfg1":
+    elif which == "cube":
         c, s = _sys1(10, levels=3)
         info = C.oracle_system1(10)
         used_nodes = info["used"]
@@ -128,11 +128,11 @@ def test_coarse_levels_equal_galerkin(which):
         c, s, info, used_nodes = _patch_case()
         nlev = 6
     else:
-        c, s = _sys2(levels=3, as_terms=(which == "cfg2"))
+        c, s = _sys2(levels=3, as_terms=(which == "hindfoot"))
         info = C.oracle_system2()
         used_nodes = c["orig"][info["used"]]
     g = c["grid"]
-    if which != "cfg2semi":
+    if which != "hindfoot_semi":
         mg = _oracle_hierarchy(info["K"], used_nodes, g, nlev)
     lv = s.levels()
     assert len(lv) == nlev
@@ -163,8 +163,8 @@ def test_level_rule_matches_reference(last_axes, expect):
     """<= 6000 DOFs or 6 levels (fe/gmg.py:80,87-90): 32^3 gives 3 levels (107,811 -> 14,739 ->
     2,187 DOFs) under the reference rule (every axis coarsened).  The default semi-coarsening of
     the last step keeps z: 8 x 8 x 16 cells, 9 * 9 * 17 * 3 = 4,131 DOFs (still <= 6000)."""
-    c = synth.config1(n=32)
-    extra, _ = synth.solve_inputs_config1(c)
+    c = synth.cube_inclusions(n=32)
+    extra, _ = synth.solve_inputs_cube(c)
     s = VoxelSystem(c["grid"], sri=False, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
                     mu_tab=c["mu_tab"], extra=extra, device=DEV, last_axes=last_axes)
     assert [l["ndof"] for l in s.levels()] == expect

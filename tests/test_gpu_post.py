@@ -14,7 +14,7 @@ DEV = "cuda:0"
 
 
 def test_sample_fe_matches_oracle():
-    c = C.cfg2((24, 16, 16), 3.0)
+    c = C.hindfoot((24, 16, 16), 3.0)
     og = C.oracle_grid(c)
     rng = np.random.default_rng(11)
     nn = len(c["orig"])
@@ -38,7 +38,7 @@ def test_sample_nodes_matches_oracle(sparse):
     """rb/rbsim.py:143-161 on the GPU: bit-exact float32 against the oracle (scipy feature transform),
     split copies in the connectivity, NaN (unused) entries, an analysis grid reaching outside the FE
     box; ``sparse`` keeps 3 % of the active cells, so most nodes resolve a nearest cell with ties."""
-    c = C.cfg2((24, 16, 16), 3.0)
+    c = C.hindfoot((24, 16, 16), 3.0)
     og = C.oracle_grid(c)
     rng = np.random.default_rng(12)
     act = c["active"].copy()
@@ -55,7 +55,7 @@ def test_sample_nodes_matches_oracle(sparse):
 
 
 def test_sample_nodes_rejects_empty_mask():
-    c = C.cfg2((24, 16, 16), 3.0)
+    c = C.hindfoot((24, 16, 16), 3.0)
     og = C.oracle_grid(c)
     gm = O.Grid(np.arange(0.0, 70.0, 10.0), np.arange(0.0, 50.0, 10.0), np.arange(0.0, 50.0, 10.0))
     with pytest.raises(ValueError):
@@ -64,7 +64,7 @@ def test_sample_nodes_rejects_empty_mask():
 
 
 def test_solve_cn_with_reference_shaped_grid():
-    c = C.cfg2((24, 16, 16), 3.0)
+    c = C.hindfoot((24, 16, 16), 3.0)
     og = C.oracle_grid(c)
     U, info, R = SF.solve_cn(og, c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"], c["loads"],
                              extra=c["extra_cb"], tol=1e-10, device=DEV)

@@ -71,7 +71,7 @@ def test_split_connectivity_gpu_bit_exact(seed):
 def test_split_connectivity_gpu_hindfoot_raster():
     """The synthetic hindfoot raster's bones: GPU == the product's host form == oracle."""
     from comech_amd import synth
-    c = synth.config2(dims=(40, 24, 24), h=2.0)
+    c = synth.hindfoot_raster(dims=(40, 24, 24), h=2.0)
     g = c["grid"]
     cn, orig, ns = SF.split_connectivity_gpu(g, c["active"], c["part"], c["bone_share"], device=DEV)
     cnh, origh, nsh = SF.split_connectivity_fast(g, c["active"], c["part"], c["bone_share"])

@@ -28,7 +28,7 @@ def rel(a, b):
 
 @pytest.fixture(scope="module")
 def ref1():
-    c = C.cfg1(12)
+    c = C.cube(12)
     og = C.oracle_grid(c)
     U, info = O.solve(og, c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="direct")
     sig, vm, _ = O.cell_stress(og, c["lam"], c["mu"], U, np.arange(og.ncell))
@@ -37,7 +37,7 @@ def ref1():
 
 @pytest.mark.parametrize("mode", ["fp64", "ref", "mixed"])
 @pytest.mark.parametrize("smoother", ["chebyshev", "jacobi"])
-def test_solve_config1(ref1, mode, smoother):
+def test_solve_cube_inclusions(ref1, mode, smoother):
     c, Uref, vmref = ref1
     U, info = FS.solve(c["grid"], c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="gmg", tol=1e-10,
                        mode=mode, smoother=smoother, device=DEV)
@@ -50,7 +50,7 @@ def test_solve_config1(ref1, mode, smoother):
 
 @pytest.fixture(scope="module")
 def ref2():
-    c = C.cfg2((24, 16, 16), 3.0)
+    c = C.hindfoot((24, 16, 16), 3.0)
     og = C.oracle_grid(c)
     U, info, R = O.solve_cn(og, c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"], c["loads"],
                             extra=c["extra_cb"], method="direct")
@@ -60,7 +60,7 @@ def ref2():
 
 
 @pytest.mark.parametrize("mode", ["fp64", "ref", "mixed"])
-def test_solve_cn_config2(ref2, mode):
+def test_solve_cn_hindfoot_raster(ref2, mode):
     c, Uref, Rref, vmref = ref2
     U, info, R = SF.solve_cn(c["grid"], c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"], c["loads"],
                              extra=c["extra_cb"], tol=1e-10, mode=mode, device=DEV)
@@ -182,8 +182,8 @@ def test_batched_concurrent_solves_match_sequential():
     cases = []
     for i, (dims, act) in enumerate([((24, 16, 16), "baseline"), ((28, 16, 18), "resection"),
                                      ((24, 18, 16), "fusion"), ((26, 16, 16), "load2")]):
-        c = synth.config2(dims=dims, h=3.0, seed=2100 + i, action=act)
-        extra, terms, f = synth.solve_inputs_config2(c)
+        c = synth.hindfoot_raster(dims=dims, h=3.0, seed=2100 + i, action=act)
+        extra, terms, f = synth.solve_inputs_hindfoot(c)
         irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
         cases.append((c, extra, terms, f, irr_c, irr_n))
 

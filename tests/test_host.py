@@ -171,7 +171,7 @@ def test_product_split_connectivity_bit_exact(seed):
 
 
 def test_used_nodes_and_irregular_cells():
-    c = synth.config2(dims=(20, 12, 12), h=4.0)
+    c = synth.hindfoot_raster(dims=(20, 12, 12), h=4.0)
     g = c["grid"]
     irr_c, irr_n = E.split_cells(g, c["active"], c["cn"])
     used = E.used_nodes(g, c["active"], len(c["orig"]), irr_c, irr_n)
@@ -193,15 +193,15 @@ def test_compress_materials_exact():
 
 
 def test_synth_deterministic_and_sized():
-    a, b = synth.config2(dims=(24, 16, 16), h=3.0), synth.config2(dims=(24, 16, 16), h=3.0)
+    a, b = synth.hindfoot_raster(dims=(24, 16, 16), h=3.0), synth.hindfoot_raster(dims=(24, 16, 16), h=3.0)
     for k in ("labels", "cn", "orig", "active"):
         assert np.array_equal(a[k], b[k])
     assert a["n_lig"] > 0 and a["n_copies"] > 0
     r, c_, v = a["extra"]
     assert len(r) == len(c_) == len(v)
-    c1 = synth.config1(n=32)
+    c1 = synth.cube_inclusions(n=32)
     assert c1["grid"].ncell == 32 ** 3 and set(np.unique(c1["labels"])) == {0, 1}
-    sizes = synth.config3_sizes()
+    sizes = synth.batch_sizes()
     assert len(sizes) == 64 and all(96 <= s[0] <= 192 and 64 <= s[1] <= 112 and 64 <= s[2] <= 112 for s in sizes)
 
 

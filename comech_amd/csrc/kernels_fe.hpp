@@ -286,6 +286,32 @@ __device__ __forceinline__ void lds_put3(double* p, const double v[3]) {
   *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], 0.0);
 }
 
+// LDS layout of n 3-vectors.  FP32: 16-byte slots as above.  FP64: one plane of n values per
+// component; 8-byte accesses of consecutive lanes are conflict-free, and the planes need 3/4 of the
+// LDS of 32-byte slots, which lets a fourth FP64 march block share a CU.
+template <typename T>
+struct V3Lds {
+  static constexpr bool slots = sizeof(T) == 4;
+  static constexpr int size(int n) { return slots ? 4 * n : 3 * n; }
+  static __device__ __forceinline__ int idx(int n, int i, int c) { return slots ? 4 * i + c : c * n + i; }
+  static __device__ __forceinline__ void get(const T* base, int n, int i, T v[3]) {
+    if constexpr (slots) {
+      lds_get3(base + 4 * i, v);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) v[c] = base[c * n + i];
+    }
+  }
+  static __device__ __forceinline__ void put(T* base, int n, int i, const T v[3]) {
+    if constexpr (slots) {
+      lds_put3(base + 4 * i, v);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) base[c * n + i] = v[c];
+    }
+  }
+};
+
 // Extra stiffness of a coarse node (kinds 2, 3): au += ex[n], ex[n] = 0.
 template <typename T, class M>
 __device__ __forceinline__ void add_extra(const M& mat, long long n, T au[3]) {
@@ -314,10 +340,12 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
                                                  TileMap tm, const int* __restrict__ order) {
   constexpr int NST = TN * TN * 3;                 // scalars per staged node layer
   constexpr int NSTP = (NST + BLOCK - 1) / BLOCK;  // per thread
-  __shared__ __align__(16) T s_u[2][TN * TN * 4];
+  using VL = V3Lds<T>;
+  constexpr int NL = TN * TN;                      // nodes per staged layer
+  __shared__ __align__(16) T s_u[2][VL::size(NL)];
   // corners 1, 2, 3 of every thread's cell, one array per corner, 4 values (xyz, pad) per thread:
   // consecutive threads write and read consecutive 16-byte slots (no LDS bank conflicts)
-  __shared__ __align__(16) T s_g[3][BLOCK * 4];
+  __shared__ __align__(16) T s_g[3][VL::size(BLOCK)];
   __shared__ T s_tL[M::kind == 0 ? 256 : 1], s_tM[M::kind == 0 ? 256 : 1];
   __shared__ T s_ph[M::kind == 3 ? 2 * TILE * 6 : 1];
   const int tid = threadIdx.x;
@@ -386,9 +414,9 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     }
   }
   // staging plan (fixed for all layers): scalar e = tid + m*BLOCK of a layer is node row r,
-  // entry q of the row's 3*TN contiguous scalars.  Branch-free: a slot outside the grid loads an
-  // in-bounds address and keeps 0; a slot past the layer (e >= NST, only in the last round)
-  // writes the unused 4th value of a node's LDS entry.
+  // entry q of the row's 3*TN contiguous scalars.  A slot outside the grid loads an in-bounds
+  // address and keeps 0; a slot past the layer (e >= NST, only in the last round) writes the
+  // unused 4th value of a node's FP32 LDS slot, or nothing in FP64.
   static_assert((NSTP - 1) * BLOCK < NST && NST <= NSTP * BLOCK, "staging rounds");
   int goff[NSTP], loff[NSTP];
   bool sv[NSTP];
@@ -399,7 +427,8 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     const int gj = J0 - 1 + r, gi = I0 - 1 + q / 3;
     sv[m] = e < NST && gj >= 0 && gj <= ny && gi >= 0 && gi <= nx;
     goff[m] = sv[m] ? (int)((gj * nxn + gi) * 3 + q % 3) : 0;
-    loff[m] = e < NST ? (r * TN + q / 3) * 4 + q % 3 : (tid % (TN * TN)) * 4 + 3;
+    // past the layer: FP32 writes the unused 4th value of a slot, FP64 (planes) writes nothing
+    loff[m] = e < NST ? VL::idx(NL, r * TN + q / 3, q % 3) : (VL::slots ? (tid % NL) * 4 + 3 : -1);
   }
   T pre[NSTP];
   auto gload = [&](int kk) {
@@ -415,14 +444,16 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   auto lstore = [&](int kk) {
     T* dst = s_u[kk & 1];
 #pragma unroll
-    for (int m = 0; m < NSTP; ++m) dst[loff[m]] = pre[m];
+    for (int m = 0; m < NSTP; ++m) {
+      if (VL::slots || loff[m] >= 0) dst[loff[m]] = pre[m];
+    }
   };
   // xy transform of the cell's face on node layer kk: f[4*comp + (mx + 2 my)]
   auto face = [&](int kk, T* f) {
     const T* src = s_u[kk & 1];
     T v[4][3];
 #pragma unroll
-    for (int ab = 0; ab < 4; ++ab) lds_get3(src + ((ty + (ab >> 1)) * TN + tx + (ab & 1)) * 4, v[ab]);
+    for (int ab = 0; ab < 4; ++ab) VL::get(src, NL, (ty + (ab >> 1)) * TN + tx + (ab & 1), v[ab]);
     face_fwd(v, f);
   };
   // material of cell layer kk, loaded one step ahead (kinds 0, 1 and 3)
@@ -502,7 +533,7 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     face(k + 1, uhi);
     T sn[3] = {T(0), T(0), T(0)};
     if (EM == EPI_CHEB || want_dot) {
-      lds_get3(s_u[k & 1] + (ty * TN + tx) * 4, sn);
+      VL::get(s_u[k & 1], NL, ty * TN + tx, sn);
     }
     T Flo[12], Fhi[12];
 #pragma unroll
@@ -573,15 +604,15 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     // back to the 4 bottom corners (corner a + 2b)
     T cr[4][3];
     face_bwd(Flo, cr);
-    lds_put3(s_g[0] + tid * 4, cr[1]);
-    lds_put3(s_g[1] + tid * 4, cr[2]);
-    lds_put3(s_g[2] + tid * 4, cr[3]);
+    VL::put(s_g[0], BLOCK, tid, cr[1]);
+    VL::put(s_g[1], BLOCK, tid, cr[2]);
+    VL::put(s_g[2], BLOCK, tid, cr[3]);
     __syncthreads();
     if (tx >= 1 && ty >= 1 && k >= K0 && ci <= nx && cj <= ny) {
       T p1[3], p2[3], p3[3];
-      lds_get3(s_g[0] + (tid - 1) * 4, p1);          // cell (tx-1, ty): its corner 1
-      lds_get3(s_g[1] + (tid - TILE) * 4, p2);       // cell (tx, ty-1): its corner 2
-      lds_get3(s_g[2] + (tid - TILE - 1) * 4, p3);   // cell (tx-1, ty-1): its corner 3
+      VL::get(s_g[0], BLOCK, tid - 1, p1);          // cell (tx-1, ty): its corner 1
+      VL::get(s_g[1], BLOCK, tid - TILE, p2);       // cell (tx, ty-1): its corner 2
+      VL::get(s_g[2], BLOCK, tid - TILE - 1, p3);   // cell (tx-1, ty-1): its corner 3
       T au[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) au[c] = cr[0][c] + p1[c] + p2[c] + p3[c];

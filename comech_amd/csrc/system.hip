@@ -1643,6 +1643,7 @@ struct System {
     int it = 0, restarts = 0, stalls = 0, flag = rn <= thr ? 0 : 1;
     bool flexible = false;
     double last_true = 1e300;
+    double target = thr;   // threshold of the recurrence; lowered by the drift a confirmation saw
     while (flag && it < maxiter) {
       apply_dot(0, p64.p, Ap64.p, S_PAP);
       // FP32 V-cycle without rigid-mode correction: the residual's FP32 copy comes out of the
@@ -1654,14 +1655,17 @@ struct System {
       ++it;
       if (it < check_every || it % check_every == 0) {
         rn = std::sqrt(read_slot(S_RR));
-        if (rn <= thr) {
+        if (rn <= target) {
           // the recurrence says converged: confirm with the FP64 true residual b - A x; on a
           // stiff system (springs 1e6 beside soft tissue) the two drift apart, and then the
-          // recurrence restarts from the true residual (residual replacement)
+          // recurrence restarts from the true residual (residual replacement) and aims lower by
+          // the drift it saw, so that the next confirmation has that margin
+          const double rec = rn;
           apply<double>(0, x, r64.p, MODE_RESID, b);
           dot(n, r64.p, r64.p, S_RR);
           rn = std::sqrt(read_slot(S_RR));
           if (rn <= thr) { flag = 0; break; }
+          target = std::max(0.25 * thr, thr - (rn - rec));
           // three restarts in a row that do not halve the true residual: the attainable accuracy
           // of this system (the residual of its solution rounded to FP64) is above tol; stop (not
           // converged) instead of iterating to maxiter.  A system whose floor lies just below tol

@@ -395,6 +395,14 @@ __global__ void k_dense_fix_empty(int n, double* __restrict__ A) {
 //   Dk = A_KK^-1;  A_KJ <- Dk A_KJ;  A_IJ -= A_IK A_KJ;  A_IK <- -A_IK Dk;  A_KK <- Dk
 // which is GJB consecutive steps of scalar Gauss-Jordan.  Pivot blocks start at multiples of
 // GJB, so the K rows/columns are exactly one 64-wide tile of every kernel below.
+//
+// Symmetry: starting from a symmetric A, after the pivot blocks P have been processed
+// A_JI = s A_IJ^T with s = -1 when exactly one of I, J is in P (else +1), so only the tiles I >= J
+// are kept.  With pivots taken in order (P = the blocks before K) and C_I = A_IK (old), the old row
+// is A_KJ = t_J C_J^T with t_J = +1 for J > K, -1 for J < K, and with W_I = C_I Dk the step is
+//   A_IJ -= t_J W_I C_J^T  (I >= J, both != K);   A_IK <- -W_I (I > K);   A_KI <- -W_I^T (I < K)
+// -- half the trailing update of the unsymmetric form.  At the end every block is processed, A is
+// the symmetric inverse, and the upper tiles are filled from the lower ones.
 constexpr int GJB = 64;
 
 // (1) Dk = inverse of the nb x nb diagonal block at k0, padded with the identity to 64 x 64
@@ -468,59 +476,75 @@ __device__ __forceinline__ void gjb_tile_mul(const double (*X)[GJB + 1], const d
   }
 }
 
-// (2) A_KJ <- Dk A_KJ for every 64-column tile J (in place; the K tile gets Dk itself).
-__global__ __launch_bounds__(256) void k_gjb_rowpanel(int n, int k0, int nb, double* __restrict__ A,
-                                                      const double* __restrict__ Dk) {
-  __shared__ double sd[GJB][GJB + 1], sr[GJB][GJB + 1];
-  const int j0 = blockIdx.x * 64;
-  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-  if (j0 == k0) {
-    for (int t = threadIdx.x; t < nb * GJB; t += blockDim.x) {
+// (2) Panels of pivot block K, one 64-row tile I per block: C_I = A_IK (from the kept tile (I, K),
+// or -(A_KI)^T from the kept tile (K, I) when I < K) and W_I = C_I Dk, stored k-major
+// (Ct[c][i] = C_I[i][c], Wt likewise, 64 x n) for the update; then the new K tiles of the kept
+// triangle: A_IK = -W_I (I > K), A_KI = -W_I^T (I < K), A_KK = Dk.  Each block reads and writes
+// only its own tile of the K row/column.
+__global__ __launch_bounds__(256) void k_gjs_panel(int n, int k0, int nb, double* __restrict__ A,
+                                                   const double* __restrict__ Dk, double* __restrict__ Ct,
+                                                   double* __restrict__ Wt) {
+  __shared__ double sc[GJB][GJB + 1], sd[GJB][GJB + 1];
+  const int i0 = blockIdx.x * 64;
+  if (i0 == k0) {
+    for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
       const int q = t >> 6, c = t & 63;
-      if (c < nb) A[(long long)(k0 + q) * n + k0 + c] = Dk[q * GJB + c];
+      if (q < nb && c < nb) A[(long long)(k0 + q) * n + k0 + c] = Dk[q * GJB + c];
+      if (k0 + c < n) { Ct[(long long)q * n + k0 + c] = 0.0; Wt[(long long)q * n + k0 + c] = 0.0; }
     }
     return;
   }
+  const bool below = i0 > k0;
   for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
-    const int q = t >> 6, c = t & 63;
-    sd[q][c] = (q < nb && c < nb) ? Dk[q * GJB + c] : 0.0;
-    sr[q][c] = (q < nb && j0 + c < n) ? A[(long long)(k0 + q) * n + j0 + c] : 0.0;
+    const int r = t >> 6, c = t & 63;
+    // below: row-major tile (I, K); above: tile (K, I) read transposed (c rows, r columns)
+    double v = 0.0;
+    if (i0 + r < n && c < nb)
+      v = below ? A[(long long)(i0 + r) * n + k0 + c] : -A[(long long)(k0 + c) * n + i0 + r];
+    sc[r][c] = v;
+    sd[r][c] = (r < nb && c < nb) ? Dk[r * GJB + c] : 0.0;
   }
   __syncthreads();
+  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
   double acc[4][4];
-  gjb_tile_mul(sd, sr, tr, tc, acc);
+  gjb_tile_mul(sc, sd, tr, tc, acc);
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
-    const int q = tr + 16 * a;
-    if (q >= nb) continue;
+    const int r = tr + 16 * a, i = i0 + r;
+    if (i >= n) continue;
 #pragma unroll
     for (int b = 0; b < 4; ++b) {
-      const int j = j0 + tc + 16 * b;
-      if (j < n) A[(long long)(k0 + q) * n + j] = acc[a][b];
+      const int c = tc + 16 * b;
+      Ct[(long long)c * n + i] = sc[r][c];
+      Wt[(long long)c * n + i] = acc[a][b];
+      if (c < nb) {
+        if (below) A[(long long)i * n + k0 + c] = -acc[a][b];
+        else A[(long long)(k0 + c) * n + i] = -acc[a][b];
+      }
     }
   }
 }
 
-// (3) A_IJ -= A_IK A_KJ for every 64 x 64 tile (I, J) outside row and column K, on the FP64
-// matrix cores.  The K-column tile of A_I. is stored transposed in LDS (sa[k][i]) and the row
-// panel as is (sr[k][j]); wave w computes the 32 x 32 quarter (w / 2, w % 2) as 2 x 2 products of
+// (3) A_IJ -= t_J W_I C_J^T for every kept 64 x 64 tile (I >= J) outside row and column K, on the
+// FP64 matrix cores.  sa[k][i] = W_I[i][k], sb[k][j] = C_J[j][k] straight from the k-major panels;
+// wave w computes the 32 x 32 quarter (w / 2, w % 2) as 2 x 2 products of
 // v_mfma_f64_16x16x4f64 over the 64-deep K.  Operand lanes: A[l % 16][l / 16], B[l / 16][l % 16];
 // result lane l, register r: D[l / 16 + 4 r][l % 16] (the layout of the builtin on gfx950).
 typedef double gjb_d4 __attribute__((ext_vector_type(4)));
 constexpr int GJP = GJB + 4;   // LDS row pitch (doubles)
-__global__ __launch_bounds__(256) void k_gjb_update(int n, int k0, int nb, double* __restrict__ A) {
-  __shared__ double sa[GJB][GJP];   // sa[k][i] = A[i0 + i][k0 + k]
-  __shared__ double sr[GJB][GJP];   // sr[k][j] = A[k0 + k][j0 + j]
+__global__ __launch_bounds__(256) void k_gjs_update(int n, int k0, const double* __restrict__ Ct,
+                                                    const double* __restrict__ Wt, double* __restrict__ A) {
+  __shared__ double sa[GJB][GJP];
+  __shared__ double sb[GJB][GJP];
   const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
-  if (i0 == k0 || j0 == k0) return;
+  if (j0 > i0 || i0 == k0 || j0 == k0) return;
   for (int t = threadIdx.x; t < 64 * GJB; t += blockDim.x) {
-    const int r = t >> 6, q = t & 63;
-    const int i = i0 + r;
-    sa[q][r] = (i < n && q < nb) ? A[(long long)i * n + k0 + q] : 0.0;
-    const int j = j0 + q;
-    sr[r][q] = (j < n && r < nb) ? A[(long long)(k0 + r) * n + j] : 0.0;
+    const int q = t >> 6, r = t & 63;
+    sa[q][r] = (i0 + r < n) ? Wt[(long long)q * n + i0 + r] : 0.0;
+    sb[q][r] = (j0 + r < n) ? Ct[(long long)q * n + j0 + r] : 0.0;
   }
   __syncthreads();
+  const double tj = j0 > k0 ? 1.0 : -1.0;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int wr = (w >> 1) * 32, wc = (w & 1) * 32;
   const int lr = l & 15, lk = l >> 4;
@@ -536,7 +560,7 @@ __global__ __launch_bounds__(256) void k_gjb_update(int n, int k0, int nb, doubl
 #pragma unroll
     for (int a = 0; a < 2; ++a) av[a] = sa[kk][wr + 16 * a + lr];
 #pragma unroll
-    for (int b = 0; b < 2; ++b) bv[b] = sr[kk][wc + 16 * b + lr];
+    for (int b = 0; b < 2; ++b) bv[b] = sb[kk][wc + 16 * b + lr];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
@@ -550,35 +574,25 @@ __global__ __launch_bounds__(256) void k_gjb_update(int n, int k0, int nb, doubl
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int i = i0 + wr + 16 * a + lk + 4 * r;
-        if (i < n && j < n) A[(long long)i * n + j] -= acc[a][b][r];
+        if (i < n && j < n) A[(long long)i * n + j] -= tj * acc[a][b][r];
       }
     }
 }
 
-// (4) A_IK <- -A_IK Dk for every 64-row tile I outside K.
-__global__ __launch_bounds__(256) void k_gjb_colpanel(int n, int k0, int nb, double* __restrict__ A,
-                                                      const double* __restrict__ Dk) {
-  __shared__ double sa[GJB][GJB + 1], sd[GJB][GJB + 1];
-  const int i0 = blockIdx.x * 64;
-  if (i0 == k0) return;
+// (4) The finished inverse is symmetric: copy every kept tile below the diagonal to its mirror.
+// 64 x 64 tiles (I > J) through LDS, so that reads and writes are both row-contiguous.
+__global__ __launch_bounds__(256) void k_gjs_mirror(int n, double* __restrict__ A) {
+  __shared__ double st[GJB][GJB + 1];
+  const int i0 = blockIdx.y * 64, j0 = blockIdx.x * 64;
+  if (j0 >= i0) return;
   for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
     const int r = t >> 6, c = t & 63;
-    sa[r][c] = (i0 + r < n && c < nb) ? A[(long long)(i0 + r) * n + k0 + c] : 0.0;
-    sd[r][c] = (r < nb && c < nb) ? Dk[r * GJB + c] : 0.0;
+    st[r][c] = (i0 + r < n && j0 + c < n) ? A[(long long)(i0 + r) * n + j0 + c] : 0.0;
   }
   __syncthreads();
-  const int tr = threadIdx.x >> 4, tc = threadIdx.x & 15;
-  double acc[4][4];
-  gjb_tile_mul(sa, sd, tr, tc, acc);
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const int i = i0 + tr + 16 * a;
-    if (i >= n) continue;
-#pragma unroll
-    for (int b = 0; b < 4; ++b) {
-      const int c = tc + 16 * b;
-      if (c < nb) A[(long long)i * n + k0 + c] = -acc[a][b];
-    }
+  for (int t = threadIdx.x; t < GJB * GJB; t += blockDim.x) {
+    const int r = t >> 6, c = t & 63;     // destination (j0 + r, i0 + c) = source (i0 + c, j0 + r)
+    if (j0 + r < n && i0 + c < n) A[(long long)(j0 + r) * n + i0 + c] = st[c][r];
   }
 }
 

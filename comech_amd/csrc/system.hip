@@ -965,15 +965,15 @@ struct System {
   // In-place blocked Gauss-Jordan inverse of the SPD matrix Ainv (m x m).
   void invert_dense(int m) {
     const long long n = m;
-    DevBuf<double> Dk(GJB * GJB);
+    DevBuf<double> Dk(GJB * GJB), Ct((size_t)GJB * n), Wt((size_t)GJB * n);
     const unsigned nt = (unsigned)div_up(n, 64);
     for (int k0 = 0; k0 < n; k0 += GJB) {
       const int nb = (int)std::min<long long>(GJB, n - k0);
       hipLaunchKernelGGL(k_gjb_diag, dim3(1), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
-      hipLaunchKernelGGL(k_gjb_rowpanel, dim3(nt), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
-      hipLaunchKernelGGL(k_gjb_update, dim3(nt, nt), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p);
-      hipLaunchKernelGGL(k_gjb_colpanel, dim3(nt), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p);
+      hipLaunchKernelGGL(k_gjs_panel, dim3(nt), dim3(256), 0, stream, (int)n, k0, nb, Ainv.p, Dk.p, Ct.p, Wt.p);
+      hipLaunchKernelGGL(k_gjs_update, dim3(nt, nt), dim3(256), 0, stream, (int)n, k0, Ct.p, Wt.p, Ainv.p);
     }
+    hipLaunchKernelGGL(k_gjs_mirror, dim3(nt, nt), dim3(256), 0, stream, (int)n, Ainv.p);
     CM_LAUNCH_CHECK();
     CM_HIP(hipStreamSynchronize(stream));
   }

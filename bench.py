@@ -77,7 +77,7 @@ def cpu_baseline(sample_dims=(80, 48, 48)):
     dt = time.time() - t0
     scale = (160 * 96 * 96) / float(g.ncell)
     return dict(value=1.0 / (dt * scale), unit="solves/s", cores=1, kind="port",
-                sample=(f"config-2 anatomy at {sample_dims[0]}x{sample_dims[1]}x{sample_dims[2]} cells "
+                sample=(f"hindfoot anatomy at {sample_dims[0]}x{sample_dims[1]}x{sample_dims[2]} cells "
                         f"({g.ncell} cells, {info['ndof']} DOFs): one FP64 oracle solve (assembly, Galerkin "
                         f"hierarchy, Chebyshev GMG-PCG, SuperLU coarse) to rtol 1e-8 in {dt:.2f} s, "
                         f"{info['iters']} iterations, extrapolated x{scale:.0f} in cell count"))

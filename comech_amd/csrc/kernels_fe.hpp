@@ -865,14 +865,6 @@ __global__ void k_fill(T* __restrict__ p, long long n, T v) {
   if (i < n) p[i] = v;
 }
 
-// Residual fix for the copies region when the structured kernel computed y = b - A_s u:
-// copies have no structured part, so y = b there before the irregular/extra passes subtract.
-template <typename T>
-__global__ void k_copy_range(T* __restrict__ dst, const T* __restrict__ src, long long n) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = src[i];
-}
-
 // -------------------------------------------------------------------------------------------
 // Grid transfers between level l (fine, nodes nf per axis) and l+1 (coarse).
 // Prolongation per axis: fine node i -> coarse index ia[i] and weight w[i] of ia[i]+1

@@ -44,13 +44,6 @@ __global__ void k_cheb0(long long nn, T* __restrict__ x, const T* __restrict__ b
   x[3 * n + 2] = c * (m[2] * r0 + m[4] * r1 + m[5] * r2);
 }
 
-// r -= t
-template <typename T>
-__global__ void k_sub(long long n, T* __restrict__ r, const T* __restrict__ t) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) r[i] -= t[i];
-}
-
 // y = M x (block-Jacobi apply), for the power iteration
 template <typename T>
 __global__ void k_block_apply(long long nn, const T* __restrict__ Dinv, const T* __restrict__ x, T* __restrict__ y) {

@@ -262,6 +262,21 @@ def test_edt_feature_transform_is_nearest_with_kji_ties(density):
         assert np.allclose(dist.ravel() ** 2, d2)
 
 
+def test_edt_tie_rule_against_stored_scipy_indices():
+    """tests/golden/edt_ties.npz holds scipy's feature transform of three tie-rich rasters: the
+    enumeration rule (smallest squared distance, then smallest (k, j, i)) reproduces it, and the
+    installed scipy still does."""
+    import os
+    from scipy import ndimage
+    z = np.load(os.path.join(os.path.dirname(__file__), "golden", "edt_ties.npz"))
+    for r in range(3):
+        act, ind = z[f"active{r}"], z[f"indices{r}"].astype(np.int64)
+        best, _ = O.nearest_active_bruteforce(act.ravel(), act.shape, np.arange(act.size))
+        assert np.array_equal(best, ind.reshape(3, -1).T)
+        _, now = ndimage.distance_transform_edt(~act, return_indices=True)
+        assert np.array_equal(now, ind)
+
+
 def test_sample_nodes_nearest_cell_and_linear_field():
     """Nodes in active cells reproduce a linear field; nodes in empty cells take the field of the
     brute-force nearest active cell at the clamped local coordinates."""

@@ -7,7 +7,7 @@
 
 namespace cm {
 
-constexpr int RED_BLOCKS = 512;
+constexpr int RED_BLOCKS = 2048;
 constexpr int RED_THREADS = 256;
 
 // partial[block] = sum a[i] * b[i]

@@ -596,20 +596,21 @@ __global__ __launch_bounds__(256) void k_gjs_mirror(int n, double* __restrict__ 
   }
 }
 
-// Coarse solve x = Ainv b (dense FP64), one wavefront per row.
-template <typename T>
+// Coarse solve x = Ainv b (dense FP64), one 256-thread block per row: four waves share a row of a
+// few thousand doubles, four times the loads in flight of one wavefront per row.
 // The inverse is over the level's used DOFs only: map[i] = full DOF of compact DOF i (nullptr: the
 // identity).  x at DOFs outside the map is not written (kept zero).
-__global__ void k_dense_gemv(int n, const double* __restrict__ Ainv, const int* __restrict__ map,
-                             const T* __restrict__ b, T* __restrict__ x) {
-  const int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wid >= n) return;
-  const double* row = Ainv + (long long)wid * n;
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_gemv(int n, const double* __restrict__ Ainv,
+                                                    const int* __restrict__ map, const T* __restrict__ b,
+                                                    T* __restrict__ x) {
+  const int row = blockIdx.x;
+  if (row >= n) return;   // block-uniform
+  const double* r = Ainv + (long long)row * n;
   double s = 0.0;
-  for (int c = lane; c < n; c += 64) s += row[c] * (double)b[map ? map[c] : c];
-  s = wave_sum(s);
-  if (lane == 0) x[map ? map[wid] : wid] = T(s);
+  for (int c = threadIdx.x; c < n; c += 256) s += r[c] * (double)b[map ? map[c] : c];
+  s = block_sum<256>(s);
+  if (threadIdx.x == 0) x[map ? map[row] : row] = T(s);
 }
 
 // Ac[i][j] = A[map[i]][map[j]] (m x m from n x n)

@@ -1487,7 +1487,7 @@ struct System {
         const long long nv = 3 * lv(0).nn;
         hipLaunchKernelGGL(k_fill<T>, grid1(nv), dim3(256), 0, stream, x, nv, T(0));
       }
-      hipLaunchKernelGGL(k_dense_gemv<T>, grid1(64LL * nc_dense), dim3(256), 0, stream, nc_dense, Ainv.p, dmap.p, b, x);
+      hipLaunchKernelGGL(k_dense_gemv<T>, dim3((unsigned)nc_dense), dim3(256), 0, stream, nc_dense, Ainv.p, dmap.p, b, x);
       CM_LAUNCH_CHECK();
       return;
     }

@@ -331,13 +331,125 @@ __device__ __forceinline__ void add_extra(const M& mat, long long n, T au[3]) {
   }
 }
 
+// Sparse pass: everything the structured march leaves out, accumulated into a compact vector tc
+// over the level's "touched" nodes (3 values per listed node):
+//   * irregular cells (level 0: cells touching split copies), 8 explicit node ids;
+//   * extra stiffness rows (springs, penalties, ligaments, their Galerkin products), CSR;
+//   * rank-one terms k w w^T with a support wider than one node neighbourhood (coarse levels).
+// One launch, three block ranges.  Atomics: rows and cells overlap on the touched nodes.
+// Extra stiffness of a coarse level, accumulated into ex (all zero on entry; the march that
+// follows adds it to its nodes and clears it).  Two block ranges:
+//   * rank-one terms k_t w_t w_t^T, one wavefront per term: d = k_t (w_t . s) reduced over the
+//     wavefront, then ex[dof] += d w for every entry of the term;
+//   * block entries (row node brow_e, column node bcol_e, 3x3 block B_e row-major), one thread each:
+//     ex[brow_e] += B_e s[bcol_e].
+template <typename T>
+struct CoarseExtra {
+  int nterm, nblk;
+  int nb_term;                       // blocks of the first range
+  int rw;                            // lanes per rank-one term (power of two <= 64)
+  const int* __restrict__ r1ptr;
+  const int* __restrict__ r1dof;
+  const T* __restrict__ r1w;
+  const T* __restrict__ r1k;
+  const int* __restrict__ brow;
+  const int* __restrict__ bcol;
+  const T* __restrict__ bval;
+};
+// The pass runs as extra blocks of the level-0 march launch (it does not depend on the march, the
+// fix-up that follows reads both), so it costs no launch of its own.
+template <typename T>
+struct SparseArgs {
+  int nb;                            // blocks of the pass (0: none)
+  int nirr, nrow, nterm;
+  int nb_irr, nb_row;                // blocks of the first two ranges
+  int xw;                            // lanes per CSR row (power of two <= 64)
+  int rw;                            // lanes per rank-one term (power of two <= 64)
+  const int* __restrict__ irr_nodes; // [nirr][8] node ids
+  const int* __restrict__ irr_ci;    // [nirr][8] compact node index
+  const int* __restrict__ irr_ijk;   // [nirr][3]
+  const T* __restrict__ irrL;
+  const T* __restrict__ irrM;
+  const int* __restrict__ xrc;       // compact DOF of each CSR row
+  const int* __restrict__ xptr;
+  const int* __restrict__ xcol;      // full DOF
+  const T* __restrict__ xval;
+  const int* __restrict__ r1ptr;
+  const int* __restrict__ r1dof;     // full DOF
+  const int* __restrict__ r1c;       // compact DOF
+  const T* __restrict__ r1w;
+  const T* __restrict__ r1k;
+  T* __restrict__ tc;                // accumulator over the touched nodes
+};
+
+template <typename T, bool SRI>
+__device__ __forceinline__ void sparse_body(const SparseArgs<T>& a, const GridArgs<T>& g, const T* __restrict__ s,
+                                            int b) {
+  T* __restrict__ tc = a.tc;
+  if (b < a.nb_irr) {
+    const int e = b * 256 + threadIdx.x;
+    if (e >= a.nirr) return;
+    const int i = a.irr_ijk[3 * e], j = a.irr_ijk[3 * e + 1], k = a.irr_ijk[3 * e + 2];
+    const T hx = g.hx[i], hy = g.hy[j], hz = g.hz[k];
+    const T V = hx * hy * hz;
+    CellW<T> w{T(0.25) / hx, T(0.25) / hy, T(0.25) / hz, a.irrL[e] * V, a.irrM[e] * V};
+    T ue[24], f[24];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const long long nd = a.irr_nodes[8 * e + l];
+      ue[3 * l] = s[3 * nd];
+      ue[3 * l + 1] = s[3 * nd + 1];
+      ue[3 * l + 2] = s[3 * nd + 2];
+    }
+    elem_apply_const<T, SRI>(ue, f, w);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+      const int q = 3 * a.irr_ci[8 * e + l];
+      atomicAdd(&tc[q], f[3 * l]);
+      atomicAdd(&tc[q + 1], f[3 * l + 1]);
+      atomicAdd(&tc[q + 2], f[3 * l + 2]);
+    }
+    return;
+  }
+  b -= a.nb_irr;
+  if (b < a.nb_row) {
+    const int W = a.xw;
+    const long long gid = (long long)b * 256 + threadIdx.x;
+    const int r = (int)(gid / W), lane = (int)(gid % W);
+    if (r >= a.nrow) return;   // the W lanes of a row leave together
+    T acc = T(0);
+    for (int q = a.xptr[r] + lane; q < a.xptr[r + 1]; q += W) acc += a.xval[q] * s[a.xcol[q]];
+    for (int off = W / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, W);
+    if (lane == 0) atomicAdd(&tc[a.xrc[r]], acc);
+    return;
+  }
+  b -= a.nb_row;
+  const int rw = a.rw;
+  const long long gid = (long long)b * 256 + threadIdx.x;
+  const int t = (int)(gid / rw), lane = (int)(gid % rw);
+  if (t >= a.nterm) return;   // the rw lanes of a term leave together
+  const int q0 = a.r1ptr[t], q1 = a.r1ptr[t + 1];
+  T acc = T(0);
+  for (int q = q0 + lane; q < q1; q += rw) acc += a.r1w[q] * s[a.r1dof[q]];
+  for (int off = rw / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, rw);
+  acc *= a.r1k[t];
+  for (int q = q0 + lane; q < q1; q += rw) atomicAdd(&tc[a.r1c[q]], acc * a.r1w[q]);
+}
+
 // order (nullptr: TileMap order): the tile of every block, ~tile for a tile none of whose cells has
 // stiffness.  Such a block's structured part is zero at all its output nodes; it runs only the
 // epilogue there.  The order puts the tiles with stiffness first, dealt to the XCDs in contiguous
 // runs, so the VALU-heavy blocks spread evenly over all CUs.
 template <typename T, class M, int EM>
 __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* __restrict__ s, Epi<T> ep, int kz,
-                                                 TileMap tm, const int* __restrict__ order) {
+                                                 TileMap tm, const int* __restrict__ order, SparseArgs<T> sp) {
+  if constexpr (M::kind <= 1) {
+    const int nbm = tm.ntx * tm.nty * tm.ntz;
+    if ((int)blockIdx.x >= nbm) {   // block-uniform: the sparse pass rides on this launch
+      sparse_body<T, M::sri>(sp, g, s, (int)blockIdx.x - nbm);
+      return;
+    }
+  }
   constexpr int NST = TN * TN * 3;                 // scalars per staged node layer
   constexpr int NSTP = (NST + BLOCK - 1) / BLOCK;  // per thread
   using VL = V3Lds<T>;
@@ -664,109 +776,6 @@ __global__ __launch_bounds__(BLOCK) void k_block_flags(GridArgs<T> g, M mat, int
 }
 
 // -------------------------------------------------------------------------------------------
-// Sparse pass: everything the structured march leaves out, accumulated into a compact vector tc
-// over the level's "touched" nodes (3 values per listed node):
-//   * irregular cells (level 0: cells touching split copies), 8 explicit node ids;
-//   * extra stiffness rows (springs, penalties, ligaments, their Galerkin products), CSR;
-//   * rank-one terms k w w^T with a support wider than one node neighbourhood (coarse levels).
-// One launch, three block ranges.  Atomics: rows and cells overlap on the touched nodes.
-template <typename T>
-struct SparseArgs {
-  int nirr, nrow, nterm;
-  int nb_irr, nb_row;                // blocks of the first two ranges
-  int rw;                            // lanes per rank-one term (power of two <= 64)
-  const int* __restrict__ irr_nodes; // [nirr][8] node ids
-  const int* __restrict__ irr_ci;    // [nirr][8] compact node index
-  const int* __restrict__ irr_ijk;   // [nirr][3]
-  const T* __restrict__ irrL;
-  const T* __restrict__ irrM;
-  const int* __restrict__ xrc;       // compact DOF of each CSR row
-  const int* __restrict__ xptr;
-  const int* __restrict__ xcol;      // full DOF
-  const T* __restrict__ xval;
-  const int* __restrict__ r1ptr;
-  const int* __restrict__ r1dof;     // full DOF
-  const int* __restrict__ r1c;       // compact DOF
-  const T* __restrict__ r1w;
-  const T* __restrict__ r1k;
-};
-
-template <typename T, bool SRI, int W>
-__global__ __launch_bounds__(256) void k_sparse(SparseArgs<T> a, GridArgs<T> g, const T* __restrict__ s,
-                                                T* __restrict__ tc) {
-  int b = blockIdx.x;
-  if (b < a.nb_irr) {
-    const int e = b * 256 + threadIdx.x;
-    if (e >= a.nirr) return;
-    const int i = a.irr_ijk[3 * e], j = a.irr_ijk[3 * e + 1], k = a.irr_ijk[3 * e + 2];
-    const T hx = g.hx[i], hy = g.hy[j], hz = g.hz[k];
-    const T V = hx * hy * hz;
-    CellW<T> w{T(0.25) / hx, T(0.25) / hy, T(0.25) / hz, a.irrL[e] * V, a.irrM[e] * V};
-    T ue[24], f[24];
-#pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      const long long nd = a.irr_nodes[8 * e + l];
-      ue[3 * l] = s[3 * nd];
-      ue[3 * l + 1] = s[3 * nd + 1];
-      ue[3 * l + 2] = s[3 * nd + 2];
-    }
-    elem_apply_const<T, SRI>(ue, f, w);
-#pragma unroll
-    for (int l = 0; l < 8; ++l) {
-      const int q = 3 * a.irr_ci[8 * e + l];
-      atomicAdd(&tc[q], f[3 * l]);
-      atomicAdd(&tc[q + 1], f[3 * l + 1]);
-      atomicAdd(&tc[q + 2], f[3 * l + 2]);
-    }
-    return;
-  }
-  b -= a.nb_irr;
-  if (b < a.nb_row) {
-    const long long gid = (long long)b * 256 + threadIdx.x;
-    const int r = (int)(gid / W), lane = (int)(gid % W);
-    if (r >= a.nrow) return;   // the W lanes of a row leave together
-    T acc = T(0);
-    for (int q = a.xptr[r] + lane; q < a.xptr[r + 1]; q += W) acc += a.xval[q] * s[a.xcol[q]];
-    if (W > 1) {
-#pragma unroll
-      for (int off = W / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, W);
-    }
-    if (lane == 0) atomicAdd(&tc[a.xrc[r]], acc);
-    return;
-  }
-  b -= a.nb_row;
-  const int rw = a.rw;
-  const long long gid = (long long)b * 256 + threadIdx.x;
-  const int t = (int)(gid / rw), lane = (int)(gid % rw);
-  if (t >= a.nterm) return;   // the rw lanes of a term leave together
-  const int q0 = a.r1ptr[t], q1 = a.r1ptr[t + 1];
-  T acc = T(0);
-  for (int q = q0 + lane; q < q1; q += rw) acc += a.r1w[q] * s[a.r1dof[q]];
-  for (int off = rw / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, rw);
-  acc *= a.r1k[t];
-  for (int q = q0 + lane; q < q1; q += rw) atomicAdd(&tc[a.r1c[q]], acc * a.r1w[q]);
-}
-
-// Extra stiffness of a coarse level, accumulated into ex (all zero on entry; the march that
-// follows adds it to its nodes and clears it).  Two block ranges:
-//   * rank-one terms k_t w_t w_t^T, one wavefront per term: d = k_t (w_t . s) reduced over the
-//     wavefront, then ex[dof] += d w for every entry of the term;
-//   * block entries (row node brow_e, column node bcol_e, 3x3 block B_e row-major), one thread each:
-//     ex[brow_e] += B_e s[bcol_e].
-template <typename T>
-struct CoarseExtra {
-  int nterm, nblk;
-  int nb_term;                       // blocks of the first range
-  int rw;                            // lanes per rank-one term (power of two <= 64)
-  const int* __restrict__ r1ptr;
-  const int* __restrict__ r1dof;
-  const T* __restrict__ r1w;
-  const T* __restrict__ r1k;
-  const int* __restrict__ brow;
-  const int* __restrict__ bcol;
-  const T* __restrict__ bval;
-};
-
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T* __restrict__ s, T* __restrict__ ex) {
   if ((int)blockIdx.x < a.nb_term) {

@@ -1082,10 +1082,8 @@ struct System {
   void apply_ep(int l, const T* s, const Epi<T>& ep) {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
-    const GridArgs<T> g = grid_args<T>(l);
-    if (v.ntouch) launch_sparse<T>(v, d, g, s);
     if (l > 0 && d.ex.p) launch_extra<T>(l, s);
-    launch_march<T, EM>(l, s, ep);
+    launch_march<T, EM>(l, s, ep, v.ntouch ? sparse_args<T>(v, d) : SparseArgs<T>{});
     if (v.ntouch) {
       hipLaunchKernelGGL((k_fixup<T, EM>), grid1(v.ntouch), dim3(256), 0, stream, v.ntouch, v.tnode.p, v.nng, d.tc.p, s,
                          ep);
@@ -1127,10 +1125,8 @@ struct System {
     CM_CHECK(partial.n >= (size_t)(nbm + nbf), "partial buffer too small for the fused dot");
     Epi<double> ep{nullptr, Ap, nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, partial.p};
     LevelT<double>& d = v.dat<double>();
-    const GridArgs<double> g = grid_args<double>(l);
-    if (v.ntouch) launch_sparse<double>(v, d, g, p);
     if (l > 0 && d.ex.p) launch_extra<double>(l, p);
-    launch_march<double, EPI_APPLY>(l, p, ep);
+    launch_march<double, EPI_APPLY>(l, p, ep, v.ntouch ? sparse_args<double>(v, d) : SparseArgs<double>{});
     if (v.ntouch) {
       Epi<double> ef = ep;
       ef.part = partial.p + nbm;
@@ -1146,31 +1142,24 @@ struct System {
     return (size_t)march_shape<double>(l, pick_kz<double>(v)).nb + (v.ntouch ? div_up(v.ntouch, 256) : 0);
   }
 
+  // the level-0 sparse pass, run as extra blocks of the march launch
   template <typename T>
-  void launch_sparse(Level& v, LevelT<T>& d, const GridArgs<T>& g, const T* s) {
+  SparseArgs<T> sparse_args(Level& v, LevelT<T>& d) {
     SparseArgs<T> a{};
     a.nirr = v.nirr;
     a.nrow = (int)v.xrow_h.size();
     a.nterm = (int)v.r1k_h.size();
+    a.xw = v.xw;
     a.nb_irr = (int)div_up(a.nirr, 256);
     a.nb_row = (int)div_up((long long)a.nrow * v.xw, 256);
     a.rw = term_lanes(v);
     const int nb_r1 = (int)div_up((long long)a.rw * a.nterm, 256);
+    a.nb = a.nb_irr + a.nb_row + nb_r1;
     a.irr_nodes = v.irr_nodes.p; a.irr_ci = v.irr_ci.p; a.irr_ijk = v.irr_ijk.p; a.irrL = d.irrL.p; a.irrM = d.irrM.p;
     a.xrc = v.xrc.p; a.xptr = v.xptr.p; a.xcol = v.xcol.p; a.xval = d.xval.p;
     a.r1ptr = v.r1ptr.p; a.r1dof = v.r1dof.p; a.r1c = v.r1c.p; a.r1w = d.r1w.p; a.r1k = d.r1k.p;
-    const dim3 grid((unsigned)(a.nb_irr + a.nb_row + nb_r1));
-    if (grid.x == 0) return;
-    switch (v.xw) {
-#define CM_XW(W)                                                                                         \
-  case W:                                                                                                \
-    if (sri) hipLaunchKernelGGL((k_sparse<T, true, W>), grid, dim3(256), 0, stream, a, g, s, d.tc.p);  \
-    else hipLaunchKernelGGL((k_sparse<T, false, W>), grid, dim3(256), 0, stream, a, g, s, d.tc.p);     \
-    break;
-      CM_XW(1) CM_XW(2) CM_XW(4) CM_XW(8) CM_XW(16) CM_XW(32) CM_XW(64)
-#undef CM_XW
-      default: throw Error("bad extra width");
-    }
+    a.tc = d.tc.p;
+    return a;
   }
 
   // Calls f with the cell-material policy of level l's structured march.
@@ -1275,7 +1264,7 @@ struct System {
 
   // structured march of level l with epilogue EM
   template <typename T, int EM>
-  void launch_march(int l, const T* s, const Epi<T>& ep) {
+  void launch_march(int l, const T* s, const Epi<T>& ep, const SparseArgs<T>& sp = SparseArgs<T>{}) {
     Level& v = lv(l);
     const GridArgs<T> g = grid_args<T>(l);
     const int slot = std::is_same<T, float>::value ? 1 : 0;
@@ -1284,8 +1273,9 @@ struct System {
     const int* fl = (v.bflag_kz[slot] == ms.kz && use_flags) ? v.border[slot].p : nullptr;
     with_mat<T>(l, [&](auto m) {
       using M = decltype(m);
-      hipLaunchKernelGGL((k_march<T, M, EM>), dim3((unsigned)ms.nb), dim3(BLOCK), 0, stream, g, m, s, ep, ms.kz, tm,
-                         fl);
+      CM_CHECK(sp.nb == 0 || M::kind <= 1, "the sparse pass rides only on level-0 marches");
+      hipLaunchKernelGGL((k_march<T, M, EM>), dim3((unsigned)(ms.nb + sp.nb)), dim3(BLOCK), 0, stream, g, m, s, ep,
+                         ms.kz, tm, fl, sp);
     });
   }
 

@@ -893,7 +893,8 @@ struct AxisR {
 // rc = P^T r over the grid nodes (structured part).
 template <typename T>
 __global__ void k_restrict(int nfx, int nfy, int nfz, int ncx, int ncy, int ncz, AxisR<T> rx, AxisR<T> ry,
-                           AxisR<T> rz, const T* __restrict__ r, T* __restrict__ rc) {
+                           AxisR<T> rz, const T* __restrict__ r, T* __restrict__ rc, const T* __restrict__ dinv_c,
+                           T* __restrict__ xc, T c0) {
   const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long nn = (long long)ncx * ncy * ncz;
   if (id >= nn) return;
@@ -921,6 +922,12 @@ __global__ void k_restrict(int nfx, int nfy, int nfz, int ncx, int ncy, int ncz,
     }
   }
   rc[3 * id] = s0; rc[3 * id + 1] = s1; rc[3 * id + 2] = s2;
+  if (xc) {   // the coarse level's first smoothing step from zero: xc = c0 Dinv rc (as k_cheb0)
+    const T* m = dinv_c + 6 * id;
+    xc[3 * id] = c0 * (m[0] * s0 + m[1] * s1 + m[2] * s2);
+    xc[3 * id + 1] = c0 * (m[1] * s0 + m[3] * s1 + m[4] * s2);
+    xc[3 * id + 2] = c0 * (m[2] * s0 + m[4] * s1 + m[5] * s2);
+  }
 }
 
 // Interpolated coarse correction at fine grid node (i,j,k).
@@ -950,12 +957,26 @@ __device__ __forceinline__ void interp_node(int i, int j, int k, int ncx, int nc
   }
 }
 
-// x += P ec on used fine grid nodes; unused nodes are set to zero (they are not unknowns).
+// x += P ec on used fine grid nodes; unused nodes are set to zero (they are not unknowns).  The
+// split copies (ncopy, interpolated like their originals, rb/splitfe.py:153) are the blocks after
+// the grid nodes' nbg blocks of the same launch: both write disjoint nodes.
 template <typename T>
 __global__ void k_prolong_add(int nfx, int nfy, int nfz, int ncx, int ncy, AxisP<T> px, AxisP<T> py, AxisP<T> pz,
-                              const T* __restrict__ ec, const uint8_t* __restrict__ used, T* __restrict__ x) {
-  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+                              const T* __restrict__ ec, const uint8_t* __restrict__ used, T* __restrict__ x,
+                              int nbg, int ncopy, const int* __restrict__ orig) {
   const long long nn = (long long)nfx * nfy * nfz;
+  if ((int)blockIdx.x >= nbg) {   // block-uniform
+    const int q = (blockIdx.x - nbg) * blockDim.x + threadIdx.x;
+    if (q >= ncopy) return;
+    const int o = orig[q];
+    const int i = o % nfx, j = (o / nfx) % nfy, k = o / (nfx * nfy);
+    T e[3];
+    interp_node(i, j, k, ncx, ncy, px, py, pz, ec, e);
+    const long long n = nn + q;
+    x[3 * n] += e[0]; x[3 * n + 1] += e[1]; x[3 * n + 2] += e[2];
+    return;
+  }
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= nn) return;
   if (!used[id]) {
     x[3 * id] = x[3 * id + 1] = x[3 * id + 2] = T(0);
@@ -965,22 +986,6 @@ __global__ void k_prolong_add(int nfx, int nfy, int nfz, int ncx, int ncy, AxisP
   T e[3];
   interp_node(i, j, k, ncx, ncy, px, py, pz, ec, e);
   x[3 * id] += e[0]; x[3 * id + 1] += e[1]; x[3 * id + 2] += e[2];
-}
-
-// Split copies: a copy is interpolated like its original grid node (rb/splitfe.py:153,
-// used_nodes = orig[nused]).  Prolongation of copies:
-template <typename T>
-__global__ void k_prolong_copies(int ncopy, long long nng, const int* __restrict__ orig, int nfx, int nfy, int ncx,
-                                 int ncy, AxisP<T> px, AxisP<T> py, AxisP<T> pz, const T* __restrict__ ec,
-                                 T* __restrict__ x) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ncopy) return;
-  const int o = orig[q];
-  const int i = o % nfx, j = (o / nfx) % nfy, k = o / (nfx * nfy);
-  T e[3];
-  interp_node(i, j, k, ncx, ncy, px, py, pz, ec, e);
-  const long long n = nng + q;
-  x[3 * n] += e[0]; x[3 * n + 1] += e[1]; x[3 * n + 2] += e[2];
 }
 
 // Restriction of copies: rc += P[orig]^T r[copy] (atomics; few copies).

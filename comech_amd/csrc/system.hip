@@ -1402,8 +1402,15 @@ struct System {
     d.lmin = d.lmax / 30.0;
   }
 
+  // coefficient c of the first smoothing step from zero, x = c Dinv b
   template <typename T>
-  void smooth(int l, const T* b, T* x, bool zero_start, int nsweep) {
+  double first_step_coef(int l) {
+    LevelT<T>& d = lv(l).dat<T>();
+    return opt.smoother == 1 ? opt.omega : 1.0 / (0.5 * (d.lmax + d.lmin));
+  }
+  // x_started: zero_start with that first step already taken (by the restriction)
+  template <typename T>
+  void smooth(int l, const T* b, T* x, bool zero_start, int nsweep, bool x_started = false) {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
     const long long nn = v.nn;
@@ -1411,7 +1418,8 @@ struct System {
       // damped block Jacobi: x += omega Dinv (b - A x); the step lands in d, then x += d
       int s0 = 0;
       if (zero_start) {
-        hipLaunchKernelGGL(k_cheb0<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(opt.omega));
+        if (!x_started)
+          hipLaunchKernelGGL(k_cheb0<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(opt.omega));
         s0 = 1;
       }
       for (int s = s0; s < nsweep; ++s) {
@@ -1438,7 +1446,8 @@ struct System {
     bool pending;          // s is a direction not yet added to x
     int nd = 0;            // next direction buffer
     if (zero_start) {
-      hipLaunchKernelGGL(k_cheb0<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(1.0 / theta));
+      if (!x_started)
+        hipLaunchKernelGGL(k_cheb0<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(1.0 / theta));
       s = x;
       pending = false;
     } else {
@@ -1471,7 +1480,7 @@ struct System {
   }
 
   template <typename T>
-  void vcycle(int l, const T* b, T* x) {
+  void vcycle(int l, const T* b, T* x, bool x_started = false) {
     if (l == nlev() - 1) {
       if (l == 0) {   // one level: x is the caller's vector; the solve writes only the used DOFs
         const long long nv = 3 * lv(0).nn;
@@ -1485,23 +1494,26 @@ struct System {
     LevelT<T>& d = v.dat<T>();
     Level& c = lv(l + 1);
     LevelT<T>& dc = c.dat<T>();
-    smooth<T>(l, b, x, true, opt.npre);
+    smooth<T>(l, b, x, true, opt.npre, x_started);
     apply<T>(l, x, d.r.p, MODE_RESID, b);
     AxisR<T> rx{v.rs[0].p, d.rw[0].p}, ry{v.rs[1].p, d.rw[1].p}, rz{v.rs[2].p, d.rw[2].p};
-    hipLaunchKernelGGL(k_restrict<T>, grid1(c.nn), dim3(256), 0, stream, v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1,
-                       c.ny + 1, c.nz + 1, rx, ry, rz, d.r.p, dc.b.p);
     AxisP<T> px{v.pia[0].p, d.pw[0].p}, py{v.pia[1].p, d.pw[1].p}, pz{v.pia[2].p, d.pw[2].p};
     const int ncopy = (int)(v.nn - v.nng);
+    // without copies to add afterwards, the restriction also takes the coarse level's first
+    // smoothing step from zero (one launch less); the coarsest level is solved directly instead
+    const bool start_c = ncopy == 0 && l + 2 < nlev();
+    hipLaunchKernelGGL(k_restrict<T>, grid1(c.nn), dim3(256), 0, stream, v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1,
+                       c.ny + 1, c.nz + 1, rx, ry, rz, d.r.p, dc.b.p, start_c ? dc.Dinv.p : (const T*)nullptr,
+                       start_c ? dc.x.p : (T*)nullptr, T(start_c ? first_step_coef<T>(l + 1) : 0.0));
     if (ncopy)
       hipLaunchKernelGGL(k_restrict_copies<T>, grid1(ncopy), dim3(256), 0, stream, ncopy, v.nng, v.copy_orig.p,
                          v.nx + 1, v.ny + 1, c.nx + 1, c.ny + 1, px, py, pz, d.r.p, dc.b.p);
     CM_LAUNCH_CHECK();
-    vcycle<T>(l + 1, dc.b.p, dc.x.p);
-    hipLaunchKernelGGL(k_prolong_add<T>, grid1(v.nng), dim3(256), 0, stream, v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1,
-                       c.ny + 1, px, py, pz, dc.x.p, v.used.p, x);
-    if (ncopy)
-      hipLaunchKernelGGL(k_prolong_copies<T>, grid1(ncopy), dim3(256), 0, stream, ncopy, v.nng, v.copy_orig.p,
-                         v.nx + 1, v.ny + 1, c.nx + 1, c.ny + 1, px, py, pz, dc.x.p, x);
+    vcycle<T>(l + 1, dc.b.p, dc.x.p, start_c);
+    const int nbg = div_up(v.nng, 256);
+    hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)(nbg + div_up(ncopy, 256))), dim3(256), 0, stream, v.nx + 1,
+                       v.ny + 1, v.nz + 1, c.nx + 1, c.ny + 1, px, py, pz, dc.x.p, v.used.p, x, nbg, ncopy,
+                       v.copy_orig.p);
     CM_LAUNCH_CHECK();
     smooth<T>(l, b, x, false, opt.npost);
   }

@@ -158,6 +158,8 @@ struct Epi {
   T* x;               // CHEB: iterate, updated in place (nullptr: not updated)
   T c1, c2, xa;
   double* part;       // APPLY: per-block partial sums of s . (A s) (nullptr: none)
+  const int* fold;    // RESID: original grid node of every split copy; the copy's residual is
+                      // added to its original's (a copy restricts like its original), nullptr: none
 };
 
 template <typename T>
@@ -842,14 +844,24 @@ __global__ __launch_bounds__(256) void k_fixup(int ntouch, const int* __restrict
       for (int c = 0; c < 3; ++c) sn[c] = s[3 * n + c];
     }
     epilogue<T, EM>(ep, n, t, sn);
+    if (EM == EPI_RESID && ep.fold) {
+      const long long o = ep.fold[n - nng];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) atomicAdd(&ep.y[3 * o + c], ep.y[3 * n + c]);
+    }
     return;
   }
   if (EM == EPI_APPLY) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) ep.y[3 * n + c] += t[c];
   } else if (EM == EPI_RESID) {
+    if (ep.fold) {   // copies of this node may add to it concurrently
 #pragma unroll
-    for (int c = 0; c < 3; ++c) ep.y[3 * n + c] -= t[c];
+      for (int c = 0; c < 3; ++c) atomicAdd(&ep.y[3 * n + c], -t[c]);
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) ep.y[3 * n + c] -= t[c];
+    }
   } else {
     if (ep.y) {
 #pragma unroll
@@ -988,36 +1000,5 @@ __global__ void k_prolong_add(int nfx, int nfy, int nfz, int ncx, int ncy, AxisP
   x[3 * id] += e[0]; x[3 * id + 1] += e[1]; x[3 * id + 2] += e[2];
 }
 
-// Restriction of copies: rc += P[orig]^T r[copy] (atomics; few copies).
-template <typename T>
-__global__ void k_restrict_copies(int ncopy, long long nng, const int* __restrict__ orig, int nfx, int nfy, int ncx,
-                                  int ncy, AxisP<T> px, AxisP<T> py, AxisP<T> pz, const T* __restrict__ r,
-                                  T* __restrict__ rc) {
-  const int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ncopy) return;
-  const int o = orig[q];
-  const int i = o % nfx, j = (o / nfx) % nfy, k = o / (nfx * nfy);
-  const long long n = nng + q;
-  const T v0 = r[3 * n], v1 = r[3 * n + 1], v2 = r[3 * n + 2];
-  const int a = px.ia[i], b = py.ia[j], c = pz.ia[k];
-  const T wx = px.w[i], wy = py.w[j], wz = pz.w[k];
-  for (int dz = 0; dz < 2; ++dz) {
-    const T fz = dz ? wz : T(1) - wz;
-    if (fz == T(0)) continue;
-    for (int dy = 0; dy < 2; ++dy) {
-      const T fy = dy ? wy : T(1) - wy;
-      if (fy == T(0)) continue;
-      for (int dx = 0; dx < 2; ++dx) {
-        const T fx = dx ? wx : T(1) - wx;
-        if (fx == T(0)) continue;
-        const T ww = fx * fy * fz;
-        const long long m = ((long long)(c + dz) * ncy + (b + dy)) * ncx + (a + dx);
-        atomicAdd(&rc[3 * m], ww * v0);
-        atomicAdd(&rc[3 * m + 1], ww * v1);
-        atomicAdd(&rc[3 * m + 2], ww * v2);
-      }
-    }
-  }
-}
 
 }  // namespace cm

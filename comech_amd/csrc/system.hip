@@ -1495,19 +1495,21 @@ struct System {
     Level& c = lv(l + 1);
     LevelT<T>& dc = c.dat<T>();
     smooth<T>(l, b, x, true, opt.npre, x_started);
-    apply<T>(l, x, d.r.p, MODE_RESID, b);
+    const int ncopy = (int)(v.nn - v.nng);
+    {
+      // residual for the restriction; split copies fold theirs into their originals
+      Epi<T> ep{b, d.r.p, nullptr, nullptr, nullptr, T(0), T(0), T(0)};
+      ep.fold = ncopy ? v.copy_orig.p : nullptr;
+      apply_ep<T, EPI_RESID>(l, x, ep);
+    }
     AxisR<T> rx{v.rs[0].p, d.rw[0].p}, ry{v.rs[1].p, d.rw[1].p}, rz{v.rs[2].p, d.rw[2].p};
     AxisP<T> px{v.pia[0].p, d.pw[0].p}, py{v.pia[1].p, d.pw[1].p}, pz{v.pia[2].p, d.pw[2].p};
-    const int ncopy = (int)(v.nn - v.nng);
-    // without copies to add afterwards, the restriction also takes the coarse level's first
-    // smoothing step from zero (one launch less); the coarsest level is solved directly instead
-    const bool start_c = ncopy == 0 && l + 2 < nlev();
+    // the restriction also takes the coarse level's first smoothing step from zero (one launch
+    // less); the coarsest level is solved directly instead
+    const bool start_c = l + 2 < nlev();
     hipLaunchKernelGGL(k_restrict<T>, grid1(c.nn), dim3(256), 0, stream, v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1,
                        c.ny + 1, c.nz + 1, rx, ry, rz, d.r.p, dc.b.p, start_c ? dc.Dinv.p : (const T*)nullptr,
                        start_c ? dc.x.p : (T*)nullptr, T(start_c ? first_step_coef<T>(l + 1) : 0.0));
-    if (ncopy)
-      hipLaunchKernelGGL(k_restrict_copies<T>, grid1(ncopy), dim3(256), 0, stream, ncopy, v.nng, v.copy_orig.p,
-                         v.nx + 1, v.ny + 1, c.nx + 1, c.ny + 1, px, py, pz, d.r.p, dc.b.p);
     CM_LAUNCH_CHECK();
     vcycle<T>(l + 1, dc.b.p, dc.x.p, start_c);
     const int nbg = div_up(v.nng, 256);

@@ -47,6 +47,7 @@ SIGNATURES = {
     "cm_sys_timings": [c_void_p, P_double, c_int],
     "cm_sys_set_kz": [c_void_p, c_int],
     "cm_sys_set_block_skip": [c_void_p, c_int],
+    "cm_sys_set_copy_blocks": [c_void_p, c_int],
     "cm_sys_set_last_coarsening": [c_void_p, c_int],
     "cm_sys_set_graphs": [c_void_p, c_int],
     "cm_lattice_materials": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int,

@@ -171,4 +171,50 @@ __device__ __forceinline__ double block_sum(double v) {
 
 inline int div_up(long long a, long long b) { return (int)((a + b - 1) / b); }
 
+// Level-0 smoother blocks over split-copy groups: a grid node and its split copies, coupled by
+// interface springs, take one dense inverse of their principal submatrix of A in the smoother
+// (block Jacobi over the group) instead of one 3x3 inverse per node.  Member nodes carry a zero
+// 3x3 inverse in Dinv, so the per-node smoother steps leave them to the group pass.
+constexpr int GROUP_MAX = 8;   // members per group (larger groups keep per-node blocks)
+
+template <typename T>
+struct CopyGroups {
+  int ng = 0;                        // groups (0: none)
+  const int* gptr = nullptr;         // [ng + 1] member ranges
+  const int* gslot = nullptr;        // members as slots of the touched-node list
+  const int* tnode = nullptr;        // touched-node list (slot -> node)
+  const uint8_t* gmem = nullptr;     // [ntouch] 1 if the slot is a group member
+  const int* goff = nullptr;         // [ng + 1] offsets of the dense (3m)^2 inverses
+  const T* ginv = nullptr;           // dense symmetric inverses, row-major per group
+};
+
+// A zero diagonal marks a group member's 3x3 slot in Dinv (a node's own inverse is never zero:
+// singular blocks become the identity).
+template <typename T>
+__device__ __forceinline__ bool group_member(const T* m) {
+  return m[0] == T(0) && m[3] == T(0) && m[5] == T(0);
+}
+
+// z = c Ginv r over the members of group g, gathered from and scattered to node vectors.
+template <typename T>
+__device__ __forceinline__ void group_apply(const CopyGroups<T>& ga, int g, const T* __restrict__ r,
+                                            T* __restrict__ z, T c) {
+  const int b = ga.gptr[g], m = ga.gptr[g + 1] - b, N = 3 * m;
+  const T* G = ga.ginv + ga.goff[g];
+  T rg[3 * GROUP_MAX];
+  for (int j = 0; j < m; ++j) {
+    const long long n = ga.tnode[ga.gslot[b + j]];
+    for (int q = 0; q < 3; ++q) rg[3 * j + q] = r[3 * n + q];
+  }
+  for (int j = 0; j < m; ++j) {
+    const long long n = ga.tnode[ga.gslot[b + j]];
+    for (int q = 0; q < 3; ++q) {
+      T acc = T(0);
+      const T* row = G + (size_t)(3 * j + q) * N;
+      for (int k = 0; k < N; ++k) acc += row[k] * rg[k];
+      z[3 * n + q] = c * acc;
+    }
+  }
+}
+
 }  // namespace cm

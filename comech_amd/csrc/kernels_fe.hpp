@@ -814,9 +814,54 @@ __global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T*
 // is affine in A s, so the correction is -tc (RESID), +tc (APPLY), or for a smoothing step
 // r -= t, d -= c2 Dinv t, x -= c2 Dinv t.  Split copies have no structured part: their epilogue
 // runs here in full with A s = tc.
+//
+// Smoothing steps with split-copy groups (threads past ntouch, one per group): the members' 3x3
+// blocks in Dinv are zero, so the march and the copies' step below leave d = c1 sn and
+// x += xa sn + d; the group thread completes every member's residual r (ep.y, always kept when
+// groups exist), then adds c2 Ginv r_group to d and x.
+template <typename T>
+__device__ void group_fixup(const CopyGroups<T>& ga, int g, long long nng, T* __restrict__ tc, const T* __restrict__ s,
+                            const Epi<T>& ep) {
+  const int b = ga.gptr[g], m = ga.gptr[g + 1] - b, N = 3 * m;
+  T rg[3 * GROUP_MAX];
+  for (int j = 0; j < m; ++j) {
+    const int slot = ga.gslot[b + j];
+    const long long n = ga.tnode[slot];
+    for (int q = 0; q < 3; ++q) {
+      const T t = tc[3 * slot + q];
+      tc[3 * slot + q] = T(0);
+      T r;
+      if (n >= nng) {   // split copy: its whole step, with a zero 3x3 block
+        const T sn = s[3 * n + q];
+        r = ep.rin[3 * n + q] - t;
+        const T dn = ep.c1 * sn;
+        if (ep.dout) ep.dout[3 * n + q] = dn;
+        if (ep.x) ep.x[3 * n + q] += ep.xa * sn + dn;
+      } else {          // grid node: the march wrote r without the sparse part t
+        r = ep.y[3 * n + q] - t;
+      }
+      ep.y[3 * n + q] = r;
+      rg[3 * j + q] = r;
+    }
+  }
+  const T* G = ga.ginv + ga.goff[g];
+  for (int j = 0; j < m; ++j) {
+    const long long n = ga.tnode[ga.gslot[b + j]];
+    for (int q = 0; q < 3; ++q) {
+      T acc = T(0);
+      const T* row = G + (size_t)(3 * j + q) * N;
+      for (int k = 0; k < N; ++k) acc += row[k] * rg[k];
+      const T dz = ep.c2 * acc;
+      if (ep.dout) ep.dout[3 * n + q] += dz;
+      if (ep.x) ep.x[3 * n + q] += dz;
+    }
+  }
+}
+
 template <typename T, int EM>
 __global__ __launch_bounds__(256) void k_fixup(int ntouch, const int* __restrict__ tnode, long long nng,
-                                               T* __restrict__ tc, const T* __restrict__ s, Epi<T> ep) {
+                                               T* __restrict__ tc, const T* __restrict__ s, Epi<T> ep,
+                                               CopyGroups<T> ga = CopyGroups<T>{}) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (EM == EPI_APPLY && ep.part) {
     // s . tc over the touched nodes: A s = structured part (its dot came from the march) + tc
@@ -828,6 +873,13 @@ __global__ __launch_bounds__(256) void k_fixup(int ntouch, const int* __restrict
     }
     d = block_sum<256>(d);
     if (threadIdx.x == 0) ep.part[blockIdx.x] = d;
+  }
+  if (EM == EPI_CHEB && ga.ng) {
+    if (i >= ntouch) {
+      if (i - ntouch < ga.ng) group_fixup<T>(ga, i - ntouch, nng, tc, s, ep);
+      return;
+    }
+    if (ga.gmem[i]) return;
   }
   if (i >= ntouch) return;
   const long long n = tnode[i];

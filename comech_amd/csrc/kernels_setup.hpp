@@ -346,6 +346,59 @@ __global__ void k_block_inverse(long long nn, const double* __restrict__ D, T* _
   o[5] = T((m[0] * m[4] - m[1] * m[3]) * id);
 }
 
+// Split-copy group inverses, one thread per group: W = the group's principal submatrix of A
+// (off-diagonal member blocks from the host in off, diagonal blocks from D), inverted in place by
+// Gauss-Jordan without pivoting (W is SPD), symmetrised into ginv; the members' 3x3 slots of Dinv
+// become zero.  A non-positive pivot keeps the members' own 3x3 inverses (block diagonal ginv).
+template <typename T>
+__global__ void k_group_inverse(int ng, const int* __restrict__ gptr, const int* __restrict__ gslot,
+                                const int* __restrict__ tnode, const int* __restrict__ goff,
+                                const double* __restrict__ off, const double* __restrict__ D,
+                                double* __restrict__ W, T* __restrict__ Dinv, T* __restrict__ ginv) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  const int b = gptr[g], m = gptr[g + 1] - b, N = 3 * m;
+  double* w = W + goff[g];
+  T* gi = ginv + goff[g];
+  for (int k = 0; k < N * N; ++k) w[k] = off[goff[g] + k];
+  for (int j = 0; j < m; ++j) {
+    const double* dm = D + 9 * (long long)tnode[gslot[b + j]];
+    for (int p = 0; p < 3; ++p)
+      for (int q = 0; q < 3; ++q) w[(3 * j + p) * N + 3 * j + q] = dm[3 * p + q];
+  }
+  bool ok = true;
+  for (int k = 0; k < N && ok; ++k) {
+    const double piv = w[k * N + k];
+    if (!(piv > 0.0) || !isfinite(piv)) { ok = false; break; }
+    const double ip = 1.0 / piv;
+    w[k * N + k] = 1.0;
+    for (int j = 0; j < N; ++j) w[k * N + j] *= ip;
+    for (int i = 0; i < N; ++i) {
+      if (i == k) continue;
+      const double f = w[i * N + k];
+      if (f == 0.0) continue;
+      w[i * N + k] = 0.0;
+      for (int j = 0; j < N; ++j) w[i * N + j] -= f * w[k * N + j];
+    }
+  }
+  if (ok) {
+    for (int i = 0; i < N; ++i)
+      for (int j = 0; j < N; ++j) gi[i * N + j] = T(0.5 * (w[i * N + j] + w[j * N + i]));
+  } else {
+    for (int k = 0; k < N * N; ++k) gi[k] = T(0);
+    for (int j = 0; j < m; ++j) {
+      const T* o = Dinv + 6 * (long long)tnode[gslot[b + j]];
+      const int idx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+      for (int p = 0; p < 3; ++p)
+        for (int q = 0; q < 3; ++q) gi[(3 * j + p) * N + 3 * j + q] = o[idx[p][q]];
+    }
+  }
+  for (int j = 0; j < m; ++j) {
+    T* o = Dinv + 6 * (long long)tnode[gslot[b + j]];
+    for (int k = 0; k < 6; ++k) o[k] = T(0);
+  }
+}
+
 // Dense coarsest operator: one block of 24 threads per coarse cell, thread = element column.
 __global__ __launch_bounds__(64) void k_dense_assemble(int nx, int ny, int nz, const double* __restrict__ hx, const double* __restrict__ hy,
                                  const double* __restrict__ hz, const double* __restrict__ mom, int n,

@@ -32,24 +32,35 @@ __global__ __launch_bounds__(RED_THREADS) void k_finalize(int nb, const double* 
 
 // ---- smoothers -------------------------------------------------------------------------
 // Dinv holds the symmetric 3x3 block inverses as 6 values per node (00 01 02 11 12 22).
-// First step of a smoother from x = 0: x = c * Dinv b.
+// First step of a smoother from x = 0: x = c * Dinv b.  Threads past nn take the split-copy
+// groups (x = c Ginv b over each group); the per-node threads skip group members.
 template <typename T>
-__global__ void k_cheb0(long long nn, T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ Dinv, T c) {
+__global__ void k_cheb0(long long nn, T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ Dinv, T c,
+                        CopyGroups<T> ga) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
+  if (n >= nn) {
+    if (n - nn < ga.ng) group_apply(ga, (int)(n - nn), b, x, c);
+    return;
+  }
   const T* m = Dinv + 6 * n;
+  if (ga.ng && group_member(m)) return;
   const T r0 = b[3 * n], r1 = b[3 * n + 1], r2 = b[3 * n + 2];
   x[3 * n] = c * (m[0] * r0 + m[1] * r1 + m[2] * r2);
   x[3 * n + 1] = c * (m[1] * r0 + m[3] * r1 + m[4] * r2);
   x[3 * n + 2] = c * (m[2] * r0 + m[4] * r1 + m[5] * r2);
 }
 
-// y = M x (block-Jacobi apply), for the power iteration
+// y = M x (block-Jacobi apply, split-copy groups as in k_cheb0), for the power iteration
 template <typename T>
-__global__ void k_block_apply(long long nn, const T* __restrict__ Dinv, const T* __restrict__ x, T* __restrict__ y) {
+__global__ void k_block_apply(long long nn, const T* __restrict__ Dinv, const T* __restrict__ x, T* __restrict__ y,
+                              CopyGroups<T> ga) {
   const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= nn) return;
+  if (n >= nn) {
+    if (n - nn < ga.ng) group_apply(ga, (int)(n - nn), x, y, T(1));
+    return;
+  }
   const T* m = Dinv + 6 * n;
+  if (ga.ng && group_member(m)) return;
   const T x0 = x[3 * n], x1 = x[3 * n + 1], x2 = x[3 * n + 2];
   y[3 * n] = m[0] * x0 + m[1] * x1 + m[2] * x2;
   y[3 * n + 1] = m[1] * x0 + m[3] * x1 + m[4] * x2;

@@ -44,6 +44,7 @@ struct LevelT {
   DevBuf<T> xval;           // extra stiffness values
   DevBuf<T> r1w, r1k;       // rank-one terms
   DevBuf<T> x, b, r, t, d;  // work vectors
+  DevBuf<T> ginv;           // level 0: dense inverses of the split-copy groups
   double lmax = 0.0, lmin = 0.0;
   bool ready = false;
 };
@@ -106,6 +107,11 @@ struct Level {
   int ncopy = 0, nirr = 0;
   std::vector<int> copy_orig_h;
   DevBuf<int> copy_orig, irr_nodes, irr_ijk;
+  // split-copy groups of the smoother (a grid node and its copies coupled by extra stiffness)
+  int ngroup = 0;
+  DevBuf<int> gptr, gslot, goff;
+  DevBuf<uint8_t> gmem;
+  DevBuf<double> goffd64, gwork64;
   std::vector<int> irr_nodes_h;
   std::vector<double> irrL_h, irrM_h;
   // coarse-level moments (FP64 master)
@@ -633,6 +639,7 @@ struct System {
   }
   long long compact_dofs(int l) { return compact_dofs_of(lv(l)); }
   int last_axes = 3;   // axes the step to the coarsest level coarsens (7 = all, the reference rule)
+  bool copy_blocks = true;   // level-0 smoother blocks over split-copy groups (false: 3x3 per node)
 
   // ------------------------------------------------------------------ build
   // h / 16 when every cell of the level is the same cube (edge lengths equal to 1e-13 relative),
@@ -721,6 +728,79 @@ struct System {
     v.r1c.upload(rc, stream);
   }
 
+  // Split-copy groups of level 0: a grid node and its split copies, all in the touched list, with
+  // extra stiffness between them (interface springs).  Keeps the members (as touched slots) and the
+  // off-diagonal member blocks of A, dense per group; k_group_inverse adds the diagonal blocks.
+  void build_groups(Level& v) {
+    v.ngroup = 0;
+    if (!copy_blocks || v.nn == v.nng || v.ntouch == 0) return;
+    auto touched = [&](long long n) { return std::binary_search(v.tnode_h.begin(), v.tnode_h.end(), (int)n); };
+    std::map<int, std::vector<int>> by_orig;
+    for (long long c = v.nng; c < v.nn; ++c) by_orig[v.copy_orig_h[c - v.nng]].push_back((int)c);
+    std::vector<std::vector<int>> mem;
+    std::vector<int> gid(v.nn, -1), gix(v.nn, -1);
+    for (auto& kv : by_orig) {
+      std::vector<int> m;
+      if (touched(kv.first)) m.push_back(kv.first);
+      for (int c : kv.second)
+        if (touched(c)) m.push_back(c);
+      if (m.size() < 2 || m.size() > (size_t)GROUP_MAX) continue;
+      for (size_t j = 0; j < m.size(); ++j) { gid[m[j]] = (int)mem.size(); gix[m[j]] = (int)j; }
+      mem.push_back(std::move(m));
+    }
+    if (mem.empty()) return;
+    std::vector<std::vector<double>> off(mem.size());
+    for (size_t g = 0; g < mem.size(); ++g) off[g].assign(9 * mem[g].size() * mem[g].size(), 0.0);
+    auto add = [&](int da, int db, double val) {
+      const int a = da / 3, b = db / 3;
+      if (a == b || gid[a] < 0 || gid[a] != gid[b]) return;
+      const int N = 3 * (int)mem[gid[a]].size();
+      off[gid[a]][(size_t)(3 * gix[a] + da % 3) * N + 3 * gix[b] + db % 3] += val;
+    };
+    for (size_t r = 0; r < v.xrow_h.size(); ++r)
+      for (int q = v.xptr_h[r]; q < v.xptr_h[r + 1]; ++q) add(v.xrow_h[r], v.xcol_h[q], v.xval_h[q]);
+    for (size_t k = 0; k + 1 < v.r1ptr_h.size(); ++k)
+      for (int p = v.r1ptr_h[k]; p < v.r1ptr_h[k + 1]; ++p)
+        for (int q = v.r1ptr_h[k]; q < v.r1ptr_h[k + 1]; ++q)
+          add(v.r1dof_h[p], v.r1dof_h[q], v.r1k_h[k] * v.r1w_h[p] * v.r1w_h[q]);
+    std::vector<int> gptr{0}, gslot, goff{0};
+    std::vector<double> offd;
+    std::vector<uint8_t> gmem(v.ntouch, 0);
+    for (size_t g = 0; g < mem.size(); ++g) {
+      bool coupled = false;
+      for (double x : off[g]) coupled |= (x != 0.0);
+      if (!coupled) continue;
+      for (int n : mem[g]) {
+        const int s = compact_of(v.tnode_h, n);
+        gslot.push_back(s);
+        gmem[s] = 1;
+      }
+      gptr.push_back((int)gslot.size());
+      offd.insert(offd.end(), off[g].begin(), off[g].end());
+      goff.push_back((int)offd.size());
+    }
+    v.ngroup = (int)gptr.size() - 1;
+    if (!v.ngroup) return;
+    v.gptr.upload(gptr, stream);
+    v.gslot.upload(gslot, stream);
+    v.goff.upload(goff, stream);
+    v.gmem.upload(gmem, stream);
+    v.goffd64.upload(offd, stream);
+    v.gwork64.alloc(offd.size());
+  }
+
+  template <typename T>
+  CopyGroups<T> groups_of(int l) {
+    Level& v = lv(l);
+    LevelT<T>& d = v.dat<T>();
+    CopyGroups<T> ga{};
+    if (l != 0 || !v.ngroup || !d.ginv.n) return ga;
+    ga.ng = v.ngroup;
+    ga.gptr = v.gptr.p; ga.gslot = v.gslot.p; ga.tnode = v.tnode.p; ga.gmem = v.gmem.p;
+    ga.goff = v.goff.p; ga.ginv = d.ginv.p;
+    return ga;
+  }
+
   void build(int min_dofs, int max_levels, int fixed_levels, const Options& o) {
     CM_CHECK(!L.empty(), "system has no fine level");
     opt = o;
@@ -802,8 +882,12 @@ struct System {
       v.r1dof.upload(v.r1dof_h, stream);
       v.r1w64.upload(v.r1w_h, stream);
       v.r1k64.upload(v.r1k_h, stream);
-      if (l == 0) build_touch(v);
-      else build_coarse_rows(v);
+      if (l == 0) {
+        build_touch(v);
+        build_groups(v);
+      } else {
+        build_coarse_rows(v);
+      }
     }
     prepare_prec<double>(0, false);
     lap(2);
@@ -1044,6 +1128,12 @@ struct System {
       d.Dinv.alloc(6 * (size_t)v.nn);
       hipLaunchKernelGGL(k_block_inverse<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, D64[l].p, d.Dinv.p);
       CM_LAUNCH_CHECK();
+      if (l == 0 && v.ngroup) {
+        d.ginv.alloc(v.gwork64.n);
+        hipLaunchKernelGGL(k_group_inverse<T>, grid1(v.ngroup), dim3(256), 0, stream, v.ngroup, v.gptr.p, v.gslot.p,
+                           v.tnode.p, v.goff.p, v.goffd64.p, D64[l].p, v.gwork64.p, d.Dinv.p, d.ginv.p);
+        CM_LAUNCH_CHECK();
+      }
       if (opt.smoother == 0) power_iteration<T>(l);
     }
     refresh_flags<T>(l);
@@ -1085,8 +1175,10 @@ struct System {
     if (l > 0 && d.ex.p) launch_extra<T>(l, s);
     launch_march<T, EM>(l, s, ep, v.ntouch ? sparse_args<T>(v, d) : SparseArgs<T>{});
     if (v.ntouch) {
-      hipLaunchKernelGGL((k_fixup<T, EM>), grid1(v.ntouch), dim3(256), 0, stream, v.ntouch, v.tnode.p, v.nng, d.tc.p, s,
-                         ep);
+      const CopyGroups<T> ga = EM == EPI_CHEB ? groups_of<T>(l) : CopyGroups<T>{};
+      CM_CHECK(!ga.ng || ep.y, "split-copy groups need the residual output");
+      hipLaunchKernelGGL((k_fixup<T, EM>), grid1(v.ntouch + ga.ng), dim3(256), 0, stream, v.ntouch, v.tnode.p, v.nng,
+                         d.tc.p, s, ep, ga);
     }
     CM_LAUNCH_CHECK();
   }
@@ -1131,7 +1223,7 @@ struct System {
       Epi<double> ef = ep;
       ef.part = partial.p + nbm;
       hipLaunchKernelGGL((k_fixup<double, EPI_APPLY>), grid1(v.ntouch), dim3(256), 0, stream, v.ntouch, v.tnode.p,
-                         v.nng, d.tc.p, p, ef);
+                         v.nng, d.tc.p, p, ef, CopyGroups<double>{});
     }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, nbm + nbf, partial.p, S.p, slot);
     CM_LAUNCH_CHECK();
@@ -1381,7 +1473,9 @@ struct System {
     DevBuf<double> s(4);
     for (int it = 0; it < 12; ++it) {
       apply<T>(l, d.x.p, d.t.p, MODE_APPLY, nullptr);
-      hipLaunchKernelGGL(k_block_apply<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, d.Dinv.p, d.t.p, d.d.p);
+      const CopyGroups<T> ga = groups_of<T>(l);
+      hipLaunchKernelGGL(k_block_apply<T>, grid1(v.nn + ga.ng), dim3(256), 0, stream, v.nn, d.Dinv.p, d.t.p, d.d.p,
+                         ga);
       hipLaunchKernelGGL((k_dot<T, T>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, d.d.p, d.d.p, partial.p);
       hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, s.p, 0);
       if (it == 11) {
@@ -1414,16 +1508,20 @@ struct System {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
     const long long nn = v.nn;
+    // split-copy groups need every step's residual (their fix-up completes it)
+    const CopyGroups<T> ga = groups_of<T>(l);
+    T* rkeep = ga.ng ? d.r.p : nullptr;
     if (opt.smoother == 1) {
       // damped block Jacobi: x += omega Dinv (b - A x); the step lands in d, then x += d
       int s0 = 0;
       if (zero_start) {
         if (!x_started)
-          hipLaunchKernelGGL(k_cheb0<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(opt.omega));
+          hipLaunchKernelGGL(k_cheb0<T>, grid1(nn + ga.ng), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(opt.omega),
+                             ga);
         s0 = 1;
       }
       for (int s = s0; s < nsweep; ++s) {
-        const Epi<T> ep{b, nullptr, d.Dinv.p, d.d.p, nullptr, T(0), T(opt.omega), T(0)};
+        const Epi<T> ep{b, rkeep, d.Dinv.p, d.d.p, nullptr, T(0), T(opt.omega), T(0)};
         apply_ep<T, EPI_CHEB>(l, x, ep);
         hipLaunchKernelGGL((k_add<T, T>), grid1(3 * nn), dim3(256), 0, stream, 3 * nn, x, d.d.p);
       }
@@ -1447,7 +1545,7 @@ struct System {
     int nd = 0;            // next direction buffer
     if (zero_start) {
       if (!x_started)
-        hipLaunchKernelGGL(k_cheb0<T>, grid1(nn), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(1.0 / theta));
+        hipLaunchKernelGGL(k_cheb0<T>, grid1(nn + ga.ng), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(1.0 / theta), ga);
       s = x;
       pending = false;
     } else {
@@ -1463,7 +1561,7 @@ struct System {
       const bool last = (k == opt.degree - 1);
       const bool upd = (s != x);
       T* dout = (!last || !upd) ? dirs[nd] : nullptr;
-      const Epi<T> ep{rin, last ? nullptr : r, d.Dinv.p, dout, upd ? x : nullptr, T(rho_new * rho),
+      const Epi<T> ep{rin, last ? rkeep : r, d.Dinv.p, dout, upd ? x : nullptr, T(rho_new * rho),
                       T(2.0 * rho_new / delta), T(pending ? 1 : 0)};
       apply_ep<T, EPI_CHEB>(l, s, ep);
       pending = !upd;
@@ -2217,6 +2315,14 @@ int cm_sys_set_last_coarsening(cm_sys* h, int axes) {
   CM_CHECK(!h->s.built, "set the coarsening before cm_sys_build");
   CM_CHECK(axes >= 1 && axes <= 7, "axes is a bit mask of x (1), y (2), z (4)");
   h->s.last_axes = axes;
+  CM_API_END
+}
+
+int cm_sys_set_copy_blocks(cm_sys* h, int on) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  CM_CHECK(!h->s.built, "set the smoother blocks before cm_sys_build");
+  h->s.copy_blocks = on != 0;
   CM_API_END
 }
 

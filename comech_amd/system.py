@@ -53,12 +53,14 @@ class VoxelSystem:
                  copy_orig=None, irr_cells=None, irr_nodes=None, extra=None, rank1=None, used=None, min_dofs=6000,
                  max_levels=6,
                  levels=None, smoother="chebyshev", degree=3, omega=0.6, npre=2, npost=2, materials="auto",
-                 last_axes=3, device=None):
+                 last_axes=3, copy_blocks=True, device=None):
         """grid: Grid; active: (ncell,) bool in reference cell order; materials either per cell
         (lam, mu) or as labels + tables; split connectivity as copy originals and irregular cells
         with their 8 node ids; extra: (rows, cols, vals) COO over level-0 DOFs.  last_axes: axes
         (x 1, y 2, z 4) the step to the coarsest level coarsens; 7 is the reference rule of
-        fe/gmg.py, the default 3 keeps the vertical resolution on the coarsest grid."""
+        fe/gmg.py, the default 3 keeps the vertical resolution on the coarsest grid.  copy_blocks:
+        level-0 smoother blocks over each grid node and its split copies when extra stiffness couples
+        them (False: the per-node 3x3 block Jacobi of fe/gmg.py:29-44)."""
         self.device = require_gpu(device)
         self.lib = _lib.load()
         self.grid = grid
@@ -88,6 +90,7 @@ class VoxelSystem:
                 check(self.lib.cm_sys_set_used(self.h, len(um), ptr(um, ctypes.c_uint8)))
             fixed = 0 if levels is None else int(levels)
             check(self.lib.cm_sys_set_last_coarsening(self.h, int(last_axes)))
+            check(self.lib.cm_sys_set_copy_blocks(self.h, int(bool(copy_blocks))))
             with torch.cuda.device(self.device):
                 check(self.lib.cm_sys_build(self.h, int(min_dofs), int(max_levels), fixed, SMOOTHERS[smoother],
                                             int(degree), float(omega), int(npre), int(npost)))

@@ -224,6 +224,13 @@ int cm_bank_pack(void* stream, int64_t ncell, int ncls, const float* vm, const f
 /* Override the z extent of a structured block (0 = automatic). */
 int cm_sys_set_kz(cm_sys* sys, int kz);
 
+/* Level-0 smoother blocks over split-copy groups (on by default): a grid node and its split
+ * copies, coupled by extra stiffness (interface springs), are smoothed with the inverse of their
+ * joint diagonal block instead of one 3x3 block per node; 0 = the per-node 3x3 block Jacobi of
+ * fe/gmg.py:29-44.  Same solution to the tolerance; stiff interfaces (fused joints) converge in
+ * fewer iterations.  Call before cm_sys_build.  No reference counterpart. */
+int cm_sys_set_copy_blocks(cm_sys* sys, int on);
+
 /* Structured-march blocks whose cells have no stiffness run only the epilogue (on by default;
  * 0 = every block runs the full march).  Both give the same results; no reference counterpart. */
 int cm_sys_set_block_skip(cm_sys* sys, int on);

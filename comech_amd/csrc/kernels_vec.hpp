@@ -163,25 +163,34 @@ __global__ void k_axpy_scaled(long long n, double* __restrict__ x, const TD* __r
 
 // Rigid-mode (deflation) correction of rb/gpusolve.py:217-228 / fe/gmg.py:127-135:
 // z += Z Einv Z^T r with Z given as CSR rows over DOFs (few columns).
-// Step 1: w = Z^T r -- one block per column j, sum over its entries (CSC layout).
-__global__ __launch_bounds__(RED_THREADS) void k_zt_r(int ncol, const int* __restrict__ cptr,
-                                                      const int* __restrict__ rows, const double* __restrict__ vals,
-                                                      const double* __restrict__ r, double* __restrict__ w) {
-  const int j = blockIdx.x;
-  if (j >= ncol) return;
+// Step 1: partials of w = Z^T r -- one block per chunk of at most ZT_CHUNK entries of one column
+// (CSC layout), so that the few long columns spread over many CUs.
+constexpr int ZT_CHUNK = 2048;
+__global__ __launch_bounds__(RED_THREADS) void k_zt_r(const int* __restrict__ chk, const int* __restrict__ rows,
+                                                      const double* __restrict__ vals, const double* __restrict__ r,
+                                                      double* __restrict__ part) {
+  const int b = blockIdx.x;
   double s = 0.0;
-  for (int q = cptr[j] + threadIdx.x; q < cptr[j + 1]; q += RED_THREADS) s += vals[q] * r[rows[q]];
+  for (int q = chk[b] + threadIdx.x; q < chk[b + 1]; q += RED_THREADS) s += vals[q] * r[rows[q]];
   s = block_sum<RED_THREADS>(s);
-  if (threadIdx.x == 0) w[j] = s;
+  if (threadIdx.x == 0) part[b] = s;
 }
-// Step 2: v = Einv w (one thread per column, ncol small)
-__global__ void k_small_gemv(int ncol, const double* __restrict__ Einv, const double* __restrict__ w,
-                             double* __restrict__ v) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ncol) return;
-  double s = 0.0;
-  for (int j = 0; j < ncol; ++j) s += Einv[(long long)i * ncol + j] * w[j];
-  v[i] = s;
+// Step 2 (one block): w_j = the column's chunk partials in order, then v = Einv w
+__global__ __launch_bounds__(RED_THREADS) void k_rigid_small(int ncol, const int* __restrict__ cchk,
+                                                             const double* __restrict__ part,
+                                                             const double* __restrict__ Einv, double* __restrict__ w,
+                                                             double* __restrict__ v) {
+  for (int j = threadIdx.x; j < ncol; j += RED_THREADS) {
+    double s = 0.0;
+    for (int c = cchk[j]; c < cchk[j + 1]; ++c) s += part[c];
+    w[j] = s;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ncol; i += RED_THREADS) {
+    double s = 0.0;
+    for (int j = 0; j < ncol; ++j) s += Einv[(long long)i * ncol + j] * w[j];
+    v[i] = s;
+  }
 }
 // Step 3: z[row] += sum_j Z[row, j] v[j] (CSR over the touched rows)
 __global__ void k_z_v(int nrow, const int* __restrict__ rows, const int* __restrict__ ptr,

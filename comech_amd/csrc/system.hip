@@ -152,9 +152,9 @@ struct System {
   DevBuf<double> r64, z64, p64, Ap64;
   DevBuf<float> r32, z32, p32, Ap32, d32;
   // rigid-mode correction
-  int zcol = 0;
-  DevBuf<int> zcptr, zrows_c, zrow, zptr, zcols;
-  DevBuf<double> zvals_c, zvals, Einv, zw, zv;
+  int zcol = 0, zchunks = 0;
+  DevBuf<int> zchk, zcchk, zrows_c, zrow, zptr, zcols;   // zchk: chunk bounds in the CSC; zcchk: chunks per column
+  DevBuf<double> zvals_c, zvals, Einv, zw, zv, zpart;
   // coarsest dense inverse
   DevBuf<double> Ainv;        // inverse of the coarsest operator over its used DOFs
   DevBuf<int> dmap;           // compact DOF -> full DOF of the coarsest level
@@ -1689,8 +1689,8 @@ struct System {
   // rigid-mode correction z += Z Einv Z^T r (FP64)
   void rigid_correct(const double* r, double* z) {
     if (!zcol) return;
-    hipLaunchKernelGGL(k_zt_r, dim3(zcol), dim3(RED_THREADS), 0, stream, zcol, zcptr.p, zrows_c.p, zvals_c.p, r, zw.p);
-    hipLaunchKernelGGL(k_small_gemv, dim3(div_up(zcol, 64)), dim3(64), 0, stream, zcol, Einv.p, zw.p, zv.p);
+    hipLaunchKernelGGL(k_zt_r, dim3(zchunks), dim3(RED_THREADS), 0, stream, zchk.p, zrows_c.p, zvals_c.p, r, zpart.p);
+    hipLaunchKernelGGL(k_rigid_small, dim3(1), dim3(RED_THREADS), 0, stream, zcol, zcchk.p, zpart.p, Einv.p, zw.p, zv.p);
     const int nr = (int)(zrow.n);
     hipLaunchKernelGGL(k_z_v, grid1(nr), dim3(256), 0, stream, nr, zrow.p, zptr.p, zcols.p, zvals.p, zv.p, z);
     CM_LAUNCH_CHECK();
@@ -2113,7 +2113,19 @@ int cm_sys_set_rigid(cm_sys* h, int ncol, int64_t nnz, const int64_t* rows, cons
     rval.push_back(vals[q]);
   }
   rptr.push_back((int)rcol.size());
-  s.zcptr.upload(cptr, s.stream); s.zrows_c.upload(crow, s.stream); s.zvals_c.upload(cval, s.stream);
+  // chunks of at most ZT_CHUNK entries inside each column
+  std::vector<int> chk{0}, cchk{0};
+  for (int j = 0; j < ncol; ++j) {
+    for (int q = cptr[j]; q < cptr[j + 1];) {
+      q = std::min(q + cm::ZT_CHUNK, cptr[j + 1]);
+      chk.push_back(q);
+    }
+    cchk.push_back((int)chk.size() - 1);
+  }
+  s.zchk.upload(chk, s.stream); s.zcchk.upload(cchk, s.stream);
+  s.zchunks = (int)chk.size() - 1;
+  s.zpart.alloc(std::max(1, s.zchunks));
+  s.zrows_c.upload(crow, s.stream); s.zvals_c.upload(cval, s.stream);
   s.zrow.upload(rrow, s.stream); s.zptr.upload(rptr, s.stream); s.zcols.upload(rcol, s.stream);
   s.zvals.upload(rval, s.stream);
   std::vector<double> E(Einv, Einv + (size_t)ncol * ncol);

@@ -414,7 +414,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24, help="solves in the timed region (the surgical actions, cycled)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed passes over the action set")
-    ap.add_argument("--concurrency", type=int, default=12, help="solves in flight on the GPU")
+    ap.add_argument("--concurrency", type=int, default=16, help="solves in flight on the GPU")
     ap.add_argument("--mode", default="ref", choices=["mixed", "ref", "fp64"])
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")

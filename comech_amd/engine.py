@@ -100,7 +100,7 @@ def rigid_coo(xyz, used, node_groups):
     return (np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)), col
 
 
-MG_KEYS = ("min_dofs", "max_levels", "levels", "smoother", "degree", "omega", "npre", "npost")
+MG_KEYS = ("min_dofs", "max_levels", "levels", "smoother", "degree", "omega", "npre", "npost", "copy_blocks")
 SOLVE_KEYS = ("mode", "check_every", "inner_tol")
 
 

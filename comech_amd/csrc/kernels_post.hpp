@@ -131,17 +131,14 @@ __global__ __launch_bounds__(256) void k_bank_pack(long long ncell, int ncls, co
   const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= ncell) return;
   vm16[c] = __float2half(vm[c]);
-  // FP32 mean of the 8 node values, summed pairwise ((0+1)+(2+3))+((4+5)+(6+7)) as numpy sums
-  // eight float32 values
-  float v[8][3];
+  // FP32 mean of the 8 node values: numpy's mean over axis 1 of a (ncell, 8, 3) float32 array
+  // accumulates the node values in index order, v0 + v1 + ... + v7, in float32, then divides by 8
+  float s[3] = {0.f, 0.f, 0.f};
   for (int l = 0; l < 8; ++l) {
     const long long n = cell_nodes[8 * c + l];
-    for (int q = 0; q < 3; ++q) v[l][q] = u_nodes[3 * n + q];
+    for (int q = 0; q < 3; ++q) s[q] = (l == 0) ? u_nodes[3 * n + q] : s[q] + u_nodes[3 * n + q];
   }
-  for (int q = 0; q < 3; ++q) {
-    const float s = ((v[0][q] + v[1][q]) + (v[2][q] + v[3][q])) + ((v[4][q] + v[5][q]) + (v[6][q] + v[7][q]));
-    uc16[3 * c + q] = __float2half(s / 8.0f);
-  }
+  for (int q = 0; q < 3; ++q) uc16[3 * c + q] = __float2half(s[q] / 8.0f);
   lab8[c] = (int8_t)lab[c];
   for (int k = 0; k < ncls; ++k) {
     // FP32 product, round half to even (numpy's round of a float32 array)

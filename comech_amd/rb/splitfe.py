@@ -23,59 +23,66 @@ SRI = True
 
 
 def split_connectivity_fast(g, active, part, bone_share):
-    """Split connectivity (rb/splitfe.py:261-305, same result as the loop form :17-55).
+    """Split connectivity with the numbering of rb/splitfe.py:261-305 (the loop form :17-55 gives
+    the same result).
 
-    A grid node touched by cells of two or more bones gets one id per bone: the lowest bone keeps
-    the grid id, every further (node, bone) pair gets ``nnode + rank`` in (node, bone) order.  A
-    part-0 (joint/soft) cell at a split node uses the bone with the largest share in that cell, or,
-    when the cell holds none of those bones, the bone with most cells at the node.
+    Rule: a grid node touched by active cells of two or more bones is split.  Its lowest bone keeps
+    the grid id; every further (node, bone) pair gets ``nnode + rank``, ranked in (node, bone)
+    order.  A bone cell uses the id of its own bone at each corner.  A part-0 (joint/soft) cell at a
+    split node uses the node's bone with the largest share in that cell (the first in bone order
+    on a tie), or, when none of the node's bones has a positive share there, the node's bone with
+    the most cells at the node (again the first on a tie).
+
+    Computed from a dense table count[node, bone] over the nodes bone cells touch: the split
+    nodes are its rows with two or more non-zero columns, the copy ids a running count over the
+    row-major table, and the soft-cell choice two masked argmax over that row.
     Returns (cn (ncell,8), orig (nnode+ncopy,), number of split nodes)."""
     cells = np.flatnonzero(active)
     corn = corners_of(g, cells)
     pc = np.asarray(part)[cells].astype(np.int64)
-    nb = bone_share.shape[1]
-    # incidence of (node, bone) pairs over bone cells, with multiplicity
-    isb = pc > 0
-    pair_keys = (corn[isb] * nb + pc[isb][:, None]).ravel()
-    pairs, mult = np.unique(pair_keys, return_counts=True)
-    p_node, p_bone = np.divmod(pairs, nb)
-    bones_at = np.bincount(p_node, minlength=g.nnode)
-    is_split = bones_at >= 2
-    lead = np.ones(len(pairs), bool)
-    lead[1:] = p_node[1:] != p_node[:-1]
-    gets_copy = is_split[p_node] & ~lead
-    copy_id = np.full(len(pairs), -1, np.int64)
-    copy_id[gets_copy] = g.nnode + np.arange(int(gets_copy.sum()))
-    orig = np.concatenate([np.arange(g.nnode, dtype=np.int64), p_node[gets_copy]])
-    first_pair = np.searchsorted(p_node, np.arange(g.nnode))
+    nb = int(bone_share.shape[1])
     cn = g.cell_nodes.copy()
-    r, l = np.nonzero(is_split[corn])
+    bc = pc > 0
+    if not bc.any():
+        return cn, np.arange(g.nnode, dtype=np.int64), 0
+    bnodes = corn[bc]
+    touched = np.zeros(g.nnode, bool)
+    touched[bnodes.ravel()] = True
+    tn = np.flatnonzero(touched)
+    row_of = np.full(g.nnode, -1, np.int64)
+    row_of[tn] = np.arange(len(tn))
+    count = np.bincount((row_of[bnodes] * nb + pc[bc][:, None]).ravel(),
+                        minlength=len(tn) * nb).reshape(len(tn), nb)
+    count[:, 0] = 0
+    rows = np.flatnonzero((count > 0).sum(1) >= 2)
+    split_nodes = tn[rows]
+    cnt = count[rows]                                   # (nsplit, nb)
+    has = cnt > 0
+    lowest = has.argmax(1)
+    ids = np.full(has.shape, -1, np.int64)
+    more = has.copy()
+    more[np.arange(len(rows)), lowest] = False
+    flat = np.flatnonzero(more.ravel())                 # row-major: (node, bone) order
+    ids.ravel()[flat] = g.nnode + np.arange(len(flat))
+    ids[np.arange(len(rows)), lowest] = split_nodes
+    orig = np.concatenate([np.arange(g.nnode, dtype=np.int64), np.repeat(split_nodes, more.sum(1))])
+    srow = np.full(g.nnode, -1, np.int64)
+    srow[split_nodes] = np.arange(len(rows))
+    q = srow[corn]
+    r, l = np.nonzero(q >= 0)
     if len(r):
-        node = corn[r, l]
-        cell = cells[r]
+        s_ = q[r, l]
         bone = pc[r].copy()
         soft = np.flatnonzero(bone == 0)
         if len(soft):
-            sn = node[soft]
-            sc = cell[soft]
-            k = bones_at[sn]
-            best_share, best_by_share = np.full(len(soft), -1.0), np.zeros(len(soft), np.int64)
-            best_mult, best_by_mult = np.full(len(soft), -1), np.zeros(len(soft), np.int64)
-            for j in range(int(k.max())):
-                m = np.flatnonzero(j < k)
-                pidx = first_pair[sn[m]] + j
-                b = p_bone[pidx]
-                share = bone_share[sc[m], b]
-                up = share > best_share[m]           # strict: the first maximum wins, as argmax
-                best_share[m[up]] = share[up]; best_by_share[m[up]] = b[up]
-                up = mult[pidx] > best_mult[m]
-                best_mult[m[up]] = mult[pidx][up]; best_by_mult[m[up]] = b[up]
-            bone[soft] = np.where(best_share > 0, best_by_share, best_by_mult)
-        pos = np.searchsorted(pairs, node * nb + bone)
-        cid = copy_id[pos]
-        sel = cid >= 0
-        cn[cell[sel], l[sel]] = cid[sel]
-    return cn, orig, int(is_split.sum())
+            cand = has[s_[soft]]
+            share = np.where(cand, np.asarray(bone_share)[cells[r[soft]]], -np.inf)
+            by_share = share.argmax(1)
+            top = share[np.arange(len(soft)), by_share]
+            by_count = np.where(cand, cnt[s_[soft]], -1).argmax(1)
+            bone[soft] = np.where(top > 0, by_share, by_count)
+        cn[cells[r], l] = ids[s_, bone]
+    return cn, orig, len(rows)
 
 
 split_connectivity = split_connectivity_fast

@@ -230,12 +230,31 @@ class VoxelSystem:
         rows = np.ascontiguousarray(rows, np.int64)
         cols = np.ascontiguousarray(cols, np.int64)
         vals = np.ascontiguousarray(vals, np.float64)
+        ncol = int(ncol)
+        if ncol == 0 or len(rows) == 0:
+            return
         n = 3 * self.nn
-        Zd = torch.zeros((ncol, n), dtype=torch.float64, device=self.device)
-        Zd[torch.from_numpy(cols).to(self.device), torch.from_numpy(rows).to(self.device)] = \
-            torch.from_numpy(vals).to(self.device)
-        AZ = torch.stack([self.apply(Zd[j].contiguous()) for j in range(ncol)])
-        E = (Zd @ AZ.T).cpu().numpy()
+        # E = Z^T A Z one column at a time: scatter z_j into one scratch vector, apply A, gather
+        # Z^T (A z_j) from the sparse entries (Z holds a few entries per node and column)
+        order = np.argsort(cols, kind="stable")
+        rows_s, cols_s, vals_s = rows[order], cols[order], vals[order]
+        bounds = np.searchsorted(cols_s, np.arange(ncol + 1))
+        rd = torch.from_numpy(rows_s).to(self.device)
+        vd = torch.from_numpy(vals_s).to(self.device)
+        cd = torch.from_numpy(cols_s).to(self.device)
+        z = torch.zeros(n, dtype=torch.float64, device=self.device)
+        az = torch.empty_like(z)
+        E = np.zeros((ncol, ncol))
+        for j in range(ncol):
+            a, b = int(bounds[j]), int(bounds[j + 1])
+            if a == b:
+                continue
+            z.zero_()
+            z.index_put_((rd[a:b],), vd[a:b], accumulate=True)
+            self.apply(z, az)
+            col = torch.zeros(ncol, dtype=torch.float64, device=self.device)
+            col.index_add_(0, cd, vd * az[rd])
+            E[:, j] = col.cpu().numpy()
         Einv = np.ascontiguousarray(np.linalg.pinv(0.5 * (E + E.T)))
         check(self.lib.cm_sys_set_rigid(self.h, int(ncol), len(rows), ptr(rows, ctypes.c_longlong),
                                         ptr(cols, ctypes.c_longlong), ptr(vals, ctypes.c_double),

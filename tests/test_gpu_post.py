@@ -82,7 +82,9 @@ def test_bank_pack_and_files(tmp_path):
     from comech_amd import bank
     from comech_amd.grid import Grid
     rng = np.random.default_rng(5)
-    g = Grid(np.arange(9.0), np.arange(7.0), np.arange(6.0))
+    # 120,000 cells: the float32 summation order of the cell mean differs from the index order in
+    # about half of the cells, so a wrong order fails here (float16 of the mean differs in ~1e-4)
+    g = Grid(np.arange(61.0), np.arange(51.0), np.arange(41.0))
     Urel = rng.standard_normal((g.nnode, 3)).astype(np.float32) * 3
     vm = rng.uniform(0, 40, g.ncell).astype(np.float32)
     lab = rng.integers(-1, 9, g.ncell)
@@ -90,7 +92,11 @@ def test_bank_pack_and_files(tmp_path):
     frac[0, :2] = [0.5 / 255, 1.5 / 255]                  # rounding ties
     p = bank.pack_action(vm, Urel, g.cell_nodes, lab, frac, device=DEV)
     assert np.array_equal(p["vm"], vm.astype(np.float16))
-    uc = Urel[g.cell_nodes].mean(1).astype(np.float16)
+    v = Urel[g.cell_nodes]
+    uc = v.mean(1).astype(np.float16)
+    pairwise = ((((v[:, 0] + v[:, 1]) + (v[:, 2] + v[:, 3])) + ((v[:, 4] + v[:, 5]) + (v[:, 6] + v[:, 7])))
+                / np.float32(8)).astype(np.float16)
+    assert not np.array_equal(pairwise, uc)              # the sample separates the two orders
     assert np.array_equal(p["ucell"], uc)
     assert np.array_equal(p["lab"], lab.astype(np.int8))
     assert np.array_equal(p["frac"], np.round(frac * 255).astype(np.uint8))

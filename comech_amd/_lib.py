@@ -37,6 +37,9 @@ SIGNATURES = {
     "cm_sys_level_stats": [c_void_p, c_int, P_ll, c_int],
     "cm_sys_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p],
     "cm_sys_vcycle": [c_void_p, c_int, c_void_p, c_void_p],
+    "cm_sys_level_data": [c_void_p, c_int, c_int, c_int, c_void_p],
+    "cm_sys_power": [c_void_p, c_int, c_int, c_void_p, P_double],
+    "cm_sys_smooth": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int],
     "cm_sys_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_int, c_double, ctypes.POINTER(SolveInfo)],
     "cm_cell_stress": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_void_p,
                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -25,6 +25,7 @@ namespace cm {
 
 static thread_local std::string g_last_error;
 
+
 // ---------------------------------------------------------------------------------------------
 // Per-level, per-precision device data
 template <typename T>
@@ -1464,12 +1465,16 @@ struct System {
   // lambda_max of M A by 12 power steps from a fixed pseudo-normal start (fe/gmg.py:51-59):
   // lam = |y| / |x| of the last step, then lmax = 1.1 lam, lmin = lmax / 30.  The iterate is
   // normalised on the device; the host reads the result once.
+  // x0: start vector in the level's layout (null: the built-in pseudo-normal start)
   template <typename T>
-  void power_iteration(int l) {
+  double power_iteration(int l, const T* x0 = nullptr) {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
     const long long n = 3 * v.nn;
-    hipLaunchKernelGGL(k_randn<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, v.used.p, 0ULL, d.x.p);
+    if (x0)
+      CM_HIP(hipMemcpyAsync(d.x.p, x0, n * sizeof(T), hipMemcpyDeviceToDevice, stream));
+    else
+      hipLaunchKernelGGL(k_randn<T>, grid1(v.nn), dim3(256), 0, stream, v.nn, v.used.p, 0ULL, d.x.p);
     DevBuf<double> s(4);
     for (int it = 0; it < 12; ++it) {
       apply<T>(l, d.x.p, d.t.p, MODE_APPLY, nullptr);
@@ -1494,6 +1499,7 @@ struct System {
     const double lam = std::sqrt(hs[0]) / std::sqrt(hs[1]);
     d.lmax = 1.1 * lam;
     d.lmin = d.lmax / 30.0;
+    return lam;
   }
 
   // coefficient c of the first smoothing step from zero, x = c Dinv b
@@ -2200,6 +2206,67 @@ int cm_sys_vcycle(cm_sys* h, int prec, const void* b, void* x) {
   OwnStream os(s);
   if (prec == 0) s.vcycle_top<double>(cp<double>(b), mp<double>(x));
   else s.vcycle_top<float>(cp<float>(b), mp<float>(x));
+  CM_API_END
+}
+
+int cm_sys_level_data(cm_sys* h, int level, int prec, int what, void* out) {
+  CM_API_BEGIN
+  need_level(h, level);
+  CM_CHECK(out != nullptr, "out is null");
+  System& s = h->s;
+  if (what == 2) {
+    const cm::Level& v = s.lv(level);
+    std::memcpy(out, v.used_h.data(), v.used_h.size());
+    return 0;
+  }
+  CM_CHECK(level + 1 < s.nlev(), "the coarsest level has no smoother");
+  if (prec == 0) s.prepare_prec<double>(level, true);
+  else s.prepare_prec<float>(level, true);
+  cm::Level& v = s.lv(level);
+  if (what == 0) {
+    double* o = static_cast<double*>(out);
+    o[0] = prec == 0 ? v.d64.lmax : v.d32.lmax;
+    o[1] = prec == 0 ? v.d64.lmin : v.d32.lmin;
+  } else if (what == 1) {
+    const size_t bytes = 6 * (size_t)v.nn * (prec == 0 ? sizeof(double) : sizeof(float));
+    const void* src = prec == 0 ? (const void*)v.d64.Dinv.p : (const void*)v.d32.Dinv.p;
+    CM_HIP(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToDevice, s.stream));
+    CM_HIP(hipStreamSynchronize(s.stream));
+  } else {
+    CM_CHECK(false, "what must be 0 (Chebyshev bounds), 1 (block-Jacobi inverses) or 2 (used nodes)");
+  }
+  CM_API_END
+}
+
+int cm_sys_power(cm_sys* h, int level, int prec, const void* x0, double* lam) {
+  CM_API_BEGIN
+  need_level(h, level);
+  CM_CHECK(x0 != nullptr && lam != nullptr, "x0 and lam must not be null");
+  System& s = h->s;
+  CM_CHECK(level + 1 < s.nlev(), "the coarsest level has no smoother");
+  if (prec == 0) {
+    s.prepare_prec<double>(level, true);
+    *lam = s.power_iteration<double>(level, cp<double>(x0));
+  } else {
+    s.prepare_prec<float>(level, true);
+    *lam = s.power_iteration<float>(level, cp<float>(x0));
+  }
+  s.drop_graphs();   // the captured V-cycle holds the old Chebyshev coefficients
+  CM_API_END
+}
+
+int cm_sys_smooth(cm_sys* h, int level, int prec, const void* b, void* x, int zero_start) {
+  CM_API_BEGIN
+  need_level(h, level);
+  System& s = h->s;
+  CM_CHECK(level + 1 < s.nlev(), "the coarsest level has no smoother");
+  if (prec == 0) {
+    s.prepare_prec<double>(level, true);
+    s.smooth<double>(level, cp<double>(b), mp<double>(x), zero_start != 0, s.opt.npre);
+  } else {
+    s.prepare_prec<float>(level, true);
+    s.smooth<float>(level, cp<float>(b), mp<float>(x), zero_start != 0, s.opt.npre);
+  }
   CM_API_END
 }
 

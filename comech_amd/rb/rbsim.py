@@ -92,11 +92,13 @@ def sample_nodes(o, grid_ml, Urel, device=None):
     fe = [t(np.asarray(e, np.float64), torch.float64) for e in (g.xe, g.ye, g.ze)]
     c = _dev(cn, torch.int64, dev)
     U = _dev(Urel, torch.float64, dev)
-    work = torch.empty((2, len(kq) * g.nx * g.ny), dtype=torch.int32, device=dev)
     mx, my, mz = len(grid_ml.xe) - 1, len(grid_ml.ye) - 1, len(grid_ml.ze) - 1
     out = torch.empty(((mx + 1) * (my + 1) * (mz + 1), 3), dtype=torch.float32, device=dev)
     vp = lambda x: ctypes.c_void_p(x.data_ptr())
-    _lib.check(lib.cm_sample_nodes(ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream), mx, my, mz,
-                                   *[vp(x) for x in me], *[vp(x) for x in ce], len(kq), vp(kqd), vp(qzd), g.nx, g.ny,
-                                   g.nz, *[vp(x) for x in fe], vp(a), vp(c), vp(U), vp(work[0]), vp(work[1]), vp(out)))
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    args = (mx, my, mz, *[vp(x) for x in me], *[vp(x) for x in ce], len(kq), vp(kqd), vp(qzd), g.nx, g.ny, g.nz,
+            *[vp(x) for x in fe], vp(a), vp(c), vp(U))
+    with torch.cuda.device(dev):
+        work = torch.empty((2, len(kq) * g.nx * g.ny), dtype=torch.int32, device=dev)
+        _lib.check(lib.cm_sample_nodes(st, *args, vp(work[0]), vp(work[1]), vp(out)))
     return out.cpu().numpy()

@@ -250,9 +250,14 @@ def batch_sizes(n_feet=64, seed=3000):
     return [tuple(int(2 * round(v / 2)) for v in rng.uniform(lo, hi)) for _ in range(n_feet)]
 
 
-def batch_case(foot, action, n_feet=64, h=0.5):
+def batch_case(foot, action, n_feet=64, h=0.5, scale=1):
+    """Foot ``foot`` of the batched set under ``action``.  ``scale`` > 1 draws the same foot on
+    coarser voxels (each grid dimension / scale, rounded to even, voxel size h * scale): the same
+    anatomy at a size the FP64 oracle solves in seconds."""
     dims = batch_sizes(n_feet)[foot]
-    return hindfoot_raster(dims=dims, h=h, seed=3000 + foot, action=action, angle_seed=3500 + foot)
+    if scale != 1:
+        dims = tuple(max(4, int(2 * round(d / scale / 2))) for d in dims)
+    return hindfoot_raster(dims=dims, h=h * scale, seed=3000 + foot, action=action, angle_seed=3500 + foot)
 
 
 def hindfoot_large(dims=(384, 256, 256), h=0.25, seed=5000):

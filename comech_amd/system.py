@@ -201,6 +201,69 @@ class VoxelSystem:
                    t_solve_ms=info.t_solve_ms, t_setup_ms=info.t_setup_ms, mode=mode, device=str(self.device))
         return x, out
 
+    def bounds(self, level, dtype=torch.float64):
+        """Chebyshev bounds (lmax, lmin) of one level (fe/gmg.py:57-58)."""
+        out = np.zeros(2)
+        check(self.lib.cm_sys_level_data(self.h, int(level), PREC[dtype], 0, out.ctypes.data_as(ctypes.c_void_p)))
+        return float(out[0]), float(out[1])
+
+    def block_inverses(self, level, dtype=torch.float64):
+        """Block-Jacobi inverses of one level as (nodes, 3, 3) (fe/gmg.py:29-44)."""
+        nn = self.nn if level == 0 else self.levels()[level]["nodes"]
+        d = torch.empty(6 * nn, dtype=dtype, device=self.device)
+        check(self.lib.cm_sys_level_data(self.h, int(level), PREC[dtype], 1, ctypes.c_void_p(d.data_ptr())))
+        v = d.view(nn, 6).double().cpu().numpy()
+        idx = np.array([[0, 1, 2], [1, 3, 4], [2, 4, 5]])
+        return v[:, idx]
+
+    def power(self, level, x0):
+        """12 power steps of Dinv A from x0 (device vector, the level's layout): sets the level's
+        Chebyshev bounds to 1.1 lam and lmax / 30 and returns lam (fe/gmg.py:51-58)."""
+        nn = self.nn if level == 0 else self.levels()[level]["nodes"]
+        self._ck(x0, x0.dtype, 3 * nn)
+        lam = ctypes.c_double()
+        check(self.lib.cm_sys_power(self.h, int(level), PREC[x0.dtype], ctypes.c_void_p(x0.data_ptr()),
+                                    ctypes.byref(lam)))
+        return lam.value
+
+    def reference_bounds(self, level0_nodes=None, dtype=torch.float64):
+        """Chebyshev bounds of every smoothed level from the reference's start vector: the Level
+        constructor of fe/gmg.py:51 draws numpy.random.default_rng(0).standard_normal(n) over the
+        level's compact DOFs (level 0: node ``level0_nodes[i]`` holds compact DOFs 3i..3i+2, default
+        the used nodes in ascending order; coarser levels: their used nodes in ascending order).
+        Returns the estimates lam per level."""
+        lv = self.levels()
+        lams = []
+        for l in range(len(lv) - 1):
+            nn = self.nn if l == 0 else lv[l]["nodes"]
+            nodes = level0_nodes if (l == 0 and level0_nodes is not None) else np.flatnonzero(self.used_mask(l))
+            nodes = np.asarray(nodes, np.int64)
+            v = np.random.default_rng(0).standard_normal(3 * len(nodes))
+            x = np.zeros(3 * nn)
+            x[(3 * nodes[:, None] + np.arange(3)).ravel()] = v
+            lams.append(self.power(l, torch.tensor(x, dtype=dtype, device=self.device)))
+        return lams
+
+    def used_mask(self, level):
+        """Nodes of one level that carry unknowns (the reference's kept DOFs / 3)."""
+        nn = self.nn if level == 0 else self.levels()[level]["nodes"]
+        m = np.zeros(nn, np.uint8)
+        check(self.lib.cm_sys_level_data(self.h, int(level), 0, 2, m.ctypes.data_as(ctypes.c_void_p)))
+        return m.astype(bool)
+
+    def smooth(self, b, x=None, level=0, zero_start=False):
+        """One call of the level's smoother in place (fe/gmg.py:61-76): x = smooth(x, b); with
+        zero_start, x = smooth(0, b).  Returns x."""
+        nn = self.nn if level == 0 else self.levels()[level]["nodes"]
+        self._ck(b, b.dtype, 3 * nn)
+        if x is None:
+            x = torch.zeros_like(b)
+            zero_start = True
+        self._ck(x, b.dtype, 3 * nn)
+        check(self.lib.cm_sys_smooth(self.h, int(level), PREC[b.dtype], ctypes.c_void_p(b.data_ptr()),
+                                     ctypes.c_void_p(x.data_ptr()), int(bool(zero_start))))
+        return x
+
     def time_apply(self, u, y, nrep=20, structured_only=False, level=0):
         ms = ctypes.c_double()
         check(self.lib.cm_sys_time_apply(self.h, level, PREC[u.dtype], ctypes.c_void_p(u.data_ptr()),

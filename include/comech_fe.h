@@ -121,6 +121,26 @@ int cm_sys_apply(cm_sys* sys, int level, int prec, const void* u, void* y);
 /* x = V-cycle(b) on level 0 (GPUMG.vcycle, rb/gpusolve.py:163-171). */
 int cm_sys_vcycle(cm_sys* sys, int prec, const void* b, void* x);
 
+/* Multigrid level internals (fe/gmg.py:29-76; tests and diagnostics).  Levels below the
+ * coarsest only.  what 0: the Chebyshev bounds (lmax, lmin) of the level in precision prec, 2
+ * host doubles at out.  what 1: the block-Jacobi inverses of fe/gmg.py:29-44 (blocks with
+ * |det| < 1e-30 -> I), 6 values per level node (xx, xy, xz, yy, yz, zz of the symmetric 3x3
+ * inverse), copied to device memory at out in precision prec.  Level 0 with split-copy smoother
+ * blocks (cm_sys_set_copy_blocks) holds zero blocks for the group members.  what 2 (any level):
+ * the level's used-node mask, one host byte per node (the kept DOFs of fe/gmg.py:95-96, / 3). */
+int cm_sys_level_data(cm_sys* sys, int level, int prec, int what, void* out);
+
+/* Chebyshev bounds of one level from 12 power steps of Dinv A started at x0 (device, the level's
+ * node layout, precision prec): the Level constructor of fe/gmg.py:51-58 (there x0 is
+ * numpy.random.default_rng(0).standard_normal on the level's compact DOFs).  lmax = 1.1 lam and
+ * lmin = lmax / 30 replace the level's bounds; *lam receives the estimate. */
+int cm_sys_power(cm_sys* sys, int level, int prec, const void* x0, double* lam);
+
+/* One call of the level's smoother in place: Level.smooth(x, b) of fe/gmg.py:61-76 (Chebyshev)
+ * or npre damped block-Jacobi sweeps x += omega Dinv (b - A x).  zero_start = 1: x = smooth(0, b)
+ * (x is not read).  b, x: device, the level's node layout, precision prec. */
+int cm_sys_smooth(cm_sys* sys, int level, int prec, const void* b, void* x, int zero_start);
+
 /* PCG to ||b - A x|| <= tol ||b|| (rb/gpusolve.py:206-262, fe/gmg.py:120-137).  b, x: FP64
  * device, level-0 layout; x holds the initial guess (x0) on entry.
  * mode 0: FP64 PCG, FP64 V-cycle.  mode 1: FP64 PCG, FP32 V-cycle (mg_dtype="float32" of

@@ -1,0 +1,192 @@
+"""Solve parity of the benchmark configurations against the FP64 oracle (the reference CPU solver:
+assembled CSR, Galerkin GMG-PCG with the Chebyshev / block-Jacobi smoother, SuperLU coarse
+solve, rtol 1e-12) on the same seeded inputs.
+
+Bars (the solve path's): displacement relative L2 <= 1e-6, von Mises relative L2 <= 1e-5, the
+same unused (NaN) nodes, ground reactions relative L2 <= 1e-6.  Independently of the oracle's own
+iterate, every GPU answer is checked in the oracle's assembled system: ||K u - f|| / ||f||,
+evaluated in extended precision, within the requested tolerance (1 % slack for the FP64
+rounding of the GPU's own confirmation).
+
+Sizes: configuration 1 as stated (32^3, 3 levels); configuration 2's raster under the 8 surgical
+actions at 40x24x24 voxels of 2 mm; configuration 3's first three feet at a quarter of their
+grid dimensions (4x the voxel size, same seeds) solved batched, 4 in flight; configuration 5's
+anatomy (4-voxel shells at 0.25 mm, i.e. 1-voxel shells at 1 mm) at 96x64x64 voxels against the
+oracle, and at its full 384x256x256 size through properties (FP64 true residual in every mode,
+the mixed forms' von Mises against the plain FP64 answer).  The oracle's solves run on a pool of
+host processes."""
+import numpy as np
+import pytest
+import torch
+
+from comech_amd import synth
+from comech_amd.fe import solver as FS
+from comech_amd.rb import splitfe as SF
+from tests import cases as C
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+U_TOL, VM_TOL = 1e-6, 1e-5
+
+
+def rel(a, b):
+    a, b = np.nan_to_num(np.asarray(a, float)).ravel(), np.nan_to_num(np.asarray(b, float)).ravel()
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def oracle_residual(ans, U):
+    """||K u - f|| / ||f|| in the oracle's assembled matrix, evaluated in extended precision (x87
+    long double, 64-bit significand) so that the FP64 rounding of the product itself -- large
+    where stiff springs (fusion, k = 1e6) cancel in a row -- does not enter the check."""
+    u = np.nan_to_num(np.asarray(U))[ans["used"]].ravel().astype(np.longdouble)
+    K = ans["K"]
+    rows = np.add.reduceat(K.data.astype(np.longdouble) * u[K.indices], K.indptr[:-1])
+    rows[np.diff(K.indptr) == 0] = 0
+    f = ans["f"].astype(np.longdouble)
+    r = rows - f
+    return float(np.sqrt((r * r).sum() / (f * f).sum()))
+
+
+def check_cn(c, ans, tol, label, **opts):
+    U, info, R = SF.solve_cn(c["grid"], c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"], c["loads"],
+                             extra=c["extra_cb"], tol=tol, maxiter=3000, device=DEV, **opts)
+    act = np.flatnonzero(c["active"])
+    _, vm, _ = SF.cell_stress_cn(c["grid"], c["lam"], c["mu"], U, act, c["cn"], device=DEV)
+    eu, ev, er = rel(U, ans["U"]), rel(vm, ans["vm"]), rel(R, ans["R"])
+    rr = oracle_residual(ans, U)
+    print(f"{label}: iters {info['iters']} (oracle {ans['iters']}) U {eu:.1e} VM {ev:.1e} R {er:.1e} "
+          f"residual in oracle K {rr:.1e}", flush=True)
+    assert info["flag"] == 0, (label, info)
+    assert np.array_equal(np.isnan(U), np.isnan(ans["U"])), label
+    assert eu <= U_TOL and ev <= VM_TOL and er <= U_TOL, label
+    assert rr <= tol * 1.01, label
+    return info
+
+
+# ---------------------------------------------------------------- configuration 1 (32^3)
+@pytest.fixture(scope="module")
+def cube32():
+    c = synth.cube_inclusions(32)
+    ans = C.oracle_answers([("cube", dict(n=32), dict(smoother="chebyshev"))], workers=1)[0]
+    return c, ans
+
+
+@pytest.mark.parametrize("mode", ["fp64", "ref", "mixed"])
+@pytest.mark.parametrize("smoother", ["jacobi", "chebyshev"])
+def test_config1_cube32(cube32, smoother, mode):
+    """32^3 at 1 mm, 6 inclusions, bottom held by k = 1e6 per area, 1 MPa on top; 3 levels
+    (the reference rule gives 107,811 -> 14,739 -> 2,187 DOFs), damped Jacobi omega 0.6 with 2+2
+    sweeps as the configuration states, and the reference's Chebyshev; rtol 1e-8."""
+    c, ans = cube32
+    tol = 1e-8
+    U, info = FS.solve(c["grid"], c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="gmg", tol=tol,
+                       mode=mode, smoother=smoother, omega=0.6, npre=2, npost=2, levels=3, device=DEV)
+    _, vm, _ = FS.cell_stress(c["grid"], c["lam"], c["mu"], U, np.arange(c["grid"].ncell), device=DEV)
+    eu, ev = rel(U, ans["U"]), rel(vm, ans["vm"])
+    rr = oracle_residual(ans, U)
+    print(f"cube32 {smoother} {mode}: iters {info['iters']} levels {info['levels']} U {eu:.1e} VM {ev:.1e} "
+          f"residual in oracle K {rr:.1e}", flush=True)
+    assert info["flag"] == 0 and info["levels"] == 3
+    assert eu <= U_TOL and ev <= VM_TOL and rr <= tol * 1.01
+    assert not np.isnan(U).any()
+
+
+# ---------------------------------------------------------------- configuration 2: 8 actions
+@pytest.fixture(scope="module")
+def actions40():
+    specs = [("hindfoot", dict(dims=(40, 24, 24), h=2.0, action=a), {}) for a in synth.ACTIONS]
+    return C.oracle_answers(specs)
+
+
+@pytest.mark.parametrize("copy_blocks", [False, True])
+@pytest.mark.parametrize("k", range(len(synth.ACTIONS)))
+def test_config2_actions(actions40, k, copy_blocks):
+    """Every surgical action of the hindfoot raster (osteotomy and interposition change the split
+    connectivity, fusion raises the interface springs to 1e6) in mode ref (FP64 PCG, FP32
+    V-cycle, the reference GPU solver's placement), with the reference's per-node smoother blocks
+    and with the split-copy group blocks; rtol 1e-9."""
+    a = synth.ACTIONS[k]
+    c = synth.hindfoot_raster(dims=(40, 24, 24), h=2.0, action=a)
+    check_cn(c, actions40[k], 1e-9, f"40x24x24 {a} copy_blocks={copy_blocks}", mode="ref", copy_blocks=copy_blocks)
+
+
+@pytest.mark.parametrize("mode", ["fp64", "mixed"])
+@pytest.mark.parametrize("a", ["osteotomy", "fusion"])
+def test_config2_actions_other_modes(actions40, a, mode):
+    k = synth.ACTIONS.index(a)
+    c = synth.hindfoot_raster(dims=(40, 24, 24), h=2.0, action=a)
+    check_cn(c, actions40[k], 1e-9, f"40x24x24 {a} {mode}", mode=mode)
+
+
+# ---------------------------------------------------------------- configuration 3: batched
+def test_config3_uneven_batch_against_oracle():
+    """The first three feet of the 64-foot set (uneven grids, each its own seed) x 8 actions at a
+    quarter of their grid dimensions: 24 systems solved batched, 4 in flight on one GPU, rtol
+    1e-9; every batched answer against the oracle."""
+    from comech_amd.batch import run_batch
+    feet = [0, 1, 2]
+    specs = [("batch", dict(foot=f, action=a, scale=4), {}) for f in feet for a in synth.ACTIONS]
+    answers = C.oracle_answers(specs)
+    cases = [C.make_case(*s[:2]) for s in specs]
+    assert len({c["grid"].shape for c in cases}) == len(feet)        # uneven grids
+
+    def job(c):
+        def run():
+            U, info, R = SF.solve_cn(c["grid"], c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"],
+                                     c["loads"], extra=c["extra_cb"], tol=1e-9, maxiter=3000, device=DEV)
+            act = np.flatnonzero(c["active"])
+            _, vm, _ = SF.cell_stress_cn(c["grid"], c["lam"], c["mu"], U, act, c["cn"], device=DEV)
+            return U, vm, R, info
+        return run
+    res, wall = run_batch([job(c) for c in cases], concurrency=4, device=DEV)
+    for (U, vm, R, info), ans, spec, c in zip(res, answers, specs, cases):
+        label = f"foot {spec[1]['foot']} {c['grid'].shape} {spec[1]['action']}"
+        eu, ev, er = rel(U, ans["U"]), rel(vm, ans["vm"]), rel(R, ans["R"])
+        rr = oracle_residual(ans, U)
+        print(f"{label}: iters {info['iters']} U {eu:.1e} VM {ev:.1e} R {er:.1e} residual {rr:.1e}", flush=True)
+        assert info["flag"] == 0 and np.array_equal(np.isnan(U), np.isnan(ans["U"])), label
+        assert eu <= U_TOL and ev <= VM_TOL and er <= U_TOL and rr <= 1e-9 * 1.01, label
+
+
+# ---------------------------------------------------------------- configuration 5
+def test_config5_anatomy_small_against_oracle():
+    """Configuration 5's anatomy (cortical shells of 1 mm) at 96x64x64 voxels of 1 mm: modes fp64,
+    ref and mixed against the oracle, rtol 1e-9."""
+    spec = ("hindfoot", dict(dims=(96, 64, 64), h=1.0, shell=1, seed=5000), {})
+    ans = C.oracle_answers([spec], workers=1)[0]
+    c = C.make_case(*spec[:2])
+    for mode in ("fp64", "ref", "mixed"):
+        check_cn(c, ans, 1e-9, f"96x64x64 {mode}", mode=mode)
+
+
+def test_config5_full_size_properties():
+    """384x256x256 voxels at 0.25 mm (25.2 M cells, ~40 M DOFs), 4-voxel cortical shells: every
+    mode reaches an FP64 true residual <= 1e-8, and the mixed forms' von Mises fields equal the
+    plain FP64 answer's to 1e-5 (relative L2), their displacements to 1e-6."""
+    from comech_amd.engine import split_cells
+    from comech_amd.system import VoxelSystem, cell_stress_gpu
+    c = synth.hindfoot_large()
+    extra, terms, f = synth.solve_inputs_hindfoot(c)
+    irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
+    s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
+                    mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c, irr_nodes=irr_n,
+                    extra=extra, rank1=terms, device=DEV)
+    b = torch.from_numpy(f).to(DEV)
+    bn = torch.linalg.vector_norm(b)
+    cells = np.flatnonzero(c["active"])
+    nodes8 = c["cn"][cells]
+    out = {}
+    for mode in ("fp64", "ref", "mixed"):
+        x, info = s.solve(b, tol=1e-8, maxiter=3000, mode=mode)
+        true = float(torch.linalg.vector_norm(b - s.apply(x)) / bn)
+        print(f"384x256x256 {mode}: iters {info['iters']} true residual {true:.2e} "
+              f"t_solve {info['t_solve_ms']:.0f} ms", flush=True)
+        assert info["flag"] == 0 and true <= 1e-8 * (1 + 1e-9), (mode, info, true)
+        out[mode] = x
+    vm = {m: cell_stress_gpu(c["grid"], c["lam"], c["mu"], out[m], cells, nodes8=nodes8)[1] for m in out}
+    for m in ("ref", "mixed"):
+        ev = float(np.linalg.norm(vm[m] - vm["fp64"]) / np.linalg.norm(vm["fp64"]))
+        eu = float(torch.linalg.vector_norm(out[m] - out["fp64"]) / torch.linalg.vector_norm(out["fp64"]))
+        print(f"384x256x256 {m} vs fp64: U {eu:.1e} VM {ev:.1e}", flush=True)
+        assert ev <= VM_TOL and eu <= U_TOL, m
+    s.close()

@@ -265,3 +265,21 @@ def test_copy_group_blocks_same_solution_fewer_iterations():
         x1, it1 = out[True, mode]
         assert np.linalg.norm(x1 - x0) <= 1e-6 * np.linalg.norm(x0), mode
         assert it1 < it0, (mode, it0, it1)
+
+
+def test_solve_with_penalties_matches_oracle():
+    """Rank-one penalties of fe/solver.py:222-232 (the parametric engine's weighted-mean
+    constraints: k w w^T, w normalised to sum 1) inside the GPU solve: same answer as the oracle's
+    solve of the same system, and a visibly different one from the solve without them."""
+    c = C.cube(12)
+    g, og = c["grid"], C.oracle_grid(c)
+    top = np.flatnonzero(g.node_xyz[:, 2] == g.ze[-1])
+    pen = [(top, np.ones(len(top)), 0, 1e4), (top[::3], np.linspace(1.0, 2.0, len(top[::3])), 1, 5e3)]
+    U, info = FS.solve(g, c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="gmg", tol=1e-10,
+                       penalties=pen, device=DEV)
+    Uo, _ = O.solve(og, c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="gmg", tol=1e-12,
+                    penalties=pen)
+    U0, _ = FS.solve(g, c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="gmg", tol=1e-10,
+                     device=DEV)
+    assert info["flag"] == 0 and rel(U, Uo) <= U_TOL
+    assert rel(U0, Uo) > 1e-4

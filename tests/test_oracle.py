@@ -524,3 +524,99 @@ def test_spring_end_bones_match_kdtree():
         assert ea == min(trees, key=lambda nm: trees[nm].query(s["a"])[0])
         assert eb == min(trees, key=lambda nm: trees[nm].query(s["b"])[0])
     assert got[-1] == ("a", "c")
+
+
+def _free_body(seed=0, dims=(4, 3, 5)):
+    # uneven spacings exact at 6 decimals: the element families of fe/solver.py:112 are built for
+    # the cell sizes rounded to 6 decimals, so other sizes make rigid motions cost ~1e-9 of |K|
+    rng = np.random.default_rng(seed)
+    sizes = np.array([0.5, 0.75, 1.0, 1.25, 2.0])
+    g = O.Grid(*[np.cumsum(np.r_[0, rng.choice(sizes, n)]) for n in dims])
+    lam, mu = O.lame(rng.uniform(100, 2000, g.ncell), rng.uniform(0.1, 0.45, g.ncell))
+    return g, lam, mu
+
+
+def test_rigid_modes_span_the_kernel_of_a_free_body():
+    """fe/solver.py:177-199: on a body without springs the 3 translations and 3 rotations of a
+    group are exact zero-energy modes, K Z = 0 (a wrong sign, a swapped component or an
+    uncentred/misplaced rotation axis gives K Z != 0), and the 6 columns are independent."""
+    g, lam, mu = _free_body()
+    K, nmap, used, _, _ = O.assemble(g, lam, mu, np.ones(g.ncell, bool))
+    Z = O.rigid_modes(g.node_xyz, nmap, K.shape[0], [np.arange(g.nnode)]).toarray()
+    assert Z.shape == (K.shape[0], 6)
+    assert np.abs(K @ Z).max() <= 1e-11 * abs(K).max() * np.abs(Z).max()
+    assert np.linalg.matrix_rank(Z) == 6
+    # a genuine deformation (stretch along x) is not in the span of the modes
+    x = g.node_xyz[used]
+    stretch = np.zeros((len(used), 3)); stretch[:, 0] = x[:, 0]
+    assert np.abs(K @ stretch.ravel()).max() > 1e-3 * abs(K).max()
+    # groups of fewer than 4 used nodes give no columns (fe/solver.py:186-187)
+    assert O.rigid_modes(g.node_xyz, nmap, K.shape[0], [np.arange(3)]).shape[1] == 0
+
+
+def test_rigid_modes_per_bone_of_split_bodies():
+    """Two bones meeting at a plane with no joint cells: split connectivity gives each bone its own
+    nodes, so each bone's 6 modes (rb/splitfe.py:105-120 on orig coordinates) are zero-energy
+    modes of the assembled split system -- 12 in all -- while the modes of the bones taken as one
+    body are not."""
+    g = O.Grid(np.arange(7.0), np.arange(4.0), np.arange(4.0))
+    active = np.ones(g.ncell, bool)
+    part = np.where(g.cI < 3, 1, 2)
+    share = np.zeros((g.ncell, 3), np.float32)
+    cn, orig, nsplit = O.split_connectivity_fast(g, active, part, share)
+    assert nsplit == 16                            # the 4 x 4 nodes of the plane x = 3
+    lam, mu = O.lame(np.full(g.ncell, 700.0), np.full(g.ncell, 0.3))
+    K, nmap, used, cells, _ = O.assemble(g, lam, mu, active, KLf=g.KL_sri, cn=cn)
+    xyz = g.node_xyz[orig]
+    groups = [np.unique(cn[part == b]) for b in (1, 2)]
+    Z = O.rigid_modes(xyz, nmap, K.shape[0], groups).toarray()
+    assert Z.shape[1] == 12 and np.linalg.matrix_rank(Z) == 12
+    assert np.abs(K @ Z).max() <= 1e-11 * abs(K).max()
+    Zb = O.rigid_modes(xyz, nmap, K.shape[0], [np.arange(len(orig))]).toarray()
+    assert np.abs(K @ Zb).max() <= 1e-11 * abs(K).max()      # one body: rigid too (both move together)
+    # a relative rotation of the two bones is free (no joint), a relative stretch inside one is not
+    assert np.abs(K @ (Z[:, 5] - Z[:, 11])).max() <= 1e-11 * abs(K).max()
+
+
+def test_element_forces_scatter_to_the_assembled_product():
+    """rb/splitfe.py:178-193: the per-cell forces K_e u_e, summed into their nodes, are K u of the
+    assembled matrix (assemble_cn, pinned by the brute-force element loop) -- a transposed K_e, a
+    wrong corner order or a dropped lambda term breaks the sum; SRI and full integration."""
+    from comech_amd import synth
+    c = synth.hindfoot_raster(dims=(12, 8, 8), h=3.0)
+    og = O.Grid(c["grid"].xe, c["grid"].ye, c["grid"].ze)
+    rng = np.random.default_rng(4)
+    for sri in (True, False):
+        K, nmap, used, cells, _ = O.assemble(og, c["lam"], c["mu"], c["active"], KLf=og.KL_sri if sri else og.KL,
+                                             cn=c["cn"])
+        U = np.full((len(c["orig"]), 3), np.nan)
+        U[used] = rng.standard_normal((len(used), 3))
+        F = O.element_forces_cn(og, c["lam"], c["mu"], U, cells, c["cn"], sri=sri)
+        node_f = np.zeros((len(c["orig"]), 3))
+        np.add.at(node_f, c["cn"][cells].ravel(), F.reshape(-1, 3))
+        Ku = (K @ U[used].ravel()).reshape(-1, 3)
+        assert np.abs(node_f[used] - Ku).max() <= 1e-12 * np.abs(Ku).max()
+        # an inactive cell contributes nothing; the NaN of unused nodes counts as zero
+        assert not np.isnan(F).any()
+
+
+def test_penalties_are_rank_one_with_unit_weight_sum():
+    """fe/solver.py:222-232: each penalty adds k w w^T with w the given weights on component comp of
+    the nodes, normalised to sum 1 (duplicate nodes add their weights)."""
+    g, lam, mu = _free_body(1)
+    active = np.ones(g.ncell, bool)
+    nodes = np.array([3, 7, 7, 11, 20])
+    weights = np.array([1.0, 2.0, 0.5, 1.5, 3.0])
+    k = 250.0
+    _, i0 = O.solve(g, lam, mu, active, [], [], method="none")
+    _, i1 = O.solve(g, lam, mu, active, [], [], method="none", penalties=[(nodes, weights, 2, k)])
+    D = (i1["K"] - i0["K"]).toarray()
+    w = np.zeros(D.shape[0])
+    for n_, wt in zip(nodes, weights):
+        w[3 * i0["nmap"][n_] + 2] += wt
+    w /= weights.sum()
+    assert abs(w.sum() - 1.0) < 1e-15
+    assert np.abs(D - k * np.outer(w, w)).max() <= 1e-13 * k
+    # the penalty pulls the weighted mean of u_z: a rigid z translation t costs k t^2
+    t = np.zeros(D.shape[0]); t[2::3] = 1.0
+    assert abs(t @ D @ t - k) <= 1e-12 * k

@@ -68,6 +68,8 @@ SIGNATURES = {
     "cm_sample_fe": [c_void_p] + [c_int] * 7 + [c_void_p] * 9 + [c_void_p] * 5 + [c_void_p] * 4,
     "cm_sample_nodes": [c_void_p] + [c_int] * 3 + [c_void_p] * 6 + [c_int] + [c_void_p] * 2 + [c_int] * 3 +
                        [c_void_p] * 9,
+    "cm_sample_nodes_noalloc": [c_void_p] + [c_int] * 3 + [c_void_p] * 6 + [c_int] + [c_void_p] * 2 + [c_int] * 3 +
+                               [c_void_p] * 7,
 }
 _RET = {"cm_last_error": ctypes.c_char_p, "cm_version": c_int}
 

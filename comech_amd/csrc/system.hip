@@ -25,6 +25,17 @@ namespace cm {
 
 static thread_local std::string g_last_error;
 
+// ---------------------------------------------------------------------------------------------
+// Library-held scratch of the allocation-free entry point (cm_sample_nodes_noalloc): the two
+// stage-1 tables of the node resampling, CM_SAMPLE_NODES_MAX_COLUMNS ints each (32 MiB).  Every
+// device the library runs on holds its own copy until the process ends; calls on one device share
+// it.  Host code never uses the array's name as an address: the per-device address comes from
+// hipGetSymbolAddress with that device current, once, at the entry point's first call there.
+static __device__ __align__(256) int scratch[2 * CM_SAMPLE_NODES_MAX_COLUMNS];
+static std::mutex scratch_mu;
+static std::vector<int*> scratch_dev;
+
+static int* scratch_on_current_device();
 
 // ---------------------------------------------------------------------------------------------
 // Per-level, per-precision device data
@@ -1938,6 +1949,22 @@ using cm::System;
     return -1;                                            \
   }
 
+namespace cm {
+static int* scratch_on_current_device() {
+  int dev = 0;
+  CM_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if ((int)scratch_dev.size() <= dev) scratch_dev.resize(dev + 1, nullptr);
+  if (!scratch_dev[dev]) {
+    void* p = nullptr;
+    CM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    CM_CHECK(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+    scratch_dev[dev] = static_cast<int*>(p);
+  }
+  return scratch_dev[dev];
+}
+}  // namespace cm
+
 struct cm_sys {
   System s;
 };
@@ -2577,6 +2604,31 @@ int cm_sample_nodes(void* stream, int mx, int my, int mz, const double* mxe, con
   CM_CHECK(nq >= 1 && nq <= mz + 1, "bad query layers");
   const hipStream_t st = (hipStream_t)stream;
   const long long ncol = (long long)nx * ny;
+  hipLaunchKernelGGL(cm::k_column_nearest, cm::grid1(ncol * nq), dim3(256), 0, st, nx, ny, nz, nq, kq, active, dz2,
+                     kbest);
+  const long long mnode = (long long)(mx + 1) * (my + 1) * (mz + 1);
+  hipLaunchKernelGGL(cm::k_sample_nodes, cm::grid1(mnode), dim3(256), 0, st, mx + 1, my + 1, mz + 1, mxe, mye, mze, ci,
+                     cj, ck, qz, nx, ny, nz, xe, ye, ze, active, dz2, kbest, (const long long*)cn, Urel, out);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
+int cm_sample_nodes_noalloc(void* stream, int mx, int my, int mz, const double* mxe, const double* mye,
+                            const double* mze, const int* ci, const int* cj, const int* ck, int nq, const int* kq,
+                            const int* qz, int nx, int ny, int nz, const double* xe, const double* ye, const double* ze,
+                            const uint8_t* active, const int64_t* cn, const double* Urel, float* out) {
+  CM_API_BEGIN
+  CM_CHECK(mx > 0 && my > 0 && mz > 0 && nx > 0 && ny > 0 && nz > 0, "bad grids");
+  CM_CHECK(nq >= 1 && nq <= mz + 1, "bad query layers");
+  const long long ncol = (long long)nx * ny;
+  CM_CHECK(ncol * nq <= (long long)CM_SAMPLE_NODES_MAX_COLUMNS,
+           "cm_sample_nodes_noalloc: nq * nx * ny = " + std::to_string(ncol * nq) + " exceeds the scratch limit of " +
+               std::to_string((long long)CM_SAMPLE_NODES_MAX_COLUMNS) + " (use cm_sample_nodes with work arrays)");
+  // both stage-1 tables live in this device's copy of the scratch array; k_column_nearest writes
+  // every entry the second kernel reads, so nothing needs clearing between calls
+  int* dz2 = cm::scratch_on_current_device();
+  int* kbest = dz2 + CM_SAMPLE_NODES_MAX_COLUMNS;
+  const hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(cm::k_column_nearest, cm::grid1(ncol * nq), dim3(256), 0, st, nx, ny, nz, nq, kq, active, dz2,
                      kbest);
   const long long mnode = (long long)(mx + 1) * (my + 1) * (mz + 1);

@@ -68,11 +68,12 @@ def sample_fe(o, grid_ml, Urel, sub=2, device=None):
     return vm.cpu().numpy(), u.view(ncm, 3).cpu().numpy(), cov.cpu().numpy()
 
 
-def sample_nodes(o, grid_ml, Urel, device=None):
+def sample_nodes(o, grid_ml, Urel, device=None, noalloc=False):
     """rb/rbsim.py:143-161.  o: dict with the FE grid ``g``, connectivity ``cn`` and ``active``;
     grid_ml: the analysis grid; Urel: (nodes, 3) relative displacement (NaN for unused nodes).
     cn, active and Urel may be numpy arrays or device tensors.  Returns float32 (nnode_ml, 3) in
-    the analysis grid's node order."""
+    the analysis grid's node order.  ``noalloc``: run the library's allocation-free entry point
+    (its work tables in the library's per-device scratch) instead of passing work arrays."""
     dev = require_gpu(device)
     lib = _lib.load()
     g, cn = o["g"], o["cn"]
@@ -99,6 +100,9 @@ def sample_nodes(o, grid_ml, Urel, device=None):
     args = (mx, my, mz, *[vp(x) for x in me], *[vp(x) for x in ce], len(kq), vp(kqd), vp(qzd), g.nx, g.ny, g.nz,
             *[vp(x) for x in fe], vp(a), vp(c), vp(U))
     with torch.cuda.device(dev):
-        work = torch.empty((2, len(kq) * g.nx * g.ny), dtype=torch.int32, device=dev)
-        _lib.check(lib.cm_sample_nodes(st, *args, vp(work[0]), vp(work[1]), vp(out)))
+        if noalloc:
+            _lib.check(lib.cm_sample_nodes_noalloc(st, *args, vp(out)))
+        else:
+            work = torch.empty((2, len(kq) * g.nx * g.ny), dtype=torch.int32, device=dev)
+            _lib.check(lib.cm_sample_nodes(st, *args, vp(work[0]), vp(work[1]), vp(out)))
     return out.cpu().numpy()

@@ -286,6 +286,21 @@ int cm_sample_nodes(void* stream, int mx, int my, int mz, const double* mxe, con
                     int nz, const double* xe, const double* ye, const double* ze, const uint8_t* active,
                     const int64_t* cn, const double* Urel, int* dz2, int* kbest, float* out);
 
+/* Allocation-free variant of cm_sample_nodes.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and result as cm_sample_nodes without the work arrays
+ * dz2 and kbest: their tables live in a scratch array the library holds per device
+ * (2 * CM_SAMPLE_NODES_MAX_COLUMNS ints, 32 MiB, kept until the process ends).  Requires
+ * nq * nx * ny <= CM_SAMPLE_NODES_MAX_COLUMNS, else fails with a message; the current device must
+ * be the one the stream and buffers belong to.  The call allocates nothing (no hipMalloc, hipFree
+ * or allocating library routine).  Calls that overlap on one device share the scratch and race:
+ * serialise them (one stream, or events). */
+#define CM_SAMPLE_NODES_MAX_COLUMNS 4194304
+int cm_sample_nodes_noalloc(void* stream, int mx, int my, int mz, const double* mxe, const double* mye,
+                            const double* mze, const int* ci, const int* cj, const int* ck, int nq, const int* kq,
+                            const int* qz, int nx, int ny, int nz, const double* xe, const double* ye, const double* ze,
+                            const uint8_t* active, const int64_t* cn, const double* Urel, float* out);
+
 #ifdef __cplusplus
 }
 #endif

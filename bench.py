@@ -1,19 +1,23 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X voxel FE solve path (one GPU).
 
-Workload: the synthetic hindfoot raster (160x96x96 voxels at 0.5 mm, ~2.38M active DOFs, split
-nodes with interface springs, 2000 ground springs, 200 ligament springs, 700 N load) under the 8
-surgical actions (baseline, resection, osteotomy, interposition, fusion, load x0.5/x1.5/x2), each
-solved to relative residual 1e-8 with mixed-precision GMG-PCG: FP32 operator, smoothers, transfers
-and V-cycle; FP64 Krylov vectors, alpha/beta reductions and dense coarsest solve; every answer
-checked by an FP64 true residual.  One step = one complete solve (GPU multigrid setup from the
-level-0 description, PCG, check); ``--concurrency`` solves are in flight at once (batched).
+Headline workload: the batched uneven-grid set -- 64 seeded hindfoot-like feet with grids drawn
+uniformly between 96x64x64 and 192x112x112 voxels at 0.5 mm, each under 8 surgical actions
+(baseline, resection, osteotomy, interposition, fusion at k = 1e6, loads x0.5/x1.5/x2), 512 solves
+to relative residual 1e-8, every answer checked by its FP64 true residual.  One step is one foot
+of the set under its 8 actions; ``--steps K`` feet are spread evenly over the 64 (K = 64 is the
+whole set).  A step includes everything a solve needs after its inputs are on the device: the
+multigrid hierarchy built on the GPU from the level-0 description (labels, split lists, spring
+lists; the load actions reuse the baseline's hierarchy, the same operator), PCG and the check.
+``--concurrency`` systems are in flight at once.
 
-Prints ONE JSON line.  ``value`` = solves/s (batched); ``sequential`` = the same solves one at a
-time; ``plain_fp64`` = all-FP64 GMG-PCG batched the same way; ``roofline`` = the FP32 fine-level
-stencil kernel (the matvec) against the 8 TB/s HBM peak; ``cpu_baseline`` = the FP64 CPU oracle
-(assembled CSR, Chebyshev GMG-PCG, SuperLU coarse solve -- the reference algorithm) on a bounded
-sample.
+Prints ONE JSON line.  ``value`` = solves/s in mode ``ref`` (FP64 Krylov vectors, alpha/beta and
+outer product, FP32 V-cycle: operator, smoothers and transfers, FP64 coarsest solve); also
+``mixed`` (FP32 PCG inside FP64 iterative refinement) and ``plain_fp64`` (all-FP64 GMG-PCG) on
+the same systems, the error of every result against the plain FP64 answer, ``roofline`` (the
+FP32 fine-level K*u on the 384x256x256 grid, whose 635 MB working set streams from HBM) and
+``cpu_baseline`` (the FP64 CPU oracle -- assembled CSR, Chebyshev GMG-PCG, SuperLU coarse solve,
+the reference algorithm -- on one foot of the set, one core and a pool of host processes).
 """
 import argparse
 import glob
@@ -26,106 +30,363 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+N_FEET = 64
+# solves that share one system: the load actions have the baseline's operator
+GROUPS = (("baseline", "load0.5", "load1.5", "load2"), ("resection",), ("osteotomy",), ("interposition",),
+          ("fusion",))
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_inputs(dims=(160, 96, 96), h=0.5, shell=2):
-    from comech_amd import synth
-    from comech_amd.engine import split_cells
-    c = synth.hindfoot_raster(dims=dims, h=h, shell=shell)
-    extra, terms, f = synth.solve_inputs_hindfoot(c)
-    irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
-    return c, (extra, terms), f, irr_c, irr_n
+def host_info():
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    ram = None
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal"):
+                ram = round(int(line.split()[1]) / 1024 ** 2, 1)
+    except OSError:
+        pass
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return dict(nproc=os.cpu_count(), cores_available=avail, cpu_model=model, ram_gib=ram)
 
 
-def make_system(c, extra, irr_c, irr_n, kz_cache=None):
-    from comech_amd.system import VoxelSystem
-    s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
-                    mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c, irr_nodes=irr_n,
-                    extra=extra[0], rank1=extra[1], device="cuda:0", smoother="chebyshev")
-    s.autotune()          # structured-block z extents: timed once per level shape, then cached
-    return s
+def pool(n):
+    """Host worker processes (spawned; they import numpy/scipy only and never touch the GPU),
+    single-threaded BLAS as the reference's production setting (exp/bank.py:10)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        os.environ[k] = "1"
+    return ProcessPoolExecutor(max_workers=n, mp_context=mp.get_context("spawn"))
 
 
-def one_solve(c, extra, f_dev, irr_c, irr_n, mode, tol):
+def timed_feet(k):
+    """K feet spread evenly over the 64-foot set (all 64 for K = 64)."""
+    return sorted({int(i * N_FEET // k) % N_FEET for i in range(k)}) if k <= N_FEET else \
+        [i % N_FEET for i in range(k)]
+
+
+def foot_job(descs, bs, fi, grp, mode, tol, dev, ref_x=None, keep=False):
+    """One system (the first action of ``grp``) solved for every action of ``grp`` in turn."""
     import torch
-    s = make_system(c, extra, irr_c, irr_n)
-    x, info = s.solve(f_dev, tol=tol, maxiter=2000, mode=mode)
-    info["ndof"] = s.levels()[0]["ndof"]
-    r = f_dev - s.apply(x)                               # FP64 true residual
-    relres = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(f_dev))
-    s.close()
-    info["true_relres"] = relres
-    return info
+    from comech_amd.batch import build_system
+    from comech_amd.synth import ACTIONS
+    idx = [ACTIONS.index(a) for a in grp]
+
+    def run():
+        s = build_system(descs[idx[0]], device=dev)
+        ndof = s.levels()[0]["ndof"]
+        out = []
+        for k in idx:
+            b = bs[k]
+            x, info = s.solve(b, tol=tol, maxiter=3000, mode=mode)
+            info["true_relres"] = float(torch.linalg.vector_norm(b - s.apply(x)) / torch.linalg.vector_norm(b))
+            info.update(ndof=ndof, foot=fi, action=ACTIONS[k])
+            if ref_x is not None:
+                xr = ref_x[(fi, k)]
+                info["u_rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(x - xr) / torch.linalg.vector_norm(xr))
+            out.append(((fi, k), x if keep else None, info))
+        s.close()
+        return out
+    return run
 
 
-def cpu_baseline(sample_dims=(80, 48, 48)):
-    """Oracle (reference algorithm, FP64, one thread) on a bounded sample of the same anatomy;
-    solves/s extrapolated linearly in the number of cells to the full raster."""
-    os.environ["OMP_NUM_THREADS"] = "1"
-    os.environ["OPENBLAS_NUM_THREADS"] = "1"
+def batch_leg(feet, descs, bs, mode, tol, conc, dev, ref_x=None, keep=False):
+    """All solves of ``feet`` (8 each) batched; returns (infos, xs, wall seconds)."""
+    from comech_amd.batch import run_batch
+    jobs = []
+    for fi in feet:
+        for grp in GROUPS:
+            w = len(grp) * descs[fi][0]["ncell"]
+            jobs.append((w, foot_job(descs[fi], bs[fi], fi, grp, mode, tol, dev, ref_x, keep)))
+    jobs.sort(key=lambda j: -j[0])                # longest first: the tail of the batch stays short
+    res, wall = run_batch([j[1] for j in jobs], concurrency=conc, device=dev)
+    infos, xs = [], {}
+    for out in res:
+        for key, x, info in out:
+            infos.append(info)
+            if keep:
+                xs[key] = x
+    return infos, xs, wall
+
+
+def summarize(infos, wall, extra=None):
+    import numpy as np
+    iters = [i["iters"] for i in infos]
+    d = {"value": round(len(infos) / wall, 3), "unit": "solves/s", "solves": len(infos),
+         "wall_s": round(wall, 3), "iters_mean": round(float(np.mean(iters)), 1), "iters_max": int(max(iters)),
+         "true_relres_max": max(i["true_relres"] for i in infos),
+         "dof_iterations_per_s": round(sum(i["ndof"] * i["iters"] for i in infos) / wall, 1)}
+    if extra:
+        d.update(extra)
+    return d
+
+
+SAMPLE_SCALE = 2     # the CPU sample: one foot of the set at twice the voxel size (1/8 of the cells)
+
+
+def oracle_sample(foot, tol):
+    """CPU worker: the FP64 oracle on one foot of the set (baseline action) drawn at twice the voxel
+    size.  Returns the solve's own wall interval (generation excluded), iterations, displacement
+    and von Mises."""
+    import numpy as np
     from comech_amd import synth
     from oracle import fe_oracle as O
-    c = synth.hindfoot_raster(dims=sample_dims, h=80.0 / sample_dims[0])
-    g = O.Grid(c["grid"].xe, c["grid"].ye, c["grid"].ze)
+    c = synth.batch_case(foot, "baseline", scale=SAMPLE_SCALE)
+    g = c["grid"]
+    og = O.Grid(g.xe, g.ye, g.ze)
     t0 = time.time()
-    U, info, R = O.solve_cn(g, c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"], c["loads"],
-                            extra=c["extra_cb"], tol=1e-8, maxiter=2000)
-    dt = time.time() - t0
-    scale = (160 * 96 * 96) / float(g.ncell)
-    return dict(value=1.0 / (dt * scale), unit="solves/s", cores=1, kind="port",
-                sample=(f"hindfoot anatomy at {sample_dims[0]}x{sample_dims[1]}x{sample_dims[2]} cells "
-                        f"({g.ncell} cells, {info['ndof']} DOFs): one FP64 oracle solve (assembly, Galerkin "
-                        f"hierarchy, Chebyshev GMG-PCG, SuperLU coarse) to rtol 1e-8 in {dt:.2f} s, "
-                        f"{info['iters']} iterations, extrapolated x{scale:.0f} in cell count"))
+    U, info, R = O.solve_cn(og, c["lam"], c["mu"], c["active"], c["cn"], c["orig"], c["fixed"], c["loads"],
+                            extra=c["extra_cb"], tol=tol, maxiter=3000)
+    t1 = time.time()
+    act = np.flatnonzero(c["active"])
+    _, vm, _ = O.cell_stress(og, c["lam"], c["mu"], U, act, cn=c["cn"])
+    return dict(t0=t0, t1=t1, iters=int(info["iters"]), ndof=int(info["ndof"]), U=U, vm=vm, ncell=g.ncell)
 
 
-def pmc_traffic(kernel="k_march"):
-    """HBM bytes per launch of the fine stencil from the committed rocprofv3 --pmc summary."""
+def cpu_baseline(ex, nworkers, sample_foot, tol, cells_per_solve, gpu_answer):
+    """The oracle (reference algorithm, FP64) on one foot of the set: alone on one core (latency),
+    then ``nworkers`` copies at once on the host cores (throughput, the reference's process pool,
+    exp/rb_run_engine_v4.py:123); solves/s extrapolated to the timed set by cell count."""
+    import numpy as np
+    one = ex.submit(oracle_sample, sample_foot, tol).result()
+    dt1 = one["t1"] - one["t0"]
+    many = list(ex.map(oracle_sample, [sample_foot] * nworkers, [tol] * nworkers))
+    t0, t1 = min(m["t0"] for m in many), max(m["t1"] for m in many)
+    scale = cells_per_solve / one["ncell"]
+    pool_rate = nworkers / (t1 - t0) / scale
+    info = host_info()
+    out = {"value": round(pool_rate, 5), "unit": "solves/s", "cores": nworkers, "kind": "port",
+           "sample": (f"foot {sample_foot} of the set at 1 mm voxels ({one['ncell']} cells, {one['ndof']} DOFs, "
+                      f"baseline action; the set is at 0.5 mm): the FP64 oracle (assembled CSR, Galerkin GMG-PCG with Chebyshev/block-Jacobi, SuperLU "
+                      f"coarse solve) to rtol {tol:g}, {one['iters']} iterations; {nworkers} single-threaded processes "
+                      f"solving it at once took {t1 - t0:.1f} s, extrapolated to the timed set's mean of "
+                      f"{cells_per_solve:.0f} cells per solve (x{scale:.2f} in cell count)"),
+           "single_core": {"value": round(1.0 / (dt1 * scale), 5), "unit": "solves/s", "solve_s": round(dt1, 2)}}
+    out.update(info)
+    if gpu_answer is not None:
+        U, vm = gpu_answer
+        Uo = np.nan_to_num(one["U"])
+        out["gpu_vs_oracle"] = {"u_rel_l2": float(np.linalg.norm(np.nan_to_num(U) - Uo) / np.linalg.norm(Uo)),
+                                "vm_rel_l2": float(np.linalg.norm(vm - one["vm"]) / np.linalg.norm(one["vm"])),
+                                "what": "mode ref on the GPU vs the oracle, both to the same rtol, same system"}
+    return out
+
+
+def roofline_case():
+    """CPU worker: level-0 description of the 384x256x256 grid at 0.25 mm (4-voxel shells)."""
+    from comech_amd import synth
+    return synth.level0_description(synth.hindfoot_large())
+
+
+def roofline(desc, dev, nrep=20):
+    """Fine-level kernels on the HBM-resident grid, timed with HIP events on the system's stream:
+    K*u (FP32 and FP64) and one fused Chebyshev smoothing step (FP32)."""
+    import torch
+    from comech_amd.batch import build_system
+    s = build_system(desc, device=dev)
+    nodes = s.grid.nnode
+    ncell = desc["ncell"]
+    u32 = torch.randn(3 * s.nn, dtype=torch.float32, device=dev)
+    y32 = torch.empty_like(u32)
+    ms32 = s.time_apply(u32, y32, nrep=nrep, structured_only=True)
+    ms64 = s.time_apply(u32.double(), y32.double(), nrep=nrep, structured_only=True)
+    mss = s.time_apply(u32, y32, nrep=nrep, smooth_step=True)
+    s.close()
+    alg = 2 * 3 * 4 * nodes + ncell                      # read u, write y (FP32) + one label byte per cell
+    alg64 = 2 * 3 * 8 * nodes + ncell
+    algs = 24 * 4 * nodes + ncell                        # s, rin, Dinv (6), x in; r, d, x out (FP32)
+    gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
+    return dict(nodes=nodes, cells=ncell, kz=None, ms32=ms32, ms64=ms64, mss=mss, alg=alg, alg64=alg64, algs=algs,
+                gbs32=gbs(alg, ms32), gbs64=gbs(alg64, ms64), gbss=gbs(algs, mss))
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary of ``kernel``."""
     for p in sorted(glob.glob(os.path.join(ROOT, "profiles", "pmc_*.json"))):
         try:
             d = json.load(open(p))
         except Exception:
             continue
-        if d.get("kernel", "").startswith(kernel):
+        if d.get("kernel") == kernel:
             return d.get("traffic_bytes")
     return None
 
 
-def action_inputs(actions):
-    """Inputs of the hindfoot raster under each surgical action (same anatomy seed)."""
+def batch_workload(args):
+    import numpy as np
     from comech_amd import synth
-    from comech_amd.engine import split_cells
-    out = []
-    for a in actions:
-        c = synth.hindfoot_raster(dims=(160, 96, 96), h=0.5, action=a)
-        extra, terms, f = synth.solve_inputs_hindfoot(c)
-        irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
-        out.append((c, (extra, terms), f, irr_c, irr_n))
-    return out
+    feet = timed_feet(args.steps)
+    warm = [f for f in reversed(range(N_FEET)) if f not in feet][:args.warmup] or feet[:args.warmup]
+    nw = max(1, min(16, host_info()["cores_available"] or 1))
+    ex = pool(nw)
+    t0 = time.time()
+    fut_large = ex.submit(roofline_case)              # the first submission starts the host workers
+    futs = {f: ex.submit(synth.batch_foot, f) for f in sorted(set(feet) | set(warm))}
+    import torch
+    from comech_amd.batch import build_system, rhs
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    descs = {f: fu.result() for f, fu in futs.items()}
+    bs = {f: [rhs(d, dev) for d in descs[f]] for f in descs}
+    torch.cuda.synchronize()
+    sizes = {f: tuple(len(e) - 1 for e in descs[f][0]["edges"]) for f in descs}
+    log(f"inputs of {len(descs)} feet ready in {time.time() - t0:.1f} s on {nw} host workers")
+    conc = args.concurrency
+    if warm:
+        batch_leg(warm, descs, bs, "ref", args.tol, conc, dev)
+        log("warmup done")
+    plain, plain_x, w_plain = batch_leg(feet, descs, bs, "fp64", args.tol, conc, dev, keep=True)
+    log(f"plain fp64: {len(plain) / w_plain:.2f} solves/s")
+    ref, _, w_ref = batch_leg(feet, descs, bs, "ref", args.tol, conc, dev, ref_x=plain_x)
+    log(f"ref: {len(ref) / w_ref:.2f} solves/s")
+    mixed, _, w_mixed = batch_leg(feet, descs, bs, "mixed", args.tol, conc, dev, ref_x=plain_x)
+    log(f"mixed: {len(mixed) / w_mixed:.2f} solves/s")
+    del plain_x
+    seq, _, w_seq = batch_leg(feet[:1], descs, bs, "ref", args.tol, 1, dev)
+    bad = [dict(foot=i["foot"], action=i["action"], mode=i["mode"], flag=i["flag"], true=i["true_relres"])
+           for i in plain + ref + mixed if i["flag"] or i["true_relres"] > args.tol * 1.0001]
+    for b in bad:
+        log("missed", b)
+    if any(b["flag"] == 0 for b in bad) or any(i["flag"] for i in ref):
+        log(f"bench.py: a solve missed rtol {args.tol}")
+        sys.exit(1)
+    rl = roofline(fut_large.result(), dev)
+    log(f"roofline: K*u FP32 {rl['ms32'] * 1e3:.1f} us, smoothing step {rl['mss'] * 1e3:.1f} us")
+    cells = float(np.mean([descs[f][0]["ncell"] for f in feet]))
+    # the oracle's sample: the smallest foot of the timed set, baseline action; the GPU's answer to
+    # the same system in mode ref for the accuracy against the oracle
+    sample = min(feet, key=lambda f: descs[f][0]["ncell"])
+    cpu = None
+    if not args.no_cpu_baseline:
+        from comech_amd.grid import Grid, corners_of
+        from comech_amd.system import cell_stress_gpu
+        d0 = ex.submit(synth.batch_foot, sample, scale=SAMPLE_SCALE, actions=["baseline"]).result()[0]
+        s = build_system(d0, device=dev)
+        x, _ = s.solve(rhs(d0, dev), tol=args.tol, maxiter=3000, mode="ref")
+        U = x.view(-1, 3).cpu().numpy().copy()
+        U[~s.used_mask(0)] = np.nan
+        s.close()
+        g = Grid(*d0["edges"])
+        lab = np.asarray(d0["labels"])
+        act = np.flatnonzero(lab != 255)
+        nodes8 = corners_of(g, act)
+        pos = np.searchsorted(act, d0["irr_cells"])
+        nodes8[pos] = d0["irr_nodes"]
+        lam = np.where(lab != 255, np.asarray(d0["lam_tab"])[np.minimum(lab, len(d0["lam_tab"]) - 1)], 0.0)
+        mu = np.where(lab != 255, np.asarray(d0["mu_tab"])[np.minimum(lab, len(d0["mu_tab"]) - 1)], 0.0)
+        vm = cell_stress_gpu(g, lam, mu, x, act, nodes8=nodes8, device=dev)[1]
+        cpu = cpu_baseline(ex, nw, sample, args.tol, cells, (U, vm))
+        log(f"cpu baseline: {cpu['value']} solves/s on {nw} cores, one core {cpu['single_core']['value']}")
+    ex.shutdown()
+    nsolve = len(ref)
+    err = lambda infos: [round(i["u_rel_l2_vs_fp64"], 15) for i in infos]
+    kname = "k_march<float, LabelMat<float,SRI,ISO>, EPI_APPLY> (fine-level K*u)"
+    line = {
+        "metric": "solves/s to rtol 1e-8 (batched) + matvec HBM GB/s; learned/mixed vs plain",
+        "value": round(nsolve / w_ref, 4),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * w_ref / len(feet), 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": (f"batched uneven grids: {len(feet)} of the 64 seeded feet (grids 96x64x64 to 192x112x112 "
+                                f"voxels at 0.5 mm, each its own seed) x 8 surgical actions = {nsolve} solves to rtol "
+                                f"{args.tol:g}, each checked by its FP64 true residual; one step = one foot's 8 "
+                                f"actions (hierarchy built on the GPU per system, the load actions reuse the "
+                                f"baseline's); mode ref: FP64 PCG with an FP32 V-cycle"),
+                   "feet": feet, "grid_min": list(min(sizes[f] for f in feet)),
+                   "grid_max": list(max(sizes[f] for f in feet)), "solves": nsolve, "concurrency": conc,
+                   "mode": "ref", **{k: v for k, v in summarize(ref, w_ref).items() if k not in ("value", "unit")}},
+        "mixed": summarize(mixed, w_mixed, {"ratio_over_plain": round(w_plain / w_mixed, 3)}),
+        "plain_fp64": summarize(plain, w_plain),
+        "ratio_ref_over_plain": round(w_plain / w_ref, 3),
+        "sequential": {"value": round(len(seq) / w_seq, 3), "unit": "solves/s", "concurrency": 1,
+                       "solves": len(seq), "what": f"foot {feet[0]}'s 8 actions one at a time, mode ref"},
+        "accuracy": {"what": "displacement relative L2 of every result against the plain FP64 GMG-PCG answer of the "
+                             "same system (itself pinned to the FP64 CPU oracle by the parity tests)",
+                     "ref_max": max(err(ref)), "mixed_max": max(err(mixed)), "ref": err(ref), "mixed": err(mixed),
+                     "order": [f"{i['foot']}:{i['action']}" for i in ref]},
+        "roofline": {"bound": "hbm", "achieved": round(rl["gbs32"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(rl["gbs32"] / HBM_PEAK_GBS, 4), "traffic": pmc_traffic("fine_apply_f32_384x256x256"),
+                     "kernel": kname, "workload": "384x256x256 voxels at 0.25 mm (25.2 M cells, 25.4 M nodes): "
+                                                  "working set 635 MB, above the 256 MB Infinity Cache",
+                     "avg_launch_us": round(rl["ms32"] * 1e3, 2), "alg_bytes_per_launch": rl["alg"],
+                     "alg_bytes": "24 B per node (read u, write y, FP32) + 1 B label per cell"},
+        "roofline_smoothing_step": {"kernel": "k_march<float, LabelMat<float,SRI,ISO>, EPI_CHEB> (fused Chebyshev "
+                                              "step, the solve's dominant kernel)",
+                                    "achieved": round(rl["gbss"], 1), "frac": round(rl["gbss"] / HBM_PEAK_GBS, 4),
+                                    "avg_launch_us": round(rl["mss"] * 1e3, 2), "alg_bytes_per_launch": rl["algs"],
+                                    "traffic": pmc_traffic("fine_cheb_f32_384x256x256"),
+                                    "alg_bytes": "96 B per node (u, rin, Dinv 6, x in; r, d, x out; FP32) + 1 B/cell"},
+        "roofline_fp64": {"achieved": round(rl["gbs64"], 1), "frac": round(rl["gbs64"] / HBM_PEAK_GBS, 4),
+                          "avg_launch_us": round(rl["ms64"] * 1e3, 2), "alg_bytes_per_launch": rl["alg64"]},
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+def kernel_workload(args):
+    """For rocprofv3 runs (kernel trace, or one --pmc counter set per run): the 384x256x256 system
+    built in this process (no host workers), the fine-level kernels timed as in the headline run,
+    then 5 plain device copies of 256 MiB (counter calibration).  Prints the roofline figures."""
+    import torch
+    from comech_amd import synth
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    desc = synth.level0_description(synth.hindfoot_large())
+    log(f"384x256x256 inputs ready in {time.time() - t0:.1f} s")
+    rl = roofline(desc, dev, nrep=args.nrep)
+    a = torch.randn(64 * 1024 * 1024, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "fine-level kernels, 384x256x256", **{k: v for k, v in rl.items() if k != "kz"}}),
+          flush=True)
 
 
 def large_workload(args):
-    """Large single solve: 384x256x256 voxels at 0.25 mm (the hindfoot anatomy upscaled, 4-voxel cortical
-    shells), one solve per precision form.  Prints one JSON line (not the headline metric): fine
+    """Large single solve (the fifth configuration): 384x256x256 voxels at 0.25 mm, 4-voxel cortical
+    shells, one solve per precision form.  Prints one JSON line (not the headline metric): fine
     matvec time and algorithmic GB/s per precision, V-cycle time, solve time and iterations to rtol
     1e-8 (FP64 true residual) for fp64 / ref / mixed, and the von Mises relative L2 of the mixed
     forms against the FP64 answer (ml/evaluate.py:93,96 metric)."""
     import numpy as np
     import torch
-    from comech_amd.system import cell_stress_gpu
+    from comech_amd import synth
+    from comech_amd.engine import split_cells
+    from comech_amd.system import VoxelSystem, cell_stress_gpu
     dev = torch.device("cuda:0")
     t0 = time.time()
-    c, extra, f, irr_c, irr_n = build_inputs(dims=(384, 256, 256), h=0.25, shell=4)
+    c = synth.hindfoot_large()
+    extra, terms, f = synth.solve_inputs_hindfoot(c)
+    irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
     g = c["grid"]
     log(f"large inputs ready in {time.time() - t0:.1f} s: ncell {g.ncell}, active {int(c['active'].sum())}")
     f_dev = torch.from_numpy(f).to(dev)
-    s = make_system(c, extra, irr_c, irr_n)
+    s = VoxelSystem(g, sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"], mu_tab=c["mu_tab"],
+                    copy_orig=c["orig"][g.nnode:], irr_cells=irr_c, irr_nodes=irr_n, extra=extra, rank1=terms,
+                    device=dev, levels=args.levels)
     lv = s.levels()
-    out = {"workload": "hindfoot raster 384x256x256 @0.25mm (the hindfoot anatomy upscaled, 4-voxel cortical shells), rtol 1e-8, FP64 true residual",
+    out = {"workload": "hindfoot raster 384x256x256 @0.25mm (4-voxel cortical shells), rtol 1e-8, FP64 true residual",
            "ncell": g.ncell, "dofs": lv[0]["ndof"], "levels": [l["ndof"] for l in lv],
            "t_setup_ms": round(s.timings()["total"], 1)}
     for dt, name, sz in ((torch.float32, "f32", 4), (torch.float64, "f64", 8)):
@@ -163,74 +424,6 @@ def large_workload(args):
     s.close()
     print(json.dumps(out), flush=True)
 
-
-def batch3_workload(args):
-    """Uneven-grid batch: the first ``--feet`` feet of the 64-foot set (grids drawn uniformly
-    between 96x64x64 and 192x112x112 at 0.5 mm, each with its own seed) under the 8 surgical
-    actions, solved batched (``--concurrency`` in flight) to rtol 1e-8 in mode ``--mode``, every
-    answer checked by the FP64 true residual.  One solve per grid shape runs alone first (the
-    z-extent autotuner times its kernels on an idle GPU).  Prints one JSON line (not the headline
-    metric): solves/s, DOF-iterations/s, iterations, and the fine K*u FP32 rate of the smallest and
-    the largest grid."""
-    import numpy as np
-    import torch
-    from comech_amd import synth
-    from comech_amd.batch import run_batch
-    from comech_amd.engine import split_cells
-    from comech_amd.synth import ACTIONS
-    dev = torch.device("cuda:0")
-    t0 = time.time()
-    sizes = synth.batch_sizes(64)[:args.feet]
-    cases = []
-    for foot in range(args.feet):
-        for a in ACTIONS:
-            c = synth.batch_case(foot, a)
-            extra, terms, f = synth.solve_inputs_hindfoot(c)
-            irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
-            cases.append((c, (extra, terms), torch.from_numpy(f).to(dev), irr_c, irr_n))
-        log(f"batch3: foot {foot} {sizes[foot]} inputs ready ({time.time() - t0:.1f} s)")
-
-    def job(i):
-        c, extra, f_dev, irr_c, irr_n = cases[i]
-        return lambda: one_solve(c, extra, f_dev, irr_c, irr_n, args.mode, args.tol)
-    for foot in range(args.feet):                      # tune every shape alone
-        run_batch([job(foot * len(ACTIONS))], concurrency=1, device=dev)
-    log("batch3: shapes tuned")
-    infos, wall = run_batch([job(i) for i in range(len(cases))], concurrency=args.concurrency, device=dev)
-    worst = max(i["true_relres"] for i in infos)
-    missed = [dict(foot=i // len(ACTIONS), action=ACTIONS[i % len(ACTIONS)], iters=r["iters"], flag=r["flag"],
-                   outer=r["outer"], relres=r["relres"], true_relres=r["true_relres"])
-              for i, r in enumerate(infos) if r["flag"] or r["true_relres"] > args.tol * 1.0001]
-    for m in missed:
-        log("batch3: missed", m)
-    rate = {}
-    for name, foot in (("smallest", int(np.argmin([np.prod(z) for z in sizes]))),
-                       ("largest", int(np.argmax([np.prod(z) for z in sizes])))):
-        c, extra, f_dev, irr_c, irr_n = cases[foot * len(ACTIONS)]
-        sy = make_system(c, extra, irr_c, irr_n)
-        u = torch.randn(3 * sy.nn, dtype=torch.float32, device=dev)
-        y = torch.empty_like(u)
-        ms = sy.time_apply(u, y, nrep=20, structured_only=True)
-        g = c["grid"]
-        alg = 2 * 3 * 4 * g.nnode + g.ncell
-        rate[name] = {"grid": list(sizes[foot]), "us": round(ms * 1e3, 2), "alg_bytes": alg,
-                      "GBps": round(alg / (ms * 1e-3) / 1e9, 1)}
-        sy.close()
-    iters = [i["iters"] for i in infos]
-    line = {"workload": f"uneven-grid batch: {args.feet} feet (grids 96x64x64 .. 192x112x112 at 0.5 mm) x "
-                        f"{len(ACTIONS)} surgical actions, rtol {args.tol}, FP64 true residual",
-            "mode": args.mode, "concurrency": args.concurrency, "solves": len(infos),
-            "grids": [list(z) for z in sizes], "solves_per_s": round(len(infos) / wall, 2),
-            "dof_iterations_per_s": round(sum(i["ndof"] * i["iters"] for i in infos) / wall, 1),
-            "iters_mean": round(float(np.mean(iters)), 1), "iters_max": int(max(iters)),
-            "true_relres_max": worst, "missed": missed, "matvec_f32": rate}
-    print(json.dumps(line), flush=True)
-    # a solve the solver reports as converged must meet the tolerance; a solve it reports as not
-    # converged (flag != 0: restarts from the true residual stopped improving, the attainable FP64
-    # accuracy of that system lies above the tolerance) is listed above, not hidden
-    if any(m["flag"] == 0 for m in missed):
-        log(f"bench.py: a solve reported converged but missed rtol {args.tol}")
-        sys.exit(1)
 
 
 def prep_workload(args):
@@ -409,145 +602,30 @@ def prep_workload(args):
     print(json.dumps(line), flush=True)
 
 
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=24, help="solves in the timed region (the surgical actions, cycled)")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed passes over the action set")
-    ap.add_argument("--concurrency", type=int, default=16, help="solves in flight on the GPU")
-    ap.add_argument("--mode", default="ref", choices=["mixed", "ref", "fp64"])
+    ap.add_argument("--steps", type=int, default=N_FEET,
+                    help="feet of the 64-foot set in the timed region (8 solves each), spread evenly; 64 = all")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed feet before the timed region")
+    ap.add_argument("--concurrency", type=int, default=16, help="systems in flight on the GPU")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="hindfoot", choices=["hindfoot", "large", "prep", "batch3"],
-                    help="large: the 384x256x256 single solve; prep: engine preparation and resampling; "
-                         "batch3: uneven grids of the 64-foot batch (separate JSON lines, not the headline metric)")
-    ap.add_argument("--feet", type=int, default=8, help="batch3: feet of the 64-foot set (x 8 actions)")
-    ap.add_argument("--kernel-only", action="store_true",
-                    help="for rocprofv3 --pmc runs: 10 launches of the FP32 fine stencil on the workload, then "
-                         "5 plain device copies of 256 MiB (counter calibration)")
+    ap.add_argument("--workload", default="batch", choices=["batch", "kernels", "large", "prep"],
+                    help="batch: the headline (uneven-grid set); kernels: the fine-level kernels of the "
+                         "384x256x256 grid alone, for rocprofv3; large: the 384x256x256 solve in every mode; "
+                         "prep: engine preparation and resampling (separate JSON lines, not the headline)")
+    ap.add_argument("--nrep", type=int, default=20, help="kernels: launches timed per kernel")
+    ap.add_argument("--levels", type=int, default=None, help="large: force the number of multigrid levels")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; every solve is independent (batched solves "
               "shard by solve across GPUs), so run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
-    import torch
-    from comech_amd.batch import run_batch
-    from comech_amd.synth import ACTIONS
-    torch.cuda.init()
-    dev = torch.device("cuda:0")
-    t0 = time.time()
-    if args.workload == "large":
-        large_workload(args)
-        return
-    if args.workload == "prep":
-        prep_workload(args)
-        return
-    if args.workload == "batch3":
-        batch3_workload(args)
-        return
-    if args.kernel_only:
-        c, extra, f, irr_c, irr_n = build_inputs()
-        s = make_system(c, extra, irr_c, irr_n)
-        u = torch.randn(3 * s.nn, dtype=torch.float32, device=dev)
-        y = torch.empty_like(u)
-        for _ in range(5):
-            s.time_apply(u, y, nrep=1, structured_only=True)          # 2 launches each
-        a = torch.randn(64 * 1024 * 1024, dtype=torch.float32, device=dev)
-        b = torch.empty_like(a)
-        for _ in range(5):
-            b.copy_(a)
-        torch.cuda.synchronize()
-        s.close()
-        log("kernel-only probe done")
-        return
-
-    inputs = action_inputs(ACTIONS)
-    f_devs = [torch.from_numpy(i[2]).to(dev) for i in inputs]
-    c0 = inputs[0][0]
-    log(f"inputs ready in {time.time() - t0:.1f} s: {len(inputs)} actions, ncell {c0['grid'].ncell}, "
-        f"active {int(c0['active'].sum())}")
-
-    def job(i, mode):
-        c, extra, f, irr_c, irr_n = inputs[i % len(inputs)]
-        return lambda: one_solve(c, extra, f_devs[i % len(inputs)], irr_c, irr_n, mode, args.tol)
-
-    def timed(mode, n, conc):
-        res, wall = run_batch([job(i, mode) for i in range(n)], concurrency=conc, device=dev)
-        return res, wall
-
-    # the first solve runs alone: the z-extent autotuner (cached per level shape) must time its
-    # kernels on an otherwise idle GPU, not between the kernels of concurrent solves
-    timed(args.mode, 1, 1)
-    for _ in range(args.warmup):
-        res, _ = timed(args.mode, len(inputs), args.concurrency)
-        log("warmup", [round(r["t_solve_ms"], 1) for r in res])
-    infos, dt = timed(args.mode, args.steps, args.concurrency)
-    for i in infos:
-        log("step", {k: i[k] for k in ("iters", "flag", "outer", "levels", "relres", "true_relres", "t_setup_ms",
-                                       "t_solve_ms", "mode")})
-    worst = max(i["true_relres"] for i in infos)
-    if worst > args.tol * 1.0001 or any(i["flag"] for i in infos):
-        log(f"bench.py: a solve missed rtol {args.tol}: {worst}")
-        sys.exit(1)
-    # the same jobs one at a time, and plain FP64 GMG-PCG (the reference method) batched the same way
-    seq, dt_seq = timed(args.mode, len(inputs), 1)
-    plain, dt_plain = timed("fp64", args.steps, args.concurrency)
-    sps = args.steps / dt
-    plain_sps = args.steps / dt_plain
-    iters = [i["iters"] for i in infos]
-    dof_it = sum(i["ndof"] * i["iters"] for i in infos) / dt
-
-    # roofline of the dominant kernel: FP32 fine-level stencil (HIP events on the solver's stream)
-    c, extra, f, irr_c, irr_n = inputs[0]
-    s = make_system(c, extra, irr_c, irr_n)
-    g = c["grid"]
-    u = torch.randn(3 * s.nn, dtype=torch.float32, device=dev)
-    y = torch.empty_like(u)
-    nrep = 50
-    ms = s.time_apply(u, y, nrep=nrep, structured_only=True)
-    ms64 = s.time_apply(u.double(), y.double(), nrep=nrep, structured_only=True)
-    s.close()
-    alg_bytes = 2 * 3 * 4 * g.nnode + g.ncell             # read u, write y (FP32) + one label byte per cell
-    achieved = alg_bytes / (ms * 1e-3) / 1e9
-    line = {
-        "metric": "solves/s to rtol 1e-8 (batched) + matvec HBM GB/s; learned/mixed vs plain",
-        "value": round(sps, 4),
-        "unit": "solves/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(1e3 * dt / args.steps, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64",
-        "data": "synthetic",
-        "config": {"workload": "hindfoot raster 160x96x96 @0.5mm under the 8 surgical actions, mixed-precision "
-                               "GMG-PCG to rtol 1e-8 checked by an FP64 true residual; one step = one solve "
-                               "(GPU multigrid setup + PCG + check)",
-                   "dofs": infos[0]["ndof"], "mode": args.mode, "concurrency": args.concurrency,
-                   "iters_mean": round(float(np.mean(iters)), 1), "levels": infos[-1]["levels"],
-                   "t_setup_ms": round(float(np.mean([i["t_setup_ms"] for i in infos])), 2),
-                   "t_solve_ms": round(float(np.mean([i["t_solve_ms"] for i in infos])), 2),
-                   "true_relres_max": worst,
-                   "dof_iterations_per_s": round(dof_it, 1)},
-        "sequential": {"value": round(len(inputs) / dt_seq, 4), "unit": "solves/s", "concurrency": 1},
-        "plain_fp64": {"value": round(plain_sps, 4), "unit": "solves/s", "concurrency": args.concurrency,
-                       "iters_mean": round(float(np.mean([i["iters"] for i in plain])), 1),
-                       "ratio_mixed_over_plain": round(sps / plain_sps, 3)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(),
-                     "kernel": "k_march<float, LabelMat<float,SRI,ISO>, APPLY> (fine-level K*u)",
-                     "avg_launch_us": round(ms * 1e3, 2),
-                     "alg_bytes_per_launch": alg_bytes, "fp64_avg_launch_us": round(ms64 * 1e3, 2)},
-    }
-    if not args.no_cpu_baseline:
-        try:
-            line["cpu_baseline"] = cpu_baseline()
-        except Exception as e:          # the baseline is reported, never required
-            line["cpu_baseline"] = {"value": None, "error": repr(e)[:200]}
-    print(json.dumps(line), flush=True)
+    {"batch": batch_workload, "kernels": kernel_workload, "large": large_workload, "prep": prep_workload}[
+        args.workload](args)
 
 
 if __name__ == "__main__":

@@ -62,4 +62,25 @@ def run_batch(jobs, concurrency=4, device=None):
     return results, wall
 
 
-__all__ = ["run_batch"]
+def build_system(desc, device=None, **mg):
+    """VoxelSystem of a level-0 description (``synth.level0_description``): labels + tables, split
+    copies and irregular cells, ground springs (COO) and spring terms (rank one)."""
+    import numpy as np
+    from .grid import Grid
+    from .system import VoxelSystem
+    g = Grid(*desc["edges"])
+    return VoxelSystem(g, sri=True, active=np.asarray(desc["labels"]) != 255, labels=desc["labels"],
+                       lam_tab=desc["lam_tab"], mu_tab=desc["mu_tab"], copy_orig=desc["copy_orig"],
+                       irr_cells=desc["irr_cells"], irr_nodes=desc["irr_nodes"], extra=desc["extra"],
+                       rank1=desc["terms"], device=device, **mg)
+
+
+def rhs(desc, device=None):
+    """Nodal force vector of a level-0 description on the device (FP64, 3 per node)."""
+    dev = require_gpu(device)
+    b = torch.zeros(3 * desc["nn"], dtype=torch.float64, device=dev)
+    b[torch.as_tensor(desc["f_idx"], device=dev)] = torch.as_tensor(desc["f_val"], dtype=torch.float64, device=dev)
+    return b
+
+
+__all__ = ["run_batch", "build_system", "rhs"]

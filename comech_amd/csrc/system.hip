@@ -1390,6 +1390,15 @@ struct System {
     launch_march<T, EPI_APPLY>(l, u, ep);
     CM_LAUNCH_CHECK();
   }
+  // The structured march of one middle Chebyshev step alone (timing): r = b - A s, d = c1 s +
+  // c2 Dinv r, x += s + d on the level's own work vectors (b, x cleared by the caller once)
+  template <typename T>
+  void smooth_step_structured(int l, const T* s) {
+    LevelT<T>& d = lv(l).dat<T>();
+    Epi<T> ep{d.b.p, d.r.p, d.Dinv.p, d.d.p, d.x.p, T(0.5), T(0.25), T(1)};
+    launch_march<T, EPI_CHEB>(l, s, ep);
+    CM_LAUNCH_CHECK();
+  }
 
   int kz_override = 0;
   bool use_flags = true;      // empty blocks of the structured march take the epilogue-only path
@@ -2359,17 +2368,34 @@ int cm_sys_time_apply(cm_sys* h, int level, int prec, const void* u, void* y, in
   CM_API_BEGIN
   need_level(h, level);
   CM_CHECK(nrep >= 1, "nrep must be positive");
+  CM_CHECK(structured_only >= 0 && structured_only <= 2, "structured_only must be 0, 1 or 2");
   System& s = h->s;
-  if (prec == 0) s.prepare_prec<double>(level, false);
-  else s.prepare_prec<float>(level, false);
+  const bool step = structured_only == 2;
+  CM_CHECK(!step || level + 1 < s.nlev(), "the coarsest level has no smoother");
+  if (prec == 0) s.prepare_prec<double>(level, step);
+  else s.prepare_prec<float>(level, step);
+  if (step) {
+    cm::Level& v = s.lv(level);
+    const long long nv = 3 * v.nn;
+    if (prec == 0) {
+      hipLaunchKernelGGL(cm::k_fill<double>, cm::grid1(nv), dim3(256), 0, s.stream, v.d64.b.p, nv, 0.0);
+      hipLaunchKernelGGL(cm::k_fill<double>, cm::grid1(nv), dim3(256), 0, s.stream, v.d64.x.p, nv, 0.0);
+    } else {
+      hipLaunchKernelGGL(cm::k_fill<float>, cm::grid1(nv), dim3(256), 0, s.stream, v.d32.b.p, nv, 0.f);
+      hipLaunchKernelGGL(cm::k_fill<float>, cm::grid1(nv), dim3(256), 0, s.stream, v.d32.x.p, nv, 0.f);
+    }
+    CM_LAUNCH_CHECK();
+  }
   hipEvent_t e0, e1;
   CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
   auto once = [&]() {
     if (prec == 0) {
-      if (structured_only) s.apply_structured<double>(level, cp<double>(u), mp<double>(y));
+      if (step) s.smooth_step_structured<double>(level, cp<double>(u));
+      else if (structured_only) s.apply_structured<double>(level, cp<double>(u), mp<double>(y));
       else s.apply<double>(level, cp<double>(u), mp<double>(y), cm::MODE_APPLY, nullptr);
     } else {
-      if (structured_only) s.apply_structured<float>(level, cp<float>(u), mp<float>(y));
+      if (step) s.smooth_step_structured<float>(level, cp<float>(u));
+      else if (structured_only) s.apply_structured<float>(level, cp<float>(u), mp<float>(y));
       else s.apply<float>(level, cp<float>(u), mp<float>(y), cm::MODE_APPLY, nullptr);
     }
   };

@@ -12,22 +12,14 @@ import scipy.sparse as sp
 import torch
 
 from .grid import corners_of
+from .split import irregular_cells
 from .system import VoxelSystem, cell_stress_gpu, require_gpu
 
 
 def split_cells(grid, active, cn):
     """Active cells whose connectivity differs from the grid corners (split-copy cells).
     Returns (irr_cells, irr_nodes(n,8))."""
-    cells = np.flatnonzero(active)
-    out_c, out_n = [], []
-    for a in range(0, len(cells), 1 << 20):
-        c = cells[a:a + (1 << 20)]
-        row = cn[c]
-        diff = (row != corners_of(grid, c)).any(1)
-        out_c.append(c[diff]); out_n.append(row[diff])
-    if not out_c:
-        return np.zeros(0, np.int64), np.zeros((0, 8), np.int64)
-    return np.concatenate(out_c).astype(np.int64), np.concatenate(out_n).astype(np.int64)
+    return irregular_cells(grid, active, cn)
 
 
 def used_nodes(grid, active, nn, irr_cells=None, irr_nodes=None):

@@ -22,7 +22,7 @@ from scipy import ndimage
 
 from .grid import Grid, corners_of, face_node_areas, lame
 from .rb.replica3 import cat_terms, spring_pairs, spring_terms
-from .rb.splitfe import split_connectivity_fast
+from .split import irregular_cells, split_connectivity_fast
 
 
 def _rotation(rng, max_angle=None):
@@ -101,6 +101,23 @@ def hindfoot_raster(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000
             k_lig=100.0, load=700.0, action="baseline", angle_seed=None):
     """Hindfoot-like raster.  action: baseline | resection | osteotomy | interposition | fusion |
     load0.5 | load1.5 | load2 (the surgical actions of the batched set)."""
+    return hindfoot_family(dims, h, seed, shell, n_ground, n_lig, k_iface, k_ground, k_lig, load, [action],
+                           angle_seed)[action]
+
+
+# actions that keep the baseline's active cells and bone parts: they share its split connectivity,
+# ground-spring nodes and ligament ends (the seeded draws depend on those only)
+_SAME_TOPOLOGY = ("baseline", "resection", "fusion", "load0.5", "load1.5", "load2")
+
+
+def hindfoot_family(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000, n_lig=200, k_iface=1e3,
+                    k_ground=50.0, k_lig=100.0, load=700.0, actions=None, angle_seed=None):
+    """``{action: hindfoot_raster(..., action=action)}`` for several actions of one foot, with the
+    anatomy drawn once and the split connectivity, ground nodes and ligament ends computed once for
+    all the actions that keep the baseline's cells (every action except osteotomy and
+    interposition).  Same arrays as separate hindfoot_raster calls."""
+    import copy
+    actions = list(ACTIONS if actions is None else actions)
     rng = np.random.default_rng(seed)
     nx, ny, nz = dims
     g = Grid(np.arange(nx + 1) * h, np.arange(ny + 1) * h, np.arange(nz + 1) * h)
@@ -117,12 +134,14 @@ def hindfoot_raster(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000
     rad = np.stack([_ellipsoid_radius(P, c, a, R) for (c, a), R in zip(bones, rots)], 1)
     inb = (rad <= 1.0) & env[:, None]
     bone = np.where(inb.any(1), np.argmin(np.where(inb, rad, np.inf), 1) + 1, 0)
+    del rad, inb
     # coalition: cylinder of radius 4 voxels bridging bones 1 and 2 (cells outside every bone)
     a0, a1 = bones[0][0], bones[1][0]
     d = a1 - a0
     t = np.clip(((P - a0) @ d) / (d @ d), 0, 1)
     dist = np.linalg.norm(P - (a0 + t[:, None] * d), axis=1)
     coal = (dist <= 4 * h) & env & (bone == 0)
+    del t, dist
     # cortical shell: bone cells within `shell` voxels of the bone boundary
     shape = (nx, ny, nz)
     cort = np.zeros(g.ncell, bool)
@@ -130,39 +149,61 @@ def hindfoot_raster(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000
         m = (bone == b).reshape(shape)
         core = ndimage.binary_erosion(m, iterations=shell)
         cort |= (m & ~core).ravel()
-    labels = np.zeros(g.ncell, np.uint8)           # 0 soft, 1 coalition, 2+2(b-1) cortical, 3+2(b-1) cancellous
-    labels[coal] = 1
+    labels0 = np.zeros(g.ncell, np.uint8)          # 0 soft, 1 coalition, 2+2(b-1) cortical, 3+2(b-1) cancellous
+    labels0[coal] = 1
     isb = bone > 0
-    labels[isb] = (2 + 2 * (bone[isb] - 1) + (~cort[isb])).astype(np.uint8)
-    active = env.copy()
+    labels0[isb] = (2 + 2 * (bone[isb] - 1) + (~cort[isb])).astype(np.uint8)
     E_lab = [FOOT_E["soft"], FOOT_E["coalition"]] + [FOOT_E["cortical"], FOOT_E["cancellous"]] * 3
-    E_tab = np.array([e for e, _ in E_lab], float)
-    nu_tab = np.array([v for _, v in E_lab], float)
-    part = np.where(isb, bone, 0).astype(np.int64)
-    k_if = k_iface
-    scale = 1.0
-    if action == "resection":                       # coalition relabelled to soft tissue
-        labels[coal] = 0
-    elif action in ("osteotomy", "interposition"):  # 3-voxel planar slab through bone 1 removed
+    E_tab0 = np.array([e for e, _ in E_lab], float)
+    nu_tab0 = np.array([v for _, v in E_lab], float)
+    part0 = np.where(isb, bone, 0).astype(np.int64)
+    rng_after_anatomy = copy.deepcopy(rng)
+    slab = None
+    if "osteotomy" in actions or "interposition" in actions:   # 3-voxel planar slab through bone 1
         arng = np.random.default_rng(seed if angle_seed is None else angle_seed)
         nrm = _rotation(arng, 0.5) @ np.array([1.0, 0.0, 0.0])
         slab = (np.abs((P - bones[0][0]) @ nrm) <= 1.5 * h) & (bone == 1)
-        if action == "osteotomy":
-            active &= ~slab
-        else:                                      # gap filled with E = 5 MPa
-            E_tab = np.append(E_tab, 5.0); nu_tab = np.append(nu_tab, 0.3)
-            labels[slab] = len(E_tab) - 1
-        part[slab] = 0
-    elif action == "fusion":
-        k_if = 1e6
-    elif action.startswith("load"):
-        scale = float(action[4:])
-    labels[~active] = 255
-    lam_tab, mu_tab = lame(E_tab, nu_tab)
-    lab_safe = np.where(active, labels, 0)
-    lam = np.where(active, lam_tab[lab_safe], 0.0)
-    mu = np.where(active, mu_tab[lab_safe], 0.0)
-    part[~active] = 0
+    del P
+    topo_cache = {}
+    out = {}
+    for action in actions:
+        labels, active, part = labels0.copy(), env.copy(), part0.copy()
+        E_tab, nu_tab = E_tab0, nu_tab0
+        k_if, scale = k_iface, 1.0
+        if action == "resection":                       # coalition relabelled to soft tissue
+            labels[coal] = 0
+        elif action in ("osteotomy", "interposition"):
+            if action == "osteotomy":
+                active &= ~slab
+            else:                                       # gap filled with E = 5 MPa
+                E_tab = np.append(E_tab, 5.0); nu_tab = np.append(nu_tab, 0.3)
+                labels[slab] = len(E_tab) - 1
+            part[slab] = 0
+        elif action == "fusion":
+            k_if = 1e6
+        elif action.startswith("load"):
+            scale = float(action[4:])
+        elif action != "baseline":
+            raise ValueError(f"unknown action {action!r}")
+        labels[~active] = 255
+        lam_tab, mu_tab = lame(E_tab, nu_tab)
+        lab_safe = np.where(active, labels, 0)
+        lam = np.where(active, lam_tab[lab_safe], 0.0)
+        mu = np.where(active, mu_tab[lab_safe], 0.0)
+        part[~active] = 0
+        key = "base" if action in _SAME_TOPOLOGY else action
+        if key not in topo_cache:
+            topo_cache[key] = _topology(g, active, part, copy.deepcopy(rng_after_anatomy), n_ground, n_lig)
+        tp = topo_cache[key]
+        out[action] = _assemble_case(g, action, labels, active, part, lam, mu, lam_tab, mu_tab, tp, k_if, k_ground,
+                                     k_lig, scale, load)
+    return out
+
+
+def _topology(g, active, part, rng, n_ground, n_lig):
+    """Split connectivity, used nodes, ground-spring nodes and ligament ends of one foot's cells
+    (the draws after the anatomy, in the generator's order)."""
+    nx, ny, nz = g.nx, g.ny, g.nz
     share = np.zeros((g.ncell, 4), np.float32)
     share[np.arange(g.ncell), part] = 1.0
     share[:, 0] = 0.0
@@ -172,9 +213,6 @@ def hindfoot_raster(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000
     used = np.zeros(nn, bool)
     used[cn[active].ravel()] = True
     xyz = g.node_xyz[orig]
-    # interface springs: copy <-> its original grid node
-    copies = np.arange(g.nnode, nn)
-    iface = spring_pairs(xyz, np.stack([copies, orig[copies]], 1), k_if, directional=False)
     # plantar ground springs on used bottom grid nodes
     bottom = np.flatnonzero(used[: g.nnode] & (np.arange(g.nnode) // ((nx + 1) * (ny + 1)) == 0))
     gsel = np.sort(rng.choice(bottom, size=min(n_ground, len(bottom)), replace=False))
@@ -196,14 +234,28 @@ def hindfoot_raster(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000
         b_ = rng.choice(near)
         if np.linalg.norm(xyz[b_] - xyz[a]) > 0:
             pairs.append((a, b_))
-    lig = spring_pairs(xyz, np.asarray(pairs, np.int64), k_lig, directional=True)
-    terms = cat_terms(spring_terms(xyz, np.stack([copies, orig[copies]], 1), k_if, directional=False),
-                      spring_terms(xyz, np.asarray(pairs, np.int64), k_lig, directional=True))
+    return dict(share=share, cn=cn, orig=orig, nsplit=nsplit, used=used, xyz=xyz, gsel=gsel,
+                pairs=np.asarray(pairs, np.int64).reshape(-1, 2))
+
+
+def _assemble_case(g, action, labels, active, part, lam, mu, lam_tab, mu_tab, tp, k_if, k_ground, k_lig, scale,
+                   load):
+    """One action's case dict from its materials, the foot's topology and the spring/load values."""
+    cn, orig, xyz, pairs = tp["cn"], tp["orig"], tp["xyz"], tp["pairs"]
+    nn = len(orig)
+    # interface springs: copy <-> its original grid node
+    copies = np.arange(g.nnode, nn)
+    cpairs = np.stack([copies, orig[copies]], 1)
+    iface = spring_pairs(xyz, cpairs, k_if, directional=False)
+    lig = spring_pairs(xyz, pairs, k_lig, directional=True)
+    terms = cat_terms(spring_terms(xyz, cpairs, k_if, directional=False),
+                      spring_terms(xyz, pairs, k_lig, directional=True))
     # load: downward total over active cells of the top 10 layers, 1/8 per corner
-    topc = np.flatnonzero(active & (g.cK >= nz - 10))
+    topc = np.flatnonzero(active & (g.cK >= g.nz - 10))
     ln = cn[topc].ravel()
     la = np.full(len(ln), scale * load / len(ln))
     loads = [(ln, la, np.array([0.0, 0.0, -1.0]))]
+    gsel = tp["gsel"]
     fixed = (gsel, np.full(len(gsel), k_ground / 1e6))
     extra = (np.concatenate([iface[0], lig[0]]), np.concatenate([iface[1], lig[1]]),
              np.concatenate([iface[2], lig[2]]))
@@ -217,8 +269,8 @@ def hindfoot_raster(dims=(160, 96, 96), h=0.5, seed=2000, shell=2, n_ground=2000
         return sp.csr_matrix((v[ok], (rr, cc)), shape=(n, n))
 
     return dict(name="hindfoot_raster", grid=g, lam=lam, mu=mu, active=active, labels=labels, lam_tab=lam_tab,
-                mu_tab=mu_tab, part=part, bone_share=share, cn=cn, orig=orig, nsplit=nsplit, fixed=fixed,
-                loads=loads, extra=extra, extra_cb=extra_cb, terms=terms, sri=True, action=action, used=used,
+                mu_tab=mu_tab, part=part, bone_share=tp["share"], cn=cn, orig=orig, nsplit=tp["nsplit"], fixed=fixed,
+                loads=loads, extra=extra, extra_cb=extra_cb, terms=terms, sri=True, action=action, used=tp["used"],
                 n_ground=len(gsel), n_lig=len(pairs), n_copies=len(copies))
 
 
@@ -262,3 +314,28 @@ def batch_case(foot, action, n_feet=64, h=0.5, scale=1):
 
 def hindfoot_large(dims=(384, 256, 256), h=0.25, seed=5000):
     return hindfoot_raster(dims=dims, h=h, seed=seed, shell=4)
+
+
+def level0_description(case):
+    """The GPU solver's inputs of a hindfoot case in compact form (numpy only): grid edges, labels
+    and their (lambda, mu) tables (255 = inactive), split copies and the cells touching them, ground
+    springs as COO, interface and ligament springs as rank-one terms, and the non-zero nodal loads
+    (the load acts on a few top layers only)."""
+    g = case["grid"]
+    extra, terms, f = solve_inputs_hindfoot(case)
+    irr_c, irr_n = irregular_cells(g, case["active"], case["cn"])
+    nz = np.flatnonzero(f)
+    return dict(edges=(g.xe, g.ye, g.ze), labels=case["labels"], lam_tab=case["lam_tab"], mu_tab=case["mu_tab"],
+                copy_orig=case["orig"][g.nnode:], irr_cells=irr_c, irr_nodes=irr_n, extra=extra, terms=terms,
+                f_idx=nz, f_val=f[nz], nn=len(case["orig"]), action=case["action"], ncell=g.ncell,
+                nactive=int(case["active"].sum()))
+
+
+def batch_foot(foot, n_feet=64, h=0.5, scale=1, actions=None):
+    """level0_description of one foot of the batched set under its surgical actions (all 8 by
+    default, in ACTIONS order); the anatomy and the shared topology are computed once."""
+    dims = batch_sizes(n_feet)[foot]
+    if scale != 1:
+        dims = tuple(max(4, int(2 * round(d / scale / 2))) for d in dims)
+    fam = hindfoot_family(dims, h * scale, 3000 + foot, actions=actions, angle_seed=3500 + foot)
+    return [level0_description(fam[a]) for a in (ACTIONS if actions is None else actions)]

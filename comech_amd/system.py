@@ -264,11 +264,13 @@ class VoxelSystem:
                                      ctypes.c_void_p(x.data_ptr()), int(bool(zero_start))))
         return x
 
-    def time_apply(self, u, y, nrep=20, structured_only=False, level=0):
+    def time_apply(self, u, y, nrep=20, structured_only=False, level=0, smooth_step=False):
+        """Average ms of nrep launches (HIP events): the whole product, the structured stencil kernel
+        alone (structured_only), or the structured kernel of one fused Chebyshev step (smooth_step)."""
         ms = ctypes.c_double()
+        what = 2 if smooth_step else int(bool(structured_only))
         check(self.lib.cm_sys_time_apply(self.h, level, PREC[u.dtype], ctypes.c_void_p(u.data_ptr()),
-                                         ctypes.c_void_p(y.data_ptr()), int(nrep), int(bool(structured_only)),
-                                         ctypes.byref(ms)))
+                                         ctypes.c_void_p(y.data_ptr()), int(nrep), what, ctypes.byref(ms)))
         return ms.value
 
     def autotune(self, precisions=("f32", "f64"), nrep=5):

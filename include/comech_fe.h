@@ -172,8 +172,11 @@ int cm_element_forces(void* stream, int nx, int ny, int nz, const double* hx, co
                       int sri, int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* lam_ref,
                       const double* mu_ref, const double* U, double* out);
 
-/* Average kernel time of nrep launches of the level-0 operator (structured_only = 1: the
- * structured stencil kernel alone), measured with HIP events on the system's stream. */
+/* Average kernel time of nrep launches of one level's operator, measured with HIP events on the
+ * system's stream: structured_only = 0 the whole product (structured stencil, sparse pass,
+ * fix-up), 1 the structured stencil kernel alone, 2 the structured kernel of one fused Chebyshev
+ * smoothing step (r = b - A u, d = c1 u + c2 Dinv r, x += u + d on the level's work vectors; y is
+ * not used). */
 int cm_sys_time_apply(cm_sys* sys, int level, int prec, const void* u, void* y, int nrep, int structured_only,
                       double* avg_ms);
 

@@ -228,3 +228,26 @@ def test_gpusolve_failure_hook_raises_without_cpu_fallback(monkeypatch):
     with pytest.raises(RuntimeError, match="simulated GPU failure"):
         GS.pcg_gmg_gpu(sp.identity(3, format="csr"), np.ones(3), np.array([0]), (np.arange(2.0),) * 3,
                        device="cuda:0")
+
+
+def test_hindfoot_family_equals_separate_generation():
+    """synth.hindfoot_family computes the anatomy once and the split/spring topology once for the
+    actions that keep the baseline's cells; every action's arrays equal a separate generation
+    (whose topology is computed from its own cells)."""
+    from comech_amd import synth
+    fam = synth.hindfoot_family((24, 16, 16), 3.0, 2100, angle_seed=2600)
+    keys = ("lam", "mu", "active", "labels", "lam_tab", "mu_tab", "part", "cn", "orig", "used", "terms", "extra",
+            "fixed", "loads")
+
+    def same(a, b):
+        if isinstance(a, np.ndarray):
+            return a.dtype == b.dtype and np.array_equal(a, b)
+        if isinstance(a, (tuple, list)):
+            return len(a) == len(b) and all(same(x, y) for x, y in zip(a, b))
+        return a == b
+    for a in synth.ACTIONS:
+        one = synth.hindfoot_raster(dims=(24, 16, 16), h=3.0, seed=2100, action=a, angle_seed=2600)
+        for k in keys:
+            assert same(fam[a][k], one[k]), (a, k)
+    assert fam["fusion"]["terms"][3].max() == 1e6 and fam["baseline"]["terms"][3].max() < 1e6
+    assert not np.array_equal(fam["osteotomy"]["active"], fam["baseline"]["active"])

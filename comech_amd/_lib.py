@@ -38,6 +38,7 @@ SIGNATURES = {
     "cm_sys_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p],
     "cm_sys_vcycle": [c_void_p, c_int, c_void_p, c_void_p],
     "cm_sys_level_data": [c_void_p, c_int, c_int, c_int, c_void_p],
+    "cm_sys_set_fallback": [c_void_p, c_int, c_double],
     "cm_sys_power": [c_void_p, c_int, c_int, c_void_p, P_double],
     "cm_sys_smooth": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int],
     "cm_sys_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_int, c_double, ctypes.POINTER(SolveInfo)],
@@ -71,7 +72,33 @@ SIGNATURES = {
     "cm_sample_nodes_noalloc": [c_void_p] + [c_int] * 3 + [c_void_p] * 6 + [c_int] + [c_void_p] * 2 + [c_int] * 3 +
                                [c_void_p] * 7,
 }
-_RET = {"cm_last_error": ctypes.c_char_p, "cm_version": c_int}
+# include/comech_learned.h (the learned warm start's kernels, same library)
+c_float = ctypes.c_float
+SIGNATURES_NN = {
+    "cm_nn_last_error": [],
+    "cm_nn_gemm_tn": [c_void_p, c_int, c_int, c_int, c_void_p, c_ll, c_void_p, c_ll, c_void_p, c_ll, c_void_p, c_void_p,
+                      c_ll, c_int, c_int, c_int, c_float],
+    "cm_nn_im2col": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_ll],
+    "cm_nn_transpose": [c_void_p, c_ll, c_ll, c_int, c_void_p, c_void_p, c_void_p],
+    "cm_nn_weights": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p],
+    "cm_nn_adam": [c_void_p, c_ll, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_float, c_float, c_float, c_int],
+    "cm_nn_colsum": [c_void_p, c_ll, c_int, c_void_p, c_void_p],
+    "cm_nn_relu_bwd": [c_void_p, c_ll, c_void_p, c_void_p, c_void_p, c_void_p],
+    "cm_nn_pool": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p],
+    "cm_nn_pool_bwd": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p],
+    "cm_nn_up": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p],
+    "cm_nn_up_bwd": [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p],
+    "cm_nn_mse": [c_void_p, c_ll, c_int, c_int, c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p],
+    "cm_nn_features": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p,
+                       c_void_p, c_void_p, c_float, c_void_p],
+    "cm_nn_node_to_cell": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p],
+    "cm_nn_cell_to_node": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_ll,
+                           c_void_p, c_double, c_void_p],
+    "cm_nn_fill": [c_void_p, c_ll, c_void_p, c_float],
+    "cm_nn_cast": [c_void_p, c_ll, c_void_p, c_void_p, c_int],
+    "cm_nn_axpy": [c_void_p, c_ll, c_float, c_void_p, c_void_p],
+}
+_RET = {"cm_last_error": ctypes.c_char_p, "cm_version": c_int, "cm_nn_last_error": ctypes.c_char_p}
 
 _lib = None
 
@@ -85,7 +112,7 @@ def load(path=LIB_PATH):
         raise OSError(f"{path} is missing: build it with `python -m comech_amd.build` (hipcc, gfx950); "
                       "the solver has no CPU fallback")
     lib = ctypes.CDLL(path)
-    for name, args in SIGNATURES.items():
+    for name, args in list(SIGNATURES.items()) + list(SIGNATURES_NN.items()):
         f = getattr(lib, name)
         f.argtypes = args
         f.restype = _RET.get(name, c_int)
@@ -100,6 +127,13 @@ class CmError(RuntimeError):
 def check(rc):
     if rc != 0:
         raise CmError(load().cm_last_error().decode())
+    return rc
+
+
+def check_nn(rc):
+    """Status of a cm_nn_* call (include/comech_learned.h)."""
+    if rc != 0:
+        raise CmError(load().cm_nn_last_error().decode())
     return rc
 
 

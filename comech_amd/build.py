@@ -16,7 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared", "-munsafe-fp-atomics",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
 
-SOURCES = [os.path.join(CSRC, "system.hip")]
+SOURCES = [os.path.join(CSRC, "system.hip"), os.path.join(CSRC, "learned.hip")]
 HOST_LIB = os.path.join(HERE, "libcomech_host.so")
 HOST_SOURCES = [os.path.join(CSRC, "host_element.cpp")]
 

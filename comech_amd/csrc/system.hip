@@ -25,6 +25,9 @@ namespace cm {
 
 static thread_local std::string g_last_error;
 
+// cm_solve_info.flag of a solve stopped by the early-exit rule of cm_sys_set_fallback
+constexpr int FLAG_FALLBACK = -2;
+
 // ---------------------------------------------------------------------------------------------
 // Library-held scratch of the allocation-free entry point (cm_sample_nodes_noalloc): the two
 // stage-1 tables of the node resampling, CM_SAMPLE_NODES_MAX_COLUMNS ints each (32 MiB).  Every
@@ -1747,6 +1750,11 @@ struct System {
 
   // Slots of S
   enum { S_RR = 0, S_BB = 1, S_PAP = 2, S_RZP = 3, S_RZ0 = 4, S_RZ1 = 5, S_IR_RR = 8 };
+  // early-exit rule for a solve started from a predicted x0 (modes fp64 / ref): after fb_iters
+  // iterations the FP64 true residual must be below 1 / fb_drop of the initial one, else the
+  // solve stops with flag FLAG_FALLBACK (the caller then solves plainly); fb_iters = 0: off
+  int fb_iters = 0;
+  double fb_drop = 10.0;
 
   // FP64 PCG (rb/gpusolve.py:229-254), V-cycle in FP64 (vprec 0) or FP32 (vprec 1)
   void pcg64(int vprec, const double* b, double* x, double tol, int maxiter, int check_every, cm_solve_info* info) {
@@ -1771,6 +1779,7 @@ struct System {
     dot(n, r64.p, z64.p, cur);
     dot(n, r64.p, r64.p, S_RR);
     double rn = std::sqrt(read_slot(S_RR));
+    const double r_initial = rn;   // r64 = b - A x0 exactly here
     int it = 0, restarts = 0, stalls = 0, flag = rn <= thr ? 0 : 1;
     bool flexible = false;
     double last_true = 1e300;
@@ -1784,6 +1793,17 @@ struct System {
                          r64.p, p64.p, Ap64.p, S.p, cur, S_PAP, partial.p, z32_path ? r32.p : (float*)nullptr);
       finalize_to<double>(S_RR);
       ++it;
+      if (fb_iters > 0 && it == fb_iters) {
+        // Ap64 is free until the next product: the true residual b - A x goes there
+        apply<double>(0, x, Ap64.p, MODE_RESID, b);
+        dot(n, Ap64.p, Ap64.p, S_IR_RR);
+        const double rt = std::sqrt(read_slot(S_IR_RR));
+        if (rt > thr && rt * fb_drop > r_initial) {
+          rn = rt;
+          flag = cm::FLAG_FALLBACK;
+          break;
+        }
+      }
       if (it < check_every || it % check_every == 0) {
         rn = std::sqrt(read_slot(S_RR));
         if (rn <= target) {
@@ -1832,7 +1852,7 @@ struct System {
       CM_LAUNCH_CHECK();
       std::swap(cur, nxt);
     }
-    if (flag) {
+    if (flag > 0) {
       apply<double>(0, x, r64.p, MODE_RESID, b);
       dot(n, r64.p, r64.p, S_RR);
       rn = std::sqrt(read_slot(S_RR));
@@ -2242,6 +2262,15 @@ int cm_sys_vcycle(cm_sys* h, int prec, const void* b, void* x) {
   OwnStream os(s);
   if (prec == 0) s.vcycle_top<double>(cp<double>(b), mp<double>(x));
   else s.vcycle_top<float>(cp<float>(b), mp<float>(x));
+  CM_API_END
+}
+
+int cm_sys_set_fallback(cm_sys* h, int iters, double drop) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  CM_CHECK(iters >= 0 && drop >= 1.0, "fallback: iters >= 0 and drop >= 1");
+  h->s.fb_iters = iters;
+  h->s.fb_drop = drop;
   CM_API_END
 }
 

@@ -325,6 +325,15 @@ class VoxelSystem:
                                         ptr(cols, ctypes.c_longlong), ptr(vals, ctypes.c_double),
                                         ptr(Einv, ctypes.c_double)))
 
+    def clear_rigid(self):
+        """Remove the additive coarse correction set by set_rigid."""
+        check(self.lib.cm_sys_set_rigid(self.h, 0, 0, None, None, None, None))
+
+    def set_fallback(self, iters=5, drop=10.0):
+        """Early exit of a solve from a predicted x0: unless the FP64 true residual has dropped by
+        ``drop`` after ``iters`` iterations, solve() stops with flag -2 (iters = 0: off)."""
+        check(self.lib.cm_sys_set_fallback(self.h, int(iters), float(drop)))
+
     def sync(self):
         check(self.lib.cm_sys_sync(self.h))
 

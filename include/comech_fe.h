@@ -32,7 +32,8 @@ typedef struct cm_sys cm_sys;
 /* Result of cm_sys_solve; mirrors the info dict of rb/gpusolve.py:259-260,285. */
 typedef struct cm_solve_info {
   int iters;        /* PCG iterations (all inner iterations in mixed mode) */
-  int flag;         /* 0 converged, else maxiter (rb/gpusolve.py:252-254) */
+  int flag;         /* 0 converged, else maxiter (rb/gpusolve.py:252-254); -2: the early-exit
+                       rule of cm_sys_set_fallback stopped the solve */
   int outer;        /* refinement steps (mixed mode), recurrence starts otherwise */
   int levels;       /* multigrid levels */
   double relres;    /* ||b - A x|| / ||b|| (FP64) at exit */
@@ -120,6 +121,13 @@ int cm_sys_apply(cm_sys* sys, int level, int prec, const void* u, void* y);
 
 /* x = V-cycle(b) on level 0 (GPUMG.vcycle, rb/gpusolve.py:163-171). */
 int cm_sys_vcycle(cm_sys* sys, int prec, const void* b, void* x);
+
+/* Early exit of a solve started from a predicted initial guess (modes 0 and 1): at iteration
+ * `iters` the FP64 true residual ||b - A x|| must be below ||b - A x0|| / drop (or already below
+ * tol), else cm_sys_solve stops with info->flag = -2 and the caller solves again without the
+ * prediction (plain GMG-PCG as the fallback).  iters = 0 turns the rule off (default).  No
+ * reference counterpart: the reference's rb engine ignores U0 (fe/simulate.py:17). */
+int cm_sys_set_fallback(cm_sys* sys, int iters, double drop);
 
 /* Multigrid level internals (fe/gmg.py:29-76; tests and diagnostics).  Levels below the
  * coarsest only.  what 0: the Chebyshev bounds (lmax, lmin) of the level in precision prec, 2

@@ -15,7 +15,10 @@ The oracle is a plain numpy/scipy restatement of the reference algorithm (xtli12
                          ``assemble_cn``, ``solve_cn``, ``cell_stress_cn``, ``element_forces_cn``;
 * ``rb/replica3.py`` -- ligament ``spring_matrix_cn``;
 * ``rb/gpusolve.py`` -- the hand-rolled PCG with its residual-check cadence;
-* ``rb/rbsim.py``    -- ``sample_fe`` resampling.
+* ``rb/rbsim.py``    -- ``sample_fe`` resampling;
+* ``net_oracle.py``  -- the learned warm-start network (no reference counterpart) in FP64, the
+                         checker of the GPU training kernels, gradient-checked by central
+                         differences as the reference's ml/test_ad.py:7-22 checks its autodiff.
 
 Each function cites the reference file:line it follows.  How every function is pinned
 (closed forms, invariants, brute force, library routines) is listed in ``oracle/fe_oracle.py``'s

@@ -20,13 +20,14 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_library_exports_every_declared_symbol():
-    hdr = open(os.path.join(ROOT, "include", "comech_fe.h")).read()
-    names = set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+(cm_\w+)\s*\(", hdr, re.M))
-    assert len(names) >= 20
     lib = _lib.load()
-    for n in sorted(names):
-        assert hasattr(lib, n), n
-    assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
+    for header, table in (("comech_fe.h", _lib.SIGNATURES), ("comech_learned.h", _lib.SIGNATURES_NN)):
+        hdr = open(os.path.join(ROOT, "include", header)).read()
+        names = set(re.findall(r"^\s*(?:const\s+)?\w+\*?\s+(cm_\w+)\s*\(", hdr, re.M))
+        assert len(names) >= 15, header
+        for n in sorted(names):
+            assert hasattr(lib, n), n
+        assert names == set(table), names ^ set(table)
     assert lib.cm_version() == 1
 
 

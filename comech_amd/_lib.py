@@ -94,6 +94,13 @@ SIGNATURES_NN = {
     "cm_nn_node_to_cell": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_float, c_void_p],
     "cm_nn_cell_to_node": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_ll,
                            c_void_p, c_double, c_void_p],
+    "cm_nn_node_groups": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_ll, c_void_p],
+    "cm_nn_group_centres": [c_void_p, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+                            c_int, c_void_p],
+    "cm_nn_zt": [c_void_p, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                 c_void_p, c_void_p, c_void_p, c_void_p],
+    "cm_nn_zc": [c_void_p, c_int, c_int, c_int, c_ll, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+                 c_void_p, c_void_p, c_void_p, c_void_p],
     "cm_nn_fill": [c_void_p, c_ll, c_void_p, c_float],
     "cm_nn_cast": [c_void_p, c_ll, c_void_p, c_void_p, c_int],
     "cm_nn_axpy": [c_void_p, c_ll, c_float, c_void_p, c_void_p],

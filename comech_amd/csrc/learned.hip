@@ -416,6 +416,120 @@ __global__ void k_cell_to_node_fin(long long nn, const float* __restrict__ acc, 
   for (int d = 0; d < 3; ++d) x[3 * n + d] = cnt > 0.f ? scale * (double)(acc[4 * n + d] / cnt) : 0.0;
 }
 
+// ---- Galerkin coarse space of the prediction ------------------------------------------------
+// Node groups: -1 unused, 0 soft tissue (a node of active part-0 cells only), g >= 1 bone part g
+// (a node of a cell of that part; split copies belong to one bone).  Columns of Z: 0..2 the
+// prediction's x, y, z components on soft nodes; 3 + 6 (g - 1) + q the rigid modes of bone g
+// (translations x, y, z, then rotations about x, y, z of r = (p - centre_g) / 20, the modes of
+// fe/solver.py:177-199).
+__global__ void k_node_groups(int nx, int ny, int nz, const uint8_t* __restrict__ lab, const uint8_t* __restrict__ part,
+                              const int* __restrict__ irr_of, const long long* __restrict__ irr_nodes,
+                              int* __restrict__ group) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= (long long)nx * ny * nz || lab[v] == 255) return;
+  const int p = part[v];
+  for (int c = 0; c < 8; ++c) atomicMax(group + cell_node(nx, ny, nz, v, c, irr_of, irr_nodes), p);
+}
+
+// grid position of level-0 node n (a split copy sits at its original node)
+__device__ __forceinline__ void node_pos(long long n, long long nnode, const long long* __restrict__ copy_orig, int nx,
+                                         int ny, const double* __restrict__ xe, const double* __restrict__ ye,
+                                         const double* __restrict__ ze, double p[3]) {
+  const long long g = n < nnode ? n : copy_orig[n - nnode];
+  p[0] = xe[g % (nx + 1)];
+  p[1] = ye[(g / (nx + 1)) % (ny + 1)];
+  p[2] = ze[g / ((long long)(nx + 1) * (ny + 1))];
+}
+
+struct Groups {
+  const int* group;                  // [nn]
+  const long long* copy_orig;        // [nn - nnode]
+  const double *xe, *ye, *ze;
+  long long nn, nnode;
+  int nx, ny, ngroup;                // ngroup: bone groups (g = 1..ngroup)
+  const double* centre;              // [ngroup][3]
+  const double* pred;                // [nn][3] FP64 prediction
+};
+
+__device__ __forceinline__ void zrow(const Groups& G, long long n, int& g, double z[6][3]) {
+  g = G.group[n];
+  if (g < 0) return;
+  if (g == 0) {
+    for (int c = 0; c < 3; ++c)
+      for (int d = 0; d < 3; ++d) z[c][d] = c == d ? G.pred[3 * n + d] : 0.0;
+    return;
+  }
+  double p[3];
+  node_pos(n, G.nnode, G.copy_orig, G.nx, G.ny, G.xe, G.ye, G.ze, p);
+  const double* cg = G.centre + 3 * (g - 1);
+  const double r0 = (p[0] - cg[0]) / 20.0, r1 = (p[1] - cg[1]) / 20.0, r2 = (p[2] - cg[2]) / 20.0;
+  const double m[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, -r2, r1}, {r2, 0, -r0}, {-r1, r0, 0}};
+  for (int q = 0; q < 6; ++q)
+    for (int d = 0; d < 3; ++d) z[q][d] = m[q][d];
+}
+__device__ __forceinline__ int zcol0(int g) { return g == 0 ? 0 : 3 + 6 * (g - 1); }
+__device__ __forceinline__ int zncol(int g) { return g == 0 ? 3 : 6; }
+
+// group centres: sums of node positions and counts per bone group into acc[ngroup][4]
+__global__ void k_group_centres(Groups G, double* __restrict__ acc) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.nn) return;
+  const int g = G.group[n];
+  if (g < 1) return;
+  double p[3];
+  node_pos(n, G.nnode, G.copy_orig, G.nx, G.ny, G.xe, G.ye, G.ze, p);
+  for (int d = 0; d < 3; ++d) atomicAdd(acc + 4 * (g - 1) + d, p[d]);
+  atomicAdd(acc + 4 * (g - 1) + 3, 1.0);
+}
+
+// out[k] += (Z^T v)[k] for ncol = 3 + 6 ngroup columns; block partial sums in LDS, then atomics
+__global__ __launch_bounds__(256) void k_zt(Groups G, const double* __restrict__ v, double* __restrict__ out) {
+  extern __shared__ double sh[];
+  const int ncol = 3 + 6 * G.ngroup;
+  for (int k = threadIdx.x; k < ncol; k += blockDim.x) sh[k] = 0.0;
+  __syncthreads();
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n < G.nn) {
+    int g;
+    double z[6][3];
+    zrow(G, n, g, z);
+    if (g >= 0) {
+      const int c0 = zcol0(g), nc = zncol(g);
+      for (int q = 0; q < nc; ++q) {
+        const double s = z[q][0] * v[3 * n] + z[q][1] * v[3 * n + 1] + z[q][2] * v[3 * n + 2];
+        if (s != 0.0) atomicAdd(sh + c0 + q, s);
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < ncol; k += blockDim.x)
+    if (sh[k] != 0.0) atomicAdd(out + k, sh[k]);
+}
+
+// x = Z c (every node: its group's columns)
+__global__ void k_zc(Groups G, const double* __restrict__ c, double* __restrict__ x) {
+  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= G.nn) return;
+  int g;
+  double z[6][3];
+  zrow(G, n, g, z);
+  double o[3] = {0.0, 0.0, 0.0};
+  if (g >= 0) {
+    const int c0 = zcol0(g), nc = zncol(g);
+    for (int q = 0; q < nc; ++q)
+      for (int d = 0; d < 3; ++d) o[d] += z[q][d] * c[c0 + q];
+  }
+  for (int d = 0; d < 3; ++d) x[3 * n + d] = o[d];
+}
+
+__global__ void k_fill_i32(long long n, int* __restrict__ p, int v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+__global__ void k_fill_f64(long long n, double* __restrict__ p, double v) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 __global__ void k_fill_f32(long long n, float* __restrict__ p, float v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -625,6 +739,58 @@ int cm_nn_cell_to_node(void* stream, int nx, int ny, int nz, const uint8_t* lab,
   hipLaunchKernelGGL(k_cell_to_node_acc, g1((long long)nx * ny * nz), dim3(256), 0, st, nx, ny, nz, lab, irr_of,
                      irr_nodes, pred, ldp, acc);
   hipLaunchKernelGGL(k_cell_to_node_fin, g1(nn), dim3(256), 0, st, nn, acc, scale, x);
+  CM_LAUNCH_CHECK();
+  NN_END
+}
+
+int cm_nn_node_groups(void* stream, int nx, int ny, int nz, const uint8_t* lab, const uint8_t* part, const int* irr_of,
+                      const long long* irr_nodes, long long nn, int* group) {
+  NN_BEGIN
+  const hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill_i32, g1(nn), dim3(256), 0, st, nn, group, -1);
+  hipLaunchKernelGGL(k_node_groups, g1((long long)nx * ny * nz), dim3(256), 0, st, nx, ny, nz, lab, part, irr_of,
+                     irr_nodes, group);
+  CM_LAUNCH_CHECK();
+  NN_END
+}
+
+static Groups groups_of(int nx, int ny, int nz, long long nn, const int* group, const long long* copy_orig,
+                        const double* xe, const double* ye, const double* ze, int ngroup, const double* centre,
+                        const double* pred) {
+  return Groups{group, copy_orig, xe, ye, ze, nn, (long long)(nx + 1) * (ny + 1) * (nz + 1), nx, ny, ngroup, centre,
+                pred};
+}
+
+int cm_nn_group_centres(void* stream, int nx, int ny, int nz, long long nn, const int* group, const long long* copy_orig,
+                        const double* xe, const double* ye, const double* ze, int ngroup, double* acc) {
+  NN_BEGIN
+  const hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill_f64, g1(4LL * ngroup), dim3(256), 0, st, 4LL * ngroup, acc, 0.0);
+  hipLaunchKernelGGL(k_group_centres, g1(nn), dim3(256), 0, st,
+                     groups_of(nx, ny, nz, nn, group, copy_orig, xe, ye, ze, ngroup, nullptr, nullptr), acc);
+  CM_LAUNCH_CHECK();
+  NN_END
+}
+
+int cm_nn_zt(void* stream, int nx, int ny, int nz, long long nn, const int* group, const long long* copy_orig,
+             const double* xe, const double* ye, const double* ze, int ngroup, const double* centre, const double* pred,
+             const double* v, double* out) {
+  NN_BEGIN
+  CM_CHECK(ngroup >= 0 && ngroup <= 40, "zt: at most 40 bone groups");
+  const hipStream_t st = (hipStream_t)stream;
+  const int ncol = 3 + 6 * ngroup;
+  hipLaunchKernelGGL(k_zt, g1(nn), dim3(256), ncol * sizeof(double), st,
+                     groups_of(nx, ny, nz, nn, group, copy_orig, xe, ye, ze, ngroup, centre, pred), v, out);
+  CM_LAUNCH_CHECK();
+  NN_END
+}
+
+int cm_nn_zc(void* stream, int nx, int ny, int nz, long long nn, const int* group, const long long* copy_orig,
+             const double* xe, const double* ye, const double* ze, int ngroup, const double* centre, const double* pred,
+             const double* c, double* x) {
+  NN_BEGIN
+  hipLaunchKernelGGL(k_zc, g1(nn), dim3(256), 0, (hipStream_t)stream,
+                     groups_of(nx, ny, nz, nn, group, copy_orig, xe, ye, ze, ngroup, centre, pred), c, x);
   CM_LAUNCH_CHECK();
   NN_END
 }

@@ -77,20 +77,29 @@ def encode(desc, device):
     ftot = float(np.abs(fv).sum())
     lmax = float(np.abs(load).max()) if len(load) else 0.0
     t = lambda a, dt: torch.as_tensor(np.ascontiguousarray(a), dtype=dt, device=dev)
-    lab = t(desc["labels"], torch.uint8)
+    # every device input stays referenced by the returned dict, so no buffer goes back to the
+    # allocator while a kernel that reads it may still be queued
+    keep = dict(lab=t(desc["labels"], torch.uint8), cls=t(cls, torch.int8), nu=t(nus, torch.float32),
+                split=t(split, torch.uint8), ground=t(ground, torch.float32), lig=t(lig, torch.float32),
+                load=t(load, torch.float32))
     F = torch.empty(V * IN_CH, dtype=torch.int16, device=dev)
     st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
-    check_nn(lib.cm_nn_features(st, nx, ny, nz, _vp(lab), _vp(t(cls, torch.int8)), _vp(t(nus, torch.float32)),
-                                _vp(t(split, torch.uint8)), kif, _vp(t(ground, torch.float32)),
-                                _vp(t(lig, torch.float32)), _vp(t(load, torch.float32)),
+    check_nn(lib.cm_nn_features(st, nx, ny, nz, _vp(keep["lab"]), _vp(keep["cls"]), _vp(keep["nu"]),
+                                _vp(keep["split"]), kif, _vp(keep["ground"]), _vp(keep["lig"]), _vp(keep["load"]),
                                 1.0 / lmax if lmax > 0 else 0.0, _vp(F)))
     irr_of = np.full(V, -1, np.int32)
     irr_of[np.asarray(desc["irr_cells"], np.int64)] = np.arange(len(desc["irr_cells"]), dtype=np.int32)
     irr_nodes = np.asarray(desc["irr_nodes"], np.int64).reshape(-1, 8)
     if len(irr_nodes) == 0:
         irr_nodes = np.zeros((1, 8), np.int64)
+    lab = keep["lab"]
+    if "part" in desc:
+        keep["part"] = t(desc["part"], torch.uint8)
+    keep["copy_orig"] = t(copy_orig if len(copy_orig) else np.zeros(1, np.int64), torch.int64)
+    keep["edges"] = [t(np.asarray(e, float), torch.float64) for e in desc["edges"]]
     return dict(F=F, lab=lab, irr_of=t(irr_of, torch.int32), irr_nodes=t(irr_nodes, torch.int64), dims=(nx, ny, nz),
-                nn=int(desc["nn"]), ftot=ftot, mask=(lab != 255).to(torch.uint8))
+                nn=int(desc["nn"]), ftot=ftot, mask=(lab != 255).to(torch.uint8), inputs=keep, kif=kif,
+                load_scale=1.0 / lmax if lmax > 0 else 0.0)
 
 
 def targets(enc, x, tscale):
@@ -113,7 +122,9 @@ def to_nodes(enc, pred, tscale):
     nx, ny, nz = enc["dims"]
     dev = pred.device
     nn = enc["nn"]
-    acc = torch.empty(4 * nn, dtype=torch.float32, device=dev)
+    acc = enc.get("acc")
+    if acc is None or acc.numel() != 4 * nn or acc.device != dev:
+        acc = enc["acc"] = torch.empty(4 * nn, dtype=torch.float32, device=dev)
     x = torch.empty(3 * nn, dtype=torch.float64, device=dev)
     st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
     check_nn(lib.cm_nn_cell_to_node(st, nx, ny, nz, _vp(enc["lab"]), _vp(enc["irr_of"]), _vp(enc["irr_nodes"]),

@@ -318,14 +318,15 @@ def hindfoot_large(dims=(384, 256, 256), h=0.25, seed=5000):
 
 def level0_description(case):
     """The GPU solver's inputs of a hindfoot case in compact form (numpy only): grid edges, labels
-    and their (lambda, mu) tables (255 = inactive), split copies and the cells touching them, ground
-    springs as COO, interface and ligament springs as rank-one terms, and the non-zero nodal loads
-    (the load acts on a few top layers only)."""
+    and their (lambda, mu) tables (255 = inactive), the bone part of every cell (0 = none), split
+    copies and the cells touching them, ground springs as COO, interface and ligament springs as
+    rank-one terms, and the non-zero nodal loads (the load acts on a few top layers only)."""
     g = case["grid"]
     extra, terms, f = solve_inputs_hindfoot(case)
     irr_c, irr_n = irregular_cells(g, case["active"], case["cn"])
     nz = np.flatnonzero(f)
-    return dict(edges=(g.xe, g.ye, g.ze), labels=case["labels"], lam_tab=case["lam_tab"], mu_tab=case["mu_tab"],
+    return dict(edges=(g.xe, g.ye, g.ze), labels=case["labels"], part=np.asarray(case["part"]).astype(np.uint8),
+                lam_tab=case["lam_tab"], mu_tab=case["mu_tab"],
                 copy_orig=case["orig"][g.nnode:], irr_cells=irr_c, irr_nodes=irr_n, extra=extra, terms=terms,
                 f_idx=nz, f_val=f[nz], nn=len(case["orig"]), action=case["action"], ncell=g.ncell,
                 nactive=int(case["active"].sum()))

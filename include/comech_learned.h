@@ -99,6 +99,27 @@ int cm_nn_cell_to_node(void* stream, int nx, int ny, int nz, const uint8_t* lab,
                        const long long* irr_nodes, const float* pred, int ldp, long long nn, float* acc, double scale,
                        double* x);
 
+/* Galerkin coarse space of a prediction (the learned initial guess x0 = Z (Z^T A Z)^-1 Z^T b, the
+ * form of the additive correction of rb/gpusolve.py:217-228).  Node groups: group[n] = -1 unused, 0
+ * soft tissue (nodes of active part-0 cells only), g >= 1 bone part g (part: uint8 per cell).  Z
+ * has 3 + 6 ngroup columns: the prediction's x, y, z components on soft nodes (pred: FP64 [nn][3]),
+ * then per bone g the rigid modes of fe/solver.py:177-199 (3 translations, 3 rotations of
+ * r = (p - centre_g) / 20; p: the node's grid position, a split copy at its original node, from
+ * the edges xe, ye, ze; copy_orig[nn - nnode]). */
+int cm_nn_node_groups(void* stream, int nx, int ny, int nz, const uint8_t* lab, const uint8_t* part, const int* irr_of,
+                      const long long* irr_nodes, long long nn, int* group);
+/* acc[ngroup][4]: sums of the positions of each bone group's nodes and their count. */
+int cm_nn_group_centres(void* stream, int nx, int ny, int nz, long long nn, const int* group, const long long* copy_orig,
+                        const double* xe, const double* ye, const double* ze, int ngroup, double* acc);
+/* out[3 + 6 ngroup] += Z^T v (v: FP64 [nn][3]); ngroup <= 40. */
+int cm_nn_zt(void* stream, int nx, int ny, int nz, long long nn, const int* group, const long long* copy_orig,
+             const double* xe, const double* ye, const double* ze, int ngroup, const double* centre, const double* pred,
+             const double* v, double* out);
+/* x = Z c (FP64 [nn][3]). */
+int cm_nn_zc(void* stream, int nx, int ny, int nz, long long nn, const int* group, const long long* copy_orig,
+             const double* xe, const double* ye, const double* ze, int ngroup, const double* centre, const double* pred,
+             const double* c, double* x);
+
 /* Small vector helpers: fill, FP32 <-> BF16 casts (to_bf16 = 1: f32 -> bf16), y += a x. */
 int cm_nn_fill(void* stream, long long n, float* p, float v);
 int cm_nn_cast(void* stream, long long n, void* f32, void* bf16, int to_bf16);

@@ -44,6 +44,8 @@ SIGNATURES = {
     "cm_sys_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_int, c_double, ctypes.POINTER(SolveInfo)],
     "cm_cell_stress": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_void_p,
                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "cm_cell_stress_ortho": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p,
+                             c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "cm_element_forces": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_int, c_ll, c_void_p,
                           c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "cm_sys_time_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, P_double],

@@ -14,7 +14,8 @@ __global__ void k_cell_stress(long long n, const long long* __restrict__ cells, 
                               int nx, int ny, int nz, const double* __restrict__ hx, const double* __restrict__ hy,
                               const double* __restrict__ hz, const double* __restrict__ lam,
                               const double* __restrict__ mu, const double* __restrict__ U, double* __restrict__ sig,
-                              double* __restrict__ vm, double* __restrict__ eps) {
+                              double* __restrict__ vm, double* __restrict__ eps,
+                              const double* __restrict__ coeffs = nullptr) {
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= n) return;
   const long long c = cells[e];
@@ -34,7 +35,21 @@ __global__ void k_cell_stress(long long n, const long long* __restrict__ cells, 
     }
   }
   double ep[6], sg[6], v;
-  cell_centre_stress<double>(ue, 0.25 / hx[i], 0.25 / hy[j], 0.25 / hz[k], lam[c], mu[c], ep, sg, &v);
+  cell_centre_stress<double>(ue, 0.25 / hx[i], 0.25 / hy[j], 0.25 / hz[k], coeffs ? 0.0 : lam[c], coeffs ? 0.0 : mu[c],
+                             ep, sg, &v);
+  if (coeffs) {
+    // z-orthotropic D = sum_k c_k P_k (fe/solver.py:43-52, 267-269): normal block [[c0 c1 c2] [c1 c0 c2]
+    // [c2 c2 c3]], shear c4 (xy), c5 (yz, xz); von Mises of fe/solver.py:270-271
+    const double* cc = coeffs + 6 * c;
+    sg[0] = cc[0] * ep[0] + cc[1] * ep[1] + cc[2] * ep[2];
+    sg[1] = cc[1] * ep[0] + cc[0] * ep[1] + cc[2] * ep[2];
+    sg[2] = cc[2] * ep[0] + cc[2] * ep[1] + cc[3] * ep[2];
+    sg[3] = cc[4] * ep[3];
+    sg[4] = cc[5] * ep[4];
+    sg[5] = cc[5] * ep[5];
+    const double dxy = sg[0] - sg[1], dyz = sg[1] - sg[2], dzx = sg[2] - sg[0];
+    v = sqrt(0.5 * (dxy * dxy + dyz * dyz + dzx * dzx) + 3.0 * (sg[3] * sg[3] + sg[4] * sg[4] + sg[5] * sg[5]));
+  }
   for (int q = 0; q < 6; ++q) { sig[6 * e + q] = sg[q]; eps[6 * e + q] = ep[q]; }
   vm[e] = v;
 }

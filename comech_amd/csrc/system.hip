@@ -2374,6 +2374,20 @@ int cm_cell_stress(void* stream, int nx, int ny, int nz, const double* hx, const
   CM_API_END
 }
 
+int cm_cell_stress_ortho(void* stream, int nx, int ny, int nz, const double* hx, const double* hy, const double* hz,
+                         int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* coeffs_ref,
+                         const double* U, double* sig, double* vm, double* eps) {
+  CM_API_BEGIN
+  CM_CHECK(nx > 0 && ny > 0 && nz > 0, "bad grid");
+  CM_CHECK(coeffs_ref != nullptr, "coeffs must not be null");
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(cm::k_cell_stress, cm::grid1(n), dim3(256), 0, (hipStream_t)stream, (long long)n,
+                     (const long long*)cells_ref, (const long long*)nodes8, nx, ny, nz, hx, hy, hz, (const double*)nullptr,
+                     (const double*)nullptr, U, sig, vm, eps, coeffs_ref);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
 int cm_element_forces(void* stream, int nx, int ny, int nz, const double* hx, const double* hy, const double* hz,
                       int sri, int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* lam_ref,
                       const double* mu_ref, const double* U, double* out) {

@@ -7,6 +7,8 @@ interface springs, the ``extra(nmap, n)`` callback) become a small COO over leve
 ``3*node+c``.  Node ids are the reference's own (grid ids, then split copies), so the level-0
 vector *is* the reference's ``U`` array (nnode_total, 3) without the NaN of unused nodes.
 """
+import time
+
 import numpy as np
 import scipy.sparse as sp
 import torch
@@ -92,6 +94,10 @@ def rigid_coo(xyz, used, node_groups):
     return (np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)), col
 
 
+# coarsest-level method (the "coarse" key of rb/gpusolve.py:259-260): a dense FP64 inverse formed on
+# the GPU by blocked Gauss-Jordan, applied as a matrix-vector product
+COARSE = "dense-inverse"
+
 MG_KEYS = ("min_dofs", "max_levels", "levels", "smoother", "degree", "omega", "npre", "npost", "copy_blocks")
 SOLVE_KEYS = ("mode", "check_every", "inner_tol")
 
@@ -128,9 +134,11 @@ class Problem:
         self.nmap = -np.ones(self.nn, np.int64)
         self.nmap[self.nused] = np.arange(len(self.nused))
         self.ndof = 3 * len(self.nused)
+        t0 = time.perf_counter()
         self.sys = VoxelSystem(grid, sri=sri, active=self.active, lam=self.lam, mu=self.mu,
                                copy_orig=self.copy_orig, irr_cells=self.irr_cells, irr_nodes=self.irr_nodes,
                                extra=extra, rank1=rank1, device=device, **mg)
+        self.t_hier = time.perf_counter() - t0
         self.device = self.sys.device
         if node_groups:
             xyz = grid.node_xyz if cn is None else grid.node_xyz[np.asarray(orig)]
@@ -145,9 +153,27 @@ class Problem:
 
     def solve(self, b_full, x0_full=None, tol=1e-8, maxiter=800, **so):
         x, info = self.sys.solve(b_full, x0_full, tol=tol, maxiter=maxiter, **so)
+        info.update(t_gpu=round(info["t_solve_ms"] / 1e3, 4), t_hier=round(self.t_hier, 4), coarse=COARSE)
+        return self._host(x), info, x
+
+    def direct(self, b_full):
+        """x = A^-1 b with the dense inverse of a one-level system (the coarsest-level solve of the
+        V-cycle), and its FP64 true residual."""
+        if len(self.sys.levels()) != 1:
+            raise ValueError("direct needs a one-level system (levels=1)")
+        t0 = time.perf_counter()
+        x = self.sys.vcycle(b_full)
+        r = float(torch.linalg.vector_norm(b_full - self.sys.apply(x)) /
+                  max(float(torch.linalg.vector_norm(b_full)), 1e-300))
+        torch.cuda.synchronize(self.device)
+        info = dict(iters=0, flag=0, outer=0, levels=1, relres=r, mode="direct", device=str(self.device),
+                    t_gpu=round(time.perf_counter() - t0, 4), t_hier=round(self.t_hier, 4), coarse=COARSE)
+        return self._host(x), info, x
+
+    def _host(self, x):
         U = x.view(-1, 3).cpu().numpy().copy()
         U[~self.used] = np.nan
-        return U, info, x
+        return U
 
     def close(self):
         self.sys.close()

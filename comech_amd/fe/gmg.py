@@ -10,6 +10,8 @@ trilinear ``P`` of fe/gmg.py:7-26, formed on the host once per system as in rb/g
 smoothing, V-cycle, coarse solve and PCG run on the GPU.  Options as ``comech_amd.fe.solver.solve``
 (default: FP64 throughout, residual checked every iteration as scipy's ``cg``).
 """
+import time
+
 import numpy as np
 import scipy.sparse as sp
 import torch
@@ -49,7 +51,9 @@ def pcg_gmg(A, b, used_nodes, grid_edges, tol=1e-7, maxiter=500, x0=None, A_op=N
     so.setdefault("mode", "fp64")
     so.setdefault("check_every", 1)
     A = sp.csr_matrix(A)
+    t0 = time.perf_counter()
     sys_, node_of = csr_system(A, used_nodes, grid_edges, device=device, **mg)
+    t_hier = time.perf_counter() - t0
     try:
         if Z is not None:
             Zc = sp.coo_matrix(Z)
@@ -62,6 +66,7 @@ def pcg_gmg(A, b, used_nodes, grid_edges, tol=1e-7, maxiter=500, x0=None, A_op=N
             xf = np.zeros(n); xf[dofs] = np.asarray(x0, float)
             xf = torch.from_numpy(xf).to(sys_.device)
         x, info = sys_.solve(torch.from_numpy(bf).to(sys_.device), xf, tol=tol, maxiter=maxiter, **so)
+        info.update(t_gpu=round(info["t_solve_ms"] / 1e3, 4), t_hier=round(t_hier, 4), coarse=E.COARSE)
         return x.cpu().numpy()[dofs], info
     finally:
         sys_.close()

@@ -208,8 +208,15 @@ def main():
         f"mean {np.mean(iters):.1f} iterations")
     ntr = len(descs_tr)
     tr, te = np.arange(ntr), np.arange(ntr, len(descs))
-    act = M[:ntr].bool().unsqueeze(-1).expand(-1, -1, 3).reshape(ntr, -1)
-    tscale = 1.0 / float(torch.sqrt((T[:ntr][act] ** 2).mean()).item())
+    V = int(np.prod(DIMS))
+    ss = torch.zeros((), dtype=torch.float64, device=dev)
+    na = 0.0
+    for a in range(0, ntr, 64):                     # RMS of the active cells' targets, in chunks
+        Tc = T[a:a + 64].view(-1, V, 3).double()
+        Mc = M[a:a + 64].view(-1, V, 1).double()
+        ss += (Tc * Tc * Mc).sum()
+        na += 3.0 * float(Mc.sum().item())
+    tscale = 1.0 / float(torch.sqrt(ss / na).item())
     T.mul_(tscale)
     log(f"target scale {tscale:.4g}")
     net = Net(dev, seed=1)

@@ -1,4 +1,6 @@
-"""Drop-in for the reference's ``rb/gpusolve.py`` GPU solver entry points.
+"""Drop-in for the reference's ``rb/gpusolve.py`` GPU solver entry points (``pcg_gmg_gpu``,
+``pick_device``, ``free_gpu``, ``GPUMG``; ``hierarchy`` and ``assemble_gpu`` raise: the GPU path forms
+no CSR matrices).
 
 ``pcg_gmg_gpu(A, b, used_nodes, grid_edges, tol, maxiter, x0, Z, device, mg_dtype)`` keeps the
 reference signature (rb/gpusolve.py:265-292): FP64 PCG with the V-cycle in ``mg_dtype`` (FP32 by
@@ -43,6 +45,56 @@ def free_gpu(dev=None):
                 torch.cuda.empty_cache()
         except Exception:
             pass
+
+
+def hierarchy(A, used_nodes, xe, ye, ze, min_dofs=6000, max_levels=6):
+    """rb/gpusolve.py:39-65 returns the Galerkin levels as host scipy CSR matrices and prolongations.
+    The GPU path builds its hierarchy on the device (matrix-free fine level, exact moment cells on
+    coarse levels, a dense coarsest inverse) and never forms CSR levels, so this raises.  The level
+    table of the same rule is ``GPUMG.from_csr(A, used_nodes, (xe, ye, ze)).levels()``."""
+    raise NotImplementedError("hierarchy: the GPU solver forms no CSR levels; use GPUMG.from_csr(...).levels()")
+
+
+def assemble_gpu(g, lam, mu, cells, cn, KLf, dev, chunk=60000):
+    """rb/gpusolve.py:306-347 assembles the global CSR matrix on the GPU for the CSR solver.  The
+    matrix-free GPU path applies the element operator directly (comech_amd.rb.splitfe.solve_cn)
+    and has no CSR assembly, so this raises."""
+    raise NotImplementedError("assemble_gpu: the GPU path is matrix-free; call comech_amd.rb.splitfe.solve_cn")
+
+
+class GPUMG:
+    """The multigrid preconditioner object of rb/gpusolve.py:132-171 on the GPU hierarchy.
+
+    Build it with ``GPUMG.from_csr(A, used_nodes, grid_edges, device, dt)`` (the inputs of
+    pcg_gmg); ``vcycle(b)`` applies one V-cycle to a device vector in the compact DOF order of A
+    (dtype float32 or float64 as ``dt``); ``mode`` names the coarsest-level method.  The reference
+    constructor ``GPUMG(As, Ps, dev, dt)`` takes host CSR levels, which this path does not form
+    (see ``hierarchy``): called that way it raises."""
+    mode = "dense-inverse"
+
+    def __init__(self, As=None, Ps=None, dev=None, dt=None):
+        raise NotImplementedError("GPUMG(As, Ps, ...) needs host CSR levels; build it with GPUMG.from_csr(A, used_nodes, "
+                                  "grid_edges, device, dt)")
+
+    @classmethod
+    def from_csr(cls, A, used_nodes, grid_edges, device=None, dt=torch.float32, **mg):
+        import numpy as np
+        import scipy.sparse as sp
+        self = cls.__new__(cls)
+        self.dev = torch.device(device or pick_device())
+        self.dt = dt
+        self.sys, node_of = _gmg.csr_system(sp.csr_matrix(A), used_nodes, grid_edges, device=self.dev, **mg)
+        self.dofs = torch.as_tensor((3 * node_of[:, None] + np.arange(3)).ravel(), device=self.dev)
+        self.levels = self.sys.levels
+        return self
+
+    def vcycle(self, b):
+        full = torch.zeros(3 * self.sys.nn, dtype=self.dt, device=self.dev)
+        full[self.dofs] = b.to(self.dt)
+        return self.sys.vcycle(full)[self.dofs]
+
+    def close(self):
+        self.sys.close()
 
 
 def _attempt(A, b, used_nodes, grid_edges, tol, maxiter, x0, Z, dev, mode, **opts):

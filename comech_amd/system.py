@@ -349,9 +349,10 @@ class VoxelSystem:
             pass
 
 
-def cell_stress_gpu(grid, lam, mu, U, cells, nodes8=None, device=None):
+def cell_stress_gpu(grid, lam, mu, U, cells, nodes8=None, device=None, coeffs=None):
     """Cell-centre stress/strain/von Mises on the GPU (fe/solver.py:257-272).  U: host (nnode,3)
-    or device tensor; returns host arrays (sig, vm, eps)."""
+    or device tensor; coeffs: (ncell, 6) z-orthotropic coefficients instead of lam/mu; returns host
+    arrays (sig, vm, eps)."""
     dev = require_gpu(device)
     lib = _lib.load()
     cells = np.asarray(cells, np.int64)
@@ -368,7 +369,12 @@ def cell_stress_gpu(grid, lam, mu, U, cells, nodes8=None, device=None):
     vm = torch.empty(n, dtype=torch.float64, device=dev)
     eps = torch.empty(n * 6, dtype=torch.float64, device=dev)
     vp = lambda x: ctypes.c_void_p(0 if x is None else x.data_ptr())
-    check(lib.cm_cell_stress(ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream), grid.nx, grid.ny, grid.nz,
-                             vp(hx), vp(hy), vp(hz), n, vp(cd), vp(nd), vp(la), vp(ma), vp(Ud), vp(sig), vp(vm),
-                             vp(eps)))
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    if coeffs is not None:
+        cf = t(np.asarray(coeffs, float).reshape(-1, 6), torch.float64)
+        check(lib.cm_cell_stress_ortho(st, grid.nx, grid.ny, grid.nz, vp(hx), vp(hy), vp(hz), n, vp(cd), vp(nd), vp(cf),
+                                       vp(Ud), vp(sig), vp(vm), vp(eps)))
+    else:
+        check(lib.cm_cell_stress(st, grid.nx, grid.ny, grid.nz, vp(hx), vp(hy), vp(hz), n, vp(cd), vp(nd), vp(la),
+                                 vp(ma), vp(Ud), vp(sig), vp(vm), vp(eps)))
     return sig.view(n, 6).cpu().numpy(), vm.cpu().numpy(), eps.view(n, 6).cpu().numpy()

@@ -174,6 +174,13 @@ int cm_cell_stress(void* stream, int nx, int ny, int nz, const double* hx, const
                    int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* lam_ref,
                    const double* mu_ref, const double* U, double* sig, double* vm, double* eps);
 
+/* cm_cell_stress for the z-orthotropic materials of fe/solver.py:43-74 (the coeffs of solve and
+ * cell_stress, fe/solver.py:153-154,267-269): coeffs_ref[ncell*6] (device) in reference cell order,
+ * stress sig = D eps with D = sum_k c_k P_k. */
+int cm_cell_stress_ortho(void* stream, int nx, int ny, int nz, const double* hx, const double* hy, const double* hz,
+                         int64_t n, const int64_t* cells_ref, const int64_t* nodes8, const double* coeffs_ref,
+                         const double* U, double* sig, double* vm, double* eps);
+
 /* Nodal element forces K_e u_e (rb/splitfe.py:178-193) of n listed cells, out[n*24] in element
  * DOF order 3*l + c; arguments as cm_cell_stress, sri as cm_sys_create.  All device. */
 int cm_element_forces(void* stream, int nx, int ny, int nz, const double* hx, const double* hy, const double* hz,

@@ -232,6 +232,9 @@ def batch_workload(args):
     ex = pool(nw)
     t0 = time.time()
     fut_large = ex.submit(roofline_case)              # the first submission starts the host workers
+    from comech_amd.learned import train as learned_train
+    learned_seeds = learned_train.split_feet(500)[1][:LEARNED_FEET]
+    fut_learned = [ex.submit(learned_train.foot_inputs, sd) for sd in learned_seeds]
     futs = {f: ex.submit(synth.batch_foot, f) for f in sorted(set(feet) | set(warm))}
     import torch
     from comech_amd.batch import build_system, rhs
@@ -261,6 +264,8 @@ def batch_workload(args):
     if any(b["flag"] == 0 for b in bad) or any(i["flag"] for i in ref):
         log(f"bench.py: a solve missed rtol {args.tol}")
         sys.exit(1)
+    ll = learned_leg([d for fu in fut_learned for d in fu.result()], dev, args.tol)
+    log(f"learned: {ll['learned_iters']} vs {ll['plain_iters']} iterations, {ll['learned_ms']} vs {ll['plain_ms']} ms")
     rl = roofline(fut_large.result(), dev)
     log(f"roofline: K*u FP32 {rl['ms32'] * 1e3:.1f} us, smoothing step {rl['mss'] * 1e3:.1f} us")
     cells = float(np.mean([descs[f][0]["ncell"] for f in feet]))
@@ -316,6 +321,7 @@ def batch_workload(args):
         "mixed": summarize(mixed, w_mixed, {"ratio_over_plain": round(w_plain / w_mixed, 3)}),
         "plain_fp64": summarize(plain, w_plain),
         "ratio_ref_over_plain": round(w_plain / w_ref, 3),
+        "learned": ll,
         "sequential": {"value": round(len(seq) / w_seq, 3), "unit": "solves/s", "concurrency": 1,
                        "solves": len(seq), "what": f"foot {feet[0]}'s 8 actions one at a time, mode ref"},
         "accuracy": {"what": "displacement relative L2 of every result against the plain FP64 GMG-PCG answer of the "
@@ -340,6 +346,52 @@ def batch_workload(args):
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
+
+
+LEARNED_FEET = 2      # held-out feet of the learned warm start's training split, 8 actions each
+
+
+def learned_leg(descs, dev, tol):
+    """The learned warm start on feet the network never saw (96x64x64, the training family): every
+    system solved from zero and from the learned guess (prediction projected onto its coarse
+    space), one at a time, wall-clocked around the whole call (prediction, projection, solve)."""
+    import numpy as np
+    import torch
+    from comech_amd.batch import build_system, rhs
+    from comech_amd.learned import load_default
+    from comech_amd.learned.data import encode
+    from comech_amd.learned.warmstart import learned_solve
+    net = load_default(dev)
+    rows = []
+    for i, d in enumerate([descs[0]] + descs):       # the first system once more, untimed (warm-up)
+        s = build_system(d, device=dev)
+        b = rhs(d, dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        xp, ip = s.solve(b, tol=tol, maxiter=3000, mode="ref")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        enc = encode(d, dev)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        xl, il = learned_solve(s, d, b, net, tol=tol, mode="ref", enc=enc)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        true = float(torch.linalg.vector_norm(b - s.apply(xl)) / torch.linalg.vector_norm(b))
+        s.close()
+        if i:
+            rows.append(dict(action=d["action"], plain_iters=ip["iters"], learned_iters=il["iters"],
+                             plain_ms=(t1 - t0) * 1e3, learned_ms=(t3 - t2) * 1e3, guess_ms=il["t_guess_ms"],
+                             fallback=il["fallback"], true_relres=true))
+    m = lambda k: round(float(np.mean([r[k] for r in rows])), 2)
+    return {"what": f"{len(rows)} held-out systems (96x64x64 voxels, 8 actions each), solved one at a time in mode "
+                    f"ref from zero and from the learned initial guess; wall ms per solve (learned: input encoding "
+                    f"excluded, prediction and projection included)",
+            "plain_iters": m("plain_iters"), "learned_iters": m("learned_iters"), "plain_ms": m("plain_ms"),
+            "learned_ms": m("learned_ms"), "guess_ms": m("guess_ms"),
+            "ratio_learned_over_plain": round(m("plain_ms") / max(m("learned_ms"), 1e-9), 3),
+            "fallbacks": int(sum(r["fallback"] for r in rows)),
+            "true_relres_max": max(r["true_relres"] for r in rows)}
 
 
 def kernel_workload(args):

@@ -283,3 +283,71 @@ def test_solve_with_penalties_matches_oracle():
                      device=DEV)
     assert info["flag"] == 0 and rel(U, Uo) <= U_TOL
     assert rel(U0, Uo) > 1e-4
+
+
+def test_orthotropic_coeffs_match_oracle():
+    """fe/solver.py:43-74 materials (the parametric engine's reduced out-of-plane cartilage shear,
+    fe/simulate_parametric.py:197-201) on the cube-with-inclusions case: cells of the stiffer
+    material get c5 = 0.2 mu, a third of the others c2 = 0.6 lam; against the oracle's direct solve."""
+    c = C.cube(12)
+    og = C.oracle_grid(c)
+    coeffs = O.ortho_coeffs(c["lam"], c["mu"])
+    stiff = c["lam"] == c["lam"].max()
+    coeffs[stiff, 5] *= 0.2
+    soft = np.flatnonzero(~stiff)[::3]
+    coeffs[soft, 2] *= 0.6
+    Uo, _ = O.solve(og, c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="direct", coeffs=coeffs)
+    cells = np.arange(og.ncell)
+    _, vmo, _ = O.cell_stress(og, c["lam"], c["mu"], Uo, cells, coeffs=coeffs)
+    U, info = FS.solve(c["grid"], c["lam"], c["mu"], c["active"], c["springs"], c["loads"], tol=1e-10, coeffs=coeffs,
+                       device=DEV)
+    assert info["flag"] == 0 and info["relres"] <= 1e-10
+    assert rel(U, Uo) <= U_TOL
+    _, vm, _ = FS.cell_stress(c["grid"], c["lam"], c["mu"], U, cells, coeffs=coeffs, device=DEV)
+    assert rel(vm, vmo) <= VM_TOL
+    # isotropic coefficients reproduce the lam/mu solve
+    U2, _ = FS.solve(c["grid"], c["lam"], c["mu"], c["active"], c["springs"], c["loads"], tol=1e-10,
+                     coeffs=O.ortho_coeffs(c["lam"], c["mu"]), device=DEV)
+    U3, _ = FS.solve(c["grid"], c["lam"], c["mu"], c["active"], c["springs"], c["loads"], tol=1e-10, device=DEV)
+    assert rel(U2, U3) <= 1e-9
+
+
+def test_direct_method_and_info_keys(ref1):
+    """method="direct" (fe/solver.py:236-237) against the oracle's direct solve; the info keys of
+    the reference GPU solver (rb/gpusolve.py:259-260,285) on the GMG path; too large a system for
+    the dense inverse raises."""
+    c, Uref, vmref = ref1
+    U, info = FS.solve(c["grid"], c["lam"], c["mu"], c["active"], c["springs"], c["loads"], method="direct",
+                       device=DEV)
+    assert rel(U, Uref) <= 1e-9
+    U, info = FS.solve(c["grid"], c["lam"], c["mu"], c["active"], c["springs"], c["loads"], tol=1e-8, device=DEV)
+    assert info["t_gpu"] > 0 and info["t_hier"] > 0 and info["coarse"] == "dense-inverse"
+    big = C.cube(24)
+    with pytest.raises(ValueError, match="direct"):
+        FS.solve(big["grid"], big["lam"], big["mu"], big["active"], big["springs"], big["loads"], method="direct",
+                 device=DEV)
+
+
+def test_gpumg_from_csr_is_a_symmetric_contraction(ref2):
+    """GPUMG (rb/gpusolve.py:132-171) built from the CSR inputs of pcg_gmg: its V-cycle M is
+    symmetric (Chebyshev smoothing before and after, exact coarsest solve), and one step of
+    e <- (I - M A) e shrinks the A-norm error of random vectors."""
+    c = ref2[0]
+    og = C.oracle_grid(c)
+    info = C.oracle_system2((24, 16, 16), 3.0)
+    A = info["K"].tocsr()
+    mg = GS.GPUMG.from_csr(A, c["orig"][info["used"]], (og.xe, og.ye, og.ze), device=DEV, dt=torch.float64)
+    assert len(mg.levels()) >= 2
+    rng = np.random.default_rng(5)
+    n = A.shape[0]
+    x, y = (torch.as_tensor(rng.standard_normal(n), device=DEV) for _ in range(2))
+    Mx, My = mg.vcycle(x), mg.vcycle(y)
+    assert abs(float(torch.dot(Mx, y) - torch.dot(x, My))) <= 1e-9 * float(torch.linalg.norm(Mx) * torch.linalg.norm(y))
+    assert float(torch.dot(Mx, x)) > 0
+    e = rng.standard_normal(n)
+    Ae = A @ e
+    e1 = e - mg.vcycle(torch.as_tensor(Ae, device=DEV)).cpu().numpy()
+    assert np.sqrt(e1 @ (A @ e1)) < 0.5 * np.sqrt(e @ Ae)
+    with pytest.raises(NotImplementedError):
+        GS.GPUMG([A], [], DEV, torch.float64)
+    mg.close()

@@ -214,6 +214,66 @@ def test_patch_test_closed_forms():
     assert abs(np.ptp(U[:, 0]) - nu * 10 / E) < 1e-9
 
 
+def _ortho_D(c):
+    """Stiffness of a material symmetric about the z axis in Voigt order (xx, yy, zz, xy, yz, xz),
+    written as the matrix: c0 = D11 = D22, c1 = D12, c2 = D13 = D23, c3 = D33, c4 = D44 (xy),
+    c5 = D55 = D66 (yz, xz)."""
+    c0, c1, c2, c3, c4, c5 = c
+    return np.array([[c0, c1, c2, 0, 0, 0], [c1, c0, c2, 0, 0, 0], [c2, c2, c3, 0, 0, 0],
+                     [0, 0, 0, c4, 0, 0], [0, 0, 0, 0, c5, 0], [0, 0, 0, 0, 0, c5]], float)
+
+
+def test_orthotropic_affine_energy_and_stress():
+    """fe/solver.py:43-74: for u = G x (any displacement gradient G) the trilinear element holds the
+    field exactly and 2x2x2 Gauss integrates its constant strain exactly, so u^T (sum_k c_k KB_k) u =
+    V eps^T D eps and the centre stress is D eps; eps in engineering shear (xy, yz, xz)."""
+    rng = np.random.default_rng(3)
+    g = O.Grid(np.r_[0, 1.5, 2.0], np.r_[0, 0.7], np.r_[0, 2.5, 3.0])
+    c = np.array([9.0, 2.0, 1.5, 6.0, 3.0, 0.8])
+    coeffs = np.tile(c, (g.ncell, 1))
+    K, nmap, used, cells, dofs = O.assemble(g, None, None, np.ones(g.ncell, bool), coeffs)
+    D = _ortho_D(c)
+    assert np.allclose(np.einsum("k,kij->ij", c, O.ortho_patterns()), D)
+    for _ in range(4):
+        G = rng.standard_normal((3, 3))
+        U = g.node_xyz @ G.T
+        u = U[used].ravel()
+        eps = np.array([G[0, 0], G[1, 1], G[2, 2], G[0, 1] + G[1, 0], G[1, 2] + G[2, 1], G[0, 2] + G[2, 0]])
+        vol = g.cell_vol.sum()
+        assert np.isclose(u @ K @ u, vol * eps @ D @ eps, rtol=1e-12)
+        sig, vm, e = O.cell_stress(g, None, None, U, np.arange(g.ncell), coeffs=coeffs)
+        assert np.allclose(e, eps[None], atol=1e-12) and np.allclose(sig, (D @ eps)[None], rtol=1e-12, atol=1e-12)
+    iso = O.ortho_coeffs(np.full(g.ncell, 2.0), np.full(g.ncell, 3.0))
+    Ki, *_ = O.assemble(g, None, None, np.ones(g.ncell, bool), iso)
+    Kl, *_ = O.assemble(g, np.full(g.ncell, 2.0), np.full(g.ncell, 3.0), np.ones(g.ncell, bool))
+    assert abs(Ki - Kl).max() <= 1e-12 * abs(Kl).max()
+
+
+def test_orthotropic_column_compliance():
+    """Uniaxial z load on a free-standing column of a z-symmetric material: uniform stress
+    (0, 0, -1, 0, 0, 0), strain S sigma with S = D^-1, top settlement height * S33, lateral
+    spread width * S13."""
+    g = O.Grid(np.r_[0, 1, 3, 4], np.r_[0, 2, 5], np.r_[0, 1, 2, 4, 6, 10])
+    c = np.array([900.0, 300.0, 250.0, 600.0, 200.0, 60.0])
+    coeffs = np.tile(c, (g.ncell, 1))
+    bot, top = np.flatnonzero(g.cK == 0), np.flatnonzero(g.cK == g.nz - 1)
+    nb, ab = O.face_node_areas(g, bot, "z-")
+    nt, at = O.face_node_areas(g, top, "z+")
+    pin = np.flatnonzero(np.all(g.node_xyz == 0, 1))                # x, y of the origin and y of
+    pin_x = np.flatnonzero(np.all(g.node_xyz == [4, 0, 0], 1))      # (4, 0, 0): no rigid motion left
+    U, info = O.solve(g, None, None, np.ones(g.ncell, bool),
+                      [(nb, ab, np.array([0.0, 0.0, 1e12])), (pin, np.ones(1), np.array([1e6, 1e6, 0.0])),
+                       (pin_x, np.ones(1), np.array([0.0, 1e6, 0.0]))],
+                      [(nt, at, np.array([0, 0, -1.0]))], method="direct", coeffs=coeffs)
+    S = np.linalg.inv(_ortho_D(c))
+    sig, vm, eps = O.cell_stress(g, None, None, U, np.arange(g.ncell), coeffs=coeffs)
+    assert np.allclose(sig, [[0, 0, -1.0, 0, 0, 0]], atol=1e-8)
+    assert np.allclose(eps, (S @ [0, 0, -1.0, 0, 0, 0])[None], rtol=1e-8, atol=1e-14)
+    top_nodes = np.flatnonzero(g.node_xyz[:, 2] == 10)
+    assert np.allclose(U[top_nodes, 2], -10 * S[2, 2], rtol=1e-8)
+    assert np.isclose(np.ptp(U[:, 0]), 4 * abs(S[0, 2]), rtol=1e-8)
+
+
 def test_von_mises_closed_forms():
     s = np.array([[-1, 0, 0, 0, 0, 0], [5, 5, 5, 0, 0, 0], [0, 0, 0, 2, 0, 0], [1, 2, 3, 0, 0, 0.0]])
     vm = O.von_mises(s)

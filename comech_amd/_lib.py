@@ -61,6 +61,7 @@ SIGNATURES = {
     "cm_bank_pack": [c_void_p, c_ll, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
                      c_void_p, c_void_p],
     "cm_part_nodes": [c_void_p, c_ll, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p],
+    "cm_body_fractions": [c_void_p, c_ll, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p],
     "cm_nearest_bone": [c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p,
                         c_void_p, c_void_p],
     "cm_spring_patches": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p,

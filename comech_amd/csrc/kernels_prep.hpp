@@ -433,4 +433,28 @@ __global__ __launch_bounds__(256) void k_spring_patches(int nend, const double* 
   }
 }
 
+// Fractions of a cell's sub-samples that carry a body of each of nset body sets (at most
+// BF_MAX_SETS): mask[b] has bit s set when body b belongs to set s; out[s][c] = count / ns3, the
+// FP64 mean of the membership booleans (exact counts, one division).  One thread per cell; the
+// 16-bit labels of a cell are contiguous (row of L).
+constexpr int BF_MAX_SETS = 16;
+__global__ __launch_bounds__(256) void k_body_fractions(long long ncell, int ns3, const int16_t* __restrict__ L,
+                                                        int nlab, const uint32_t* __restrict__ mask, int nset,
+                                                        double* __restrict__ out) {
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  int cnt[BF_MAX_SETS];
+#pragma unroll
+  for (int s = 0; s < BF_MAX_SETS; ++s) cnt[s] = 0;
+  const int16_t* row = L + c * ns3;
+  for (int q = 0; q < ns3; ++q) {
+    const int b = row[q];
+    const uint32_t m = (b >= 0 && b < nlab) ? mask[b] : 0u;
+#pragma unroll
+    for (int s = 0; s < BF_MAX_SETS; ++s) cnt[s] += (int)((m >> s) & 1u);
+  }
+  const double n = (double)ns3;
+  for (int s = 0; s < nset; ++s) out[(long long)s * ncell + c] = (double)cnt[s] / n;
+}
+
 }  // namespace cm

@@ -2552,6 +2552,21 @@ int cm_lattice_materials(void* stream, int nx, int ny, int nz, int ns3, const in
   CM_API_END
 }
 
+int cm_body_fractions(void* stream, int64_t ncell, int ns3, const int16_t* L, int nlab, const uint32_t* mask, int nset,
+                      double* out) {
+  CM_API_BEGIN
+  CM_CHECK(ncell >= 0, "negative size");
+  CM_CHECK(ns3 >= 1 && ns3 <= cm::LAT_MAX_NS3, "1..64 sub-samples per cell");
+  CM_CHECK(nlab >= 1, "empty body table");
+  CM_CHECK(nset >= 1 && nset <= cm::BF_MAX_SETS, "1..16 body sets");
+  hipStream_t st = (hipStream_t)stream;
+  if (ncell)
+    hipLaunchKernelGGL(cm::k_body_fractions, cm::grid1(ncell), dim3(256), 0, st, (long long)ncell, ns3, L, nlab, mask,
+                       nset, out);
+  CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
 int cm_part_nodes(void* stream, int64_t ncell, const int64_t* cn, const int32_t* part, const uint8_t* active,
                   int64_t nnode, uint64_t* node_mask, int32_t* part_cells) {
   CM_API_BEGIN

@@ -2,7 +2,9 @@
 (``materials``, rb/rbfe.py:75-134), bone shares (``bone_share``, rb/replica3.py:17-29), the
 bone part of every cell (rb/rbfe.py:140-141), and the preparation around the solve
 (``prepare_solve``, rb/rbfe.py:146-178): ground contact, leg loads, the node sets of the bones,
-the bone of every ligament end and the ligament patches.
+the bone of every ligament end and the ligament patches; and ``solve`` (rb/rbfe.py:135-219), the
+engine step around the split-node solve with its results: the region of every cell, the forces on
+the talus and calcaneus from each region, the ground reactions by zone.
 
 A lattice ``V`` (as built by the reference's voxeliser) holds ``L`` -- (ncell, ns^3) body labels
 of the sub-samples of every cell in reference cell order (i ny + j) nz + k -- and ``cls`` -- the
@@ -346,6 +348,159 @@ def prepare_solve(V, g, cn, orig, active, part, names, roles=ROLES, forces=FORCE
     return dict(fixed=fixed, loads=loads, bone_nodes=bone_nodes, springs=springs, extra=extra, groups=groups)
 
 
-__all__ = ["COAL", "FIB", "INTERP", "FUSE", "GAP", "BONES", "CART", "CLASSES", "MAT", "ROLES", "FORCES", "bone_groups",
-           "lattice_prep", "materials", "bone_share", "part_node_sets", "spring_end_bones", "spring_patches",
-           "spring_matrix_cn", "prepare_solve"]
+# ---------------------------------------------------------------------------------------------
+# After the solve (rb/rbfe.py:184-219): stresses, region labels, forces on the talus and calcaneus,
+# ground reactions by zone
+REGIONS = list(CART) + ["coalition", "fibrous", "fat", "soft", "fascia", "bone", "other"]   # rb/rbfe.py:190
+
+
+def _labels_dev(L, dev):
+    if isinstance(L, torch.Tensor) and L.dtype == torch.int16 and L.device == dev:
+        return L.contiguous()
+    L = np.asarray(L)
+    if L.size and (L.min() < 0 or L.max() > 32767):
+        raise ValueError("body labels must fit int16")
+    return torch.as_tensor(np.ascontiguousarray(L, np.int16), device=dev)
+
+
+def body_fractions(L, sets, device=None):
+    """bf(bodies) of rb/rbfe.py:138 (the fraction of a cell's sub-samples whose body is in the set)
+    for up to 16 body sets in one GPU pass (cm_body_fractions), bit-exact to the numpy mean.
+    L: (ncell, ns3) body labels, numpy or a device int16 tensor.  Returns a device FP64 tensor
+    [len(sets)][ncell]."""
+    dev = require_gpu(device)
+    lib = _lib.load()
+    if not 1 <= len(sets) <= 16:
+        raise ValueError("1..16 body sets")
+    Ld = _labels_dev(L, dev)
+    ncell, ns3 = Ld.shape
+    nlab = max([int(b) for bodies in sets for b in bodies] + [0]) + 1
+    nlab = max(nlab, 32768 if ncell == 0 else int(Ld.max().item()) + 1)
+    mask = np.zeros(nlab, np.uint32)
+    for q, bodies in enumerate(sets):
+        for b in bodies:
+            if 0 <= int(b) < nlab:
+                mask[int(b)] |= np.uint32(1 << q)
+    md = torch.as_tensor(mask.view(np.int32), device=dev)
+    out = torch.empty((len(sets), ncell), dtype=torch.float64, device=dev)
+    st = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+    _lib.check(lib.cm_body_fractions(st, ncell, ns3, _vp(Ld), nlab, _vp(md), len(sets), _vp(out)))
+    return out
+
+
+def region_labels(F, L, roles=ROLES, device=None):
+    """rb/rbfe.py:190-201 on the GPU: region index (into ``REGIONS``-style names) of every cell --
+    'other'; the dominant class of F (soft, fascia, or bone for cortical/cancellous); any
+    coalition/fusion, fibrous, interposition body; any cartilage body, later rules winning.
+    Returns (lab int16 numpy, names)."""
+    dev = require_gpu(device)
+    cart = roles["cart"]
+    names = list(cart) + ["coalition", "fibrous", "fat", "soft", "fascia", "bone", "other"]
+    sets = [(COAL, FUSE), (FIB,), (INTERP,)] + [(b,) for b in cart.values()]
+    fr = body_fractions(L, sets, device=dev)
+    Fd = torch.as_tensor(np.ascontiguousarray(F, np.float64), device=dev)
+    dom = torch.argmax(Fd[:, 1:], 1) + 1          # first maximum, as numpy
+    lab = torch.full((Fd.shape[0],), len(names) - 1, dtype=torch.int16, device=dev)
+
+    def put(sel, name):
+        lab.masked_fill_(sel, names.index(name))
+    put(dom == CLASSES.index("soft"), "soft")
+    put(dom == CLASSES.index("fascia"), "fascia")
+    put((dom == 1) | (dom == 2), "bone")
+    for q, name in enumerate(("coalition", "fibrous", "fat")):
+        put(fr[q] > 0, name)
+    for q, key in enumerate(cart):
+        put(fr[3 + q] > 0, key)
+    return lab.cpu().numpy(), names
+
+
+def bone_forces(g, lam, mu, U, cn, part, active, k, lab, names, device=None):
+    """rb/rbfe.py:202-210: the force on bone part k from each region -- minus the element forces
+    K_e u_e (GPU, SRI) of the active cells outside the bone, on the bone's nodes, summed by the
+    region of the cell.  Returns {name: [Fx, Fy, Fz]} unrounded, for the regions present."""
+    from . import splitfe as SF
+    cn = np.asarray(cn, np.int64)
+    part, active, lab = np.asarray(part), np.asarray(active, bool), np.asarray(lab)
+    nn = max(int(cn.max()) + 1 if cn.size else 0, len(U))
+    tgt = (part == k) & active
+    on_bone = np.zeros(nn, bool)
+    on_bone[cn[tgt].ravel()] = True
+    oc = np.flatnonzero(active & ~tgt)
+    touch = on_bone[cn[oc]]
+    oc = oc[touch.any(1)]
+    fe = SF.element_forces_cn(g, lam, mu, U, oc, cn, device=device)
+    fsum = -(fe * on_bone[cn[oc]][:, :, None]).sum(1)
+    out = {}
+    for i, nm in enumerate(names):
+        sel = lab[oc] == i
+        if sel.any():
+            out[nm] = [float(v) for v in fsum[sel].sum(0)]
+    return out
+
+
+def ground_zones(R, x, scale, c0x):
+    """rb/rbfe.py:211-215: vertical reactions summed over the heel (x < xh), mid, and fore (x >= xf)
+    zones, xh = c0x + s (-40 - c0x), xf = c0x + s (20 - c0x) (unrounded)."""
+    R, x = np.asarray(R, float), np.asarray(x, float)
+    xh = c0x + scale * (-40.0 - c0x)
+    xf = c0x + scale * (20.0 - c0x)
+    heel, fore = x < xh, x >= xf
+    return {"heel": float(R[heel, 2].sum()), "mid": float(R[~heel & ~fore, 2].sum()), "fore": float(R[fore, 2].sum())}
+
+
+def solve(V, roles=ROLES, forces=FORCES, load=1.0, cart_E=10.0, tol=1e-6, rigid=True, nu_soft=0.49, keep=True,
+          c0x=None, device=None, **opts):
+    """rb/rbfe.py:135-219 on the GPU: lattice materials, bone parts, split connectivity, contact,
+    loads and ligaments, the split-node solve, stresses, and the engine's results (res with the
+    forces on the talus and calcaneus from every region and the ground reactions by zone).
+    ``c0x``: the template's c0[0] (the zone edges of rb/rbfe.py:212-213 are in its frame; also read
+    from V["c0x"]).  Returns the reference's dict (res, g, U, VM, SIG, part, bnames, cn, orig,
+    active, F, lab, names, R, bone_nodes, E, nu)."""
+    import time
+    from ..grid import Grid, lame
+    from . import splitfe as SF
+    c0x = V.get("c0x", c0x) if hasattr(V, "get") else c0x
+    if c0x is None:
+        raise ValueError("solve needs the template's c0[0] (c0x) for the ground zones (rb/rbfe.py:211-215)")
+    dev = require_gpu(device)
+    t0 = time.time()
+    g = Grid(V["xe"], V["ye"], V["ze"])
+    o = lattice_prep(V, roles, cart_E, nu_soft, device=dev)
+    E, nu, active, F, part, bnames = o["E"], o["nu"], o["active"], o["F"], o["part"], o["names"]
+    lam, mu = lame(E, nu)
+    cn, orig, nsplit = SF.split_connectivity_gpu(g, active, part, o["S"], device=dev)
+    prep = prepare_solve(V, g, cn, orig, active, part, bnames, roles, forces, load, rigid, device=dev)
+    t_prep = time.time() - t0
+    U, info, R = SF.solve_cn(g, lam, mu, active, cn, orig, prep["fixed"], prep["loads"], prep["extra"],
+                             prep["groups"], tol, device=dev, **opts)
+    t1 = time.time()
+    act = np.flatnonzero(active)
+    sig, vm, _ = SF.cell_stress_cn(g, lam, mu, U, act, cn, device=dev)
+    VM = np.zeros(g.ncell, np.float32)
+    VM[act] = vm
+    SIG = np.zeros((g.ncell, 6), np.float32)
+    SIG[act] = sig
+    res = {"ndof": int(info["ndof"]), "iters": info.get("iters"), "flag": info.get("flag"),
+           "t_prep_s": round(t_prep, 1), "t_solve_s": round(t1 - t0 - t_prep, 1),
+           "solver": {k: v for k, v in info.items() if k in ("t_hier", "t_gpu", "device", "relres", "coarse",
+                                                           "gpu_attempt", "fallback", "gpu_error")},
+           "n_split_nodes": int(nsplit), "contact_area_mm2": V.get("contact_area") if hasattr(V, "get") else None,
+           "grid": list(g.shape)}
+    lab, names = region_labels(F, V["L"], roles, device=dev)
+    for bone in ("talus", "calcaneus"):
+        k = next(i for i, nm in bnames.items() if nm == bone)
+        f = bone_forces(g, lam, mu, U, cn, part, active, k, lab, names, device=dev)
+        res[f"F_on_{bone}"] = {nm: [round(v, 2) for v in vec] for nm, vec in f.items()}
+    xyz = g.node_xyz[orig]
+    gz = ground_zones(R, xyz[:, 0], float(V["scale"]), float(c0x))
+    res["ground_Fz"] = {k: round(v, 2) for k, v in gz.items()}
+    res["ground_total_Fz"] = float(R[:, 2].sum())
+    res["time_s"] = round(time.time() - t0, 1)
+    return dict(res=res, g=g, U=U, VM=VM, SIG=SIG, part=part, bnames=bnames, cn=cn, orig=orig, active=active, F=F,
+                lab=lab, names=names, R=R, bone_nodes=prep["bone_nodes"], E=E, nu=nu)
+
+
+__all__ = ["COAL", "FIB", "INTERP", "FUSE", "GAP", "BONES", "CART", "CLASSES", "MAT", "ROLES", "FORCES", "REGIONS",
+           "bone_groups", "lattice_prep", "materials", "bone_share", "part_node_sets", "spring_end_bones",
+           "spring_patches", "spring_matrix_cn", "prepare_solve", "body_fractions", "region_labels", "bone_forces",
+           "ground_zones", "solve"]

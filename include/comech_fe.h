@@ -220,6 +220,14 @@ int cm_lattice_materials(void* stream, int nx, int ny, int nz, int ns3, const in
                          const double* mu_k, double* E, double* nu, uint8_t* active, double* F, uint8_t* ser, float* S,
                          int32_t* part);
 
+/* Body fractions of the lattice cells for several body sets at once (the bf(bodies) =
+ * np.isin(L, bodies).mean(1) of rb/rbfe.py:138 and rb/rbsim.py:168, used by the region labels of
+ * rb/rbfe.py:191-201 and the effect descriptors of rb/rbsim.py:164-203).  mask[b] (b < nlab): bit s
+ * set when body b is in set s; out[s][c] = (sub-samples of cell c whose body is in set s) / ns3
+ * (FP64, bit-exact to the numpy mean).  nset <= 16. */
+int cm_body_fractions(void* stream, int64_t ncell, int ns3, const int16_t* L, int nlab, const uint32_t* mask, int nset,
+                      double* out);
+
 /* Node sets of the bone parts (rb/rbfe.py:166-169,178: np.unique of the nodes of the active cells
  * of each part): node_mask[n] gets bit p for every active cell of part p (1..63) that uses node n;
  * part_cells[p] = active cells of part p (64 entries).  cn: (ncell, 8) int64 in reference cell

@@ -41,6 +41,17 @@ ground_contact / leg_loads  contact: areas sum to the contact-weighted bottom fa
                             the resultant is -Fz along z and its moment puts it on the load line
                             (closed form), uniform when the line passes through the centroid
 spring_end_bones            scipy cKDTree nearest-neighbour distances (library routine)
+rigid_modes                 K Z = 0 for the 6 modes of a spring-free body; per-bone blocks
+element_forces_cn           their scatter-sum equals the assembled K u
+solve(penalties)            each penalty block is k w w^T with sum(w) = 1 (closed form)
+coeffs (orthotropic)        energy of an affine field = V eps^T D eps; centre stress D eps;
+                            isotropic coefficients reproduce lam KL + mu KM; column compliance
+region_labels               hand-built cells covering every rule and its precedence
+bone_forces                 uniform stress across a face: force = sigma n A (closed form)
+ground_zones                hand-built reactions and zone edges
+rigid_fit                   exact recovery of t, w from a rigid displacement field
+effects                     hand-built fields: p99 / weighted means by the mask rules, shares,
+                            rigid motion of the talus relative to the calcaneus recovered
 ==========================  =====================================================================
 """
 import numpy as np
@@ -995,3 +1006,111 @@ def spring_end_bones(springs, bone_nodes, xyz):
             ends.append(best)
         out.append(tuple(ends))
     return out
+
+
+# ---------------------------------------------------------------------------------------------
+# rb/rbfe.py:188-219 (region labels, forces on the talus / calcaneus, ground reactions by zone) and
+# rb/rbsim.py:99-107,164-203 (rigid fit, effect descriptors)
+ENGINE_CART = {"ankle": 5, "syndesmosis": 6, "talofibular": 7, "mid": 8, "tn": 9, "ant": 10, "cc": 11, "post": 12}
+COAL, FIB, INTERP, FUSE = 101, 102, 103, 104
+
+
+def body_fraction(L, bodies):
+    """rb/rbfe.py:138 bf: fraction of a cell's sub-samples whose body is in ``bodies``."""
+    return np.isin(L, list(bodies)).mean(1)
+
+
+def region_labels(F, L, cart=ENGINE_CART):
+    """rb/rbfe.py:190-201: region of every cell, later rules overriding earlier ones: 'other', then
+    the dominant class (soft, fascia, bone = cortical or cancellous), then cells holding any
+    coalition/fusion, fibrous, interposition and finally any cartilage body.  Returns (lab, names)."""
+    names = list(cart) + ["coalition", "fibrous", "fat", "soft", "fascia", "bone", "other"]
+    lab = np.full(len(F), len(names) - 1, np.int16)
+    dom = np.argmax(F[:, 1:], 1) + 1
+    lab[dom == LATTICE_CLASSES.index("soft")] = names.index("soft")
+    lab[dom == LATTICE_CLASSES.index("fascia")] = names.index("fascia")
+    lab[np.isin(dom, [1, 2])] = names.index("bone")
+    lab[body_fraction(L, (COAL, FUSE)) > 0] = names.index("coalition")
+    lab[body_fraction(L, (FIB,)) > 0] = names.index("fibrous")
+    lab[body_fraction(L, (INTERP,)) > 0] = names.index("fat")
+    for key, b in cart.items():
+        lab[body_fraction(L, (b,)) > 0] = names.index(key)
+    return lab, names
+
+
+def bone_forces(grid, lam, mu, U, cn, part, active, k, lab, names, nnode_total, sri=True):
+    """rb/rbfe.py:202-210 without the final rounding: the force on bone part k from every region,
+    minus the element forces (K_e u_e) that the active cells outside the bone put on the bone's
+    nodes, summed by the region of the cell.  Returns {name: [Fx, Fy, Fz]} for the regions present."""
+    cells = np.arange(grid.ncell)
+    tgt = (part == k) & active
+    tn = np.zeros(nnode_total, bool)
+    tn[cn[cells[tgt]].ravel()] = True
+    oc = cells[active & ~tgt]
+    oc = oc[tn[cn[oc]].any(1)]
+    fe = element_forces_cn(grid, lam, mu, U, oc, cn, sri=sri)
+    fsum = -(fe * tn[cn[oc]][:, :, None]).sum(1)
+    return {nm: list(fsum[lab[oc] == i].sum(0)) for i, nm in enumerate(names) if (lab[oc] == i).any()}
+
+
+def ground_zones(R, x, scale, c0x):
+    """rb/rbfe.py:211-215 without the final rounding: vertical reactions summed over the heel
+    (x < xh), mid and fore (x >= xf) zones, xh = c0x + s (-40 - c0x), xf = c0x + s (20 - c0x)."""
+    xh = c0x + scale * (-40.0 - c0x)
+    xf = c0x + scale * (20.0 - c0x)
+    return {"heel": float(R[x < xh, 2].sum()), "mid": float(R[(x >= xh) & (x < xf), 2].sum()),
+            "fore": float(R[x >= xf, 2].sum())}
+
+
+def rigid_fit(X, U):
+    """rb/rbsim.py:99-107: least-squares translation t and small rotation w about the centroid c of
+    X fitting U ~ t + w x (X - c).  Returns (t, w, c)."""
+    c = X.mean(0)
+    r = X - c
+    A = np.zeros((3 * len(X), 6))
+    A[0::3, 0] = 1; A[1::3, 1] = 1; A[2::3, 2] = 1
+    A[0::3, 4] = r[:, 2]; A[0::3, 5] = -r[:, 1]
+    A[1::3, 3] = -r[:, 2]; A[1::3, 5] = r[:, 0]
+    A[2::3, 3] = r[:, 1]; A[2::3, 4] = -r[:, 0]
+    sol, *_ = np.linalg.lstsq(A, U.ravel(), rcond=None)
+    return sol[:3], sol[3:], c
+
+
+def effects(VM, L, active, part, bnames, vol, U, xyz, bone_nodes, F_on_talus, ground_Fz, ground_total_Fz,
+            cart=ENGINE_CART, dt=None, dc=None):
+    """rb/rbsim.py:164-203: cartilage p99 / volume-weighted mean von Mises, talus load shares,
+    calcaneus-relative talus translation and eversion, navicular drop, heel share, coalition p99 and
+    max; tc_p99 (the talocalcaneal band) only when the template distances dt, dc of the cell centres
+    to the talus and calcaneus are given (rb/rbsim.py:193-199).  xyz: positions of all nodes
+    (split copies at their original node).  Returns (d, Urel)."""
+    bf = lambda bodies: body_fraction(L, bodies)
+    d = {}
+    for key in ("ant", "mid", "post", "tn", "cc"):
+        fb = bf((cart[key],))
+        m = (fb >= 0.25) & active
+        mm = (fb > 0) & active
+        d[key + "_p99"] = float(np.percentile(VM[m], 99)) if m.sum() >= 3 else 0.0
+        w = fb[mm] * vol[mm]
+        d[key + "_mean"] = float((VM[mm] * w).sum() / w.sum()) if mm.any() else 0.0
+    fz = {k: max(F_on_talus.get(k, [0, 0, 0])[2], 0.0) for k in ("ant", "mid", "post")}
+    tot = sum(fz.values()) + 1e-9
+    for k in fz:
+        d["share_" + k] = float(fz[k] / tot)
+    nodes_c = bone_nodes["calcaneus"]
+    t, w, c = rigid_fit(xyz[nodes_c], np.nan_to_num(U[nodes_c]))
+    Urel = np.nan_to_num(U) - (t[None, :] + np.cross(w[None, :], xyz - c[None, :]))
+    nodes_t = bone_nodes["talus"]
+    tt, wt, ct = rigid_fit(xyz[nodes_t], Urel[nodes_t])
+    d["talus_dz"] = float(tt[2]); d["talus_dy"] = float(tt[1]); d["talus_evert"] = float(np.rad2deg(wt[0]))
+    d["nav_drop"] = float(-np.mean(Urel[bone_nodes["navicular"], 2]))
+    d["heel_share"] = float(ground_Fz["heel"] / max(ground_total_Fz, 1e-9))
+    if dt is not None:
+        kt = [i for i, n in bnames.items() if n == "talus"][0]
+        kc = [i for i, n in bnames.items() if n == "calcaneus"][0]
+        sub_ = bf((cart["post"], cart["mid"], cart["ant"], COAL, FUSE, FIB))
+        band = active & (dt <= 6) & (dc <= 6) & ((part == kt) | (part == kc) | (sub_ > 0))
+        d["tc_p99"] = float(np.percentile(VM[band], 99)) if band.sum() > 3 else 0.0
+    cm = (bf((COAL,)) >= 0.25) & active
+    d["coal_p99"] = float(np.percentile(VM[cm], 99)) if cm.sum() > 3 else 0.0
+    d["coal_max"] = float(VM[cm].max()) if cm.sum() > 0 else 0.0
+    return d, Urel

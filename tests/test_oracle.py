@@ -680,3 +680,111 @@ def test_penalties_are_rank_one_with_unit_weight_sum():
     # the penalty pulls the weighted mean of u_z: a rigid z translation t costs k t^2
     t = np.zeros(D.shape[0]); t[2::3] = 1.0
     assert abs(t @ D @ t - k) <= 1e-12 * k
+
+
+def test_region_labels_rules_and_precedence():
+    """rb/rbfe.py:190-201 on hand-built cells: the dominant class, then coalition/fusion, fibrous,
+    interposition, then cartilage bodies override, in that order."""
+    C = O.LATTICE_CLASSES
+    F = np.zeros((8, 10))
+    F[0, C.index("soft")] = 1.0                              # soft
+    F[1, C.index("fascia")] = 0.6; F[1, C.index("soft")] = 0.4
+    F[2, C.index("cortical")] = 0.5; F[2, C.index("soft")] = 0.5   # tie: the first maximum (cortical)
+    F[3, C.index("cancellous")] = 1.0
+    F[4, C.index("plate")] = 1.0                             # plate dominates: 'other'
+    F[5, C.index("cortical")] = 1.0
+    F[6, C.index("cortical")] = 1.0
+    F[7, C.index("fibrous")] = 1.0
+    L = np.zeros((8, 4), np.int64)
+    L[5, 0] = O.FUSE                                         # bone with one fusion sample -> coalition
+    L[6, 1] = O.ENGINE_CART["post"]; L[6, 2] = O.COAL        # cartilage wins over coalition
+    L[7, :] = O.FIB; L[7, 3] = O.INTERP                      # interposition wins over fibrous
+    lab, names = O.region_labels(F, L)
+    assert names[-7:] == ["coalition", "fibrous", "fat", "soft", "fascia", "bone", "other"]
+    assert [names[i] for i in lab] == ["soft", "fascia", "bone", "bone", "other", "coalition", "post", "fat"]
+
+
+def test_bone_forces_uniform_stress_closed_form():
+    """Two cells along x, the first the bone: under a uniform strain the second cell's element
+    forces on the shared face's nodes sum to -sigma e_x (hy hz), so the force on the bone from that
+    region is sigma e_x hy hz (full and one-point lambda integration alike)."""
+    g = O.Grid(np.r_[0.0, 1.5, 2.5], np.r_[0.0, 0.8], np.r_[0.0, 1.2])
+    lam, mu = np.array([3.0, 2.0]), np.array([1.0, 0.7])
+    G = np.array([[1e-3, 2e-4, -3e-4], [4e-4, -5e-4, 1e-4], [-2e-4, 3e-4, 6e-4]])
+    U = g.node_xyz @ G.T
+    eps = 0.5 * (G + G.T)
+    part = np.array([1, 0])
+    lab = np.array([0, 3])
+    names = ["a", "b", "c", "d"]
+    for sri in (True, False):
+        f = O.bone_forces(g, lam, mu, U, g.cell_nodes, part, np.ones(2, bool), 1, lab, names, g.nnode, sri=sri)
+        sig = lam[1] * np.trace(eps) * np.eye(3) + 2 * mu[1] * eps
+        assert list(f) == ["d"]
+        assert np.allclose(f["d"], sig[:, 0] * 0.8 * 1.2, rtol=1e-12, atol=1e-16)
+
+
+def test_ground_zones_hand_built():
+    R = np.zeros((6, 3))
+    R[:, 2] = [1.0, 2.0, 4.0, 8.0, 16.0, 32.0]
+    x = np.array([-50.0, -31.0, -30.0, 0.0, 25.0, 26.0])
+    z = O.ground_zones(R, x, scale=0.75, c0x=10.0)        # xh = 10 + 0.75 (-50) = -27.5, xf = 10 + 0.75 * 10 = 17.5
+    assert z == {"heel": 7.0, "mid": 8.0, "fore": 48.0}
+
+
+def test_rigid_fit_recovers_rigid_motion():
+    rng = np.random.default_rng(4)
+    X = rng.uniform(-20, 20, (50, 3)) + [3.0, -1.0, 7.0]
+    t, w = np.array([0.1, -0.2, 0.05]), np.array([1e-3, -2e-3, 5e-4])
+    c = X.mean(0)
+    U = t + np.cross(w, X - c)
+    tt, ww, cc = O.rigid_fit(X, U)
+    assert np.allclose(tt, t, atol=1e-14) and np.allclose(ww, w, atol=1e-15) and np.allclose(cc, c)
+
+
+def test_effects_hand_built():
+    """A 6x4x3 grid: von Mises known per cell, cartilage/coalition bodies in chosen cells; the talus
+    moves rigidly relative to the calcaneus (translation, rotation about x); checks every descriptor
+    against its definition."""
+    g = O.Grid(np.arange(7.0), np.arange(5.0) * 0.5, np.arange(4.0))
+    nc = g.ncell
+    rng = np.random.default_rng(8)
+    VM = rng.uniform(0, 10, nc).astype(np.float32)
+    active = np.ones(nc, bool)
+    active[5] = False
+    L = np.zeros((nc, 4), np.int64)
+    ant = np.arange(10, 20)
+    L[ant, 0] = O.ENGINE_CART["ant"]                          # fraction 0.25 -> in p99 and mean
+    L[ant[:3], 1] = O.ENGINE_CART["ant"]                      # fraction 0.5
+    L[30, 0] = O.ENGINE_CART["mid"]                           # a single mid cell: p99 = 0 (< 3 cells)
+    coal = np.arange(40, 48)
+    L[coal, :2] = O.COAL
+    part = np.zeros(nc, np.int64)
+    bnames = {1: "calcaneus", 2: "talus", 3: "navicular"}
+    xyz = g.node_xyz
+    calc, tal, nav = np.arange(0, 30), np.arange(30, 60), np.arange(60, 70)
+    base_t, base_w = np.array([0.01, 0.02, -0.05]), np.array([1e-3, 2e-4, -3e-4])
+    c_c = xyz[calc].mean(0)
+    U = base_t + np.cross(base_w, xyz - c_c)                  # everything moves with the calcaneus
+    rel_t, rel_w = np.array([0.0, 0.003, -0.02]), np.array([0.004, 0.0, 0.0])
+    c_t = xyz[tal].mean(0)
+    U[tal] += rel_t + np.cross(rel_w, xyz[tal] - c_t)
+    U[nav, 2] -= 0.01
+    bone_nodes = {"calcaneus": calc, "talus": tal, "navicular": nav}
+    Ft = {"ant": [0, 0, 30.0], "mid": [0, 0, -5.0], "post": [1, 1, 10.0], "soft": [0, 0, 99.0]}
+    gz = {"heel": 150.0, "mid": 50.0, "fore": 100.0}
+    d, Urel = O.effects(VM, L, active, part, bnames, g.cell_vol, U, xyz, bone_nodes, Ft, gz, 300.0)
+    sel = ant[active[ant]]
+    assert np.isclose(d["ant_p99"], np.percentile(VM[sel], 99))
+    w = np.where(np.isin(sel, ant[:3]), 0.5, 0.25) * g.cell_vol[sel]
+    assert np.isclose(d["ant_mean"], (VM[sel] * w).sum() / w.sum())
+    assert d["mid_p99"] == 0.0 and np.isclose(d["mid_mean"], VM[30])
+    assert d["post_p99"] == 0.0 and d["post_mean"] == 0.0
+    assert np.isclose(d["share_ant"], 30 / 40) and d["share_mid"] == 0.0 and np.isclose(d["share_post"], 10 / 40)
+    assert np.isclose(d["talus_dz"], rel_t[2]) and np.isclose(d["talus_dy"], rel_t[1])
+    assert np.isclose(d["talus_evert"], np.rad2deg(rel_w[0]))
+    assert np.isclose(d["nav_drop"], 0.01)
+    assert np.isclose(d["heel_share"], 0.5)
+    cm = coal[active[coal]]
+    assert np.isclose(d["coal_p99"], np.percentile(VM[cm], 99)) and d["coal_max"] == VM[cm].max()
+    assert "tc_p99" not in d
+    assert np.abs(Urel[calc]).max() < 1e-15

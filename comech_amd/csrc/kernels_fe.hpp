@@ -438,6 +438,134 @@ __device__ __forceinline__ void sparse_body(const SparseArgs<T>& a, const GridAr
   for (int q = q0 + lane; q < q1; q += rw) atomicAdd(&tc[a.r1c[q]], acc * a.r1w[q]);
 }
 
+// Buffer resources (128-bit descriptors held in SGPRs) over one node or cell layer.  An access at
+// a byte offset at or past the descriptor's length reads 0 and writes nothing; lanes without an
+// entry use the offset OOB.  The march then runs no 64-bit address arithmetic and no select per
+// access: a layer's base moves in scalar registers, every lane keeps its 32-bit offsets.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int RS_FLAGS = 0x00020000;   // raw (untyped) buffer, 32-bit data format
+constexpr unsigned OOB = 0x80000000u;
+
+// Descriptor of [base + off, base + off + len) bytes; ok = false gives an empty one.
+__device__ __forceinline__ Rsrc layer_rs(const void* base, long long off, long long len, bool ok) {
+  const int n = ok ? (int)min(len, 0x7fffffffLL) : 0;
+  char* p = const_cast<char*>(static_cast<const char*>(base)) + (ok ? off : 0);
+  return __builtin_amdgcn_make_buffer_rsrc(p, (short)0, n, RS_FLAGS);
+}
+__device__ __forceinline__ void bld(Rsrc r, unsigned off, float& v) {
+  v = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void bld(Rsrc r, unsigned off, double& v) {
+  v = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+// 3 consecutive values (one node's xyz).  FP32: one 96-bit access, its components taken from a
+// float3 bit-cast of the whole vector (a bit-cast of each 32-bit component compiled into a single
+// dword load, the other two components left undefined).  FP64: three 64-bit accesses.
+typedef float cm_f3 __attribute__((ext_vector_type(3)));
+typedef unsigned int cm_u3 __attribute__((ext_vector_type(3)));
+typedef unsigned int cm_u2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void bld3(Rsrc r, unsigned off, float v[3]) {
+  const cm_f3 q = __builtin_bit_cast(cm_f3, __builtin_amdgcn_raw_buffer_load_b96(r, (int)off, 0, 0));
+  v[0] = q.x;
+  v[1] = q.y;
+  v[2] = q.z;
+}
+__device__ __forceinline__ void bld3(Rsrc r, unsigned off, double v[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) bld(r, off + 8u * c, v[c]);
+}
+__device__ __forceinline__ void bst3(Rsrc r, unsigned off, const float v[3]) {
+  cm_f3 q;
+  q.x = v[0];
+  q.y = v[1];
+  q.z = v[2];
+  __builtin_amdgcn_raw_buffer_store_b96(__builtin_bit_cast(cm_u3, q), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bst3(Rsrc r, unsigned off, const double v[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(cm_u2, v[c]), r, (int)(off + 8u * c), 0, 0);
+}
+
+// Epilogue arrays of one node layer as buffer descriptors.
+template <typename T>
+struct EpiRs {
+  Rsrc rin, y, dinv, dout, x;
+};
+template <typename T, int EM>
+__device__ __forceinline__ EpiRs<T> epi_rs(const Epi<T>& ep, long long k, long long lay, bool ok) {
+  const long long b3 = 3 * lay * (long long)sizeof(T);
+  EpiRs<T> r;
+  r.y = layer_rs(ep.y, k * b3, b3, ok && ep.y != nullptr);
+  if (EM == EPI_RESID || EM == EPI_CHEB) r.rin = layer_rs(ep.rin, k * b3, b3, ok);
+  if (EM == EPI_CHEB) {
+    r.dinv = layer_rs(ep.Dinv, 2 * k * b3, 2 * b3, ok);
+    r.dout = layer_rs(ep.dout, k * b3, b3, ok && ep.dout != nullptr);
+    r.x = layer_rs(ep.x, k * b3, b3, ok && ep.x != nullptr);
+  }
+  return r;
+}
+// Epilogue inputs of one node from buffer descriptors (off3: byte offset of the node's 3-vector).
+template <typename T, int EM>
+__device__ __forceinline__ void epi_load_rs(const EpiRs<T>& r, bool has_x, unsigned off3, EpiIn<T>& q) {
+  if (EM == EPI_RESID || EM == EPI_CHEB) bld3(r.rin, off3, q.rin);
+  if (EM == EPI_CHEB) {
+    bld3(r.dinv, 2 * off3, q.dinv);
+    bld3(r.dinv, 2 * off3 + 3 * sizeof(T), q.dinv + 3);
+    bld3(r.x, off3, q.x);   // reads 0 without an iterate (empty descriptor)
+  }
+}
+// Epilogue outputs of one node: y = A s (APPLY), rin - A s (RESID) or r (CHEB); d and x (CHEB).
+template <typename T>
+struct EpiOut {
+  T y[3], d[3], x[3];
+};
+template <typename T, int EM>
+__device__ __forceinline__ void epi_compute(const Epi<T>& ep, const T au[3], const T sn[3], const EpiIn<T>& q,
+                                            EpiOut<T>& o) {
+  if (EM == EPI_APPLY) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o.y[c] = au[c];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) o.y[c] = q.rin[c] - au[c];
+  }
+  if (EM == EPI_CHEB) {
+    T z[3];
+    sym3_mul(q.dinv, o.y, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      o.d[c] = ep.c1 * sn[c] + ep.c2 * z[c];
+      o.x[c] = q.x[c] + (ep.xa * sn[c] + o.d[c]);
+    }
+  }
+}
+// Stores of the epilogue outputs; arrays the caller does not keep have empty descriptors.
+template <typename T, int EM>
+__device__ __forceinline__ void epi_store_rs(const EpiRs<T>& r, unsigned off3, const EpiOut<T>& o) {
+  bst3(r.y, off3, o.y);
+  if (EM == EPI_CHEB) {
+    bst3(r.dout, off3, o.d);
+    bst3(r.x, off3, o.x);
+  }
+}
+
+template <int P>
+struct Par {
+  static constexpr int value = P;
+};
+
+// Prefetch depths of the structured march: node layers of s (MD) and epilogue inputs (ED) are
+// loaded this many z steps before their use (1 or 2).
+#ifndef CM_MARCH_MD
+#define CM_MARCH_MD 1
+#endif
+#ifndef CM_MARCH_ED
+#define CM_MARCH_ED 1
+#endif
+constexpr int MD = CM_MARCH_MD, ED = CM_MARCH_ED;
+static_assert((MD == 1 || MD == 2) && (ED == 1 || ED == 2), "prefetch depth 1 or 2");
+
 // order (nullptr: TileMap order): the tile of every block, ~tile for a tile none of whose cells has
 // stiffness.  Such a block's structured part is zero at all its output nodes; it runs only the
 // epilogue there.  The order puts the tiles with stiffness first, dealt to the XCDs in contiguous
@@ -456,6 +584,8 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   constexpr int NSTP = (NST + BLOCK - 1) / BLOCK;  // per thread
   using VL = V3Lds<T>;
   constexpr int NL = TN * TN;                      // nodes per staged layer
+  // staged node layers: buffer (kk - (K0 - 1)) & 1 holds layer kk, so that the two z steps of an
+  // unrolled pair address fixed buffers
   __shared__ __align__(16) T s_u[2][VL::size(NL)];
   // corners 1, 2, 3 of every thread's cell, one array per corner, 4 values (xyz, pad) per thread:
   // consecutive threads write and read consecutive 16-byte slots (no LDS bank conflicts)
@@ -482,9 +612,11 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   const long long ncl = (long long)nx * ny;
   const bool want_dot = (EM == EPI_APPLY) && ep.part != nullptr;
   double dsum = 0.0;     // s . (A s) over this block's output nodes (APPLY with part)
+  const int ci = I0 - 1 + tx, cj = J0 - 1 + ty;
+  // the thread's output node (cj, ci) on every layer, or none (halo threads, past the grid)
+  const bool own = tx >= 1 && ty >= 1 && ci <= nx && cj <= ny;
   if (empty) {   // block-uniform
-    const int ci = I0 - 1 + tx, cj = J0 - 1 + ty;
-    if (tx >= 1 && ty >= 1 && ci <= nx && cj <= ny) {
+    if (own) {
       const int kend = min(K0 + kz, nz + 1);
       for (int k = K0; k < kend; ++k) {
         const long long n = (long long)k * lay + (long long)cj * nxn + ci;
@@ -505,11 +637,12 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     return;
   }
   if constexpr (M::kind == 0) {
+    // label LAB_NONE (no stiffness, or an irregular cell of the sparse pass) reads zero weights,
+    // so every cell runs the same branch-free operator
     const T sc = M::iso ? g.iso : T(1);
-    s_tL[tid] = mat.tabL[tid] * sc;
-    s_tM[tid] = mat.tabM[tid] * sc;
+    s_tL[tid] = tid == LAB_NONE ? T(0) : mat.tabL[tid] * sc;
+    s_tM[tid] = tid == LAB_NONE ? T(0) : mat.tabM[tid] * sc;
   }
-  const int ci = I0 - 1 + tx, cj = J0 - 1 + ty;
   const bool cin = (ci >= 0 && ci < nx && cj >= 0 && cj < ny);
   const long long cxy = cin ? ci + (long long)nx * cj : 0;
   T gx = T(0), gy = T(0), axy = T(0);
@@ -528,111 +661,119 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     }
   }
   // staging plan (fixed for all layers): scalar e = tid + m*BLOCK of a layer is node row r,
-  // entry q of the row's 3*TN contiguous scalars.  A slot outside the grid loads an in-bounds
-  // address and keeps 0; a slot past the layer (e >= NST, only in the last round) writes the
-  // unused 4th value of a node's FP32 LDS slot, or nothing in FP64.
+  // entry q of the row's 3*TN contiguous scalars.  A slot outside the grid reads 0 (offset OOB);
+  // a slot past the layer (e >= NST, only in the last round) writes the unused 4th value of a
+  // node's FP32 LDS slot, or nothing in FP64.
   static_assert((NSTP - 1) * BLOCK < NST && NST <= NSTP * BLOCK, "staging rounds");
-  int goff[NSTP], loff[NSTP];
-  bool sv[NSTP];
+  unsigned goff[NSTP];
+  int loff[NSTP];
 #pragma unroll
   for (int m = 0; m < NSTP; ++m) {
     const int e = tid + m * BLOCK;
     const int r = e / (TN * 3), q = e - r * (TN * 3);
     const int gj = J0 - 1 + r, gi = I0 - 1 + q / 3;
-    sv[m] = e < NST && gj >= 0 && gj <= ny && gi >= 0 && gi <= nx;
-    goff[m] = sv[m] ? (int)((gj * nxn + gi) * 3 + q % 3) : 0;
-    // past the layer: FP32 writes the unused 4th value of a slot, FP64 (planes) writes nothing
+    const bool sv = e < NST && gj >= 0 && gj <= ny && gi >= 0 && gi <= nx;
+    goff[m] = sv ? (unsigned)(((gj * nxn + gi) * 3 + q % 3) * sizeof(T)) : OOB;
     loff[m] = e < NST ? VL::idx(NL, r * TN + q / 3, q % 3) : (VL::slots ? (tid % NL) * 4 + 3 : -1);
   }
-  T pre[NSTP];
-  auto gload = [&](int kk) {
-    const bool kin = (kk >= 0 && kk <= nz);
-    const long long base = kin ? (long long)kk * lay * 3 : 0;
+  const long long lbytes = 3 * lay * (long long)sizeof(T);   // one node layer of s
+  // node layers in flight: layer kk is loaded MD z steps before it is staged in LDS, into
+  // register set (kk - K0) % MD (MD = 1 or 2; the two z steps of an unrolled pair use fixed sets)
+  T pre[MD][NSTP];
+  auto gload = [&](int kk, int set) {
+    const Rsrc r = layer_rs(s, kk * lbytes, lbytes, kk >= 0 && kk <= nz);
+#pragma unroll
+    for (int m = 0; m < NSTP; ++m) bld(r, goff[m], pre[set][m]);
+  };
+  auto lstore = [&](int buf, int set) {
+    T* dst = s_u[buf];
 #pragma unroll
     for (int m = 0; m < NSTP; ++m) {
-      const bool ok = kin && sv[m];
-      const T v = s[ok ? base + goff[m] : 0];
-      pre[m] = ok ? v : T(0);
+      if (VL::slots || loff[m] >= 0) dst[loff[m]] = pre[set][m];
     }
   };
-  auto lstore = [&](int kk) {
-    T* dst = s_u[kk & 1];
-#pragma unroll
-    for (int m = 0; m < NSTP; ++m) {
-      if (VL::slots || loff[m] >= 0) dst[loff[m]] = pre[m];
-    }
-  };
-  // xy transform of the cell's face on node layer kk: f[4*comp + (mx + 2 my)]
-  auto face = [&](int kk, T* f) {
-    const T* src = s_u[kk & 1];
+  // xy transform of the cell's face on the node layer staged in buffer buf: f[4*comp + (mx + 2 my)]
+  auto face = [&](int buf, T* f) {
+    const T* src = s_u[buf];
     T v[4][3];
 #pragma unroll
     for (int ab = 0; ab < 4; ++ab) VL::get(src, NL, (ty + (ab >> 1)) * TN + tx + (ab & 1), v[ab]);
     face_fwd(v, f);
   };
-  // material of cell layer kk, loaded one step ahead (kinds 0, 1 and 3)
-  uint8_t labn = LAB_NONE;
-  T lamn = T(0), mun = T(0);
+  // material of cell layer kk, loaded one step ahead (kinds 0, 1 and 3); kinds 0 and 1 read 0 for
+  // cells outside the grid
+  const unsigned coff = cin ? (unsigned)cxy : OOB;                   // byte offset of a label
+  const unsigned coffT = cin ? (unsigned)(cxy * sizeof(T)) : OOB;     // of a T value
+  const long long clb = ncl * (long long)sizeof(T);                    // one cell layer of T values
+  // level 0 loads its materials MM = MD steps ahead (set (kk - K0 + 1) % MM), level 1 one step
+  constexpr int MM = M::kind <= 1 ? MD : 1;
+  uint8_t labn[MM];
+  bool labok[MM];       // block-uniform: labn belongs to a cell layer of the grid
+  T lamn[MM], mun[MM];
   T cwn[M::kind == 3 ? 16 : 1];
-  auto mload = [&](int kk) {
-    const bool ok = cin && kk >= 0 && kk < nz;
-    const long long cid = ok ? cxy + ncl * kk : 0;
+  auto mload = [&](int kk, int set) {
+    const bool okz = kk >= 0 && kk < nz;
     if constexpr (M::kind == 3) {
       const long long ncell = ncl * nz;
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const T v = mat.cw[q * ncell + cid];
-        cwn[q] = ok ? v : T(0);
-      }
+      for (int q = 0; q < 16; ++q) bld(layer_rs(mat.cw, (q * ncell + kk * ncl) * (long long)sizeof(T), clb, okz), coffT, cwn[q]);
     }
     if constexpr (M::kind == 0) {
-      const uint8_t lb = mat.lab[cid];
-      labn = ok ? lb : LAB_NONE;
+      // a cell outside the grid or an empty layer reads 0, a real label: the step that uses labn
+      // replaces it by LAB_NONE there (a select at the load would wait for it)
+      labn[set] = __builtin_amdgcn_raw_buffer_load_b8(layer_rs(mat.lab, kk * ncl, ncl, okz), (int)coff, 0, 0);
+      labok[set] = okz;
     }
     if constexpr (M::kind == 1) {
-      const T l = mat.cl[cid], m = mat.cm[cid];
-      lamn = ok ? l : T(0);
-      mun = ok ? m : T(0);
+      bld(layer_rs(mat.cl, kk * clb, clb, okz), coffT, lamn[set]);
+      bld(layer_rs(mat.cm, kk * clb, clb, okz), coffT, mun[set]);
     }
   };
 
   const int kend = min(K0 + kz, nz + 1);   // exclusive end of the output node layers
-  gload(K0 - 1);
-  lstore(K0 - 1);
-  gload(K0);
-  mload(K0 - 1);
+  mload(K0 - 1, 0);
+  if (MM == 2) mload(K0, 1);
+  gload(K0 - 1, 0);
+  lstore(0, 0);
+  gload(K0, 0);
+  if (MD == 2) gload(K0 + 1, MD - 1);
   __syncthreads();
-  // level 0 (residual and smoothing steps): the epilogue inputs of the thread's output node on
-  // layer K0, then always one layer ahead
+  // epilogue inputs (residual and smoothing steps on level 0) of the thread's output node on layer
+  // K0, then always one layer ahead
   constexpr bool PF = (M::kind <= 1) && (EM != EPI_APPLY);
-  EpiIn<T> qn;
+  const unsigned off3 = own ? (unsigned)((cj * nxn + ci) * 3 * sizeof(T)) : OOB;
+  // epilogue inputs of output layer k are loaded ED steps ahead into set (k - K0 + 1) % ED
+  EpiIn<T> qn[ED];
   if constexpr (PF) {
-    if (tx >= 1 && ty >= 1 && ci <= nx && cj <= ny && K0 < kend)
-      epi_load<T, EM>(ep, (long long)K0 * lay + (long long)cj * nxn + ci, qn);
+    epi_load_rs<T, EM>(epi_rs<T, EM>(ep, K0, lay, K0 < kend), ep.x != nullptr, off3, qn[1 % ED]);
+    if (ED == 2) epi_load_rs<T, EM>(epi_rs<T, EM>(ep, K0 + 1, lay, K0 + 1 < kend), ep.x != nullptr, off3, qn[0]);
   }
-  T ulo[12], car[12];
-  face(K0 - 1, ulo);
+  // LDS slots of the corners the three in-plane neighbour cells pass to this thread's node
+  const int i1 = tx >= 1 ? tid - 1 : tid, i2 = ty >= 1 ? tid - TILE : tid, i3 = (tx >= 1 && ty >= 1) ? tid - TILE - 1 : tid;
+  T uA[12], uB[12], cA[12], cB[12];
+  face(0, uA);
 #pragma unroll
-  for (int q = 0; q < 12; ++q) car[q] = T(0);
+  for (int q = 0; q < 12; ++q) cA[q] = T(0);
 
-  for (int k = K0 - 1; k < kend; ++k) {
-    lstore(k + 1);
-    if (k + 2 <= kend) gload(k + 2);
+  // one z step: cell layer k (bottom face transform ulo, top face computed into uhi), carry in car
+  // (top-face forces of cell k-1), carry out carn (top-face forces of cell k), output node layer k
+  auto step = [&](auto PAR, const int k, T(&ulo)[12], T(&uhi)[12], T(&car)[12], T(&carn)[12]) {
+    constexpr int BLO = decltype(PAR)::value, BHI = 1 - BLO;   // buffers of layers k and k+1
+    constexpr int SET = MD == 2 ? BLO : 0, QS = ED == 2 ? BLO : 0, MSET = MM == 2 ? BLO : 0;
+    // Issue order of the memory operations of a step: the material MM layers ahead, node layer
+    // k+1+MD, (after the cell work) the epilogue inputs ED layers ahead, then this step's stores.
+    // Each prefetched value is then waited for behind loads only, never behind a store.
     // this step's cell material
-    bool act = false;
+    bool act = true;
     CellW<T> w{gx, gy, T(0), T(0), T(0)};
     if constexpr (M::kind == 0) {
-      const uint8_t lb = labn;
-      act = (lb != LAB_NONE);
-      if (act) {
-        w.wl = s_tL[lb];
-        w.wm = s_tM[lb];
-      }
+      const uint8_t lb = (cin && labok[MSET]) ? labn[MSET] : LAB_NONE;
+      w.wl = s_tL[lb];
+      w.wm = s_tM[lb];
     } else if constexpr (M::kind == 1) {
-      act = (lamn != T(0)) || (mun != T(0));
       const T sc = M::iso ? g.iso : T(1);
-      w.wl = lamn * sc;
-      w.wm = mun * sc;
+      w.wl = lamn[MSET] * sc;
+      w.wm = mun[MSET] * sc;
     } else {
       act = cin && k >= 0 && k < nz;
     }
@@ -641,80 +782,82 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
 #pragma unroll
       for (int q = 0; q < 16; ++q) cw[q] = cwn[q];
     }
-    if constexpr (M::kind != 2) mload(k + 1);
+    lstore(BHI, SET);         // node layer k+1, loaded MD steps ago
+    if constexpr (M::kind != 2) mload(k + MM, MSET);
+    gload(k + 1 + MD, SET);   // past kend: an empty descriptor (no traffic)
     __syncthreads();
-    T uhi[12];
-    face(k + 1, uhi);
+    face(BHI, uhi);
     T sn[3] = {T(0), T(0), T(0)};
-    if (EM == EPI_CHEB || want_dot) {
-      VL::get(s_u[k & 1], NL, ty * TN + tx, sn);
-    }
-    T Flo[12], Fhi[12];
-#pragma unroll
-    for (int q = 0; q < 12; ++q) {
-      Flo[q] = T(0);
-      Fhi[q] = T(0);
-    }
-    if (act) {
-      T c3[3][8];
+    if (EM == EPI_CHEB || want_dot) VL::get(s_u[BLO], NL, ty * TN + tx, sn);
+    T Flo[12];
+    if constexpr (M::kind <= 1) {
+      // level 0: every lane computes its cell (zero weights where there is no stiffness)
+      T c3[3][8], F3[3][8];
       faces_to_cell(ulo, uhi, c3);
-      T F3[3][8];
-      bool done = true;
-      if constexpr (M::kind == 2) {
-        const long long ncell = ncl * nz;
-        const long long cid = cxy + ncl * k;
-        T mL[27], mM[27];
-#pragma unroll
-        for (int q = 0; q < 27; ++q) {
-          mL[q] = mat.mom[q * ncell + cid];
-          mM[q] = mat.mom[(27 + q) * ncell + cid];
-        }
-        done = (mL[0] != T(0)) || (mM[0] != T(0));
-        if (done) elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
-      } else if constexpr (M::kind == 3) {
-        T ws = T(0);
-#pragma unroll
-        for (int q = 0; q < 16; ++q) ws += cw[q];
-        done = (ws != T(0));
-        if (done) {
-          T fxL[2][2], fxM[2][2], fyL[2][2], fyM[2][2], fzL[2][2], fzM[2][2];
-          const T* px = s_ph + 6 * tx;
-          const T* py = s_ph + TILE * 6 + 6 * ty;
-          const T* pz = mat.phz + 6 * k;
-#pragma unroll
-          for (int d = 0; d < 2; ++d) {
-            fxM[d][0] = fxL[d][0] = px[3 * d];
-            fxM[d][1] = px[3 * d + 1];
-            fxL[d][1] = M::sri ? px[3 * d + 2] : px[3 * d + 1];
-            fyM[d][0] = fyL[d][0] = py[3 * d];
-            fyM[d][1] = py[3 * d + 1];
-            fyL[d][1] = M::sri ? py[3 * d + 2] : py[3 * d + 1];
-            fzM[d][0] = fzL[d][0] = pz[3 * d];
-            fzM[d][1] = pz[3 * d + 1];
-            fzL[d][1] = M::sri ? pz[3 * d + 2] : pz[3 * d + 1];
-          }
-          T mL[27], mM[27];
-          child_moments<T>(cw, fxL, fyL, fzL, mL);
-          child_moments<T>(cw + 8, fxM, fyM, fzM, mM);
-          elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
-        }
-      } else {
-        if (!M::iso) {
-          const T hz = g.hz[k];
-          const T V = axy * hz;
-          w.gz = T(0.25) / hz;
-          w.wl *= V;
-          w.wm *= V;
-        }
-        elem_mid<T, M::sri, M::iso>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+      if (!M::iso) {
+        const T hz = g.hz[min(max(k, 0), nz - 1)];   // layers past the grid have zero weights
+        const T V = axy * hz;
+        w.gz = T(0.25) / hz;
+        w.wl *= V;
+        w.wm *= V;
       }
-      if (done) cell_to_faces(F3, Flo, Fhi);
+      elem_mid<T, M::sri, M::iso>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+      cell_to_faces(F3, Flo, carn);
+    } else {
+#pragma unroll
+      for (int q = 0; q < 12; ++q) {
+        Flo[q] = T(0);
+        carn[q] = T(0);
+      }
+      if (act) {
+        T c3[3][8];
+        faces_to_cell(ulo, uhi, c3);
+        T F3[3][8];
+        bool done = true;
+        if constexpr (M::kind == 2) {
+          const long long ncell = ncl * nz;
+          const long long cid = cxy + ncl * k;
+          T mL[27], mM[27];
+#pragma unroll
+          for (int q = 0; q < 27; ++q) {
+            mL[q] = mat.mom[q * ncell + cid];
+            mM[q] = mat.mom[(27 + q) * ncell + cid];
+          }
+          done = (mL[0] != T(0)) || (mM[0] != T(0));
+          if (done) elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
+        } else {
+          T ws = T(0);
+#pragma unroll
+          for (int q = 0; q < 16; ++q) ws += cw[q];
+          done = (ws != T(0));
+          if (done) {
+            T fxL[2][2], fxM[2][2], fyL[2][2], fyM[2][2], fzL[2][2], fzM[2][2];
+            const T* px = s_ph + 6 * tx;
+            const T* py = s_ph + TILE * 6 + 6 * ty;
+            const T* pz = mat.phz + 6 * k;
+#pragma unroll
+            for (int d = 0; d < 2; ++d) {
+              fxM[d][0] = fxL[d][0] = px[3 * d];
+              fxM[d][1] = px[3 * d + 1];
+              fxL[d][1] = M::sri ? px[3 * d + 2] : px[3 * d + 1];
+              fyM[d][0] = fyL[d][0] = py[3 * d];
+              fyM[d][1] = py[3 * d + 1];
+              fyL[d][1] = M::sri ? py[3 * d + 2] : py[3 * d + 1];
+              fzM[d][0] = fzL[d][0] = pz[3 * d];
+              fzM[d][1] = pz[3 * d + 1];
+              fzL[d][1] = M::sri ? pz[3 * d + 2] : pz[3 * d + 1];
+            }
+            T mL[27], mM[27];
+            child_moments<T>(cw, fxL, fyL, fzL, mL);
+            child_moments<T>(cw + 8, fxM, fyM, fzM, mM);
+            elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
+          }
+        }
+        if (done) cell_to_faces(F3, Flo, carn);
+      }
     }
 #pragma unroll
-    for (int q = 0; q < 12; ++q) {
-      Flo[q] += car[q];
-      car[q] = Fhi[q];
-    }
+    for (int q = 0; q < 12; ++q) Flo[q] += car[q];
     // back to the 4 bottom corners (corner a + 2b)
     T cr[4][3];
     face_bwd(Flo, cr);
@@ -722,27 +865,41 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     VL::put(s_g[1], BLOCK, tid, cr[2]);
     VL::put(s_g[2], BLOCK, tid, cr[3]);
     __syncthreads();
-    if (tx >= 1 && ty >= 1 && k >= K0 && ci <= nx && cj <= ny) {
+    // Every step issues the same memory operations (those of the first step, which has no output
+    // layer, and those of arrays the caller does not keep go to empty descriptors): with the same
+    // count of memory operations on every path, the compiler's wait before each use of a prefetched
+    // value covers only that value, instead of draining all loads in flight.
+    {   // halo lanes compute and store nowhere (offset OOB)
       T p1[3], p2[3], p3[3];
-      VL::get(s_g[0], BLOCK, tid - 1, p1);          // cell (tx-1, ty): its corner 1
-      VL::get(s_g[1], BLOCK, tid - TILE, p2);       // cell (tx, ty-1): its corner 2
-      VL::get(s_g[2], BLOCK, tid - TILE - 1, p3);   // cell (tx-1, ty-1): its corner 3
+      VL::get(s_g[0], BLOCK, i1, p1);   // cell (tx-1, ty): its corner 1
+      VL::get(s_g[1], BLOCK, i2, p2);   // cell (tx, ty-1): its corner 2
+      VL::get(s_g[2], BLOCK, i3, p3);   // cell (tx-1, ty-1): its corner 3
       T au[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) au[c] = cr[0][c] + p1[c] + p2[c] + p3[c];
-      const long long n = (long long)k * lay + (long long)cj * nxn + ci;
-      add_extra<T, M>(mat, n, au);
-      if (want_dot) dsum += (double)sn[0] * au[0] + (double)sn[1] * au[1] + (double)sn[2] * au[2];
-      if constexpr (PF) {
-        epilogue_in<T, EM>(ep, n, au, sn, qn);
-        if (k + 1 < kend) epi_load<T, EM>(ep, n + lay, qn);
-      } else {
-        epilogue<T, EM>(ep, n, au, sn);
+      if constexpr (M::kind >= 2) {
+        if (own && k >= K0) add_extra<T, M>(mat, (long long)k * lay + (long long)cj * nxn + ci, au);
       }
+      if (want_dot && own && k >= K0) dsum += (double)sn[0] * au[0] + (double)sn[1] * au[1] + (double)sn[2] * au[2];
+      const EpiRs<T> ro = epi_rs<T, EM>(ep, k, lay, k >= K0);
+      EpiOut<T> o;
+      if constexpr (PF) {
+        epi_compute<T, EM>(ep, au, sn, qn[QS], o);
+        epi_load_rs<T, EM>(epi_rs<T, EM>(ep, k + ED, lay, k + ED < kend), ep.x != nullptr, off3, qn[QS]);
+      } else {
+        EpiIn<T> q;
+        epi_load_rs<T, EM>(ro, ep.x != nullptr, off3, q);
+        epi_compute<T, EM>(ep, au, sn, q, o);
+      }
+      epi_store_rs<T, EM>(ro, off3, o);
     }
-#pragma unroll
-    for (int q = 0; q < 12; ++q) ulo[q] = uhi[q];
+  };
+  int k = K0 - 1;
+  for (; k + 1 < kend; k += 2) {
+    step(Par<0>{}, k, uA, uB, cA, cB);
+    step(Par<1>{}, k + 1, uB, uA, cB, cA);
   }
+  if (k < kend) step(Par<0>{}, k, uA, uB, cA, cB);
   if (want_dot) {
     dsum = block_sum<BLOCK>(dsum);
     if (tid == 0) ep.part[blockIdx.x] = dsum;

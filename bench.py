@@ -13,8 +13,11 @@ lists; the load actions reuse the baseline's hierarchy, the same operator), PCG 
 
 Prints ONE JSON line.  ``value`` = solves/s in mode ``ref`` (FP64 Krylov vectors, alpha/beta and
 outer product, FP32 V-cycle: operator, smoothers and transfers, FP64 coarsest solve); also
-``mixed`` (FP32 PCG inside FP64 iterative refinement) and ``plain_fp64`` (all-FP64 GMG-PCG) on
-the same systems, the error of every result against the plain FP64 answer, ``roofline`` (the
+``mixed`` (FP32 PCG inside FP64 iterative refinement) and ``plain_fp64`` (the reference's plain
+method, all FP64: its level rule and per-node block Jacobi) on the same systems, batched, and
+``plain_fp64_sequential`` (the plain method one solve at a time, as the reference runs it, on a
+subset of the feet, with the batched method on the same subset), the error of every result
+against the plain FP64 answer, ``roofline`` (the
 FP32 fine-level K*u on the 384x256x256 grid, whose 635 MB working set streams from HBM) and
 ``cpu_baseline`` (the FP64 CPU oracle -- assembled CSR, Chebyshev GMG-PCG, SuperLU coarse solve,
 the reference algorithm -- on one foot of the set, one core and a pool of host processes).
@@ -76,7 +79,15 @@ def timed_feet(k):
         [i % N_FEET for i in range(k)]
 
 
-def foot_job(descs, bs, fi, grp, mode, tol, dev, ref_x=None, keep=False):
+# Multigrid of the method as built here: the semi-coarsened coarsest level and the split-copy
+# smoother blocks (library defaults), coarsening down to 3,000 DOFs (7 levels at most).
+TUNED = dict(min_dofs=3000, max_levels=7)
+# The reference's plain method: its level rule (6,000 DOFs, 6 levels, every axis coarsened,
+# fe/gmg.py:80-96) and per-node 3x3 block Jacobi (fe/gmg.py:29-44).
+PLAIN = dict(last_axes=7, copy_blocks=False)
+
+
+def foot_job(descs, bs, fi, grp, mode, tol, dev, ref_x=None, keep=False, mg=None):
     """One system (the first action of ``grp``) solved for every action of ``grp`` in turn."""
     import torch
     from comech_amd.batch import build_system
@@ -84,7 +95,7 @@ def foot_job(descs, bs, fi, grp, mode, tol, dev, ref_x=None, keep=False):
     idx = [ACTIONS.index(a) for a in grp]
 
     def run():
-        s = build_system(descs[idx[0]], device=dev)
+        s = build_system(descs[idx[0]], device=dev, **(mg or {}))
         ndof = s.levels()[0]["ndof"]
         out = []
         for k in idx:
@@ -101,14 +112,14 @@ def foot_job(descs, bs, fi, grp, mode, tol, dev, ref_x=None, keep=False):
     return run
 
 
-def batch_leg(feet, descs, bs, mode, tol, conc, dev, ref_x=None, keep=False):
+def batch_leg(feet, descs, bs, mode, tol, conc, dev, ref_x=None, keep=False, mg=None):
     """All solves of ``feet`` (8 each) batched; returns (infos, xs, wall seconds)."""
     from comech_amd.batch import run_batch
     jobs = []
     for fi in feet:
         for grp in GROUPS:
             w = len(grp) * descs[fi][0]["ncell"]
-            jobs.append((w, foot_job(descs[fi], bs[fi], fi, grp, mode, tol, dev, ref_x, keep)))
+            jobs.append((w, foot_job(descs[fi], bs[fi], fi, grp, mode, tol, dev, ref_x, keep, mg)))
     jobs.sort(key=lambda j: -j[0])                # longest first: the tail of the batch stays short
     res, wall = run_batch([j[1] for j in jobs], concurrency=conc, device=dev)
     infos, xs = [], {}
@@ -247,18 +258,26 @@ def batch_workload(args):
     log(f"inputs of {len(descs)} feet ready in {time.time() - t0:.1f} s on {nw} host workers")
     conc = args.concurrency
     if warm:
-        batch_leg(warm, descs, bs, "ref", args.tol, conc, dev)
+        batch_leg(warm, descs, bs, "ref", args.tol, conc, dev, mg=TUNED)
+        batch_leg(warm, descs, bs, "fp64", args.tol, conc, dev, mg=PLAIN)
         log("warmup done")
-    plain, plain_x, w_plain = batch_leg(feet, descs, bs, "fp64", args.tol, conc, dev, keep=True)
-    log(f"plain fp64: {len(plain) / w_plain:.2f} solves/s")
-    ref, _, w_ref = batch_leg(feet, descs, bs, "ref", args.tol, conc, dev, ref_x=plain_x)
+    plain, plain_x, w_plain = batch_leg(feet, descs, bs, "fp64", args.tol, conc, dev, keep=True, mg=PLAIN)
+    log(f"plain fp64 (reference method): {len(plain) / w_plain:.2f} solves/s batched")
+    ref, _, w_ref = batch_leg(feet, descs, bs, "ref", args.tol, conc, dev, ref_x=plain_x, mg=TUNED)
     log(f"ref: {len(ref) / w_ref:.2f} solves/s")
-    mixed, _, w_mixed = batch_leg(feet, descs, bs, "mixed", args.tol, conc, dev, ref_x=plain_x)
+    mixed, _, w_mixed = batch_leg(feet, descs, bs, "mixed", args.tol, conc, dev, ref_x=plain_x, mg=TUNED)
     log(f"mixed: {len(mixed) / w_mixed:.2f} solves/s")
     del plain_x
-    seq, _, w_seq = batch_leg(feet[:1], descs, bs, "ref", args.tol, 1, dev)
+    seq, _, w_seq = batch_leg(feet[:1], descs, bs, "ref", args.tol, 1, dev, mg=TUNED)
+    # the plain method as the reference runs it, one solve at a time, on a subset of the timed feet,
+    # and the batched method on the same subset
+    sub = feet[::max(1, len(feet) // SEQ_FEET)][:SEQ_FEET]
+    pseq, _, w_pseq = batch_leg(sub, descs, bs, "fp64", args.tol, 1, dev, mg=PLAIN)
+    rsub, _, w_rsub = batch_leg(sub, descs, bs, "ref", args.tol, conc, dev, mg=TUNED)
+    log(f"plain fp64 one at a time: {len(pseq) / w_pseq:.2f} solves/s; ref batched on the same feet: "
+        f"{len(rsub) / w_rsub:.2f}")
     bad = [dict(foot=i["foot"], action=i["action"], mode=i["mode"], flag=i["flag"], true=i["true_relres"])
-           for i in plain + ref + mixed if i["flag"] or i["true_relres"] > args.tol * 1.0001]
+           for i in plain + ref + mixed + pseq + rsub if i["flag"] or i["true_relres"] > args.tol * 1.0001]
     for b in bad:
         log("missed", b)
     if any(b["flag"] == 0 for b in bad) or any(i["flag"] for i in ref):
@@ -277,7 +296,7 @@ def batch_workload(args):
         from comech_amd.grid import Grid, corners_of
         from comech_amd.system import cell_stress_gpu
         d0 = ex.submit(synth.batch_foot, sample, scale=SAMPLE_SCALE, actions=["baseline"]).result()[0]
-        s = build_system(d0, device=dev)
+        s = build_system(d0, device=dev, **TUNED)
         x, _ = s.solve(rhs(d0, dev), tol=args.tol, maxiter=3000, mode="ref")
         U = x.view(-1, 3).cpu().numpy().copy()
         U[~s.used_mask(0)] = np.nan
@@ -315,12 +334,22 @@ def batch_workload(args):
                                 f"{args.tol:g}, each checked by its FP64 true residual; one step = one foot's 8 "
                                 f"actions (hierarchy built on the GPU per system, the load actions reuse the "
                                 f"baseline's); mode ref: FP64 PCG with an FP32 V-cycle"),
-                   "feet": feet, "grid_min": list(min(sizes[f] for f in feet)),
-                   "grid_max": list(max(sizes[f] for f in feet)), "solves": nsolve, "concurrency": conc,
+                   "feet": feet, "grid_min": [int(min(sizes[f][a] for f in feet)) for a in range(3)],
+                   "grid_max": [int(max(sizes[f][a] for f in feet)) for a in range(3)], "solves": nsolve,
+                   "concurrency": conc,
                    "mode": "ref", **{k: v for k, v in summarize(ref, w_ref).items() if k not in ("value", "unit")}},
         "mixed": summarize(mixed, w_mixed, {"ratio_over_plain": round(w_plain / w_mixed, 3)}),
-        "plain_fp64": summarize(plain, w_plain),
+        "plain_fp64": summarize(plain, w_plain, {
+            "what": "the reference's plain method in FP64 on the GPU (its level rule: 6,000 DOFs or 6 levels, every "
+                    "axis coarsened; per-node 3x3 block Jacobi; Chebyshev degree 3), batched like the headline"}),
+        "plain_fp64_sequential": summarize(pseq, w_pseq, {
+            "what": f"the same plain method one solve at a time (as the reference runs it, one process per GPU), "
+                    f"feet {sub} x 8 actions",
+            "ref_batched_same_feet": round(len(rsub) / w_rsub, 3),
+            "ratio_ref_batched_over_plain_sequential": round((len(rsub) / w_rsub) / (len(pseq) / w_pseq), 3)}),
         "ratio_ref_over_plain": round(w_plain / w_ref, 3),
+        "multigrid": {"ref_and_mixed": dict(TUNED, last_axes=3, copy_blocks=True), "plain_fp64": dict(PLAIN, min_dofs=6000,
+                                                                                                   max_levels=6)},
         "learned": ll,
         "sequential": {"value": round(len(seq) / w_seq, 3), "unit": "solves/s", "concurrency": 1,
                        "solves": len(seq), "what": f"foot {feet[0]}'s 8 actions one at a time, mode ref"},
@@ -349,6 +378,7 @@ def batch_workload(args):
 
 
 LEARNED_FEET = 2      # held-out feet of the learned warm start's training split, 8 actions each
+SEQ_FEET = 8          # feet of the one-at-a-time plain leg
 
 
 def learned_leg(descs, dev, tol):

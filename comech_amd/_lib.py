@@ -42,6 +42,11 @@ SIGNATURES = {
     "cm_sys_power": [c_void_p, c_int, c_int, c_void_p, P_double],
     "cm_sys_smooth": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int],
     "cm_sys_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_int, c_double, ctypes.POINTER(SolveInfo)],
+    "cm_batch_create": [ctypes.POINTER(c_void_p), c_void_p, c_int],
+    "cm_batch_destroy": [c_void_p],
+    "cm_batch_add": [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, P_int],
+    "cm_batch_step": [c_void_p, c_int, P_int],
+    "cm_batch_poll": [c_void_p, P_int, ctypes.POINTER(SolveInfo)],
     "cm_cell_stress": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p, c_void_p,
                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "cm_cell_stress_ortho": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_ll, c_void_p, c_void_p,

@@ -62,6 +62,153 @@ def run_batch(jobs, concurrency=4, device=None):
     return results, wall
 
 
+class LockStep:
+    """Solves several systems together, each PCG iteration's launches covering all of them
+    (cm_batch_*, include/comech_fe.h): FP64 PCG with an FP32 V-cycle, the iteration of
+    ``VoxelSystem.solve(mode="ref")``.  A system is in the batch once at a time."""
+
+    def __init__(self, device=None, check_every=5):
+        import ctypes
+        from . import _lib
+        self.dev = require_gpu(device)
+        self.lib = _lib.load()
+        self.stream = torch.cuda.Stream(device=self.dev)
+        h = ctypes.c_void_p()
+        _lib.check(self.lib.cm_batch_create(ctypes.byref(h), ctypes.c_void_p(self.stream.cuda_stream),
+                                            int(check_every)))
+        self.h = h
+
+    def add(self, system, b, x, tol=1e-8, maxiter=3000):
+        """Starts PCG on system x = b (x: FP64 device vector holding the initial guess, updated in
+        place).  Returns the slot, or -1 when the system's launch structure differs from the
+        batch's (solve it with ``system.solve``)."""
+        import ctypes
+        from . import _lib
+        slot = ctypes.c_int()
+        _lib.check(self.lib.cm_batch_add(self.h, system.h, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr()),
+                                         float(tol), int(maxiter), ctypes.byref(slot)))
+        return slot.value
+
+    def step(self, max_iters=5):
+        """Iterations until a system has finished or max_iters ran; returns the systems still in."""
+        import ctypes
+        from . import _lib
+        n = ctypes.c_int()
+        _lib.check(self.lib.cm_batch_step(self.h, int(max_iters), ctypes.byref(n)))
+        return n.value
+
+    def poll(self):
+        """(slot, info) of one finished system, or None."""
+        import ctypes
+        from . import _lib
+        slot = ctypes.c_int()
+        info = _lib.SolveInfo()
+        _lib.check(self.lib.cm_batch_poll(self.h, ctypes.byref(slot), ctypes.byref(info)))
+        if slot.value < 0:
+            return None
+        return slot.value, dict(iters=info.iters, flag=info.flag, outer=info.outer, levels=info.levels,
+                                relres=info.relres, t_solve_ms=info.t_solve_ms, t_setup_ms=info.t_setup_ms,
+                                mode="ref", device=str(self.dev))
+
+    def close(self):
+        if self.h is not None:
+            self.lib.cm_batch_destroy(self.h)
+            self.h = None
+
+
+def solve_lockstep(items, width=16, builders=4, tol=1e-8, maxiter=3000, device=None, check_every=5):
+    """Solve many systems in lock-step batches of up to ``width`` (continuous: a finished system's
+    slot takes the next right-hand side or system).
+
+    items: callables, each building one system on the calling thread's stream and returning
+    (system, [(key, b), ...]) -- its right-hand sides, solved one after another from zero.  Systems
+    are built on ``builders`` host threads (one stream each) while the batch runs.  Returns
+    (results {key: (x, info)}, wall seconds).  A system whose launch structure differs from the
+    batch's is solved on its own (``system.solve(mode="ref")``) and its info carries
+    ``lockstep=False``."""
+    from concurrent.futures import ThreadPoolExecutor
+    dev = require_gpu(device)
+    items = list(items)
+    streams = [torch.cuda.Stream(device=dev) for _ in range(builders)]
+    tl = threading.local()
+    counter = iter(range(1 << 30))
+    lock = threading.Lock()
+
+    def build(fn):
+        if not hasattr(tl, "stream"):
+            with lock:
+                tl.stream = streams[next(counter) % builders]
+        torch.cuda.set_device(dev)
+        with torch.cuda.stream(tl.stream):
+            out = fn()
+            tl.stream.synchronize()
+        return out
+
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    results = {}
+    batch = LockStep(dev, check_every)
+    ex = ThreadPoolExecutor(builders)
+    try:
+        pending = [ex.submit(build, fn) for fn in items[:width + builders]]
+        nxt = len(pending)
+        waiting = []          # [system, rhs list, index of the next rhs]
+        inb = {}              # slot -> [system, rhs list, index, x]
+        while pending or waiting or inb:
+            # systems built since the last round
+            still = []
+            for f in pending:
+                if f.done():
+                    sysm, rhs_list = f.result()
+                    waiting.append([sysm, list(rhs_list), 0])
+                    if nxt < len(items):
+                        still.append(ex.submit(build, items[nxt]))
+                        nxt += 1
+                else:
+                    still.append(f)
+            pending = still
+            if not inb and not waiting and pending:
+                pending[0].result()
+                continue
+            # fill free slots
+            while waiting and len(inb) < width:
+                sysm, rhs_list, k = waiting.pop(0)
+                key, b = rhs_list[k]
+                x = torch.zeros_like(b)
+                slot = batch.add(sysm, b, x, tol=tol, maxiter=maxiter)
+                if slot < 0:      # another launch structure: on its own
+                    with torch.cuda.stream(batch.stream):
+                        for key2, b2 in rhs_list[k:]:
+                            x2, info = sysm.solve(b2, tol=tol, maxiter=maxiter, mode="ref", check_every=check_every)
+                            info["lockstep"] = False
+                            results[key2] = (x2, info)
+                    batch.stream.synchronize()
+                    sysm.close()
+                    continue
+                inb[slot] = [sysm, rhs_list, k, x, key]
+            if not inb:
+                continue
+            batch.step(check_every)
+            while True:
+                got = batch.poll()
+                if got is None:
+                    break
+                slot, info = got
+                sysm, rhs_list, k, x, key = inb.pop(slot)
+                info["lockstep"] = True
+                results[key] = (x, info)
+                if k + 1 < len(rhs_list):
+                    waiting.insert(0, [sysm, rhs_list, k + 1])
+                else:
+                    batch.stream.synchronize()
+                    sysm.close()
+        torch.cuda.synchronize(dev)
+    finally:
+        batch.close()
+        ex.shutdown()
+    return results, time.perf_counter() - t0
+
+
 def build_system(desc, device=None, **mg):
     """VoxelSystem of a level-0 description (``synth.level0_description``): labels + tables, split
     copies and irregular cells, ground springs (COO) and spring terms (rank one)."""
@@ -83,4 +230,4 @@ def rhs(desc, device=None):
     return b
 
 
-__all__ = ["run_batch", "build_system", "rhs"]
+__all__ = ["run_batch", "LockStep", "solve_lockstep", "build_system", "rhs"]

@@ -570,13 +570,15 @@ static_assert((MD == 1 || MD == 2) && (ED == 1 || ED == 2), "prefetch depth 1 or
 // stiffness.  Such a block's structured part is zero at all its output nodes; it runs only the
 // epilogue there.  The order puts the tiles with stiffness first, dealt to the XCDs in contiguous
 // runs, so the VALU-heavy blocks spread evenly over all CUs.
+// The march of block bid (the kernel's block index, or a block of one system in a launch that
+// covers several systems, k_multi).
 template <typename T, class M, int EM>
-__global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* __restrict__ s, Epi<T> ep, int kz,
-                                                 TileMap tm, const int* __restrict__ order, SparseArgs<T> sp) {
+__device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, const T* __restrict__ s, Epi<T> ep,
+                                           int kz, TileMap tm, const int* __restrict__ order, SparseArgs<T> sp) {
   if constexpr (M::kind <= 1) {
     const int nbm = tm.ntx * tm.nty * tm.ntz;
-    if ((int)blockIdx.x >= nbm) {   // block-uniform: the sparse pass rides on this launch
-      sparse_body<T, M::sri>(sp, g, s, (int)blockIdx.x - nbm);
+    if (bid >= nbm) {   // block-uniform: the sparse pass rides on this launch
+      sparse_body<T, M::sri>(sp, g, s, bid - nbm);
       return;
     }
   }
@@ -597,14 +599,14 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   int bx, by, bz;
   bool empty = false;
   if (order) {
-    int id = order[blockIdx.x];
+    int id = order[bid];
     empty = id < 0;
     id = empty ? ~id : id;
     bx = id % tm.ntx;
     by = (id / tm.ntx) % tm.nty;
     bz = id / (tm.ntx * tm.nty);
   } else {
-    tm.decode(blockIdx.x, bx, by, bz);
+    tm.decode(bid, bx, by, bz);
   }
   const int I0 = bx * TOUT, J0 = by * TOUT, K0 = bz * kz;
   const int nx = g.nx, ny = g.ny, nz = g.nz;
@@ -632,7 +634,7 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
     }
     if (want_dot) {
       dsum = block_sum<BLOCK>(dsum);
-      if (tid == 0) ep.part[blockIdx.x] = dsum;
+      if (tid == 0) ep.part[bid] = dsum;
     }
     return;
   }
@@ -902,9 +904,28 @@ __global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* 
   if (k < kend) step(Par<0>{}, k, uA, uB, cA, cB);
   if (want_dot) {
     dsum = block_sum<BLOCK>(dsum);
-    if (tid == 0) ep.part[blockIdx.x] = dsum;
+    if (tid == 0) ep.part[bid] = dsum;
   }
 }
+
+template <typename T, class M, int EM>
+__global__ __launch_bounds__(BLOCK) void k_march(GridArgs<T> g, M mat, const T* __restrict__ s, Epi<T> ep, int kz,
+                                                 TileMap tm, const int* __restrict__ order, SparseArgs<T> sp) {
+  march_body<T, M, EM>(blockIdx.x, g, mat, s, ep, kz, tm, order, sp);
+}
+// launch record of one march (k_one / k_multi)
+template <typename T, class M, int EM>
+struct MarchL {
+  GridArgs<T> g;
+  M mat;
+  const T* s;
+  Epi<T> ep;
+  int kz;
+  TileMap tm;
+  const int* order;
+  SparseArgs<T> sp;
+  __device__ __forceinline__ void run(int bid, int) const { march_body<T, M, EM>(bid, g, mat, s, ep, kz, tm, order, sp); }
+};
 
 // Block flags of the structured march for z extent kz: one block per tile id (x fastest, then y,
 // then z), flag = 1 when any cell the march block reads (cell columns I0-1 .. I0+14, rows
@@ -936,10 +957,11 @@ __global__ __launch_bounds__(BLOCK) void k_block_flags(GridArgs<T> g, M mat, int
 
 // -------------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T* __restrict__ s, T* __restrict__ ex) {
-  if ((int)blockIdx.x < a.nb_term) {
+__device__ __forceinline__ void coarse_extra_body(const int bid, const CoarseExtra<T>& a, const T* __restrict__ s,
+                                                  T* __restrict__ ex) {
+  if (bid < a.nb_term) {
     const int rw = a.rw;
-    const long long gid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long gid = (long long)bid * blockDim.x + threadIdx.x;
     const int t = (int)(gid / rw), lane = (int)(gid % rw);
     if (t >= a.nterm) return;   // the rw lanes of a term leave together
     const int q0 = a.r1ptr[t], q1 = a.r1ptr[t + 1];
@@ -957,7 +979,7 @@ __global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T*
     for (int q = qf + rw; q < q1; q += rw) atomicAdd(ex + a.r1dof[q], dk * a.r1w[q]);
     return;
   }
-  const int e = (blockIdx.x - a.nb_term) * blockDim.x + threadIdx.x;
+  const int e = (bid - a.nb_term) * blockDim.x + threadIdx.x;
   if (e >= a.nblk) return;
   const long long m = a.bcol[e], n = a.brow[e];
   const T* B = a.bval + 9LL * e;
@@ -965,6 +987,17 @@ __global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T*
 #pragma unroll
   for (int c = 0; c < 3; ++c) atomicAdd(ex + 3 * n + c, B[3 * c] * s0 + B[3 * c + 1] * s1 + B[3 * c + 2] * s2);
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T* __restrict__ s, T* __restrict__ ex) {
+  coarse_extra_body<T>(blockIdx.x, a, s, ex);
+}
+template <typename T>
+struct CoarseExtraL {
+  CoarseExtra<T> a;
+  const T* s;
+  T* ex;
+  __device__ __forceinline__ void run(int bid, int) const { coarse_extra_body<T>(bid, a, s, ex); }
+};
 
 // Fix-up after the structured march, one thread per touched node: adds the sparse part tc (and
 // clears it for the next pass).  Grid nodes got the structured epilogue without tc; the epilogue
@@ -1016,10 +1049,10 @@ __device__ void group_fixup(const CopyGroups<T>& ga, int g, long long nng, T* __
 }
 
 template <typename T, int EM>
-__global__ __launch_bounds__(256) void k_fixup(int ntouch, const int* __restrict__ tnode, long long nng,
-                                               T* __restrict__ tc, const T* __restrict__ s, Epi<T> ep,
-                                               CopyGroups<T> ga = CopyGroups<T>{}) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void fixup_body(const int bid, int ntouch, const int* __restrict__ tnode, long long nng,
+                                           T* __restrict__ tc, const T* __restrict__ s, const Epi<T>& ep,
+                                           const CopyGroups<T>& ga) {
+  const int i = bid * blockDim.x + threadIdx.x;
   if (EM == EPI_APPLY && ep.part) {
     // s . tc over the touched nodes: A s = structured part (its dot came from the march) + tc
     double d = 0.0;
@@ -1029,7 +1062,7 @@ __global__ __launch_bounds__(256) void k_fixup(int ntouch, const int* __restrict
       for (int c = 0; c < 3; ++c) d += (double)s[3 * n + c] * (double)tc[3 * i + c];
     }
     d = block_sum<256>(d);
-    if (threadIdx.x == 0) ep.part[blockIdx.x] = d;
+    if (threadIdx.x == 0) ep.part[bid] = d;
   }
   if (EM == EPI_CHEB && ga.ng) {
     if (i >= ntouch) {
@@ -1089,11 +1122,39 @@ __global__ __launch_bounds__(256) void k_fixup(int ntouch, const int* __restrict
   }
 }
 
+template <typename T, int EM>
+__global__ __launch_bounds__(256) void k_fixup(int ntouch, const int* __restrict__ tnode, long long nng,
+                                               T* __restrict__ tc, const T* __restrict__ s, Epi<T> ep,
+                                               CopyGroups<T> ga = CopyGroups<T>{}) {
+  fixup_body<T, EM>(blockIdx.x, ntouch, tnode, nng, tc, s, ep, ga);
+}
+template <typename T, int EM>
+struct FixupL {
+  int ntouch;
+  const int* tnode;
+  long long nng;
+  T* tc;
+  const T* s;
+  Epi<T> ep;
+  CopyGroups<T> ga;
+  __device__ __forceinline__ void run(int bid, int) const { fixup_body<T, EM>(bid, ntouch, tnode, nng, tc, s, ep, ga); }
+};
+
 template <typename T>
 __global__ void k_fill(T* __restrict__ p, long long n, T v) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
 }
+template <typename T>
+struct FillL {
+  T* p;
+  long long n;
+  T v;
+  __device__ __forceinline__ void run(int bid, int) const {
+    const long long i = (long long)bid * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+  }
+};
 
 // -------------------------------------------------------------------------------------------
 // Grid transfers between level l (fine, nodes nf per axis) and l+1 (coarse).
@@ -1113,10 +1174,11 @@ struct AxisR {
 
 // rc = P^T r over the grid nodes (structured part).
 template <typename T>
-__global__ void k_restrict(int nfx, int nfy, int nfz, int ncx, int ncy, int ncz, AxisR<T> rx, AxisR<T> ry,
-                           AxisR<T> rz, const T* __restrict__ r, T* __restrict__ rc, const T* __restrict__ dinv_c,
-                           T* __restrict__ xc, T c0) {
-  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void restrict_body(const int bid, int nfx, int nfy, int nfz, int ncx, int ncy, int ncz,
+                                              AxisR<T> rx, AxisR<T> ry, AxisR<T> rz, const T* __restrict__ r,
+                                              T* __restrict__ rc, const T* __restrict__ dinv_c, T* __restrict__ xc,
+                                              T c0) {
+  const long long id = (long long)bid * blockDim.x + threadIdx.x;
   const long long nn = (long long)ncx * ncy * ncz;
   if (id >= nn) return;
   const int I = id % ncx, J = (id / ncx) % ncy, K = id / ((long long)ncx * ncy);
@@ -1150,6 +1212,25 @@ __global__ void k_restrict(int nfx, int nfy, int nfz, int ncx, int ncy, int ncz,
     xc[3 * id + 2] = c0 * (m[2] * s0 + m[4] * s1 + m[5] * s2);
   }
 }
+template <typename T>
+__global__ void k_restrict(int nfx, int nfy, int nfz, int ncx, int ncy, int ncz, AxisR<T> rx, AxisR<T> ry,
+                           AxisR<T> rz, const T* __restrict__ r, T* __restrict__ rc, const T* __restrict__ dinv_c,
+                           T* __restrict__ xc, T c0) {
+  restrict_body<T>(blockIdx.x, nfx, nfy, nfz, ncx, ncy, ncz, rx, ry, rz, r, rc, dinv_c, xc, c0);
+}
+template <typename T>
+struct RestrictL {
+  int nfx, nfy, nfz, ncx, ncy, ncz;
+  AxisR<T> rx, ry, rz;
+  const T* r;
+  T* rc;
+  const T* dinv_c;
+  T* xc;
+  T c0;
+  __device__ __forceinline__ void run(int bid, int) const {
+    restrict_body<T>(bid, nfx, nfy, nfz, ncx, ncy, ncz, rx, ry, rz, r, rc, dinv_c, xc, c0);
+  }
+};
 
 // Interpolated coarse correction at fine grid node (i,j,k).
 template <typename T>
@@ -1182,12 +1263,13 @@ __device__ __forceinline__ void interp_node(int i, int j, int k, int ncx, int nc
 // split copies (ncopy, interpolated like their originals, rb/splitfe.py:153) are the blocks after
 // the grid nodes' nbg blocks of the same launch: both write disjoint nodes.
 template <typename T>
-__global__ void k_prolong_add(int nfx, int nfy, int nfz, int ncx, int ncy, AxisP<T> px, AxisP<T> py, AxisP<T> pz,
-                              const T* __restrict__ ec, const uint8_t* __restrict__ used, T* __restrict__ x,
-                              int nbg, int ncopy, const int* __restrict__ orig) {
+__device__ __forceinline__ void prolong_body(const int bid, int nfx, int nfy, int nfz, int ncx, int ncy, AxisP<T> px,
+                                             AxisP<T> py, AxisP<T> pz, const T* __restrict__ ec,
+                                             const uint8_t* __restrict__ used, T* __restrict__ x, int nbg, int ncopy,
+                                             const int* __restrict__ orig) {
   const long long nn = (long long)nfx * nfy * nfz;
-  if ((int)blockIdx.x >= nbg) {   // block-uniform
-    const int q = (blockIdx.x - nbg) * blockDim.x + threadIdx.x;
+  if (bid >= nbg) {   // block-uniform
+    const int q = (bid - nbg) * blockDim.x + threadIdx.x;
     if (q >= ncopy) return;
     const int o = orig[q];
     const int i = o % nfx, j = (o / nfx) % nfy, k = o / (nfx * nfy);
@@ -1197,7 +1279,7 @@ __global__ void k_prolong_add(int nfx, int nfy, int nfz, int ncx, int ncy, AxisP
     x[3 * n] += e[0]; x[3 * n + 1] += e[1]; x[3 * n + 2] += e[2];
     return;
   }
-  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long id = (long long)bid * blockDim.x + threadIdx.x;
   if (id >= nn) return;
   if (!used[id]) {
     x[3 * id] = x[3 * id + 1] = x[3 * id + 2] = T(0);
@@ -1209,5 +1291,25 @@ __global__ void k_prolong_add(int nfx, int nfy, int nfz, int ncx, int ncy, AxisP
   x[3 * id] += e[0]; x[3 * id + 1] += e[1]; x[3 * id + 2] += e[2];
 }
 
+
+template <typename T>
+__global__ void k_prolong_add(int nfx, int nfy, int nfz, int ncx, int ncy, AxisP<T> px, AxisP<T> py, AxisP<T> pz,
+                              const T* __restrict__ ec, const uint8_t* __restrict__ used, T* __restrict__ x,
+                              int nbg, int ncopy, const int* __restrict__ orig) {
+  prolong_body<T>(blockIdx.x, nfx, nfy, nfz, ncx, ncy, px, py, pz, ec, used, x, nbg, ncopy, orig);
+}
+template <typename T>
+struct ProlongL {
+  int nfx, nfy, nfz, ncx, ncy;
+  AxisP<T> px, py, pz;
+  const T* ec;
+  const uint8_t* used;
+  T* x;
+  int nbg, ncopy;
+  const int* orig;
+  __device__ __forceinline__ void run(int bid, int) const {
+    prolong_body<T>(bid, nfx, nfy, nfz, ncx, ncy, px, py, pz, ec, used, x, nbg, ncopy, orig);
+  }
+};
 
 }  // namespace cm

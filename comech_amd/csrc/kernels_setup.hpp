@@ -654,10 +654,9 @@ __global__ __launch_bounds__(256) void k_gjs_mirror(int n, double* __restrict__ 
 // The inverse is over the level's used DOFs only: map[i] = full DOF of compact DOF i (nullptr: the
 // identity).  x at DOFs outside the map is not written (kept zero).
 template <typename T>
-__global__ __launch_bounds__(256) void k_dense_gemv(int n, const double* __restrict__ Ainv,
-                                                    const int* __restrict__ map, const T* __restrict__ b,
-                                                    T* __restrict__ x) {
-  const int row = blockIdx.x;
+__device__ __forceinline__ void dense_gemv_body(const int row, int n, const double* __restrict__ Ainv,
+                                                const int* __restrict__ map, const T* __restrict__ b,
+                                                T* __restrict__ x) {
   if (row >= n) return;   // block-uniform
   const double* r = Ainv + (long long)row * n;
   double s = 0.0;
@@ -665,6 +664,21 @@ __global__ __launch_bounds__(256) void k_dense_gemv(int n, const double* __restr
   s = block_sum<256>(s);
   if (threadIdx.x == 0) x[map ? map[row] : row] = T(s);
 }
+template <typename T>
+__global__ __launch_bounds__(256) void k_dense_gemv(int n, const double* __restrict__ Ainv,
+                                                    const int* __restrict__ map, const T* __restrict__ b,
+                                                    T* __restrict__ x) {
+  dense_gemv_body<T>(blockIdx.x, n, Ainv, map, b, x);
+}
+template <typename T>
+struct GemvL {
+  int n;
+  const double* Ainv;
+  const int* map;
+  const T* b;
+  T* x;
+  __device__ __forceinline__ void run(int bid, int) const { dense_gemv_body<T>(bid, n, Ainv, map, b, x); }
+};
 
 // Ac[i][j] = A[map[i]][map[j]] (m x m from n x n)
 __global__ void k_dense_gather(int m, const int* __restrict__ map, int n, const double* __restrict__ A,

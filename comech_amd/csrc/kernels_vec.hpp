@@ -12,32 +12,64 @@ constexpr int RED_THREADS = 256;
 
 // partial[block] = sum a[i] * b[i]
 template <typename Ta, typename Tb>
-__global__ __launch_bounds__(RED_THREADS) void k_dot(long long n, const Ta* __restrict__ a, const Tb* __restrict__ b,
-                                                     double* __restrict__ partial) {
+__device__ __forceinline__ void dot_body(const int bid, const int nblk, long long n, const Ta* __restrict__ a,
+                                         const Tb* __restrict__ b, double* __restrict__ partial) {
   double s = 0.0;
-  for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (long long)gridDim.x * RED_THREADS)
+  for (long long i = (long long)bid * RED_THREADS + threadIdx.x; i < n; i += (long long)nblk * RED_THREADS)
     s += (double)a[i] * (double)b[i];
   s = block_sum<RED_THREADS>(s);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (threadIdx.x == 0) partial[bid] = s;
 }
+template <typename Ta, typename Tb>
+__global__ __launch_bounds__(RED_THREADS) void k_dot(long long n, const Ta* __restrict__ a, const Tb* __restrict__ b,
+                                                     double* __restrict__ partial) {
+  dot_body<Ta, Tb>(blockIdx.x, gridDim.x, n, a, b, partial);
+}
+template <typename Ta, typename Tb>
+struct DotL {
+  long long n;
+  const Ta* a;
+  const Tb* b;
+  double* partial;
+  __device__ __forceinline__ void run(int bid, int nblk) const { dot_body<Ta, Tb>(bid, nblk, n, a, b, partial); }
+};
 
 // S[dst] = sum partial[0..nb)
-__global__ __launch_bounds__(RED_THREADS) void k_finalize(int nb, const double* __restrict__ partial,
-                                                          double* __restrict__ S, int dst) {
+__device__ __forceinline__ void finalize_body(int nb, const double* __restrict__ partial, double* __restrict__ S,
+                                              int dst) {
   double s = 0.0;
   for (int i = threadIdx.x; i < nb; i += RED_THREADS) s += partial[i];
   s = block_sum<RED_THREADS>(s);
   if (threadIdx.x == 0) S[dst] = s;
 }
+__global__ __launch_bounds__(RED_THREADS) void k_finalize(int nb, const double* __restrict__ partial,
+                                                          double* __restrict__ S, int dst) {
+  finalize_body(nb, partial, S, dst);
+}
+struct FinalizeL {
+  int nb;
+  const double* partial;
+  double* S;
+  int dst;
+  __device__ __forceinline__ void run(int, int) const { finalize_body(nb, partial, S, dst); }
+};
+// S[dst] = S[src] (one thread)
+struct SlotCopyL {
+  double* S;
+  int dst, src;
+  __device__ __forceinline__ void run(int, int) const {
+    if (threadIdx.x == 0) S[dst] = S[src];
+  }
+};
 
 // ---- smoothers -------------------------------------------------------------------------
 // Dinv holds the symmetric 3x3 block inverses as 6 values per node (00 01 02 11 12 22).
 // First step of a smoother from x = 0: x = c * Dinv b.  Threads past nn take the split-copy
 // groups (x = c Ginv b over each group); the per-node threads skip group members.
 template <typename T>
-__global__ void k_cheb0(long long nn, T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ Dinv, T c,
-                        CopyGroups<T> ga) {
-  const long long n = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void cheb0_body(const int bid, long long nn, T* __restrict__ x, const T* __restrict__ b,
+                                           const T* __restrict__ Dinv, T c, const CopyGroups<T>& ga) {
+  const long long n = (long long)bid * blockDim.x + threadIdx.x;
   if (n >= nn) {
     if (n - nn < ga.ng) group_apply(ga, (int)(n - nn), b, x, c);
     return;
@@ -49,6 +81,21 @@ __global__ void k_cheb0(long long nn, T* __restrict__ x, const T* __restrict__ b
   x[3 * n + 1] = c * (m[1] * r0 + m[3] * r1 + m[4] * r2);
   x[3 * n + 2] = c * (m[2] * r0 + m[4] * r1 + m[5] * r2);
 }
+template <typename T>
+__global__ void k_cheb0(long long nn, T* __restrict__ x, const T* __restrict__ b, const T* __restrict__ Dinv, T c,
+                        CopyGroups<T> ga) {
+  cheb0_body<T>(blockIdx.x, nn, x, b, Dinv, c, ga);
+}
+template <typename T>
+struct Cheb0L {
+  long long nn;
+  T* x;
+  const T* b;
+  const T* Dinv;
+  T c;
+  CopyGroups<T> ga;
+  __device__ __forceinline__ void run(int bid, int) const { cheb0_body<T>(bid, nn, x, b, Dinv, c, ga); }
+};
 
 // y = M x (block-Jacobi apply, split-copy groups as in k_cheb0), for the power iteration
 template <typename T>
@@ -95,13 +142,13 @@ __global__ void k_randn(long long nn, const uint8_t* __restrict__ used, unsigned
 // ---- PCG -------------------------------------------------------------------------------
 // alpha = S[rz] / S[pap];  x += alpha p;  r -= alpha Ap;  partial ||r||^2;  optional cast of r
 template <typename TX, typename T, typename TC>
-__global__ __launch_bounds__(RED_THREADS) void k_pcg_xr(long long n, TX* __restrict__ x, T* __restrict__ r,
-                                                        const T* __restrict__ p, const T* __restrict__ Ap,
-                                                        const double* __restrict__ S, int s_rz, int s_pap,
-                                                        double* __restrict__ partial, TC* __restrict__ rc) {
+__device__ __forceinline__ void pcg_xr_body(const int bid, const int nblk, long long n, TX* __restrict__ x,
+                                            T* __restrict__ r, const T* __restrict__ p, const T* __restrict__ Ap,
+                                            const double* __restrict__ S, int s_rz, int s_pap,
+                                            double* __restrict__ partial, TC* __restrict__ rc) {
   const double alpha = S[s_rz] / S[s_pap];
   double s = 0.0;
-  for (long long i = (long long)blockIdx.x * RED_THREADS + threadIdx.x; i < n; i += (long long)gridDim.x * RED_THREADS) {
+  for (long long i = (long long)bid * RED_THREADS + threadIdx.x; i < n; i += (long long)nblk * RED_THREADS) {
     x[i] = TX((double)x[i] + alpha * (double)p[i]);
     const T ri = T((double)r[i] - alpha * (double)Ap[i]);
     r[i] = ri;
@@ -109,19 +156,55 @@ __global__ __launch_bounds__(RED_THREADS) void k_pcg_xr(long long n, TX* __restr
     s += (double)ri * (double)ri;
   }
   s = block_sum<RED_THREADS>(s);
-  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+  if (threadIdx.x == 0) partial[bid] = s;
 }
+template <typename TX, typename T, typename TC>
+__global__ __launch_bounds__(RED_THREADS) void k_pcg_xr(long long n, TX* __restrict__ x, T* __restrict__ r,
+                                                        const T* __restrict__ p, const T* __restrict__ Ap,
+                                                        const double* __restrict__ S, int s_rz, int s_pap,
+                                                        double* __restrict__ partial, TC* __restrict__ rc) {
+  pcg_xr_body<TX, T, TC>(blockIdx.x, gridDim.x, n, x, r, p, Ap, S, s_rz, s_pap, partial, rc);
+}
+template <typename TX, typename T, typename TC>
+struct PcgXrL {
+  long long n;
+  TX* x;
+  T* r;
+  const T* p;
+  const T* Ap;
+  const double* S;
+  int s_rz, s_pap;
+  double* partial;
+  TC* rc;
+  __device__ __forceinline__ void run(int bid, int nblk) const {
+    pcg_xr_body<TX, T, TC>(bid, nblk, n, x, r, p, Ap, S, s_rz, s_pap, partial, rc);
+  }
+};
 
 // beta = S[rz_new] / S[rz_old];  p = z + beta p
-template <typename T, typename TZ = T>
-__global__ void k_pcg_p(long long n, T* __restrict__ p, const TZ* __restrict__ z, const double* __restrict__ S,
-                        int s_new, int s_old, int s_sub = -1) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+template <typename T, typename TZ>
+__device__ __forceinline__ void pcg_p_body(const int bid, long long n, T* __restrict__ p, const TZ* __restrict__ z,
+                                           const double* __restrict__ S, int s_new, int s_old, int s_sub) {
+  const long long i = (long long)bid * blockDim.x + threadIdx.x;
   if (i >= n) return;
   // beta = (r.z) / (r_old.z_old); flexible (Polak-Ribiere) form subtracts r.z_old (slot s_sub)
   const double beta = (S[s_new] - (s_sub >= 0 ? S[s_sub] : 0.0)) / S[s_old];
   p[i] = T((double)z[i] + beta * (double)p[i]);
 }
+template <typename T, typename TZ = T>
+__global__ void k_pcg_p(long long n, T* __restrict__ p, const TZ* __restrict__ z, const double* __restrict__ S,
+                        int s_new, int s_old, int s_sub = -1) {
+  pcg_p_body<T, TZ>(blockIdx.x, n, p, z, S, s_new, s_old, s_sub);
+}
+template <typename T, typename TZ>
+struct PcgPL {
+  long long n;
+  T* p;
+  const TZ* z;
+  const double* S;
+  int s_new, s_old, s_sub;
+  __device__ __forceinline__ void run(int bid, int) const { pcg_p_body<T, TZ>(bid, n, p, z, S, s_new, s_old, s_sub); }
+};
 
 // out = in (with conversion)
 template <typename TO, typename TI>
@@ -136,6 +219,16 @@ __global__ void k_add(long long n, TX* __restrict__ x, const TD* __restrict__ d)
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) x[i] = TX((double)x[i] + (double)d[i]);
 }
+template <typename TX, typename TD>
+struct AddL {
+  long long n;
+  TX* x;
+  const TD* d;
+  __device__ __forceinline__ void run(int bid, int) const {
+    const long long i = (long long)bid * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = TX((double)x[i] + (double)d[i]);
+  }
+};
 
 // r = b - t (FP64) with partial ||r||^2 and optional cast (iterative-refinement residual)
 template <typename TC>
@@ -201,6 +294,32 @@ __global__ void k_z_v(int nrow, const int* __restrict__ rows, const int* __restr
   double s = 0.0;
   for (int q = ptr[r]; q < ptr[r + 1]; ++q) s += vals[q] * v[cols[q]];
   z[rows[r]] += s;
+}
+
+// out[i] = S_i[slot] for the n slot arrays S (one block): the residual norms of a batch's systems
+// read back with one copy
+__global__ void k_gather_slot(int n, double* const* __restrict__ S, int slot, double* __restrict__ out) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) out[i] = S[i][slot];
+}
+
+// ---- launch records --------------------------------------------------------------------
+// A launch record L holds a kernel's arguments and runs block bid of nblk (256 threads).
+// k_one launches one record; k_multi launches the records of several systems as one grid:
+// blocks start[s] .. start[s+1]-1 run record s (start has nsys + 1 entries).
+template <class L>
+__global__ __launch_bounds__(256) void k_one(L a) {
+  a.run(blockIdx.x, gridDim.x);
+}
+template <class L>
+__global__ __launch_bounds__(256) void k_multi(const L* __restrict__ a, const int* __restrict__ start, int nsys) {
+  const int b = blockIdx.x;
+  int lo = 0, hi = nsys - 1;   // the last s with start[s] <= b
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  a[lo].run(b - start[lo], start[lo + 1] - start[lo]);
 }
 
 }  // namespace cm

@@ -156,8 +156,37 @@ static void upload_cast(DevBuf<T>& dst, const std::vector<double>& src, hipStrea
 
 static inline dim3 grid1(long long n, int bs = 256) { return dim3((unsigned)div_up(n, bs)); }
 
+// One recorded launch: the record's bytes, its block count and the launcher of the same record
+// type over several systems at once (k_multi).
+using MultiFn = void (*)(const void* args, const int* start, int nsys, int nblocks, hipStream_t st);
+struct LaunchRec {
+  MultiFn multi;
+  size_t size;
+  std::vector<char> bytes;
+  long long nb;
+};
+template <class L>
+static void multi_launch(const void* args, const int* start, int nsys, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_multi<L>, dim3((unsigned)nblocks), dim3(256), 0, st, static_cast<const L*>(args), start, nsys);
+}
+
 struct System {
   hipStream_t stream = nullptr;
+  // launches of the operator, V-cycle and PCG steps go through go(): launched on the stream, or,
+  // while rec is set, appended to it (batched solves replay the records of several systems as
+  // single launches)
+  std::vector<LaunchRec>* rec = nullptr;
+  template <class L>
+  void go(const L& a, long long nb) {
+    static_assert(std::is_trivially_copyable<L>::value, "launch records are copied as bytes");
+    if (rec) {
+      LaunchRec r{&multi_launch<L>, sizeof(L), std::vector<char>(sizeof(L)), nb};
+      std::memcpy(r.bytes.data(), &a, sizeof(L));
+      rec->push_back(std::move(r));
+      return;
+    }
+    if (nb > 0) hipLaunchKernelGGL(k_one<L>, dim3((unsigned)nb), dim3(256), 0, stream, a);
+  }
   int sri = 1;
   std::vector<std::unique_ptr<Level>> L;
   Options opt;
@@ -1192,8 +1221,7 @@ struct System {
     if (v.ntouch) {
       const CopyGroups<T> ga = EM == EPI_CHEB ? groups_of<T>(l) : CopyGroups<T>{};
       CM_CHECK(!ga.ng || ep.y, "split-copy groups need the residual output");
-      hipLaunchKernelGGL((k_fixup<T, EM>), grid1(v.ntouch + ga.ng), dim3(256), 0, stream, v.ntouch, v.tnode.p, v.nng,
-                         d.tc.p, s, ep, ga);
+      go(FixupL<T, EM>{(int)v.ntouch, v.tnode.p, v.nng, d.tc.p, s, ep, ga}, div_up(v.ntouch + ga.ng, 256));
     }
     CM_LAUNCH_CHECK();
   }
@@ -1211,7 +1239,7 @@ struct System {
     a.r1ptr = v.r1ptr.p; a.r1dof = v.r1dof.p; a.r1w = d.r1w.p; a.r1k = d.r1k.p;
     a.brow = v.brow.p; a.bcol = v.bcol.p; a.bval = d.bval.p;
     const long long nb = a.nb_term + div_up(a.nblk, 256);
-    if (nb > 0) hipLaunchKernelGGL(k_coarse_extra<T>, dim3((unsigned)nb), dim3(256), 0, stream, a, s, d.ex.p);
+    go(CoarseExtraL<T>{a, s, d.ex.p}, nb);   // recorded even when empty: batched systems keep one launch order
   }
 
   // lanes per rank-one term in the sparse passes: the power of two >= the mean support (<= 64)
@@ -1237,10 +1265,10 @@ struct System {
     if (v.ntouch) {
       Epi<double> ef = ep;
       ef.part = partial.p + nbm;
-      hipLaunchKernelGGL((k_fixup<double, EPI_APPLY>), grid1(v.ntouch), dim3(256), 0, stream, v.ntouch, v.tnode.p,
-                         v.nng, d.tc.p, p, ef, CopyGroups<double>{});
+      go(FixupL<double, EPI_APPLY>{(int)v.ntouch, v.tnode.p, v.nng, d.tc.p, p, ef, CopyGroups<double>{}},
+         div_up(v.ntouch, 256));
     }
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, nbm + nbf, partial.p, S.p, slot);
+    go(FinalizeL{nbm + nbf, partial.p, S.p, slot}, 1);
     CM_LAUNCH_CHECK();
   }
   // partials needed by apply_dot on level l (FP64)
@@ -1381,8 +1409,7 @@ struct System {
     with_mat<T>(l, [&](auto m) {
       using M = decltype(m);
       CM_CHECK(sp.nb == 0 || M::kind <= 1, "the sparse pass rides only on level-0 marches");
-      hipLaunchKernelGGL((k_march<T, M, EM>), dim3((unsigned)(ms.nb + sp.nb)), dim3(BLOCK), 0, stream, g, m, s, ep,
-                         ms.kz, tm, fl, sp);
+      go(MarchL<T, M, EM>{g, m, s, ep, ms.kz, tm, fl, sp}, ms.nb + sp.nb);
     });
   }
 
@@ -1544,15 +1571,13 @@ struct System {
       // damped block Jacobi: x += omega Dinv (b - A x); the step lands in d, then x += d
       int s0 = 0;
       if (zero_start) {
-        if (!x_started)
-          hipLaunchKernelGGL(k_cheb0<T>, grid1(nn + ga.ng), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(opt.omega),
-                             ga);
+        if (!x_started) go(Cheb0L<T>{nn, x, b, d.Dinv.p, T(opt.omega), ga}, div_up(nn + ga.ng, 256));
         s0 = 1;
       }
       for (int s = s0; s < nsweep; ++s) {
         const Epi<T> ep{b, rkeep, d.Dinv.p, d.d.p, nullptr, T(0), T(opt.omega), T(0)};
         apply_ep<T, EPI_CHEB>(l, x, ep);
-        hipLaunchKernelGGL((k_add<T, T>), grid1(3 * nn), dim3(256), 0, stream, 3 * nn, x, d.d.p);
+        go(AddL<T, T>{3 * nn, x, d.d.p}, div_up(3 * nn, 256));
       }
       CM_LAUNCH_CHECK();
       return;
@@ -1573,8 +1598,7 @@ struct System {
     bool pending;          // s is a direction not yet added to x
     int nd = 0;            // next direction buffer
     if (zero_start) {
-      if (!x_started)
-        hipLaunchKernelGGL(k_cheb0<T>, grid1(nn + ga.ng), dim3(256), 0, stream, nn, x, b, d.Dinv.p, T(1.0 / theta), ga);
+      if (!x_started) go(Cheb0L<T>{nn, x, b, d.Dinv.p, T(1.0 / theta), ga}, div_up(nn + ga.ng, 256));
       s = x;
       pending = false;
     } else {
@@ -1601,8 +1625,7 @@ struct System {
       rin = r;
       rho = rho_new;
     }
-    if (pending && s != x)
-      hipLaunchKernelGGL((k_add<T, T>), grid1(3 * nn), dim3(256), 0, stream, 3 * nn, x, s);
+    if (pending && s != x) go(AddL<T, T>{3 * nn, x, s}, div_up(3 * nn, 256));
     CM_LAUNCH_CHECK();
   }
 
@@ -1611,9 +1634,9 @@ struct System {
     if (l == nlev() - 1) {
       if (l == 0) {   // one level: x is the caller's vector; the solve writes only the used DOFs
         const long long nv = 3 * lv(0).nn;
-        hipLaunchKernelGGL(k_fill<T>, grid1(nv), dim3(256), 0, stream, x, nv, T(0));
+        go(FillL<T>{x, nv, T(0)}, div_up(nv, 256));
       }
-      hipLaunchKernelGGL(k_dense_gemv<T>, dim3((unsigned)nc_dense), dim3(256), 0, stream, nc_dense, Ainv.p, dmap.p, b, x);
+      go(GemvL<T>{nc_dense, Ainv.p, dmap.p, b, x}, nc_dense);
       CM_LAUNCH_CHECK();
       return;
     }
@@ -1634,15 +1657,16 @@ struct System {
     // the restriction also takes the coarse level's first smoothing step from zero (one launch
     // less); the coarsest level is solved directly instead
     const bool start_c = l + 2 < nlev();
-    hipLaunchKernelGGL(k_restrict<T>, grid1(c.nn), dim3(256), 0, stream, v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1,
-                       c.ny + 1, c.nz + 1, rx, ry, rz, d.r.p, dc.b.p, start_c ? dc.Dinv.p : (const T*)nullptr,
-                       start_c ? dc.x.p : (T*)nullptr, T(start_c ? first_step_coef<T>(l + 1) : 0.0));
+    go(RestrictL<T>{v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1, c.ny + 1, c.nz + 1, rx, ry, rz, d.r.p, dc.b.p,
+                    start_c ? dc.Dinv.p : (const T*)nullptr, start_c ? dc.x.p : (T*)nullptr,
+                    T(start_c ? first_step_coef<T>(l + 1) : 0.0)},
+       div_up(c.nn, 256));
     CM_LAUNCH_CHECK();
     vcycle<T>(l + 1, dc.b.p, dc.x.p, start_c);
     const int nbg = div_up(v.nng, 256);
-    hipLaunchKernelGGL(k_prolong_add<T>, dim3((unsigned)(nbg + div_up(ncopy, 256))), dim3(256), 0, stream, v.nx + 1,
-                       v.ny + 1, v.nz + 1, c.nx + 1, c.ny + 1, px, py, pz, dc.x.p, v.used.p, x, nbg, ncopy,
-                       v.copy_orig.p);
+    go(ProlongL<T>{v.nx + 1, v.ny + 1, v.nz + 1, c.nx + 1, c.ny + 1, px, py, pz, dc.x.p, v.used.p, x, nbg, ncopy,
+                   v.copy_orig.p},
+       nbg + div_up(ncopy, 256));
     CM_LAUNCH_CHECK();
     smooth<T>(l, b, x, false, opt.npost);
   }
@@ -1700,13 +1724,11 @@ struct System {
 
   // ------------------------------------------------------------------ PCG
   template <typename T>
-  void finalize_to(int slot) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, S.p, slot);
-  }
+  void finalize_to(int slot) { go(FinalizeL{RED_BLOCKS, partial.p, S.p, slot}, 1); }
   template <typename Ta, typename Tb>
   void dot(long long n, const Ta* a, const Tb* b, int slot) {
-    hipLaunchKernelGGL((k_dot<Ta, Tb>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, a, b, partial.p);
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(RED_THREADS), 0, stream, RED_BLOCKS, partial.p, S.p, slot);
+    go(DotL<Ta, Tb>{n, a, b, partial.p}, RED_BLOCKS);
+    go(FinalizeL{RED_BLOCKS, partial.p, S.p, slot}, 1);
   }
   double read_slot(int slot) {
     double v;
@@ -2032,6 +2054,319 @@ void need_level(cm_sys* h, int level) {
   CM_CHECK(level >= 0 && level < h->s.nlev(), "level out of range");
 }
 }  // namespace
+
+// ---------------------------------------------------------------------------------------------
+// Lock-step batched PCG over several systems on one stream (mode "ref": FP64 Krylov vectors,
+// alpha/beta reductions and operator, FP32 V-cycle; the same iteration as System::pcg64).  Every
+// launch of an iteration -- the FP64 product with its dot, the vector updates, every kernel of
+// the V-cycle on every level -- covers all systems of the batch: their launch records
+// (System::rec) are laid out side by side and run as one grid (k_multi).  Systems join and leave
+// between iterations (continuous batching).  The recurrence residual is checked on the host every
+// check_every iterations of the batch, and a system whose recurrence has converged is confirmed,
+// restarted or stopped by its FP64 true residual exactly as pcg64 does.
+struct cm_batch {
+  hipStream_t st = nullptr;
+  int check_every = 5;
+  long long iter = 0;                 // iterations of the batch
+  struct Slot {
+    cm_sys* h = nullptr;
+    hipStream_t user = nullptr;       // the system's stream, restored when it leaves
+    const double* b = nullptr;
+    double* x = nullptr;
+    double thr = 0, bn = 0, rn = 0, target = 0, last_true = 1e300;
+    int it = 0, maxiter = 0, restarts = 0, stalls = 0;
+    bool flexible = false, live = false, done = false;
+    std::vector<cm::LaunchRec> seq;   // one iteration's launches
+    cm_solve_info info{};
+    std::chrono::steady_clock::time_point t0;
+  };
+  std::vector<Slot> slots;
+  std::vector<cm::MultiFn> sig;       // record types of one iteration (the batch's first system)
+  // device layout of the live slots' records
+  bool plan_ok = false;
+  std::vector<int> live;
+  std::vector<char> h_args;
+  std::vector<int> h_start;
+  cm::DevBuf<char> d_args;
+  cm::DevBuf<int> d_start;
+  std::vector<size_t> aoff, soff;
+  std::vector<long long> tot;
+  cm::DevBuf<double*> d_sptr;
+  cm::DevBuf<double> d_rr;
+  std::vector<double> h_rr;
+
+  ~cm_batch() {
+    if (st) (void)hipStreamSynchronize(st);
+  }
+
+  static long long n3(System& s) { return 3 * s.lv(0).nn; }
+
+  void record(Slot& sl) {
+    System& s = sl.h->s;
+    const long long n = n3(s);
+    sl.seq.clear();
+    s.rec = &sl.seq;
+    try {
+      // head: Ap = A p with p.Ap; x += alpha p, r -= alpha Ap, r32 = r; ||r||^2
+      s.apply_dot(0, s.p64.p, s.Ap64.p, System::S_PAP);
+      s.go(cm::PcgXrL<double, double, float>{n, sl.x, s.r64.p, s.p64.p, s.Ap64.p, s.S.p, System::S_RZ0,
+                                             System::S_PAP, s.partial.p, s.r32.p},
+           cm::RED_BLOCKS);
+      s.finalize_to<double>(System::S_RR);
+      // tail: r.z_old (flexible form), z = V(r) in FP32, r.z, p = z + beta p, rz_old = rz
+      s.dot(n, s.r64.p, s.z32.p, System::S_RZP);
+      s.vcycle<float>(0, s.r32.p, s.z32.p);
+      s.dot(n, s.r64.p, s.z32.p, System::S_RZ1);
+      s.go(cm::PcgPL<double, float>{n, s.p64.p, s.z32.p, s.S.p, System::S_RZ1, System::S_RZ0,
+                                    sl.flexible ? (int)System::S_RZP : -1},
+           cm::div_up(n, 256));
+      s.go(cm::SlotCopyL{s.S.p, System::S_RZ0, System::S_RZ1}, 1);
+    } catch (...) {
+      s.rec = nullptr;
+      throw;
+    }
+    s.rec = nullptr;
+    plan_ok = false;
+  }
+
+  bool same_sig(const Slot& sl) const {
+    if (sig.size() != sl.seq.size()) return false;
+    for (size_t j = 0; j < sig.size(); ++j)
+      if (sig[j] != sl.seq[j].multi) return false;
+    return true;
+  }
+
+  void plan() {
+    live.clear();
+    for (int k = 0; k < (int)slots.size(); ++k)
+      if (slots[k].live) live.push_back(k);
+    plan_ok = true;
+    if (live.empty()) return;
+    const int K = (int)live.size();
+    const size_t nrec = sig.size();
+    aoff.assign(nrec, 0);
+    soff.assign(nrec, 0);
+    tot.assign(nrec, 0);
+    size_t asz = 0;
+    for (size_t j = 0; j < nrec; ++j) {
+      aoff[j] = asz;
+      asz += ((slots[live[0]].seq[j].size * K + 255) / 256) * 256;
+    }
+    h_args.assign(asz, 0);
+    h_start.assign(nrec * (K + 1), 0);
+    for (size_t j = 0; j < nrec; ++j) {
+      soff[j] = j * (K + 1);
+      long long acc = 0;
+      for (int k = 0; k < K; ++k) {
+        const cm::LaunchRec& r = slots[live[k]].seq[j];
+        std::memcpy(h_args.data() + aoff[j] + k * r.size, r.bytes.data(), r.size);
+        h_start[soff[j] + k] = (int)acc;
+        acc += std::max(0LL, r.nb);
+      }
+      CM_CHECK(acc < (1LL << 31), "batched launch too large");
+      h_start[soff[j] + K] = (int)acc;
+      tot[j] = acc;
+    }
+    if (d_args.n < asz) d_args.alloc(asz);
+    if (d_start.n < h_start.size()) d_start.alloc(h_start.size());
+    CM_HIP(hipMemcpyAsync(d_args.p, h_args.data(), asz, hipMemcpyHostToDevice, st));
+    CM_HIP(hipMemcpyAsync(d_start.p, h_start.data(), h_start.size() * sizeof(int), hipMemcpyHostToDevice, st));
+    std::vector<double*> sp(K);
+    for (int k = 0; k < K; ++k) sp[k] = slots[live[k]].h->s.S.p;
+    if (d_sptr.n < (size_t)K) d_sptr.alloc(std::max(K, 64));
+    if (d_rr.n < (size_t)K) d_rr.alloc(std::max(K, 64));
+    CM_HIP(hipMemcpyAsync(d_sptr.p, sp.data(), K * sizeof(double*), hipMemcpyHostToDevice, st));
+    CM_HIP(hipStreamSynchronize(st));   // the host copies above must outlive the transfers
+  }
+
+  void replay() {
+    const int K = (int)live.size();
+    for (size_t j = 0; j < sig.size(); ++j)
+      if (tot[j] > 0) sig[j](d_args.p + aoff[j], d_start.p + soff[j], K, (int)tot[j], st);
+    CM_LAUNCH_CHECK();
+  }
+
+  void finish(Slot& sl, int flag) {
+    System& s = sl.h->s;
+    sl.info.iters = sl.it;
+    sl.info.flag = flag;
+    sl.info.outer = 1 + sl.restarts;
+    sl.info.relres = sl.rn / (sl.bn > 0 ? sl.bn : 1.0);
+    sl.info.levels = s.nlev();
+    sl.info.t_setup_ms = s.t_setup_ms;
+    sl.info.t_solve_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - sl.t0).count();
+    // hand the system back to its own stream after the batch's work on it
+    hipEvent_t ev;
+    CM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    CM_HIP(hipEventRecord(ev, st));
+    CM_HIP(hipStreamWaitEvent(sl.user, ev, 0));
+    CM_HIP(hipEventDestroy(ev));
+    s.stream = sl.user;
+    sl.live = false;
+    sl.done = true;
+    sl.seq.clear();
+    plan_ok = false;
+  }
+
+  // true residual of the slot's x (FP64), into rn
+  double true_res(Slot& sl) {
+    System& s = sl.h->s;
+    s.apply<double>(0, sl.x, s.r64.p, cm::MODE_RESID, sl.b);
+    s.dot(n3(s), s.r64.p, s.r64.p, System::S_RR);
+    return std::sqrt(s.read_slot(System::S_RR));
+  }
+
+  int add(cm_sys* h, const double* b, double* x, double tol, int maxiter) {
+    System& s = h->s;
+    CM_CHECK(s.built, "system not built (call cm_sys_build)");
+    CM_CHECK(s.zcol == 0, "batched solves do not take a rigid-mode correction");
+    for (auto& o : slots)
+      CM_CHECK(!(o.live && o.h == h), "a system can be in the batch once at a time");
+    int k = 0;   // a free slot: not live and its result taken
+    while (k < (int)slots.size() && (slots[k].live || slots[k].h != nullptr)) ++k;
+    if (k == (int)slots.size()) slots.emplace_back();
+    Slot& sl = slots[k];
+    sl = Slot{};
+    sl.h = h;
+    sl.b = b;
+    sl.x = x;
+    sl.maxiter = maxiter;
+    sl.t0 = std::chrono::steady_clock::now();
+    // the system's queued work (its build) before the batch's
+    sl.user = s.stream;
+    hipEvent_t ev;
+    CM_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    CM_HIP(hipEventRecord(ev, sl.user));
+    CM_HIP(hipStreamWaitEvent(st, ev, 0));
+    CM_HIP(hipEventDestroy(ev));
+    s.stream = st;
+    try {
+      s.alloc_pcg64();
+      s.alloc_pcg32();
+      s.ensure_prec<float>();
+      s.ensure_data<double>();
+      if (s.partial.n < s.dot_partials(0)) s.partial.alloc(std::max((size_t)cm::RED_BLOCKS, s.dot_partials(0)));
+      s.refresh_all_flags<double>();
+      s.refresh_all_flags<float>();
+      const long long n = n3(s);
+      // r = b - A x, z = V(r), p = z (pcg64's start)
+      s.apply<double>(0, x, s.r64.p, cm::MODE_RESID, b);
+      s.dot(n, b, b, System::S_BB);
+      sl.bn = std::sqrt(s.read_slot(System::S_BB));
+      sl.thr = tol * (sl.bn > 0 ? sl.bn : 1.0);
+      sl.target = sl.thr;
+      s.precondition64(1, s.r64.p, s.z64.p);
+      CM_HIP(hipMemcpyAsync(s.p64.p, s.z64.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      s.dot(n, s.r64.p, s.z64.p, System::S_RZ0);
+      s.dot(n, s.r64.p, s.r64.p, System::S_RR);
+      sl.rn = std::sqrt(s.read_slot(System::S_RR));
+      if (sl.rn <= sl.thr || maxiter <= 0) {
+        finish(sl, sl.rn <= sl.thr ? 0 : maxiter);
+        return k;
+      }
+      record(sl);
+      if (sig.empty() || std::none_of(slots.begin(), slots.end(), [](const Slot& o) { return o.live; })) {
+        sig.clear();
+        for (auto& r : sl.seq) sig.push_back(r.multi);
+      } else if (!same_sig(sl)) {
+        sl.seq.clear();
+        s.stream = sl.user;
+        sl = Slot{};
+        sl.done = false;
+        sl.h = nullptr;
+        return -1;   // another launch structure (level count, springs): solve it on its own
+      }
+      sl.live = true;
+      plan_ok = false;
+    } catch (...) {
+      s.stream = sl.user;
+      s.rec = nullptr;
+      sl = Slot{};
+      throw;
+    }
+    return k;
+  }
+
+  // host decisions of pcg64 at a check, for one live slot
+  void check(Slot& sl, double rr) {
+    System& s = sl.h->s;
+    sl.rn = std::sqrt(rr);
+    if (sl.rn <= sl.target) {
+      const double rec = sl.rn;
+      sl.rn = true_res(sl);
+      if (sl.rn <= sl.thr) {
+        finish(sl, 0);
+        return;
+      }
+      sl.target = std::max(0.25 * sl.thr, sl.thr - (sl.rn - rec));
+      sl.stalls = (sl.rn > 0.5 * sl.last_true) ? sl.stalls + 1 : 0;
+      sl.last_true = sl.rn;
+      if (sl.stalls >= 3) {
+        finish(sl, sl.it > 0 ? sl.it : sl.maxiter);
+        return;
+      }
+      ++sl.restarts;
+      const long long n = n3(s);
+      s.precondition64(1, s.r64.p, s.z64.p);
+      CM_HIP(hipMemcpyAsync(s.p64.p, s.z64.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
+      s.dot(n, s.r64.p, s.z64.p, System::S_RZ0);
+      if (!sl.flexible) {
+        sl.flexible = true;
+        record(sl);
+      }
+      return;
+    }
+    if (sl.it >= sl.maxiter) {
+      sl.rn = true_res(sl);
+      finish(sl, sl.rn <= sl.thr ? 0 : sl.it);
+    }
+  }
+
+  // iterations until a slot has finished or max_iters ran; returns the live slots left
+  int step(int max_iters) {
+    int ran = 0;
+    while (ran < max_iters) {
+      if (!plan_ok) plan();
+      if (live.empty()) break;
+      replay();
+      for (int k : live) ++slots[k].it;
+      ++iter;
+      ++ran;
+      if (iter % check_every) continue;
+      const int K = (int)live.size();
+      hipLaunchKernelGGL(cm::k_gather_slot, dim3(1), dim3(256), 0, st, K, d_sptr.p, (int)System::S_RR, d_rr.p);
+      h_rr.resize(K);
+      CM_HIP(hipMemcpyAsync(h_rr.data(), d_rr.p, K * sizeof(double), hipMemcpyDeviceToHost, st));
+      CM_HIP(hipStreamSynchronize(st));
+      const std::vector<int> now = live;
+      bool any_done = false;
+      for (int q = 0; q < K; ++q) {
+        Slot& sl = slots[now[q]];
+        check(sl, h_rr[q]);
+        any_done |= !sl.live;
+      }
+      if (any_done) break;
+    }
+    int n = 0;
+    for (auto& sl : slots) n += sl.live;
+    return n;
+  }
+
+  // a finished slot's result (slot -1: none finished)
+  void poll(int* slot, cm_solve_info* info) {
+    *slot = -1;
+    for (int k = 0; k < (int)slots.size(); ++k) {
+      Slot& sl = slots[k];
+      if (sl.done && sl.h) {
+        *slot = k;
+        *info = sl.info;
+        sl.h = nullptr;
+        return;
+      }
+    }
+  }
+};
 
 extern "C" {
 
@@ -2719,6 +3054,49 @@ int cm_sample_nodes_noalloc(void* stream, int mx, int my, int mz, const double* 
   hipLaunchKernelGGL(cm::k_sample_nodes, cm::grid1(mnode), dim3(256), 0, st, mx + 1, my + 1, mz + 1, mxe, mye, mze, ci,
                      cj, ck, qz, nx, ny, nz, xe, ye, ze, active, dz2, kbest, (const long long*)cn, Urel, out);
   CM_LAUNCH_CHECK();
+  CM_API_END
+}
+
+
+int cm_batch_create(cm_batch** out, void* stream, int check_every) {
+  CM_API_BEGIN
+  CM_CHECK(out != nullptr, "out is null");
+  auto* B = new cm_batch();
+  B->st = (hipStream_t)stream;
+  B->check_every = check_every > 0 ? check_every : 5;
+  *out = B;
+  CM_API_END
+}
+
+int cm_batch_destroy(cm_batch* B) {
+  CM_API_BEGIN
+  if (B) {
+    for (auto& sl : B->slots)
+      if (sl.live && sl.h) sl.h->s.stream = sl.user;
+    delete B;
+  }
+  CM_API_END
+}
+
+int cm_batch_add(cm_batch* B, cm_sys* h, const double* b, double* x, double tol, int maxiter, int* slot) {
+  CM_API_BEGIN
+  CM_CHECK(B != nullptr && h != nullptr && slot != nullptr, "null argument");
+  need_built(h);
+  *slot = B->add(h, b, x, tol, maxiter);
+  CM_API_END
+}
+
+int cm_batch_step(cm_batch* B, int max_iters, int* nlive) {
+  CM_API_BEGIN
+  CM_CHECK(B != nullptr && nlive != nullptr, "null argument");
+  *nlive = B->step(max_iters > 0 ? max_iters : 1);
+  CM_API_END
+}
+
+int cm_batch_poll(cm_batch* B, int* slot, cm_solve_info* info) {
+  CM_API_BEGIN
+  CM_CHECK(B != nullptr && slot != nullptr && info != nullptr, "null argument");
+  B->poll(slot, info);
   CM_API_END
 }
 

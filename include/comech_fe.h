@@ -166,6 +166,25 @@ int cm_sys_smooth(cm_sys* sys, int level, int prec, const void* b, void* x, int 
 int cm_sys_solve(cm_sys* sys, int mode, const double* b, double* x, double tol, int maxiter, int check_every,
                  double inner_tol, cm_solve_info* info);
 
+/* ---- batched solves in lock-step ----------------------------------------------------------------
+ * Replaces the reference's one-solve-per-process batch (rb/gpusolve.py:174-180,
+ * exp/rb_run_engine_v4.py:113-146) on one GPU: systems in a batch are solved together by the PCG
+ * of cm_sys_solve mode 1 (FP64 PCG, FP32 V-cycle), every launch of an iteration covering all of
+ * them.  A system joins with cm_batch_add (after its build, on any stream; its stream waits for
+ * the batch while it is in) and leaves when converged, stopped or at maxiter; cm_batch_poll hands
+ * back its cm_solve_info.  Systems whose launch structure differs from the batch's (level count,
+ * springs on a coarse level) are not added: *slot = -1, solve them with cm_sys_solve.  The
+ * residual is checked every check_every iterations of the batch. */
+typedef struct cm_batch cm_batch;
+int cm_batch_create(cm_batch** out, void* stream, int check_every);
+int cm_batch_destroy(cm_batch* batch);
+/* Starts PCG on (sys, b, x); x holds the initial guess.  *slot: the slot id, or -1 (not added). */
+int cm_batch_add(cm_batch* batch, cm_sys* sys, const double* b, double* x, double tol, int maxiter, int* slot);
+/* Runs iterations until a system has left the batch or max_iters ran; *nlive: systems still in. */
+int cm_batch_step(cm_batch* batch, int max_iters, int* nlive);
+/* One finished system: its slot and result (*slot = -1: none waiting).  The slot is free again. */
+int cm_batch_poll(cm_batch* batch, int* slot, cm_solve_info* info);
+
 /* Cell-centre strain/stress/von Mises (fe/solver.py:257-272, rb/splitfe.py:161-175) for n listed
  * cells.  hx/hy/hz: cell sizes per axis; cells_ref: reference cell ids; nodes8: 8 node ids per
  * listed cell or NULL for the grid's own corners; lam_ref/mu_ref indexed by reference cell id;

@@ -119,16 +119,58 @@ def test_config2_actions_other_modes(actions40, a, mode):
 
 
 # ---------------------------------------------------------------- configuration 3: batched
-def test_config3_uneven_batch_against_oracle():
+CONFIG3_FEET = [0, 1, 2]
+
+
+@pytest.fixture(scope="module")
+def config3_answers():
+    specs = [("batch", dict(foot=f, action=a, scale=4), {}) for f in CONFIG3_FEET for a in synth.ACTIONS]
+    return specs, C.oracle_answers(specs)
+
+
+def test_config3_lockstep_batch_against_oracle(config3_answers):
+    """The same 24 systems solved in lock-step (every launch of a PCG iteration covers all systems
+    in the batch, comech_amd.batch.solve_lockstep, cm_batch_*): width 6, so systems join and leave
+    the batch while it runs (the load actions reuse their baseline's system); every answer against
+    the oracle, every system in lock-step, every solve converged by its FP64 true residual."""
+    from comech_amd.batch import build_system, rhs, solve_lockstep
+    specs, answers = config3_answers
+    descs = {f: synth.batch_foot(f, scale=4) for f in CONFIG3_FEET}
+    groups = (("baseline", "load0.5", "load1.5", "load2"), ("resection",), ("osteotomy",), ("interposition",),
+              ("fusion",))
+    items = []
+    for f in CONFIG3_FEET:
+        for grp in groups:
+            idx = [synth.ACTIONS.index(a) for a in grp]
+
+            def mk(f=f, idx=idx):
+                s = build_system(descs[f][idx[0]], device=DEV, min_dofs=3000, max_levels=7)
+                return s, [((f, synth.ACTIONS[k]), rhs(descs[f][k], DEV)) for k in idx]
+            items.append(mk)
+    got, wall = solve_lockstep(items, width=6, builders=2, tol=1e-8, device=DEV)
+    assert len(got) == len(specs)
+    for (kind, kw, _), ans in zip(specs, answers):
+        key = (kw["foot"], kw["action"])
+        x, info = got[key]
+        U = x.view(-1, 3).cpu().numpy().copy()
+        U[np.isnan(ans["U"]).any(1)] = np.nan
+        eu = rel(U, ans["U"])
+        d = descs[kw["foot"]][synth.ACTIONS.index(kw["action"])]
+        print(f"{key}: iters {info['iters']} flag {info['flag']} relres {info['relres']:.1e} U {eu:.1e}", flush=True)
+        assert info["lockstep"] and info["flag"] == 0 and info["relres"] <= 1e-8, key
+        assert eu <= U_TOL, key
+        assert d["nn"] == len(ans["U"]), key
+
+
+def test_config3_uneven_batch_against_oracle(config3_answers):
     """The first three feet of the 64-foot set (uneven grids, each its own seed) x 8 actions at a
     quarter of their grid dimensions: 24 systems solved batched, 4 in flight on one GPU, rtol
     1e-8 (the configuration's); every batched answer against the oracle.  (At 1e-9 the fused
     joint, k = 1e6, sits at the attainable floor of an FP32 V-cycle: its true residual can stall
     within a few per cent of the tolerance, and the solver then reports non-convergence.)"""
     from comech_amd.batch import run_batch
-    feet = [0, 1, 2]
-    specs = [("batch", dict(foot=f, action=a, scale=4), {}) for f in feet for a in synth.ACTIONS]
-    answers = C.oracle_answers(specs)
+    feet = CONFIG3_FEET
+    specs, answers = config3_answers
     cases = [C.make_case(*s[:2]) for s in specs]
     assert len({c["grid"].shape for c in cases}) == len(feet)        # uneven grids
 

@@ -131,6 +131,36 @@ def batch_leg(feet, descs, bs, mode, tol, conc, dev, ref_x=None, keep=False, mg=
     return infos, xs, wall
 
 
+def lockstep_leg(feet, descs, bs, tol, width, dev, ref_x=None, mg=None):
+    """All solves of ``feet`` in lock-step (comech_amd.batch.solve_lockstep): every launch of a PCG
+    iteration covers all systems in the batch, systems join and leave as they finish; the load
+    actions reuse their baseline's system.  Returns (infos, wall seconds)."""
+    import torch
+    from comech_amd.batch import build_system, solve_lockstep
+    from comech_amd.synth import ACTIONS
+    items = []
+    for fi in feet:
+        for grp in GROUPS:
+            idx = [ACTIONS.index(a) for a in grp]
+
+            def mk(fi=fi, idx=idx):
+                s = build_system(descs[fi][idx[0]], device=dev, **(mg or {}))
+                return s, [((fi, k), bs[fi][k]) for k in idx]
+            items.append((len(idx) * descs[fi][0]["ncell"], mk))
+    items.sort(key=lambda j: -j[0])
+    got, wall = solve_lockstep([j[1] for j in items], width=width, builders=4, tol=tol, device=dev)
+    infos = []
+    for (fi, k), (x, info) in got.items():
+        b = bs[fi][k]
+        info["ndof"] = int(b.numel())
+        info.update(foot=fi, action=ACTIONS[k])
+        if ref_x is not None:
+            xr = ref_x[(fi, k)]
+            info["u_rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(x - xr) / torch.linalg.vector_norm(xr))
+        infos.append(info)
+    return infos, wall
+
+
 def summarize(infos, wall, extra=None):
     import numpy as np
     iters = [i["iters"] for i in infos]
@@ -267,6 +297,10 @@ def batch_workload(args):
     log(f"ref: {len(ref) / w_ref:.2f} solves/s")
     mixed, _, w_mixed = batch_leg(feet, descs, bs, "mixed", args.tol, conc, dev, ref_x=plain_x, mg=TUNED)
     log(f"mixed: {len(mixed) / w_mixed:.2f} solves/s")
+    lock, w_lock = lockstep_leg(feet, descs, bs, args.tol, LOCK_WIDTH, dev, ref_x=plain_x, mg=TUNED)
+    for i in lock:
+        i["true_relres"] = i["relres"]
+    log(f"lock-step: {len(lock) / w_lock:.2f} solves/s")
     del plain_x
     seq, _, w_seq = batch_leg(feet[:1], descs, bs, "ref", args.tol, 1, dev, mg=TUNED)
     # the plain method as the reference runs it, one solve at a time, on a subset of the timed feet,
@@ -277,7 +311,7 @@ def batch_workload(args):
     log(f"plain fp64 one at a time: {len(pseq) / w_pseq:.2f} solves/s; ref batched on the same feet: "
         f"{len(rsub) / w_rsub:.2f}")
     bad = [dict(foot=i["foot"], action=i["action"], mode=i["mode"], flag=i["flag"], true=i["true_relres"])
-           for i in plain + ref + mixed + pseq + rsub if i["flag"] or i["true_relres"] > args.tol * 1.0001]
+           for i in plain + ref + mixed + pseq + rsub + lock if i["flag"] or i["true_relres"] > args.tol * 1.0001]
     for b in bad:
         log("missed", b)
     if any(b["flag"] == 0 for b in bad) or any(i["flag"] for i in ref):
@@ -339,6 +373,11 @@ def batch_workload(args):
                    "concurrency": conc,
                    "mode": "ref", **{k: v for k, v in summarize(ref, w_ref).items() if k not in ("value", "unit")}},
         "mixed": summarize(mixed, w_mixed, {"ratio_over_plain": round(w_plain / w_mixed, 3)}),
+        "lockstep": summarize(lock, w_lock, {
+            "what": f"mode ref in lock-step: {LOCK_WIDTH} systems at a time, every launch of a PCG iteration covering "
+                    f"all of them (cm_batch_*), systems joining and leaving as they finish",
+            "lockstep_solves": int(sum(bool(i.get("lockstep")) for i in lock)),
+            "ref_max_vs_fp64": max(i["u_rel_l2_vs_fp64"] for i in lock)}),
         "plain_fp64": summarize(plain, w_plain, {
             "what": "the reference's plain method in FP64 on the GPU (its level rule: 6,000 DOFs or 6 levels, every "
                     "axis coarsened; per-node 3x3 block Jacobi; Chebyshev degree 3), batched like the headline"}),
@@ -379,6 +418,7 @@ def batch_workload(args):
 
 LEARNED_FEET = 2      # held-out feet of the learned warm start's training split, 8 actions each
 SEQ_FEET = 8          # feet of the one-at-a-time plain leg
+LOCK_WIDTH = 16       # systems in one lock-step batch
 
 
 def learned_leg(descs, dev, tol):

@@ -42,6 +42,7 @@ SIGNATURES = {
     "cm_sys_power": [c_void_p, c_int, c_int, c_void_p, P_double],
     "cm_sys_smooth": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int],
     "cm_sys_solve": [c_void_p, c_int, c_void_p, c_void_p, c_double, c_int, c_int, c_double, ctypes.POINTER(SolveInfo)],
+    "cm_sys_prepare_solve": [c_void_p],
     "cm_batch_create": [ctypes.POINTER(c_void_p), c_void_p, c_int],
     "cm_batch_destroy": [c_void_p],
     "cm_batch_add": [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, P_int],

@@ -141,6 +141,7 @@ def solve_lockstep(items, width=16, builders=4, tol=1e-8, maxiter=3000, device=N
         torch.cuda.set_device(dev)
         with torch.cuda.stream(tl.stream):
             out = fn()
+            out[0].prepare_solve()
             tl.stream.synchronize()
         return out
 

@@ -53,6 +53,26 @@ struct FinalizeL {
   int dst;
   __device__ __forceinline__ void run(int, int) const { finalize_body(nb, partial, S, dst); }
 };
+// S[slot] = v (one thread)
+struct SlotSetL {
+  double* S;
+  int slot;
+  double v;
+  __device__ __forceinline__ void run(int, int) const {
+    if (threadIdx.x == 0) S[slot] = v;
+  }
+};
+// out = in, converted
+template <typename TO, typename TI>
+struct CastL {
+  long long n;
+  TO* out;
+  const TI* in;
+  __device__ __forceinline__ void run(int bid, int) const {
+    const long long i = (long long)bid * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = TO(in[i]);
+  }
+};
 // S[dst] = S[src] (one thread)
 struct SlotCopyL {
   double* S;
@@ -319,7 +339,10 @@ __global__ __launch_bounds__(256) void k_multi(const L* __restrict__ a, const in
     if (start[mid] <= b) lo = mid;
     else hi = mid - 1;
   }
-  a[lo].run(b - start[lo], start[lo + 1] - start[lo]);
+  // s is the same on every lane: say so, so that the record's fields are read with scalar loads
+  // into scalar registers (not once per lane into vector registers)
+  const int s = __builtin_amdgcn_readfirstlane(lo);
+  a[s].run(b - start[s], start[s + 1] - start[s]);
 }
 
 }  // namespace cm

@@ -1761,6 +1761,19 @@ struct System {
     rigid_correct(r, z);
   }
 
+  // everything a mode-1 solve needs before its first launch: work vectors, FP32 V-cycle data,
+  // FP64 operator data, the partials of the fused dot, the march's block order (the last one reads
+  // flags back to the host, so it runs where a wait on the stream is harmless)
+  void prepare_solve() {
+    alloc_pcg64();
+    alloc_pcg32();
+    ensure_prec<float>();
+    ensure_data<double>();
+    if (partial.n < dot_partials(0)) partial.alloc(std::max((size_t)RED_BLOCKS, dot_partials(0)));
+    refresh_all_flags<double>();
+    refresh_all_flags<float>();
+  }
+
   void alloc_pcg64() {
     const size_t n = 3 * (size_t)lv(0).nn;
     if (r64.n != n) { r64.alloc(n); z64.alloc(n); p64.alloc(n); Ap64.alloc(n); }
@@ -2073,7 +2086,7 @@ struct cm_batch {
     hipStream_t user = nullptr;       // the system's stream, restored when it leaves
     const double* b = nullptr;
     double* x = nullptr;
-    double thr = 0, bn = 0, rn = 0, target = 0, last_true = 1e300;
+    double tol = 0, thr = 0, bn = 0, rn = 0, target = 0, last_true = 1e300;
     int it = 0, maxiter = 0, restarts = 0, stalls = 0;
     bool flexible = false, live = false, done = false;
     std::vector<cm::LaunchRec> seq;   // one iteration's launches
@@ -2101,19 +2114,16 @@ struct cm_batch {
 
   static long long n3(System& s) { return 3 * s.lv(0).nn; }
 
+  // One iteration, preconditioner first: z = V(r) in FP32, r.z (and r.z_old, the flexible form),
+  // p = z + beta p, rz_old = rz; then Ap = A p with p.Ap, x += alpha p, r -= alpha Ap, r32 = r and
+  // ||r||^2.  A system (re)starts with p = 0 and rz_old = +inf (beta = 0: p = z), so joining or
+  // restarting needs no V-cycle of its own and no host wait.
   void record(Slot& sl) {
     System& s = sl.h->s;
     const long long n = n3(s);
     sl.seq.clear();
     s.rec = &sl.seq;
     try {
-      // head: Ap = A p with p.Ap; x += alpha p, r -= alpha Ap, r32 = r; ||r||^2
-      s.apply_dot(0, s.p64.p, s.Ap64.p, System::S_PAP);
-      s.go(cm::PcgXrL<double, double, float>{n, sl.x, s.r64.p, s.p64.p, s.Ap64.p, s.S.p, System::S_RZ0,
-                                             System::S_PAP, s.partial.p, s.r32.p},
-           cm::RED_BLOCKS);
-      s.finalize_to<double>(System::S_RR);
-      // tail: r.z_old (flexible form), z = V(r) in FP32, r.z, p = z + beta p, rz_old = rz
       s.dot(n, s.r64.p, s.z32.p, System::S_RZP);
       s.vcycle<float>(0, s.r32.p, s.z32.p);
       s.dot(n, s.r64.p, s.z32.p, System::S_RZ1);
@@ -2121,6 +2131,11 @@ struct cm_batch {
                                     sl.flexible ? (int)System::S_RZP : -1},
            cm::div_up(n, 256));
       s.go(cm::SlotCopyL{s.S.p, System::S_RZ0, System::S_RZ1}, 1);
+      s.apply_dot(0, s.p64.p, s.Ap64.p, System::S_PAP);
+      s.go(cm::PcgXrL<double, double, float>{n, sl.x, s.r64.p, s.p64.p, s.Ap64.p, s.S.p, System::S_RZ0,
+                                             System::S_PAP, s.partial.p, s.r32.p},
+           cm::RED_BLOCKS);
+      s.finalize_to<double>(System::S_RR);
     } catch (...) {
       s.rec = nullptr;
       throw;
@@ -2174,7 +2189,7 @@ struct cm_batch {
     std::vector<double*> sp(K);
     for (int k = 0; k < K; ++k) sp[k] = slots[live[k]].h->s.S.p;
     if (d_sptr.n < (size_t)K) d_sptr.alloc(std::max(K, 64));
-    if (d_rr.n < (size_t)K) d_rr.alloc(std::max(K, 64));
+    if (d_rr.n < (size_t)(2 * K)) d_rr.alloc(std::max(2 * K, 128));
     CM_HIP(hipMemcpyAsync(d_sptr.p, sp.data(), K * sizeof(double*), hipMemcpyHostToDevice, st));
     CM_HIP(hipStreamSynchronize(st));   // the host copies above must outlive the transfers
   }
@@ -2207,6 +2222,15 @@ struct cm_batch {
     sl.done = true;
     sl.seq.clear();
     plan_ok = false;
+  }
+
+  // r32 = r64, p = 0, rz_old = +inf: the next iteration starts the recurrence from r64
+  void restart_from_r(Slot& sl) {
+    System& s = sl.h->s;
+    const long long n = n3(s);
+    s.go(cm::CastL<float, double>{n, s.r32.p, s.r64.p}, cm::div_up(n, 256));
+    s.go(cm::FillL<double>{s.p64.p, n, 0.0}, cm::div_up(n, 256));
+    s.go(cm::SlotSetL{s.S.p, System::S_RZ0, INFINITY}, 1);
   }
 
   // true residual of the slot's x (FP64), into rn
@@ -2242,27 +2266,18 @@ struct cm_batch {
     CM_HIP(hipEventDestroy(ev));
     s.stream = st;
     try {
-      s.alloc_pcg64();
-      s.alloc_pcg32();
-      s.ensure_prec<float>();
-      s.ensure_data<double>();
-      if (s.partial.n < s.dot_partials(0)) s.partial.alloc(std::max((size_t)cm::RED_BLOCKS, s.dot_partials(0)));
-      s.refresh_all_flags<double>();
-      s.refresh_all_flags<float>();
+      s.prepare_solve();   // once per system (cm_sys_prepare_solve does it on the builder's stream)
       const long long n = n3(s);
-      // r = b - A x, z = V(r), p = z (pcg64's start)
+      // r = b - A x; ||b|| is read at the batch's next check (tol and thr then)
       s.apply<double>(0, x, s.r64.p, cm::MODE_RESID, b);
       s.dot(n, b, b, System::S_BB);
-      sl.bn = std::sqrt(s.read_slot(System::S_BB));
-      sl.thr = tol * (sl.bn > 0 ? sl.bn : 1.0);
-      sl.target = sl.thr;
-      s.precondition64(1, s.r64.p, s.z64.p);
-      CM_HIP(hipMemcpyAsync(s.p64.p, s.z64.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-      s.dot(n, s.r64.p, s.z64.p, System::S_RZ0);
-      s.dot(n, s.r64.p, s.r64.p, System::S_RR);
-      sl.rn = std::sqrt(s.read_slot(System::S_RR));
-      if (sl.rn <= sl.thr || maxiter <= 0) {
-        finish(sl, sl.rn <= sl.thr ? 0 : maxiter);
+      sl.tol = tol;
+      sl.thr = -1.0;
+      restart_from_r(sl);
+      if (maxiter <= 0) {
+        sl.bn = std::sqrt(s.read_slot(System::S_BB));
+        sl.rn = true_res(sl);
+        finish(sl, maxiter);
         return k;
       }
       record(sl);
@@ -2289,8 +2304,12 @@ struct cm_batch {
   }
 
   // host decisions of pcg64 at a check, for one live slot
-  void check(Slot& sl, double rr) {
-    System& s = sl.h->s;
+  void check(Slot& sl, double rr, double bb) {
+    if (sl.thr < 0) {   // the first check of this solve: ||b|| and the threshold
+      sl.bn = std::sqrt(bb);
+      sl.thr = sl.tol * (sl.bn > 0 ? sl.bn : 1.0);
+      sl.target = sl.thr;
+    }
     sl.rn = std::sqrt(rr);
     if (sl.rn <= sl.target) {
       const double rec = sl.rn;
@@ -2307,10 +2326,7 @@ struct cm_batch {
         return;
       }
       ++sl.restarts;
-      const long long n = n3(s);
-      s.precondition64(1, s.r64.p, s.z64.p);
-      CM_HIP(hipMemcpyAsync(s.p64.p, s.z64.p, n * sizeof(double), hipMemcpyDeviceToDevice, st));
-      s.dot(n, s.r64.p, s.z64.p, System::S_RZ0);
+      restart_from_r(sl);   // r64 holds the true residual
       if (!sl.flexible) {
         sl.flexible = true;
         record(sl);
@@ -2336,14 +2352,15 @@ struct cm_batch {
       if (iter % check_every) continue;
       const int K = (int)live.size();
       hipLaunchKernelGGL(cm::k_gather_slot, dim3(1), dim3(256), 0, st, K, d_sptr.p, (int)System::S_RR, d_rr.p);
-      h_rr.resize(K);
-      CM_HIP(hipMemcpyAsync(h_rr.data(), d_rr.p, K * sizeof(double), hipMemcpyDeviceToHost, st));
+      hipLaunchKernelGGL(cm::k_gather_slot, dim3(1), dim3(256), 0, st, K, d_sptr.p, (int)System::S_BB, d_rr.p + K);
+      h_rr.resize(2 * K);
+      CM_HIP(hipMemcpyAsync(h_rr.data(), d_rr.p, 2 * K * sizeof(double), hipMemcpyDeviceToHost, st));
       CM_HIP(hipStreamSynchronize(st));
       const std::vector<int> now = live;
       bool any_done = false;
       for (int q = 0; q < K; ++q) {
         Slot& sl = slots[now[q]];
-        check(sl, h_rr[q]);
+        check(sl, h_rr[q], h_rr[K + q]);
         any_done |= !sl.live;
       }
       if (any_done) break;
@@ -3057,6 +3074,14 @@ int cm_sample_nodes_noalloc(void* stream, int mx, int my, int mz, const double* 
   CM_API_END
 }
 
+
+int cm_sys_prepare_solve(cm_sys* h) {
+  CM_API_BEGIN
+  need_built(h);
+  h->s.prepare_solve();
+  CM_HIP(hipStreamSynchronize(h->s.stream));
+  CM_API_END
+}
 
 int cm_batch_create(cm_batch** out, void* stream, int check_every) {
   CM_API_BEGIN

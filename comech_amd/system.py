@@ -201,6 +201,12 @@ class VoxelSystem:
                    t_solve_ms=info.t_solve_ms, t_setup_ms=info.t_setup_ms, mode=mode, device=str(self.device))
         return x, out
 
+    def prepare_solve(self):
+        """Everything a mode-"ref" solve needs before its first launch (work vectors, FP32 V-cycle
+        data, FP64 operator data, block orders), done now on this system's stream; solves and
+        lock-step batches then start without a wait on the host."""
+        check(self.lib.cm_sys_prepare_solve(self.h))
+
     def bounds(self, level, dtype=torch.float64):
         """Chebyshev bounds (lmax, lmin) of one level (fe/gmg.py:57-58)."""
         out = np.zeros(2)

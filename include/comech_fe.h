@@ -176,6 +176,11 @@ int cm_sys_solve(cm_sys* sys, int mode, const double* b, double* x, double tol, 
  * springs on a coarse level) are not added: *slot = -1, solve them with cm_sys_solve.  The
  * residual is checked every check_every iterations of the batch. */
 typedef struct cm_batch cm_batch;
+/* Everything a mode-1 solve needs before its first launch (work vectors, FP32 V-cycle data, FP64
+ * operator data, the march's block orders), done now on the system's stream; a system joins a
+ * batch without a host wait after this (no reference counterpart: the reference builds its
+ * hierarchy inside each solve, rb/gpusolve.py:229-236). */
+int cm_sys_prepare_solve(cm_sys* sys);
 int cm_batch_create(cm_batch** out, void* stream, int check_every);
 int cm_batch_destroy(cm_batch* batch);
 /* Starts PCG on (sys, b, x); x holds the initial guess.  *slot: the slot id, or -1 (not added). */

@@ -342,7 +342,13 @@ __global__ __launch_bounds__(256) void k_multi(const L* __restrict__ a, const in
   // s is the same on every lane: say so, so that the record's fields are read with scalar loads
   // into scalar registers (not once per lane into vector registers)
   const int s = __builtin_amdgcn_readfirstlane(lo);
-  a[s].run(b - start[s], start[s + 1] - start[s]);
+  // the record read through the constant address space: its pointers then count as global
+  // (loads and stores through them compile to global, not flat, instructions)
+#if defined(__HIP_DEVICE_COMPILE__)
+  typedef const __attribute__((address_space(4))) L CL;
+  L la = *reinterpret_cast<CL*>(reinterpret_cast<uintptr_t>(a + s));
+  la.run(b - start[s], start[s + 1] - start[s]);
+#endif
 }
 
 }  // namespace cm

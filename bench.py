@@ -80,8 +80,10 @@ def timed_feet(k):
 
 
 # Multigrid of the method as built here: the semi-coarsened coarsest level and the split-copy
-# smoother blocks (library defaults), coarsening down to 3,000 DOFs (7 levels at most).
-TUNED = dict(min_dofs=3000, max_levels=7)
+# smoother blocks (library defaults), coarsening down to 3,000 DOFs (7 levels at most), and the
+# rigid modes of the pieces of a split bone (osteotomy, interposition) as an additive coarse
+# correction.
+TUNED = dict(min_dofs=3000, max_levels=7, fragment_modes=True)
 # The reference's plain method: its level rule (6,000 DOFs, 6 levels, every axis coarsened,
 # fe/gmg.py:80-96) and per-node 3x3 block Jacobi (fe/gmg.py:29-44).
 PLAIN = dict(last_axes=7, copy_blocks=False)
@@ -387,8 +389,8 @@ def batch_workload(args):
             "ref_batched_same_feet": round(len(rsub) / w_rsub, 3),
             "ratio_ref_batched_over_plain_sequential": round((len(rsub) / w_rsub) / (len(pseq) / w_pseq), 3)}),
         "ratio_ref_over_plain": round(w_plain / w_ref, 3),
-        "multigrid": {"ref_and_mixed": dict(TUNED, last_axes=3, copy_blocks=True), "plain_fp64": dict(PLAIN, min_dofs=6000,
-                                                                                                   max_levels=6)},
+        "multigrid": {"ref_and_mixed": dict(TUNED, last_axes=3, copy_blocks=True),
+                      "plain_fp64": dict(PLAIN, min_dofs=6000, max_levels=6)},
         "learned": ll,
         "sequential": {"value": round(len(seq) / w_seq, 3), "unit": "solves/s", "concurrency": 1,
                        "solves": len(seq), "what": f"foot {feet[0]}'s 8 actions one at a time, mode ref"},

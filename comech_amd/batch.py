@@ -123,8 +123,9 @@ def solve_lockstep(items, width=16, builders=4, tol=1e-8, maxiter=3000, device=N
     items: callables, each building one system on the calling thread's stream and returning
     (system, [(key, b), ...]) -- its right-hand sides, solved one after another from zero.  Systems
     are built on ``builders`` host threads (one stream each) while the batch runs.  Returns
-    (results {key: (x, info)}, wall seconds).  A system whose launch structure differs from the
-    batch's is solved on its own (``system.solve(mode="ref")``) and its info carries
+    (results {key: (x, info)}, wall seconds).  A system with a rigid-mode correction is solved by
+    its builder thread on the builder's stream (``system.solve(mode="ref")``), and one whose launch
+    structure differs from the batch's on its own after the batch's systems; their infos carry
     ``lockstep=False``."""
     from concurrent.futures import ThreadPoolExecutor
     dev = require_gpu(device)
@@ -140,10 +141,19 @@ def solve_lockstep(items, width=16, builders=4, tol=1e-8, maxiter=3000, device=N
                 tl.stream = streams[next(counter) % builders]
         torch.cuda.set_device(dev)
         with torch.cuda.stream(tl.stream):
-            out = fn()
-            out[0].prepare_solve()
+            sysm, rhs_list = fn()
+            if getattr(sysm, "rigid_cols", 0):
+                done = []
+                for key, b in rhs_list:
+                    x, info = sysm.solve(b, tol=tol, maxiter=maxiter, mode="ref", check_every=check_every)
+                    info["lockstep"] = False
+                    done.append((key, x, info))
+                tl.stream.synchronize()
+                sysm.close()
+                return None, done
+            sysm.prepare_solve()
             tl.stream.synchronize()
-        return out
+        return sysm, rhs_list
 
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
@@ -161,7 +171,11 @@ def solve_lockstep(items, width=16, builders=4, tol=1e-8, maxiter=3000, device=N
             for f in pending:
                 if f.done():
                     sysm, rhs_list = f.result()
-                    waiting.append([sysm, list(rhs_list), 0])
+                    if sysm is None:      # solved by its builder (rigid-mode correction)
+                        for key, x, info in rhs_list:
+                            results[key] = (x, info)
+                    else:
+                        waiting.append([sysm, list(rhs_list), 0])
                     if nxt < len(items):
                         still.append(ex.submit(build, items[nxt]))
                         nxt += 1
@@ -210,17 +224,34 @@ def solve_lockstep(items, width=16, builders=4, tol=1e-8, maxiter=3000, device=N
     return results, time.perf_counter() - t0
 
 
-def build_system(desc, device=None, **mg):
+def build_system(desc, device=None, fragment_modes=False, **mg):
     """VoxelSystem of a level-0 description (``synth.level0_description``): labels + tables, split
-    copies and irregular cells, ground springs (COO) and spring terms (rank one)."""
+    copies and irregular cells, ground springs (COO) and spring terms (rank one).  fragment_modes:
+    the rigid modes of every piece of a bone split into pieces (engine.fragment_groups) as the
+    additive coarse correction of the preconditioner (none when no bone is split)."""
     import numpy as np
+    from . import engine as E
     from .grid import Grid
     from .system import VoxelSystem
     g = Grid(*desc["edges"])
-    return VoxelSystem(g, sri=True, active=np.asarray(desc["labels"]) != 255, labels=desc["labels"],
-                       lam_tab=desc["lam_tab"], mu_tab=desc["mu_tab"], copy_orig=desc["copy_orig"],
-                       irr_cells=desc["irr_cells"], irr_nodes=desc["irr_nodes"], extra=desc["extra"],
-                       rank1=desc["terms"], device=device, **mg)
+    active = np.asarray(desc["labels"]) != 255
+    s = VoxelSystem(g, sri=True, active=active, labels=desc["labels"],
+                    lam_tab=desc["lam_tab"], mu_tab=desc["mu_tab"], copy_orig=desc["copy_orig"],
+                    irr_cells=desc["irr_cells"], irr_nodes=desc["irr_nodes"], extra=desc["extra"],
+                    rank1=desc["terms"], device=device, **mg)
+    if fragment_modes and "part" in desc:
+        try:
+            groups = E.fragment_groups(g, desc["part"], active, desc["irr_cells"], desc["irr_nodes"])
+            if groups:
+                xyz = g.node_xyz
+                xyz = np.concatenate([xyz, xyz[np.asarray(desc["copy_orig"], np.int64)]])
+                Z, ncol = E.rigid_coo(xyz, s.used_mask(0), groups)
+                if ncol:
+                    s.set_rigid(*Z, ncol)
+        except Exception:
+            s.close()
+            raise
+    return s
 
 
 def rhs(desc, device=None):

@@ -2244,7 +2244,7 @@ struct cm_batch {
   int add(cm_sys* h, const double* b, double* x, double tol, int maxiter) {
     System& s = h->s;
     CM_CHECK(s.built, "system not built (call cm_sys_build)");
-    CM_CHECK(s.zcol == 0, "batched solves do not take a rigid-mode correction");
+    if (s.zcol != 0) return -1;   // a rigid-mode correction: solved on its own
     for (auto& o : slots)
       CM_CHECK(!(o.live && o.h == h), "a system can be in the batch once at a time");
     int k = 0;   // a free slot: not live and its result taken

@@ -94,6 +94,39 @@ def rigid_coo(xyz, used, node_groups):
     return (np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)), col
 
 
+def fragment_groups(grid, part, active, irr_cells=(), irr_nodes=None, min_cells=8):
+    """Node groups of the fragments of split bones: for every bone part (part > 0, reference cell
+    order) whose active cells form two or more face-connected pieces (an osteotomy cut, a resected
+    and filled gap), the nodes of each piece of at least ``min_cells`` cells -- the split copies of
+    irregular cells included (irr_nodes, 8 per listed cell).  Rigid modes of these groups
+    (rigid_coo) as the additive coarse correction of the preconditioner let the fragments move
+    apart on every level, which the grid hierarchy cannot represent (no reference counterpart: the
+    reference's engine takes one group per bone, rb/rbfe.py:180)."""
+    import scipy.ndimage as ndi
+    from .grid import corners_of
+    nx, ny, nz = grid.nx, grid.ny, grid.nz
+    part = np.asarray(part).reshape(nx, ny, nz)
+    act = np.asarray(active, bool).reshape(nx, ny, nz)
+    irr = {int(c): i for i, c in enumerate(np.asarray(irr_cells, np.int64))}
+    out = []
+    for p in np.unique(part[act & (part > 0)]):
+        comp, n = ndi.label(act & (part == p))
+        if n < 2:
+            continue
+        flat = comp.ravel()
+        for q in range(1, n + 1):
+            cells = np.flatnonzero(flat == q)
+            if len(cells) < min_cells:
+                continue
+            nodes = corners_of(grid, cells)
+            for t, c in enumerate(cells):
+                e = irr.get(int(c))
+                if e is not None:
+                    nodes[t] = irr_nodes[e]
+            out.append(np.unique(nodes.ravel()))
+    return out
+
+
 # coarsest-level method (the "coarse" key of rb/gpusolve.py:259-260): a dense FP64 inverse formed on
 # the GPU by blocked Gauss-Jordan, applied as a matrix-vector product
 COARSE = "dense-inverse"

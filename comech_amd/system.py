@@ -330,10 +330,12 @@ class VoxelSystem:
         check(self.lib.cm_sys_set_rigid(self.h, int(ncol), len(rows), ptr(rows, ctypes.c_longlong),
                                         ptr(cols, ctypes.c_longlong), ptr(vals, ctypes.c_double),
                                         ptr(Einv, ctypes.c_double)))
+        self.rigid_cols = ncol
 
     def clear_rigid(self):
         """Remove the additive coarse correction set by set_rigid."""
         check(self.lib.cm_sys_set_rigid(self.h, 0, 0, None, None, None, None))
+        self.rigid_cols = 0
 
     def set_fallback(self, iters=5, drop=10.0):
         """Early exit of a solve from a predicted x0: unless the FP64 true residual has dropped by

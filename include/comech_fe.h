@@ -173,7 +173,8 @@ int cm_sys_solve(cm_sys* sys, int mode, const double* b, double* x, double tol, 
  * them.  A system joins with cm_batch_add (after its build, on any stream; its stream waits for
  * the batch while it is in) and leaves when converged, stopped or at maxiter; cm_batch_poll hands
  * back its cm_solve_info.  Systems whose launch structure differs from the batch's (level count,
- * springs on a coarse level) are not added: *slot = -1, solve them with cm_sys_solve.  The
+ * springs on a coarse level) or that carry a rigid-mode correction (cm_sys_set_rigid) are not
+ * added: *slot = -1, solve them with cm_sys_solve.  The
  * residual is checked every check_every iterations of the batch. */
 typedef struct cm_batch cm_batch;
 /* Everything a mode-1 solve needs before its first launch (work vectors, FP32 V-cycle data, FP64

@@ -162,6 +162,36 @@ def test_config3_lockstep_batch_against_oracle(config3_answers):
         assert d["nn"] == len(ans["U"]), key
 
 
+@pytest.mark.parametrize("action", ["osteotomy", "interposition"])
+def test_fragment_rigid_modes_keep_the_answer(config3_answers, action):
+    """The rigid modes of the pieces of a cut bone (engine.fragment_groups: the osteotomy's slab and
+    the interposition's filled gap split one bone into two face-connected pieces) as the additive
+    coarse correction: 12 columns (2 pieces x 6 modes), the same answer as the oracle, fewer PCG
+    iterations than without them; a baseline foot has no cut bone and gets none."""
+    from comech_amd.batch import build_system, rhs
+    specs, answers = config3_answers
+    for f in CONFIG3_FEET:
+        descs = synth.batch_foot(f, scale=4, actions=[action, "baseline"])
+        d = descs[0]
+        ans = answers[[i for i, (_, kw, _) in enumerate(specs) if kw["foot"] == f and kw["action"] == action][0]]
+        b = rhs(d, DEV)
+        its = {}
+        for modes in (False, True):
+            s = build_system(d, device=DEV, min_dofs=3000, max_levels=7, fragment_modes=modes)
+            assert getattr(s, "rigid_cols", 0) == (12 if modes else 0), (f, modes)
+            x, info = s.solve(b, tol=1e-8, maxiter=3000, mode="ref")
+            s.close()
+            U = x.view(-1, 3).cpu().numpy().copy()
+            U[np.isnan(ans["U"]).any(1)] = np.nan
+            assert info["flag"] == 0 and rel(U, ans["U"]) <= U_TOL, (f, modes)
+            its[modes] = info["iters"]
+        print(f"foot {f} {action}: iterations {its[False]} -> {its[True]}", flush=True)
+        assert its[True] <= its[False], (f, its)
+        s = build_system(descs[1], device=DEV, min_dofs=3000, max_levels=7, fragment_modes=True)
+        assert getattr(s, "rigid_cols", 0) == 0
+        s.close()
+
+
 def test_config3_uneven_batch_against_oracle(config3_answers):
     """The first three feet of the 64-foot set (uneven grids, each its own seed) x 8 actions at a
     quarter of their grid dimensions: 24 systems solved batched, 4 in flight on one GPU, rtol

@@ -80,10 +80,8 @@ def timed_feet(k):
 
 
 # Multigrid of the method as built here: the semi-coarsened coarsest level and the split-copy
-# smoother blocks (library defaults), coarsening down to 3,000 DOFs (7 levels at most), and the
-# rigid modes of the pieces of a split bone (osteotomy, interposition) as an additive coarse
-# correction.
-TUNED = dict(min_dofs=3000, max_levels=7, fragment_modes=True)
+# smoother blocks (library defaults), coarsening down to 3,000 DOFs (7 levels at most).
+TUNED = dict(min_dofs=3000, max_levels=7)
 # The reference's plain method: its level rule (6,000 DOFs, 6 levels, every axis coarsened,
 # fe/gmg.py:80-96) and per-node 3x3 block Jacobi (fe/gmg.py:29-44).
 PLAIN = dict(last_axes=7, copy_blocks=False)

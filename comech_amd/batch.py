@@ -224,6 +224,31 @@ def solve_lockstep(items, width=16, builders=4, tol=1e-8, maxiter=3000, device=N
     return results, time.perf_counter() - t0
 
 
+def fragment_z(desc):
+    """Rigid modes of the pieces of the description's cut bones (engine.fragment_groups) as a COO
+    in (row, column) order, or None: host only (numpy/scipy), so description workers can compute
+    it beside the GPU's work; build_system(fragment_modes=True) takes desc["fragment_z"]."""
+    import numpy as np
+    from . import engine as E
+    from .grid import Grid
+    g = Grid(*desc["edges"])
+    labels = np.asarray(desc["labels"])
+    active = labels != 255
+    groups = E.fragment_groups(g, desc["part"], active, desc["irr_cells"], desc["irr_nodes"])
+    if not groups:
+        return None
+    copy_orig = np.asarray(desc["copy_orig"], np.int64)
+    xyz = g.node_xyz
+    xyz = np.concatenate([xyz, xyz[copy_orig]])
+    used = E.used_nodes(g, active, g.nnode + len(copy_orig), desc["irr_cells"], desc["irr_nodes"])
+    Z, ncol = E.rigid_coo(xyz, used, groups)
+    if not ncol:
+        return None
+    rows, cols, vals = Z
+    o = np.lexsort((cols, rows))
+    return rows[o], cols[o], vals[o], ncol
+
+
 def build_system(desc, device=None, fragment_modes=False, **mg):
     """VoxelSystem of a level-0 description (``synth.level0_description``): labels + tables, split
     copies and irregular cells, ground springs (COO) and spring terms (rank one).  fragment_modes:
@@ -241,13 +266,9 @@ def build_system(desc, device=None, fragment_modes=False, **mg):
                     rank1=desc["terms"], device=device, **mg)
     if fragment_modes and "part" in desc:
         try:
-            groups = E.fragment_groups(g, desc["part"], active, desc["irr_cells"], desc["irr_nodes"])
-            if groups:
-                xyz = g.node_xyz
-                xyz = np.concatenate([xyz, xyz[np.asarray(desc["copy_orig"], np.int64)]])
-                Z, ncol = E.rigid_coo(xyz, s.used_mask(0), groups)
-                if ncol:
-                    s.set_rigid(*Z, ncol)
+            fz = desc["fragment_z"] if "fragment_z" in desc else fragment_z(desc)
+            if fz is not None:
+                s.set_rigid(*fz, ordered=True)
         except Exception:
             s.close()
             raise

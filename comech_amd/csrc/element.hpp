@@ -201,6 +201,77 @@ CM_INLINE void elem_mid(const T* __restrict__ cx, const T* __restrict__ cy, cons
   }
 }
 
+// The cubic-cell, SRI operator of one z step in face form: lo, hi = xy transforms of the cell's
+// bottom and top faces (f[4 comp + q], q = mx + 2 my), car = top-face forces of the cell below;
+// out: flo = car + the cell's bottom-face forces, fhi = its top-face forces.  The same operator as
+// faces_to_cell + elem_mid<T, true, true> + cell_to_faces + the carry add, with the z butterflies
+// folded into the element terms.  With A = hi + lo, D = hi - lo (the z modes) and m0 = wm,
+// m1 = wm / 3, the transform-space forces of elem_mid are
+//   x: A1 = 2 m0 A1 + g7, D1 = m0 D1, A2 = g1, D2 = g2 + g6, A3 = m0 A3, D3 = 4/9 m0 D3, D0 = g5
+//   y: A1 = g1, D1 = g2 + g4, A2 = 2 m0 A2 + g7, D2 = m0 D2, A3 = m0 A3, D3 = 4/9 m0 D3, D0 = g3
+//   z: A1 = g5, A2 = g3, A3 = g4 + g6, D0 = 2 m0 D0 + g7, D1 = m0 D1, D2 = m0 D2, D3 = 4/9 m0 D3
+// (the shear terms of a mode add to its normal term: 2 m1 + m1 = m0, 2 m2 + m2 + m2 = 4/9 m0),
+// the A0 forces are zero, and the faces take lo = FA - FD, hi = FA + FD.
+template <typename T>
+CM_INLINE void cell_faces_iso_sri(const T* __restrict__ lo, const T* __restrict__ hi, const T* __restrict__ car,
+                                  T* __restrict__ flo, T* __restrict__ fhi, T wl, T wm) {
+  const T m0 = wm, m1 = wm * (T(1) / T(3)), k2 = T(2) * wm, k49 = wm * (T(4) / T(9));
+  const T kp = m0 + k49, km = m0 - k49;
+  // z modes the coupled terms need
+  const T xD0 = hi[0] - lo[0], xA1 = hi[1] + lo[1], xD1 = hi[1] - lo[1], xA2 = hi[2] + lo[2], xD2 = hi[2] - lo[2];
+  const T yD0 = hi[4] - lo[4], yA1 = hi[5] + lo[5], yD1 = hi[5] - lo[5], yA2 = hi[6] + lo[6], yD2 = hi[6] - lo[6];
+  const T zD0 = hi[8] - lo[8], zA1 = hi[9] + lo[9], zD1 = hi[9] - lo[9], zA2 = hi[10] + lo[10], zD2 = hi[10] - lo[10];
+  const T zA3 = hi[11] + lo[11], zD3 = hi[11] - lo[11];
+  const T g1 = m0 * (xA2 + yA1), g2 = m1 * (xD2 + yD1), g3 = m0 * (yD0 + zA2);
+  const T g4 = m1 * (yD1 + zA3), g5 = m0 * (xD0 + zA1), g6 = m1 * (xD2 + zA3);
+  const T g7 = wl * (xA1 + yA2 + zD0);
+  // x component
+  flo[0] = car[0] - g5;
+  fhi[0] = g5;
+  {
+    const T FA = k2 * xA1 + g7;
+    flo[1] = (car[1] + FA) - m0 * xD1;
+    fhi[1] = FA + m0 * xD1;
+  }
+  {
+    const T t = g2 + g6;
+    flo[2] = (car[2] + g1) - t;
+    fhi[2] = g1 + t;
+  }
+  flo[3] = car[3] + km * hi[3] + kp * lo[3];
+  fhi[3] = kp * hi[3] + km * lo[3];
+  // y component
+  flo[4] = car[4] - g3;
+  fhi[4] = g3;
+  {
+    const T t = g2 + g4;
+    flo[5] = (car[5] + g1) - t;
+    fhi[5] = g1 + t;
+  }
+  {
+    const T FA = k2 * yA2 + g7;
+    flo[6] = (car[6] + FA) - m0 * yD2;
+    fhi[6] = FA + m0 * yD2;
+  }
+  flo[7] = car[7] + km * hi[7] + kp * lo[7];
+  fhi[7] = kp * hi[7] + km * lo[7];
+  // z component
+  {
+    const T FD = k2 * zD0 + g7;
+    flo[8] = car[8] - FD;
+    fhi[8] = FD;
+  }
+  flo[9] = (car[9] + g5) - m0 * zD1;
+  fhi[9] = g5 + m0 * zD1;
+  flo[10] = (car[10] + g3) - m0 * zD2;
+  fhi[10] = g3 + m0 * zD2;
+  {
+    const T t = g4 + g6;
+    flo[11] = (car[11] + t) - k49 * zD3;
+    fhi[11] = t + k49 * zD3;
+  }
+}
+
 // f = (lam KL + mu KM) u for one cell with constant material.  u, f: 24 values, DOF 3*l+comp.
 // SRI = true integrates the lam term at the cell centre only (rb/splitfe.py:58-69).
 template <typename T, bool SRI>

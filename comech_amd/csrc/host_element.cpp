@@ -73,18 +73,22 @@ void cm_host_march_column(int nz, const double* hxy, const double* hz, const dou
       face(k + 1, uhi);
       double c3[3][8], F3[3][8];
       faces_to_cell(ulo, uhi, c3);
-      if (iso) {
+      if (iso && sri) {   // the kernel's face form (carry included)
+        const double f = hxy[0] / 16.0;
+        double Fc[12];
+        cell_faces_iso_sri<double>(ulo, uhi, car, Fc, Fhi, lam[k] * f, mu[k] * f);
+        for (int q = 0; q < 12; ++q) Flo[q] = Fc[q] - car[q];
+      } else if (iso) {
         const double f = hxy[0] / 16.0;
         CellW<double> w{1.0, 1.0, 1.0, lam[k] * f, mu[k] * f};
-        if (sri) elem_mid<double, true, true>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
-        else elem_mid<double, false, true>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+        elem_mid<double, false, true>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
       } else {
         const double V = hxy[0] * hxy[1] * hz[k];
         CellW<double> w{gx, gy, 0.25 / hz[k], lam[k] * V, mu[k] * V};
         if (sri) elem_mid<double, true, false>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
         else elem_mid<double, false, false>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
       }
-      cell_to_faces(F3, Flo, Fhi);
+      if (!(iso && sri)) cell_to_faces(F3, Flo, Fhi);
     }
     for (int q = 0; q < 12; ++q) {
       Flo[q] += car[q];
@@ -96,6 +100,23 @@ void cm_host_march_column(int nz, const double* hxy, const double* hz, const dou
       for (int c = 0; c < 3; ++c) y[(4 * k + ab) * 3 + c] = cr[ab][c];
     for (int q = 0; q < 12; ++q) ulo[q] = uhi[q];
   }
+}
+
+// One z step of the cubic-cell SRI operator on face transforms (lo, hi, car: 12 values each):
+// fused != 0 runs the kernel's face form (cell_faces_iso_sri), 0 the composition faces_to_cell +
+// elem_mid + cell_to_faces + carry add it replaces.
+void cm_host_cell_faces(int fused, const double* lo, const double* hi, const double* car, double wl, double wm,
+                        double* flo, double* fhi) {
+  if (fused) {
+    cell_faces_iso_sri<double>(lo, hi, car, flo, fhi, wl, wm);
+    return;
+  }
+  double c3[3][8], F3[3][8];
+  faces_to_cell(lo, hi, c3);
+  CellW<double> w{1.0, 1.0, 1.0, wl, wm};
+  elem_mid<double, true, true>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
+  cell_to_faces(F3, flo, fhi);
+  for (int q = 0; q < 12; ++q) flo[q] += car[q];
 }
 
 }  // extern "C"

@@ -288,6 +288,19 @@ __device__ __forceinline__ void lds_put3(double* p, const double v[3]) {
   *reinterpret_cast<double2*>(p + 2) = make_double2(v[2], 0.0);
 }
 
+// Value of lane l - 1 of the same 16-lane DPP row (row_shr:1), 0 in a row's first lane.  The
+// march lays out one tile row of 16 threads per DPP row, so this is the value of the thread at
+// tx - 1 (lanes at tx = 0 are halo lanes).
+__device__ __forceinline__ float row_prev(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double row_prev(double v) {
+  const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffull), 0x111, 0xF, 0xF, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), 0x111, 0xF, 0xF, true);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 // LDS layout of n 3-vectors.  FP32: 16-byte slots as above.  FP64: one plane of n values per
 // component; 8-byte accesses of consecutive lanes are conflict-free, and the planes need 3/4 of the
 // LDS of 32-byte slots, which lets a fourth FP64 march block share a CU.
@@ -589,9 +602,10 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
   // staged node layers: buffer (kk - (K0 - 1)) & 1 holds layer kk, so that the two z steps of an
   // unrolled pair address fixed buffers
   __shared__ __align__(16) T s_u[2][VL::size(NL)];
-  // corners 1, 2, 3 of every thread's cell, one array per corner, 4 values (xyz, pad) per thread:
+  // per thread: the forces its cell and the cell at tx - 1 pass to the node at ty + 1 (xyz, pad):
   // consecutive threads write and read consecutive 16-byte slots (no LDS bank conflicts)
-  __shared__ __align__(16) T s_g[3][VL::size(BLOCK)];
+  static_assert(TILE == 16, "one tile row per 16-lane DPP row");
+  __shared__ __align__(16) T s_g[VL::size(BLOCK)];
   __shared__ T s_tL[M::kind == 0 ? 256 : 1], s_tM[M::kind == 0 ? 256 : 1];
   __shared__ T s_ph[M::kind == 3 ? 2 * TILE * 6 : 1];
   const int tid = threadIdx.x;
@@ -750,8 +764,8 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
     epi_load_rs<T, EM>(epi_rs<T, EM>(ep, K0, lay, K0 < kend), ep.x != nullptr, off3, qn[1 % ED]);
     if (ED == 2) epi_load_rs<T, EM>(epi_rs<T, EM>(ep, K0 + 1, lay, K0 + 1 < kend), ep.x != nullptr, off3, qn[0]);
   }
-  // LDS slots of the corners the three in-plane neighbour cells pass to this thread's node
-  const int i1 = tx >= 1 ? tid - 1 : tid, i2 = ty >= 1 ? tid - TILE : tid, i3 = (tx >= 1 && ty >= 1) ? tid - TILE - 1 : tid;
+  // LDS slot of the forces the row below passes to this thread's node
+  const int i2 = ty >= 1 ? tid - TILE : tid;
   T uA[12], uB[12], cA[12], cB[12];
   face(0, uA);
 #pragma unroll
@@ -792,7 +806,10 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
     T sn[3] = {T(0), T(0), T(0)};
     if (EM == EPI_CHEB || want_dot) VL::get(s_u[BLO], NL, ty * TN + tx, sn);
     T Flo[12];
-    if constexpr (M::kind <= 1) {
+    if constexpr (M::kind <= 1 && M::iso && M::sri) {
+      // level 0, cubic cells, SRI: the face form of the cell operator (carry included)
+      cell_faces_iso_sri<T>(ulo, uhi, car, Flo, carn, w.wl, w.wm);
+    } else if constexpr (M::kind <= 1) {
       // level 0: every lane computes its cell (zero weights where there is no stiffness)
       T c3[3][8], F3[3][8];
       faces_to_cell(ulo, uhi, c3);
@@ -858,27 +875,35 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
         if (done) cell_to_faces(F3, Flo, carn);
       }
     }
+    if constexpr (!(M::kind <= 1 && M::iso && M::sri)) {
 #pragma unroll
-    for (int q = 0; q < 12; ++q) Flo[q] += car[q];
-    // back to the 4 bottom corners (corner a + 2b)
-    T cr[4][3];
-    face_bwd(Flo, cr);
-    VL::put(s_g[0], BLOCK, tid, cr[1]);
-    VL::put(s_g[1], BLOCK, tid, cr[2]);
-    VL::put(s_g[2], BLOCK, tid, cr[3]);
+      for (int q = 0; q < 12; ++q) Flo[q] += car[q];
+    }
+    // back to the nodes in two butterflies (face_bwd split by axis): the x butterfly adds the
+    // corners of the cell at tx - 1, read from the previous lane of the DPP row; R = corners 0 and
+    // 1 that meet at this thread's node, S = corners 2 and 3 that meet at the node at ty + 1, passed
+    // through LDS
+    T R[3], S[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T t0 = Flo[4 * c] - Flo[4 * c + 1], t1 = Flo[4 * c] + Flo[4 * c + 1];
+      const T t2 = Flo[4 * c + 2] - Flo[4 * c + 3], t3 = Flo[4 * c + 2] + Flo[4 * c + 3];
+      const T P = t0 + row_prev(t1), Q = t2 + row_prev(t3);
+      R[c] = P - Q;
+      S[c] = P + Q;
+    }
+    VL::put(s_g, BLOCK, tid, S);
     __syncthreads();
     // Every step issues the same memory operations (those of the first step, which has no output
     // layer, and those of arrays the caller does not keep go to empty descriptors): with the same
     // count of memory operations on every path, the compiler's wait before each use of a prefetched
     // value covers only that value, instead of draining all loads in flight.
     {   // halo lanes compute and store nowhere (offset OOB)
-      T p1[3], p2[3], p3[3];
-      VL::get(s_g[0], BLOCK, i1, p1);   // cell (tx-1, ty): its corner 1
-      VL::get(s_g[1], BLOCK, i2, p2);   // cell (tx, ty-1): its corner 2
-      VL::get(s_g[2], BLOCK, i3, p3);   // cell (tx-1, ty-1): its corner 3
+      T p2[3];
+      VL::get(s_g, BLOCK, i2, p2);   // cells (tx-1 .. tx, ty-1): their corners 3 and 2
       T au[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) au[c] = cr[0][c] + p1[c] + p2[c] + p3[c];
+      for (int c = 0; c < 3; ++c) au[c] = R[c] + p2[c];
       if constexpr (M::kind >= 2) {
         if (own && k >= K0) add_extra<T, M>(mat, (long long)k * lay + (long long)cj * nxn + ci, au);
       }

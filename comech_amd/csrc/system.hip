@@ -95,7 +95,9 @@ struct Level {
   std::vector<double> r1w_h, r1k_h;
   DevBuf<int> r1ptr, r1dof;
   DevBuf<double> r1w64, r1k64;
-  int kz[2] = {0, 0};         // z extent of a structured block per precision (0 = heuristic)
+  // z extent of a structured block per precision and epilogue class (0: products and residuals,
+  // 1: fused smoothing steps); 0 = heuristic
+  int kz[2][2] = {{0, 0}, {0, 0}};
   // nodes the sparse pass writes (irregular cells, extra rows, rank-one terms, all split copies),
   // sorted; the sparse pass addresses them through compact indices
   int ntouch = 0;
@@ -110,8 +112,9 @@ struct Level {
   std::vector<double> ph_h[3];
   long long nblk = 0, nrn = 0;  // block-row entries and rank-one list entries (totals)
   DevBuf<uint8_t> bflag[2];     // per precision: stiffness flags of the structured-march tiles
-  DevBuf<int> border[2];        // per precision: block order of the march (~tile = no stiffness)
-  int bflag_kz[2] = {0, 0};     // the z extent flags and order were built for (0 = none)
+  DevBuf<int> border[2][2];     // per precision and epilogue class: block order of the march
+                                // (~tile = no stiffness)
+  int bflag_kz[2][2] = {{0, 0}, {0, 0}};   // the z extent each order was built for (0 = none)
   int blk_max = 0, rn_max = 0;  // longest block row and longest rank-one list of one node
   // level 0 only
   bool label_mode = false;
@@ -1255,7 +1258,7 @@ struct System {
   // partials, summed in fixed order (no separate pass over p and Ap).
   void apply_dot(int l, const double* p, double* Ap, int slot) {
     Level& v = lv(l);
-    const int nbm = (int)march_shape<double>(l, pick_kz<double>(v)).nb;
+    const int nbm = (int)march_shape<double>(l, pick_kz<double>(v, 0)).nb;
     const int nbf = v.ntouch ? div_up(v.ntouch, 256) : 0;
     CM_CHECK(partial.n >= (size_t)(nbm + nbf), "partial buffer too small for the fused dot");
     Epi<double> ep{nullptr, Ap, nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, partial.p};
@@ -1274,7 +1277,7 @@ struct System {
   // partials needed by apply_dot on level l (FP64)
   size_t dot_partials(int l) {
     Level& v = lv(l);
-    return (size_t)march_shape<double>(l, pick_kz<double>(v)).nb + (v.ntouch ? div_up(v.ntouch, 256) : 0);
+    return (size_t)march_shape<double>(l, pick_kz<double>(v, 0)).nb + (v.ntouch ? div_up(v.ntouch, 256) : 0);
   }
 
   // the level-0 sparse pass, run as extra blocks of the march launch
@@ -1348,11 +1351,16 @@ struct System {
   // Runs outside graph capture: entry points refresh before they launch or capture.
   template <typename T>
   void refresh_flags(int l) {
+    refresh_flags_ec<T>(l, 0);
+    refresh_flags_ec<T>(l, 1);
+  }
+  template <typename T>
+  void refresh_flags_ec(int l, int ec) {
     Level& v = lv(l);
     if (!v.dat<T>().ready || l == nlev() - 1) return;
     const int slot = std::is_same<T, float>::value ? 1 : 0;
-    const int kz = pick_kz<T>(v);
-    if (v.bflag_kz[slot] == kz) return;
+    const int kz = pick_kz<T>(v, ec);
+    if (v.bflag_kz[slot][ec] == kz) return;
     const MarchShape ms = march_shape<T>(l, kz);
     if (v.bflag[slot].n < (size_t)ms.nb) v.bflag[slot].alloc((size_t)march_shape<T>(l, 1).nb);
     const GridArgs<T> g = grid_args<T>(l);
@@ -1389,8 +1397,8 @@ struct System {
     };
     deal(full, 0, false);
     deal(none, nf, true);
-    v.border[slot].upload(ord, stream);
-    v.bflag_kz[slot] = kz;
+    v.border[slot][ec].upload(ord, stream);
+    v.bflag_kz[slot][ec] = kz;
   }
   template <typename T>
   void refresh_all_flags() {
@@ -1403,9 +1411,10 @@ struct System {
     Level& v = lv(l);
     const GridArgs<T> g = grid_args<T>(l);
     const int slot = std::is_same<T, float>::value ? 1 : 0;
-    const MarchShape ms = march_shape<T>(l, pick_kz<T>(v));
+    const int ec = EM == EPI_CHEB ? 1 : 0;
+    const MarchShape ms = march_shape<T>(l, pick_kz<T>(v, ec));
     const TileMap tm{ms.ntx, ms.nty, ms.ntz, (int)(ms.nb / 8), (int)(ms.nb % 8)};
-    const int* fl = (v.bflag_kz[slot] == ms.kz && use_flags) ? v.border[slot].p : nullptr;
+    const int* fl = (v.bflag_kz[slot][ec] == ms.kz && use_flags) ? v.border[slot][ec].p : nullptr;
     with_mat<T>(l, [&](auto m) {
       using M = decltype(m);
       CM_CHECK(sp.nb == 0 || M::kind <= 1, "the sparse pass rides only on level-0 marches");
@@ -1433,9 +1442,10 @@ struct System {
   int kz_override = 0;
   bool use_flags = true;      // empty blocks of the structured march take the epilogue-only path
   template <typename T>
-  int pick_kz(const Level& v) const {
+  int pick_kz(const Level& v, int ec) const {
     if (kz_override > 0) return kz_override;
-    const int tuned = v.kz[std::is_same<T, float>::value ? 1 : 0];
+    const int slot = std::is_same<T, float>::value ? 1 : 0;
+    const int tuned = v.kz[slot][ec] > 0 ? v.kz[slot][ec] : v.kz[slot][0];
     if (tuned > 0) return tuned;
     const long long xy = (long long)div_up(v.nx + 1, TOUT) * div_up(v.ny + 1, TOUT);
     // the longest z chain that still gives >= 600 blocks (about 2.3 blocks per CU); measured on
@@ -1470,43 +1480,65 @@ struct System {
     const int kind = l > 0 ? 3 : (v.label_mode ? 1 : 2);
     return prec + 2 * (kind + 4 * ((l == 0 && sri) + 2 * (l == 0 && v.iso_fac > 0.0)));
   }
+  // Class 0 (products, residuals) is timed on the plain product, class 1 on the fused smoothing
+  // step where the level has its block-Jacobi inverse in precision T (else it takes class 0's).
   template <typename T>
-  void autotune_kz(int nrep) {
+  void autotune_kz(int nrep, bool retime = false) {
     hipEvent_t e0, e1;
     CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));
     const int slot = std::is_same<T, float>::value ? 1 : 0;
     for (int l = 0; l < nlev() - 1; ++l) {
       Level& v = lv(l);
       LevelT<T>& d = v.dat<T>();
-      const KzKey key{v.nx, v.ny, v.nz, kz_variant<T>(l)};
-      {
-        std::lock_guard<std::mutex> g(kz_mu());
-        auto it = kz_cache().find(key);
-        if (it != kz_cache().end()) {
-          v.kz[slot] = it->second;
-          refresh_flags<T>(l);
+      for (int ec = 0; ec < 2; ++ec) {
+        if (ec == 1 && !d.Dinv.n) {
+          v.kz[slot][1] = v.kz[slot][0];
+          refresh_flags_ec<T>(l, 1);
           continue;
         }
+        const KzKey key{v.nx, v.ny, v.nz, 2 * kz_variant<T>(l) + ec};
+        if (!retime) {
+          std::lock_guard<std::mutex> g(kz_mu());
+          auto it = kz_cache().find(key);
+          if (it != kz_cache().end()) {
+            v.kz[slot][ec] = it->second;
+            refresh_flags_ec<T>(l, ec);
+            continue;
+          }
+        }
+        auto run = [&]() {
+          if (ec == 0) apply_structured<T>(l, d.x.p, d.t.p);
+          else smooth_step_structured<T>(l, d.t.p);
+        };
+        // two passes over the extents, the faster of each extent's two timings kept (a single
+        // pass picked within-noise losers on large grids)
+        constexpr int KZS[6] = {1, 2, 4, 8, 16, 32};
+        float tbest[6];
+        for (float& t : tbest) t = 1e30f;
+        for (int pass = 0; pass < 2; ++pass)
+          for (int q = 0; q < 6; ++q) {
+            const int kz = KZS[q];
+            if (kz > v.nz + 1 && kz != 1) continue;
+            v.kz[slot][ec] = kz;
+            refresh_flags_ec<T>(l, ec);
+            run();
+            CM_HIP(hipEventRecord(e0, stream));
+            for (int r = 0; r < nrep; ++r) run();
+            CM_HIP(hipEventRecord(e1, stream));
+            CM_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            CM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            tbest[q] = std::min(tbest[q], ms);
+          }
+        float best = 1e30f;
+        int best_kz = 1;
+        for (int q = 0; q < 6; ++q)
+          if (tbest[q] < best) { best = tbest[q]; best_kz = KZS[q]; }
+        v.kz[slot][ec] = best_kz;
+        refresh_flags_ec<T>(l, ec);
+        std::lock_guard<std::mutex> g(kz_mu());
+        kz_cache()[key] = best_kz;
       }
-      float best = 1e30f;
-      int best_kz = 0;
-      for (int kz : {1, 2, 4, 8, 16, 32}) {
-        if (kz > v.nz + 1 && kz != 1) continue;
-        v.kz[slot] = kz;
-        refresh_flags<T>(l);
-        apply_structured<T>(l, d.x.p, d.t.p);
-        CM_HIP(hipEventRecord(e0, stream));
-        for (int r = 0; r < nrep; ++r) apply_structured<T>(l, d.x.p, d.t.p);
-        CM_HIP(hipEventRecord(e1, stream));
-        CM_HIP(hipEventSynchronize(e1));
-        float ms = 0.f;
-        CM_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) { best = ms; best_kz = kz; }
-      }
-      v.kz[slot] = best_kz;
-      refresh_flags<T>(l);
-      std::lock_guard<std::mutex> g(kz_mu());
-      kz_cache()[key] = best_kz;
     }
     CM_HIP(hipEventDestroy(e0)); CM_HIP(hipEventDestroy(e1));
   }
@@ -1737,6 +1769,14 @@ struct System {
     return v;
   }
 
+  // Recurrence threshold after a failed confirmation that saw the drift (true minus recurrence
+  // residual): at most tol minus that drift and half the previous threshold, at least tol / 16.
+  // Halving keeps a margin when the drift grows from one restart to the next (stiff joints beside
+  // soft tissue, whose attainable true residual lies near tol).
+  static double next_target(double target, double thr, double drift) {
+    return std::max(thr / 16.0, std::min(thr - drift, 0.5 * target));
+  }
+
   // rigid-mode correction z += Z Einv Z^T r (FP64)
   template <typename T>
   void rigid_correct(const double* r, T* z) {
@@ -1853,7 +1893,7 @@ struct System {
           dot(n, r64.p, r64.p, S_RR);
           rn = std::sqrt(read_slot(S_RR));
           if (rn <= thr) { flag = 0; break; }
-          target = std::max(0.25 * thr, thr - (rn - rec));
+          target = next_target(target, thr, rn - rec);
           // three restarts in a row that do not halve the true residual: the attainable accuracy
           // of this system (the residual of its solution rounded to FP64) is above tol; stop (not
           // converged) instead of iterating to maxiter.  A system whose floor lies just below tol
@@ -2320,7 +2360,7 @@ struct cm_batch {
         finish(sl, 0);
         return;
       }
-      sl.target = std::max(0.25 * sl.thr, sl.thr - (sl.rn - rec));
+      sl.target = System::next_target(sl.target, sl.thr, sl.rn - rec);
       sl.stalls = (sl.rn > 0.5 * sl.last_true) ? sl.stalls + 1 : 0;
       sl.last_true = sl.rn;
       if (sl.stalls >= 3) {
@@ -2589,10 +2629,13 @@ int cm_sys_level_stats(cm_sys* h, int level, int64_t* st, int cap) {
   CM_API_BEGIN
   need_level(h, level);
   cm::Level& v = h->s.lv(level);
-  const int64_t vals[10] = {v.ncell, v.nirr, (int64_t)v.xrow_h.size(), (int64_t)v.xcol_h.size(),
+  const System& s = h->s;
+  const int64_t vals[14] = {v.ncell, v.nirr, (int64_t)v.xrow_h.size(), (int64_t)v.xcol_h.size(),
                             (int64_t)(v.r1ptr_h.empty() ? 0 : v.r1ptr_h.size() - 1), (int64_t)v.r1dof_h.size(),
-                            v.nblk, v.blk_max, v.nrn, v.rn_max};
-  for (int q = 0; q < 10 && q < cap; ++q) st[q] = vals[q];
+                            v.nblk, v.blk_max, v.nrn, v.rn_max,
+                            s.pick_kz<double>(v, 0), s.pick_kz<double>(v, 1), s.pick_kz<float>(v, 0),
+                            s.pick_kz<float>(v, 1)};
+  for (int q = 0; q < 14 && q < cap; ++q) st[q] = vals[q];
   CM_API_END
 }
 
@@ -2814,14 +2857,15 @@ int cm_sys_time_apply(cm_sys* h, int level, int prec, const void* u, void* y, in
 int cm_sys_autotune(cm_sys* h, int prec, int nrep) {
   CM_API_BEGIN
   need_built(h);
-  CM_CHECK(nrep >= 1, "nrep must be positive");
+  CM_CHECK(nrep != 0, "nrep must not be 0");
   System& s = h->s;
+  const bool retime = nrep < 0;
   if (prec == 0) {
     s.ensure_data<double>();
-    s.autotune_kz<double>(nrep);
+    s.autotune_kz<double>(retime ? -nrep : nrep, retime);
   } else {
-    s.ensure_data<float>();
-    s.autotune_kz<float>(nrep);
+    s.ensure_prec<float>();   // an FP32 V-cycle always has its smoother data
+    s.autotune_kz<float>(retime ? -nrep : nrep, retime);
   }
   s.drop_graphs();
   CM_API_END
@@ -2844,6 +2888,19 @@ int cm_sys_set_kz(cm_sys* h, int kz) {
     h->s.refresh_all_flags<double>();
     h->s.refresh_all_flags<float>();
   }
+  CM_API_END
+}
+
+int cm_sys_set_tuned_kz(cm_sys* h, int level, int prec, int smooth, int kz) {
+  CM_API_BEGIN
+  need_level(h, level);
+  CM_CHECK(prec == 0 || prec == 1, "prec must be 0 (FP64) or 1 (FP32)");
+  CM_CHECK(kz >= 0 && kz <= 4096, "kz out of range");
+  System& s = h->s;
+  s.lv(level).kz[prec][smooth ? 1 : 0] = kz;
+  s.drop_graphs();
+  if (prec == 0) s.refresh_flags<double>(level);
+  else s.refresh_flags<float>(level);
   CM_API_END
 }
 

@@ -146,12 +146,14 @@ class VoxelSystem:
                 for r in d]
 
     def level_stats(self, level):
-        """Operator structure of one level: cell, irregular-cell, extra-row and rank-one counts and
-        the coarse per-node list lengths applied inside the structured march."""
-        st = np.zeros(10, np.int64)
-        check(self.lib.cm_sys_level_stats(self.h, int(level), ptr(st, ctypes.c_longlong), 10))
+        """Operator structure of one level: cell, irregular-cell, extra-row and rank-one counts, the
+        coarse per-node list lengths applied inside the structured march, and the z extent of a
+        march block per precision for products (kz_f64, kz_f32) and fused smoothing steps."""
+        st = np.zeros(14, np.int64)
+        check(self.lib.cm_sys_level_stats(self.h, int(level), ptr(st, ctypes.c_longlong), 14))
         keys = ("cells", "irregular", "extra_rows", "extra_nnz", "rank1_terms", "rank1_nnz", "block_entries",
-                "block_row_max", "rank1_node_entries", "rank1_node_max")
+                "block_row_max", "rank1_node_entries", "rank1_node_max", "kz_f64", "kz_f64_smooth", "kz_f32",
+                "kz_f32_smooth")
         return dict(zip(keys, (int(v) for v in st)))
 
     def timings(self):
@@ -279,10 +281,31 @@ class VoxelSystem:
                                          ctypes.c_void_p(y.data_ptr()), int(nrep), what, ctypes.byref(ms)))
         return ms.value
 
-    def autotune(self, precisions=("f32", "f64"), nrep=5):
-        """Pick the fastest structured-block z extent per level (project autotuner, HIP events)."""
+    def autotune(self, precisions=("f32", "f64"), nrep=5, retime=False):
+        """Pick the fastest structured-block z extent per level (project autotuner, HIP events);
+        retime: time level shapes the process has tuned before again."""
         for p in precisions:
-            check(self.lib.cm_sys_autotune(self.h, PREC[p], int(nrep)))
+            check(self.lib.cm_sys_autotune(self.h, PREC[p], -int(nrep) if retime else int(nrep)))
+
+    def tuning(self):
+        """z extents of every level's march blocks (the autotuner's or the heuristic's):
+        [[nx, ny, nz, kz_f64, kz_f64_smooth, kz_f32, kz_f32_smooth], ...] for the levels with a
+        smoother."""
+        out = []
+        for l, lv in enumerate(self.levels()[:-1]):
+            st = self.level_stats(l)
+            out.append([lv["nx"], lv["ny"], lv["nz"]] + [st[k] for k in ("kz_f64", "kz_f64_smooth", "kz_f32",
+                                                                        "kz_f32_smooth")])
+        return out
+
+    def set_tuning(self, table):
+        """Apply a table of ``tuning()``'s form to the levels whose shape matches (others keep theirs)."""
+        lvs = self.levels()
+        for l, row in enumerate(table):
+            if l >= len(lvs) - 1 or [lvs[l]["nx"], lvs[l]["ny"], lvs[l]["nz"]] != list(row[:3]):
+                continue
+            for prec, smooth, kz in ((0, 0, row[3]), (0, 1, row[4]), (1, 0, row[5]), (1, 1, row[6])):
+                check(self.lib.cm_sys_set_tuned_kz(self.h, l, prec, smooth, int(kz)))
 
     def set_kz(self, kz):
         check(self.lib.cm_sys_set_kz(self.h, int(kz)))

@@ -108,10 +108,11 @@ int cm_sys_set_last_coarsening(cm_sys* sys, int axes);
 /* Level table: 6 int64 per level: nx, ny, nz, nodes (incl. copies), compact DOFs, extra rows. */
 int cm_sys_levels(cm_sys* sys, int* nlev, int64_t* desc, int cap);
 
-/* Operator structure of one level (diagnostics; no reference counterpart): 10 int64 --
+/* Operator structure of one level (diagnostics; no reference counterpart): up to 14 int64 --
  * cells, irregular cells, extra-stiffness CSR rows and entries, rank-one terms and their DOF
  * entries, coarse block-row entries and the longest block row, rank-one node-list entries and the
- * longest list of one node. */
+ * longest list of one node, then the z extent of a structured-march block for FP64 products, FP64
+ * smoothing steps, FP32 products and FP32 smoothing steps. */
 int cm_sys_level_stats(cm_sys* sys, int level, int64_t* stats, int cap);
 
 /* ---- compute (device pointers) ------------------------------------------------------------- */
@@ -221,10 +222,18 @@ int cm_sys_time_apply(cm_sys* sys, int level, int prec, const void* u, void* y, 
                       double* avg_ms);
 
 /* Tile autotuner: for every level, time the structured stencil kernel in precision `prec` for
- * z extents 1..32 (nrep launches each, HIP events) and keep the fastest.  Results are cached
- * process-wide per level shape and kernel variant; a cached level is not timed again.  No
- * reference counterpart (project autotuner). */
+ * z extents 1..32 (nrep launches each, HIP events) and keep the fastest, separately for products
+ * (timed on K u) and fused smoothing steps (timed on one step; FP32 always, FP64 when the level
+ * has its FP64 smoother, else the product's extent).  Results are cached process-wide per level
+ * shape and kernel variant; a cached level is not timed again (nrep < 0: timed again with -nrep
+ * launches, replacing the cached extent).  The GPU should be otherwise idle.
+ * No reference counterpart (project autotuner). */
 int cm_sys_autotune(cm_sys* sys, int prec, int nrep);
+
+/* Set the z extent of one level's structured-march blocks (prec 0 = FP64, 1 = FP32; smooth = 1 for
+ * fused smoothing steps, 0 for products and residuals; 0 = the heuristic), e.g. from a tuning table
+ * the autotuner wrote earlier.  No reference counterpart. */
+int cm_sys_set_tuned_kz(cm_sys* sys, int level, int prec, int smooth, int kz);
 
 /* Build stage times in ms: level-0 preparation, hierarchy structure, uploads, coarse moments,
  * diagonal blocks, coarsest inverse, total, then the structure stage's used-mask and

@@ -98,6 +98,28 @@ def test_march_column_equals_assembled_operator(sri, iso):
     assert np.abs(y - ref).max() <= 1e-13 * np.abs(ref).max()
 
 
+def test_face_form_equals_transform_composition():
+    """The level-0 kernel's face form of one z step (cell_faces_iso_sri: z butterflies folded into
+    the element terms, normal and shear weights of a mode combined) equals the composition it
+    replaces (faces_to_cell, the transform-space operator, cell_to_faces, carry add) on random
+    faces, and a zero material gives back the carry."""
+    lib = _host()
+    rng = np.random.default_rng(7)
+    for trial in range(200):
+        lo, hi, car = (rng.standard_normal(12) * 10.0 ** rng.uniform(-3, 3) for _ in range(3))
+        wl, wm = 10.0 ** rng.uniform(-3, 3, 2)
+        out = {}
+        for fused in (0, 1):
+            flo, fhi = np.zeros(12), np.zeros(12)
+            lib.cm_host_cell_faces(fused, P(lo), P(hi), P(car), ctypes.c_double(wl), ctypes.c_double(wm), P(flo), P(fhi))
+            out[fused] = np.r_[flo, fhi]
+        scale = np.abs(out[0]).max()
+        assert np.abs(out[1] - out[0]).max() <= 1e-14 * scale, trial
+    flo, fhi = np.zeros(12), np.zeros(12)
+    lib.cm_host_cell_faces(1, P(lo), P(hi), P(car), ctypes.c_double(0.0), ctypes.c_double(0.0), P(flo), P(fhi))
+    assert np.array_equal(flo, car) and not fhi.any()
+
+
 @pytest.mark.parametrize("sri", [0, 1])
 def test_kernel_moments_are_exact_galerkin(sri):
     """Coarse cell operator from shifted child moments == P^T K P of its 2x2x2 children with random

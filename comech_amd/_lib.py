@@ -57,6 +57,7 @@ SIGNATURES = {
     "cm_sys_time_apply": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_int, P_double],
     "cm_sys_autotune": [c_void_p, c_int, c_int],
     "cm_sys_set_tuned_kz": [c_void_p, c_int, c_int, c_int, c_int],
+    "cm_sys_set_march_blocks": [c_void_p, c_int],
     "cm_sys_timings": [c_void_p, P_double, c_int],
     "cm_sys_set_kz": [c_void_p, c_int],
     "cm_sys_set_block_skip": [c_void_p, c_int],

@@ -310,10 +310,11 @@ CM_INLINE void mom_mul(const T* __restrict__ mom, const int (&B)[N], const T* a,
 }
 
 // Transform-space core of the moment operator: c = forward transforms of the three displacement
-// components, F = transform-space forces (element forces = wht_bwd of them).
+// components, F = transform-space forces (element forces = wht_bwd of them).  In two parts, the
+// lam part (sets F) and the mu part (adds to F), so that a caller can form each part's 27 moments
+// just before it (the two sets are not needed at once).
 template <typename T>
-CM_INLINE void elem_mid_mom(const T (&c)[3][8], T (&F)[3][8], const T* __restrict__ momL, const T* __restrict__ momM,
-                            T gx, T gy, T gz) {
+CM_INLINE void elem_mid_mom_lam(const T (&c)[3][8], T (&F)[3][8], const T* __restrict__ momL, T gx, T gy, T gz) {
   const T g[3] = {gx, gy, gz};
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -339,6 +340,10 @@ CM_INLINE void elem_mid_mom(const T (&c)[3][8], T (&F)[3][8], const T* __restric
       for (int i = 0; i < 3; ++i)
         if (!((B[k] >> i) & 1)) F[i][B[k] | (1 << i)] += g[i] * q[k];
   }
+}
+template <typename T>
+CM_INLINE void elem_mid_mom_mu(const T (&c)[3][8], T (&F)[3][8], const T* __restrict__ momM, T gx, T gy, T gz) {
+  const T g[3] = {gx, gy, gz};
   // ---- mu 2 G_ii^2: beta with beta_i = 0
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -396,6 +401,13 @@ CM_INLINE void elem_mid_mom(const T (&c)[3][8], T (&F)[3][8], const T* __restric
       if (!((B[k] >> i) & 1)) F[j][B[k] | (1 << i)] += g[i] * q[k];
     }
   }
+}
+
+template <typename T>
+CM_INLINE void elem_mid_mom(const T (&c)[3][8], T (&F)[3][8], const T* __restrict__ momL, const T* __restrict__ momM,
+                            T gx, T gy, T gz) {
+  elem_mid_mom_lam<T>(c, F, momL, gx, gy, gz);
+  elem_mid_mom_mu<T>(c, F, momM, gx, gy, gz);
 }
 
 template <typename T>

@@ -866,10 +866,17 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
               fzM[d][1] = pz[3 * d + 1];
               fzL[d][1] = M::sri ? pz[3 * d + 2] : pz[3 * d + 1];
             }
-            T mL[27], mM[27];
-            child_moments<T>(cw, fxL, fyL, fzL, mL);
-            child_moments<T>(cw + 8, fxM, fyM, fzM, mM);
-            elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
+            const T gz = T(0.25) / g.hz[k];
+            {
+              T mL[27];
+              child_moments<T>(cw, fxL, fyL, fzL, mL);
+              elem_mid_mom_lam<T>(c3, F3, mL, gx, gy, gz);
+            }
+            {
+              T mM[27];
+              child_moments<T>(cw + 8, fxM, fyM, fzM, mM);
+              elem_mid_mom_mu<T>(c3, F3, mM, gx, gy, gz);
+            }
           }
         }
         if (done) cell_to_faces(F3, Flo, carn);

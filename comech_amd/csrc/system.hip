@@ -1440,6 +1440,7 @@ struct System {
   }
 
   int kz_override = 0;
+  long long march_blocks = 600;   // the heuristic's minimum number of march blocks per launch
   bool use_flags = true;      // empty blocks of the structured march take the epilogue-only path
   template <typename T>
   int pick_kz(const Level& v, int ec) const {
@@ -1448,10 +1449,11 @@ struct System {
     const int tuned = v.kz[slot][ec] > 0 ? v.kz[slot][ec] : v.kz[slot][0];
     if (tuned > 0) return tuned;
     const long long xy = (long long)div_up(v.nx + 1, TOUT) * div_up(v.ny + 1, TOUT);
-    // the longest z chain that still gives >= 600 blocks (about 2.3 blocks per CU); measured on
-    // MI355X this is the fastest or within 5 % of it on every level of the hindfoot raster
+    // the longest z chain that still gives >= march_blocks blocks (default 600, about 2.3 blocks
+    // per CU); measured on MI355X this is the fastest or within 5 % of it on every level of the
+    // hindfoot raster solved alone
     for (int kz : {32, 16, 8, 4, 2}) {
-      if (xy * div_up(v.nz + 1, kz) >= 600) return kz;
+      if (xy * div_up(v.nz + 1, kz) >= march_blocks) return kz;
     }
     return 1;
   }
@@ -2887,6 +2889,20 @@ int cm_sys_set_kz(cm_sys* h, int kz) {
   if (h->s.built) {
     h->s.refresh_all_flags<double>();
     h->s.refresh_all_flags<float>();
+  }
+  CM_API_END
+}
+
+int cm_sys_set_march_blocks(cm_sys* h, int min_blocks) {
+  CM_API_BEGIN
+  CM_CHECK(h != nullptr, "null system");
+  CM_CHECK(min_blocks >= 1, "min_blocks must be positive");
+  System& s = h->s;
+  s.march_blocks = min_blocks;
+  s.drop_graphs();
+  if (s.built) {
+    s.refresh_all_flags<double>();
+    s.refresh_all_flags<float>();
   }
   CM_API_END
 }

@@ -53,14 +53,16 @@ class VoxelSystem:
                  copy_orig=None, irr_cells=None, irr_nodes=None, extra=None, rank1=None, used=None, min_dofs=6000,
                  max_levels=6,
                  levels=None, smoother="chebyshev", degree=3, omega=0.6, npre=2, npost=2, materials="auto",
-                 last_axes=3, copy_blocks=True, device=None):
+                 last_axes=3, copy_blocks=True, march_blocks=600, device=None):
         """grid: Grid; active: (ncell,) bool in reference cell order; materials either per cell
         (lam, mu) or as labels + tables; split connectivity as copy originals and irregular cells
         with their 8 node ids; extra: (rows, cols, vals) COO over level-0 DOFs.  last_axes: axes
         (x 1, y 2, z 4) the step to the coarsest level coarsens; 7 is the reference rule of
         fe/gmg.py, the default 3 keeps the vertical resolution on the coarsest grid.  copy_blocks:
         level-0 smoother blocks over each grid node and its split copies when extra stiffness couples
-        them (False: the per-node 3x3 block Jacobi of fe/gmg.py:29-44)."""
+        them (False: the per-node 3x3 block Jacobi of fe/gmg.py:29-44).  march_blocks: the z-extent
+        heuristic takes the longest z chain that still gives this many march blocks per launch (fewer
+        suit solves that share the GPU with others)."""
         self.device = require_gpu(device)
         self.lib = _lib.load()
         self.grid = grid
@@ -91,6 +93,7 @@ class VoxelSystem:
             fixed = 0 if levels is None else int(levels)
             check(self.lib.cm_sys_set_last_coarsening(self.h, int(last_axes)))
             check(self.lib.cm_sys_set_copy_blocks(self.h, int(bool(copy_blocks))))
+            check(self.lib.cm_sys_set_march_blocks(self.h, int(march_blocks)))
             with torch.cuda.device(self.device):
                 check(self.lib.cm_sys_build(self.h, int(min_dofs), int(max_levels), fixed, SMOOTHERS[smoother],
                                             int(degree), float(omega), int(npre), int(npost)))

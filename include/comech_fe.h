@@ -235,6 +235,11 @@ int cm_sys_autotune(cm_sys* sys, int prec, int nrep);
  * the autotuner wrote earlier.  No reference counterpart. */
 int cm_sys_set_tuned_kz(cm_sys* sys, int level, int prec, int smooth, int kz);
 
+/* Minimum number of structured-march blocks per launch the z-extent heuristic aims for (default
+ * 600; fewer, longer blocks suit a system that shares the GPU with concurrent solves).  Levels with
+ * a tuned extent keep it.  No reference counterpart. */
+int cm_sys_set_march_blocks(cm_sys* sys, int min_blocks);
+
 /* Build stage times in ms: level-0 preparation, hierarchy structure, uploads, coarse moments,
  * diagonal blocks, coarsest inverse, total, then the structure stage's used-mask and
  * extra-stiffness coarsening parts (up to cap values, at most 9). */

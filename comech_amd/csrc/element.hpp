@@ -126,10 +126,10 @@ struct CellW {
 // forces are wht_bwd of them).  ISO = true is the cubic cell (h_x = h_y = h_z = h): every term
 // carries exactly two gradient factors, so the caller passes wl = lam V g^2, wm = mu V g^2
 // (V g^2 = h / 16) and the gradient factors are compile-time ones.
-template <typename T, bool SRI, bool ISO>
-CM_INLINE void elem_mid(const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
-                        T* __restrict__ Fx, T* __restrict__ Fy, T* __restrict__ Fz, const CellW<T>& w) {
-  // ISO: cubic cell, the common factor g^2 is folded into wl and wm by the caller
+// The mu terms (exact integration of a constant mu over the cell); sets F.
+template <typename T, bool ISO>
+CM_INLINE void elem_mu_part(const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+                            T* __restrict__ Fx, T* __restrict__ Fy, T* __restrict__ Fz, const CellW<T>& w) {
   const T gx = ISO ? T(1) : w.gx, gy = ISO ? T(1) : w.gy, gz = ISO ? T(1) : w.gz;
   const T third = T(1) / T(3), ninth = T(1) / T(9);
   // monomial weights V * 3^-|beta| folded with the material factor
@@ -184,6 +184,15 @@ CM_INLINE void elem_mid(const T* __restrict__ cx, const T* __restrict__ cy, cons
     Fx[5] += zz * m1 * cx[5]; Fx[7] += zz * m2 * cx[7];
     Fz[5] += xx * m1 * cz[5]; Fz[7] += xx * m2 * cz[7];
   }
+}
+
+template <typename T, bool SRI, bool ISO>
+CM_INLINE void elem_mid(const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+                        T* __restrict__ Fx, T* __restrict__ Fy, T* __restrict__ Fz, const CellW<T>& w) {
+  // ISO: cubic cell, the common factor g^2 is folded into wl and wm by the caller
+  const T gx = ISO ? T(1) : w.gx, gy = ISO ? T(1) : w.gy, gz = ISO ? T(1) : w.gz;
+  const T third = T(1) / T(3), ninth = T(1) / T(9);
+  elem_mu_part<T, ISO>(cx, cy, cz, Fx, Fy, Fz, w);
   // ---- lam: (div u)^2.  div monomial beta collects G_ii from components with beta_i = 0.
   {
     const T l0 = w.wl;
@@ -199,6 +208,28 @@ CM_INLINE void elem_mid(const T* __restrict__ cx, const T* __restrict__ cy, cons
       Fx[7] += gx * gx * l2 * cx[7];                                             // beta 011
     }
   }
+}
+
+// Constant-material operator whose lam term is integrated with per-axis second moments q
+// (int xi^2 dV / V along that axis): q = 1/3 is exact integration, 0 the centre point (SRI), 1/4
+// the two centre points of a cell split into equal halves -- a level-1 cell whose level-0 children
+// share one material integrates its lam term with the children's SRI points, and its Galerkin
+// operator is this one (the mu term is exact on every child, so exact on the cell).
+template <typename T>
+CM_INLINE void elem_mid_q(const T* __restrict__ cx, const T* __restrict__ cy, const T* __restrict__ cz,
+                          T* __restrict__ Fx, T* __restrict__ Fy, T* __restrict__ Fz, const CellW<T>& w, T qx,
+                          T qy, T qz) {
+  const T gx = w.gx, gy = w.gy, gz = w.gz;
+  elem_mu_part<T, false>(cx, cy, cz, Fx, Fy, Fz, w);
+  const T lx = w.wl * qx, ly = w.wl * qy, lz = w.wl * qz;
+  T g = gx * cx[1] + gy * cy[2] + gz * cz[4];            // beta = 000
+  g *= w.wl; Fx[1] += gx * g; Fy[2] += gy * g; Fz[4] += gz * g;
+  g = gy * cy[3] + gz * cz[5]; g *= lx; Fy[3] += gy * g; Fz[5] += gz * g;   // beta 100
+  g = gx * cx[3] + gz * cz[6]; g *= ly; Fx[3] += gx * g; Fz[6] += gz * g;   // beta 010
+  g = gx * cx[5] + gy * cy[6]; g *= lz; Fx[5] += gx * g; Fy[6] += gy * g;   // beta 001
+  Fz[7] += gz * gz * (lx * qy) * cz[7];                                      // beta 110
+  Fy[7] += gy * gy * (lx * qz) * cy[7];                                      // beta 101
+  Fx[7] += gx * gx * (ly * qz) * cx[7];                                      // beta 011
 }
 
 // The cubic-cell, SRI operator of one z step in face form: lo, hi = xy transforms of the cell's

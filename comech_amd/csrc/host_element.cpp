@@ -32,6 +32,22 @@ void cm_host_elem_const_f32(const double* dims, double lam, double mu, int sri, 
   for (int q = 0; q < 24; ++q) f64[q] = f[q];
 }
 
+// Dense 24x24 matrix of the constant-material operator with per-axis lam second moments q
+// (elem_mid_q), cell dims h, lam and mu per unit volume.
+void cm_host_elem_q(const double* h, double lam, double mu, const double* q, double* K) {
+  const double V = h[0] * h[1] * h[2];
+  CellW<double> w{0.25 / h[0], 0.25 / h[1], 0.25 / h[2], lam * V, mu * V};
+  for (int c = 0; c < 24; ++c) {
+    double cx[8], cy[8], cz[8];
+    for (int l = 0; l < 8; ++l) { cx[l] = (3 * l == c); cy[l] = (3 * l + 1 == c); cz[l] = (3 * l + 2 == c); }
+    wht_fwd(cx); wht_fwd(cy); wht_fwd(cz);
+    double Fx[8], Fy[8], Fz[8];
+    elem_mid_q<double>(cx, cy, cz, Fx, Fy, Fz, w, q[0], q[1], q[2]);
+    wht_bwd(Fx); wht_bwd(Fy); wht_bwd(Fz);
+    for (int l = 0; l < 8; ++l) { K[(3 * l) * 24 + c] = Fx[l]; K[(3 * l + 1) * 24 + c] = Fy[l]; K[(3 * l + 2) * 24 + c] = Fz[l]; }
+  }
+}
+
 void cm_host_elem_mom(const double* momL, const double* momM, const double* h, double* K) {
   for (int c = 0; c < 24; ++c) {
     double u[24] = {0}, f[24];

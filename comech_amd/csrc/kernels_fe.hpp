@@ -33,6 +33,8 @@ constexpr uint8_t LAB_NONE = 255;
 //   kind 2: coarse levels, 27 lam + 27 mu moments per cell
 //   kind 3: level 1, the 8 lam and 8 mu weights of its level-0 child cells (constant materials),
 //           expanded in registers into the 27 + 27 moments
+//   kind 4: level 1 split by cells: cells whose children share one material take the constant-
+//           material operator, the others go through the heterogeneous-cell pass (het_body)
 // ISO = the level-0 grid is made of identical cubes (h_x = h_y = h_z = h everywhere); the common
 // factor V g^2 = h / 16 is then folded into the material weights (GridArgs::iso).
 template <typename T, bool SRI_, bool ISO_>
@@ -78,6 +80,42 @@ struct ChildMat {
   static constexpr int kind = 3;
   static constexpr bool sri = SRI_, iso = false;
 };
+
+// A level-1 cell whose existing level-0 children all carry one material and split every axis into
+// equal halves (or not at all) has the Galerkin operator of a constant material whose lam term is
+// integrated at the children's SRI points: elem_mid_q with the per-axis second moments q (1/4 for
+// two halves, 0 for one child; 1/3 without SRI).  cl, cm hold lam V and mu V of such cells and 0 for
+// every other cell.  The cells with several materials are applied by het_cell (extra blocks of the
+// coarse-springs launch before the march) into hf, 24 forces per cell; the march epilogue of each of
+// their nodes adds its corners' forces (hmap: the node's row of the slot lists, -1 for none; slots
+// 8 * cell + corner, in increasing order) -- a fixed summation order and no launch of its own.
+template <typename T, bool SRI_>
+struct SplitMat {
+  const T* __restrict__ cl;          // lam V per cell (x-fastest), 0 where the cell is not uniform
+  const T* __restrict__ cm;          // mu V per cell
+  const T* __restrict__ qx;          // lam second moment per cell column, row and layer
+  const T* __restrict__ qy;
+  const T* __restrict__ qz;
+  T* ex;
+  const int* __restrict__ hmap;      // per node: row of hptr, or -1
+  const int* __restrict__ hptr;
+  const int* __restrict__ hslot;
+  const T* __restrict__ hf;
+  static constexpr int kind = 4;
+  static constexpr bool sri = SRI_, iso = false;
+};
+
+// Per-axis factor tables of a level-1 cell's children from its ph rows (b, a^2/3 + b^2, b^2 per
+// child): f?L for the lam moments (SRI: centre points), f?M for the mu moments.
+template <typename T, bool SRI>
+__device__ __forceinline__ void child_factors(const T* __restrict__ p, T fL[2][2], T fM[2][2]) {
+#pragma unroll
+  for (int d = 0; d < 2; ++d) {
+    fM[d][0] = fL[d][0] = p[3 * d];
+    fM[d][1] = p[3 * d + 1];
+    fL[d][1] = SRI ? p[3 * d + 2] : p[3 * d + 1];
+  }
+}
 
 // Expands child weights w[8] into the 27 moments (index gx + 3 gy + 9 gz) with per-axis factor
 // tables f?[d][g] (g = 1, 2; g = 0 is 1).
@@ -327,9 +365,20 @@ struct V3Lds {
   }
 };
 
-// Extra stiffness of a coarse node (kinds 2, 3): au += ex[n], ex[n] = 0.
+// Extra stiffness of a coarse node (kinds 2, 3, 4): au += ex[n], ex[n] = 0; kind 4 also adds the
+// forces of the node's corners of several-material cells.
 template <typename T, class M>
 __device__ __forceinline__ void add_extra(const M& mat, long long n, T au[3]) {
+  if constexpr (M::kind == 4) {
+    const int h = mat.hmap[n];
+    if (h >= 0) {
+      for (int q = mat.hptr[h]; q < mat.hptr[h + 1]; ++q) {
+        const long long o = 3LL * mat.hslot[q];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) au[c] += mat.hf[o + c];
+      }
+    }
+  }
   if constexpr (M::kind >= 2) {
     if (mat.ex) {
       T e3[3];
@@ -668,6 +717,14 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
     gy = T(0.25) / hy;
     axy = hx * hy;
   }
+  // kind 4: lam second moments of the thread's cell column and row
+  T qxv = T(0), qyv = T(0);
+  if constexpr (M::kind == 4) {
+    if (cin) {
+      qxv = mat.qx[ci];
+      qyv = mat.qy[cj];
+    }
+  }
   // kind 3: child factor tables of the tile's cell columns and rows (read per step from LDS)
   if constexpr (M::kind == 3) {
     if (tid < 2 * TILE * 6) {
@@ -716,8 +773,8 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
     for (int ab = 0; ab < 4; ++ab) VL::get(src, NL, (ty + (ab >> 1)) * TN + tx + (ab & 1), v[ab]);
     face_fwd(v, f);
   };
-  // material of cell layer kk, loaded one step ahead (kinds 0, 1 and 3); kinds 0 and 1 read 0 for
-  // cells outside the grid
+  // material of cell layer kk, loaded one step ahead (kinds 0, 1, 3 and 4); kinds 0, 1 and 4 read
+  // 0 for cells outside the grid
   const unsigned coff = cin ? (unsigned)cxy : OOB;                   // byte offset of a label
   const unsigned coffT = cin ? (unsigned)(cxy * sizeof(T)) : OOB;     // of a T value
   const long long clb = ncl * (long long)sizeof(T);                    // one cell layer of T values
@@ -740,7 +797,7 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
       labn[set] = __builtin_amdgcn_raw_buffer_load_b8(layer_rs(mat.lab, kk * ncl, ncl, okz), (int)coff, 0, 0);
       labok[set] = okz;
     }
-    if constexpr (M::kind == 1) {
+    if constexpr (M::kind == 1 || M::kind == 4) {
       bld(layer_rs(mat.cl, kk * clb, clb, okz), coffT, lamn[set]);
       bld(layer_rs(mat.cm, kk * clb, clb, okz), coffT, mun[set]);
     }
@@ -786,7 +843,7 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
       const uint8_t lb = (cin && labok[MSET]) ? labn[MSET] : LAB_NONE;
       w.wl = s_tL[lb];
       w.wm = s_tM[lb];
-    } else if constexpr (M::kind == 1) {
+    } else if constexpr (M::kind == 1 || M::kind == 4) {
       const T sc = M::iso ? g.iso : T(1);
       w.wl = lamn[MSET] * sc;
       w.wm = mun[MSET] * sc;
@@ -822,6 +879,14 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
       }
       elem_mid<T, M::sri, M::iso>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w);
       cell_to_faces(F3, Flo, carn);
+    } else if constexpr (M::kind == 4) {
+      // level 1, cells of one material (zero weights elsewhere: the heterogeneous-cell pass)
+      T c3[3][8], F3[3][8];
+      faces_to_cell(ulo, uhi, c3);
+      const int kc = min(max(k, 0), nz - 1);   // layers past the grid have zero weights
+      w.gz = T(0.25) / g.hz[kc];
+      elem_mid_q<T>(c3[0], c3[1], c3[2], F3[0], F3[1], F3[2], w, qxv, qyv, mat.qz[kc]);
+      cell_to_faces(F3, Flo, carn);
     } else {
 #pragma unroll
       for (int q = 0; q < 12; ++q) {
@@ -851,21 +916,9 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
           done = (ws != T(0));
           if (done) {
             T fxL[2][2], fxM[2][2], fyL[2][2], fyM[2][2], fzL[2][2], fzM[2][2];
-            const T* px = s_ph + 6 * tx;
-            const T* py = s_ph + TILE * 6 + 6 * ty;
-            const T* pz = mat.phz + 6 * k;
-#pragma unroll
-            for (int d = 0; d < 2; ++d) {
-              fxM[d][0] = fxL[d][0] = px[3 * d];
-              fxM[d][1] = px[3 * d + 1];
-              fxL[d][1] = M::sri ? px[3 * d + 2] : px[3 * d + 1];
-              fyM[d][0] = fyL[d][0] = py[3 * d];
-              fyM[d][1] = py[3 * d + 1];
-              fyL[d][1] = M::sri ? py[3 * d + 2] : py[3 * d + 1];
-              fzM[d][0] = fzL[d][0] = pz[3 * d];
-              fzM[d][1] = pz[3 * d + 1];
-              fzL[d][1] = M::sri ? pz[3 * d + 2] : pz[3 * d + 1];
-            }
+            child_factors<T, M::sri>(s_ph + 6 * tx, fxL, fxM);
+            child_factors<T, M::sri>(s_ph + TILE * 6 + 6 * ty, fyL, fyM);
+            child_factors<T, M::sri>(mat.phz + 6 * k, fzL, fzM);
             const T gz = T(0.25) / g.hz[k];
             {
               T mL[27];
@@ -959,6 +1012,64 @@ struct MarchL {
   __device__ __forceinline__ void run(int bid, int) const { march_body<T, M, EM>(bid, g, mat, s, ep, kz, tm, order, sp); }
 };
 
+// Several-material cells of a level-1 split march (SplitMat): one thread per listed cell forms the
+// cell's moments from its child weights, applies the moment operator to s and writes the cell's 24
+// forces to hf[cell][corner][comp] (the march epilogue gathers them per node).
+template <typename T>
+struct HetArgs {
+  int n;                             // listed cells (0: none)
+  int sri;
+  const int* __restrict__ cells;     // x-fastest cell ids
+  int nx, ny, nz;
+  const T* __restrict__ hx;
+  const T* __restrict__ hy;
+  const T* __restrict__ hz;
+  const T* __restrict__ cw;          // [16][ncell] child weights
+  const T* __restrict__ phx;
+  const T* __restrict__ phy;
+  const T* __restrict__ phz;
+  T* __restrict__ hf;
+};
+template <typename T, bool SRI>
+__device__ __forceinline__ void het_cell(const HetArgs<T>& h, int t, const T* __restrict__ s) {
+  const long long ncl = (long long)h.nx * h.ny, ncell = ncl * h.nz;
+  const long long c = h.cells[t];
+  const int i = (int)(c % h.nx), j = (int)((c / h.nx) % h.ny), k = (int)(c / ncl);
+  const long long nxn = h.nx + 1, lay = nxn * (h.ny + 1);
+  T cc[3][8];
+#pragma unroll
+  for (int l = 0; l < 8; ++l) {
+    const long long nd = (i + (l & 1)) + nxn * (j + ((l >> 1) & 1)) + lay * (k + (l >> 2));
+#pragma unroll
+    for (int q = 0; q < 3; ++q) cc[q][l] = s[3 * nd + q];
+  }
+  wht_fwd(cc[0]); wht_fwd(cc[1]); wht_fwd(cc[2]);
+  T w[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) w[q] = h.cw[q * ncell + c];
+  T fxL[2][2], fxM[2][2], fyL[2][2], fyM[2][2], fzL[2][2], fzM[2][2];
+  child_factors<T, SRI>(h.phx + 6 * i, fxL, fxM);
+  child_factors<T, SRI>(h.phy + 6 * j, fyL, fyM);
+  child_factors<T, SRI>(h.phz + 6 * k, fzL, fzM);
+  const T gx = T(0.25) / h.hx[i], gy = T(0.25) / h.hy[j], gz = T(0.25) / h.hz[k];
+  T F[3][8];
+  {
+    T mL[27];
+    child_moments<T>(w, fxL, fyL, fzL, mL);
+    elem_mid_mom_lam<T>(cc, F, mL, gx, gy, gz);
+  }
+  {
+    T mM[27];
+    child_moments<T>(w + 8, fxM, fyM, fzM, mM);
+    elem_mid_mom_mu<T>(cc, F, mM, gx, gy, gz);
+  }
+  wht_bwd(F[0]); wht_bwd(F[1]); wht_bwd(F[2]);
+#pragma unroll
+  for (int l = 0; l < 8; ++l)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) h.hf[(8LL * t + l) * 3 + q] = F[q][l];
+}
+
 // Block flags of the structured march for z extent kz: one block per tile id (x fastest, then y,
 // then z), flag = 1 when any cell the march block reads (cell columns I0-1 .. I0+14, rows
 // J0-1 .. J0+14, layers K0-1 .. K0+kz-1) has stiffness.
@@ -981,6 +1092,7 @@ __global__ __launch_bounds__(BLOCK) void k_block_flags(GridArgs<T> g, M mat, int
       if constexpr (M::kind == 3) {
         for (int q = 0; q < 16; ++q) any |= mat.cw[q * ncell + cid] != T(0);
       }
+      if constexpr (M::kind == 4) any = mat.cl[cid] != T(0) || mat.cm[cid] != T(0);
     }
   }
   any = __syncthreads_or(any);
@@ -1023,12 +1135,25 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_coarse_extra(CoarseExtra<T> a, const T* __restrict__ s, T* __restrict__ ex) {
   coarse_extra_body<T>(blockIdx.x, a, s, ex);
 }
+// launch record of the coarse springs (blocks 0 .. nb_extra-1) and, on level 1, the
+// several-material cells (the blocks after them)
 template <typename T>
 struct CoarseExtraL {
   CoarseExtra<T> a;
   const T* s;
   T* ex;
-  __device__ __forceinline__ void run(int bid, int) const { coarse_extra_body<T>(bid, a, s, ex); }
+  int nb_extra;
+  HetArgs<T> h;
+  __device__ __forceinline__ void run(int bid, int) const {
+    if (bid < nb_extra) {
+      coarse_extra_body<T>(bid, a, s, ex);
+      return;
+    }
+    const int t = (bid - nb_extra) * (int)blockDim.x + (int)threadIdx.x;
+    if (t >= h.n) return;
+    if (h.sri) het_cell<T, true>(h, t, s);
+    else het_cell<T, false>(h, t, s);
+  }
 };
 
 // Fix-up after the structured march, one thread per touched node: adds the sparse part tc (and

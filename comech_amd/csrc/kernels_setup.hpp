@@ -88,6 +88,50 @@ __global__ __launch_bounds__(256) void k_child_weights(int ncx, int ncy, int ncz
   for (int q = 0; q < 16; ++q) cw[q * nc + id] = w[q];
 }
 
+// Level-1 split (SplitMat): a coarse cell whose existing level-0 children all carry the same
+// (lam, mu) and split every axis into equal halves (or not at all) gets cl = lam V, cm = mu V (the
+// sums of its child weights); a cell with stiffness that is not such a cell is heterogeneous
+// (het = 1, zero weights); a cell without stiffness gets zeros and het = 0.
+__global__ __launch_bounds__(256) void k_level1_split(int ncx, int ncy, int ncz, int nfx, int nfy, AxisNest ax,
+                                                      AxisNest ay, AxisNest az, const double* __restrict__ lam,
+                                                      const double* __restrict__ mu, const double* __restrict__ cw,
+                                                      double* __restrict__ cl, double* __restrict__ cm,
+                                                      uint8_t* __restrict__ het) {
+  const long long id = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nc = (long long)ncx * ncy * ncz;
+  if (id >= nc) return;
+  const int I = id % ncx, J = (id / ncx) % ncy, K = id / ((long long)ncx * ncy);
+  auto halves = [](const AxisNest& a, int C) {
+    if (a.cnum[C] != 2) return true;
+    const double a0 = a.a[a.cs[C]], a1 = a.a[a.cs[C] + 1];
+    return fabs(a0 - a1) <= 1e-12 * (a0 + a1);
+  };
+  bool same = halves(ax, I) && halves(ay, J) && halves(az, K);
+  double l0 = 0.0, m0 = 0.0;
+  bool first = true;
+  for (int dk = 0; dk < az.cnum[K]; ++dk)
+    for (int dj = 0; dj < ay.cnum[J]; ++dj)
+      for (int di = 0; di < ax.cnum[I]; ++di) {
+        const long long f = (ax.cs[I] + di) + (long long)nfx * ((ay.cs[J] + dj) + (long long)nfy * (az.cs[K] + dk));
+        if (first) {
+          l0 = lam[f];
+          m0 = mu[f];
+          first = false;
+        } else {
+          same = same && lam[f] == l0 && mu[f] == m0;
+        }
+      }
+  double sl = 0.0, sm = 0.0;
+  for (int q = 0; q < 8; ++q) {
+    sl += cw[q * nc + id];
+    sm += cw[(8 + q) * nc + id];
+  }
+  const bool stiff = sl != 0.0 || sm != 0.0;
+  cl[id] = (stiff && same) ? sl : 0.0;
+  cm[id] = (stiff && same) ? sm : 0.0;
+  het[id] = (stiff && !same) ? 1 : 0;
+}
+
 // Level l+1 moments from level-l moments (shift of the child moments into the parent).
 __global__ __launch_bounds__(256) void k_moments_from_moments(int ncx, int ncy, int ncz, int nfx, int nfy, int nfz, AxisNest ax, AxisNest ay,
                                        AxisNest az, const double* __restrict__ momf, double* __restrict__ mom) {

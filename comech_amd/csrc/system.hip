@@ -50,6 +50,8 @@ struct LevelT {
   DevBuf<T> irrL, irrM;     // level 0 irregular cells
   DevBuf<T> mom;            // levels >= 2: [54][ncell]
   DevBuf<T> cw, ph[3];      // level 1: child weights [16][ncell], per-axis child factor tables
+  DevBuf<T> cl1, cm1, q1[3];  // level 1 split: lam V, mu V of one-material cells, lam second moments
+  DevBuf<T> hf;             // level 1 split: forces of the several-material cells [cell][corner][3]
   DevBuf<T> Dinv;           // [nn][6] symmetric 3x3 block inverses
   DevBuf<T> tc;             // sparse-pass accumulator, 3 per touched node (kept zero between uses)
   DevBuf<T> bval;           // coarse levels: block values (3x3 row-major per block entry)
@@ -109,6 +111,14 @@ struct Level {
   std::vector<double> bval_h;
   // level 1: lam V, mu V of the level-0 children of every cell, child factor tables per axis
   DevBuf<double> cw64;
+  // level 1 split: lam V, mu V of the cells whose children share one material (0 elsewhere), the
+  // cells with several materials (x-fastest ids), per-axis lam second moments per coarse index
+  DevBuf<double> cl64, cm64;
+  DevBuf<int> het;
+  int nhet = 0;
+  // per node the row of its corner slots (8 * cell + corner, in slot order) or -1
+  DevBuf<int> hmap, hptr, hslot;
+  std::vector<double> q_h[3];
   std::vector<double> ph_h[3];
   long long nblk = 0, nrn = 0;  // block-row entries and rank-one list entries (totals)
   DevBuf<uint8_t> bflag[2];     // per precision: stiffness flags of the structured-march tiles
@@ -965,6 +975,53 @@ struct System {
               c.ph_h[a2][6 * C + 3 * d + 2] = B * B;
             }
         }
+        // split of level 1: cells of one material for the structured march, the others listed for
+        // the heterogeneous-cell pass; per-axis lam second moments of one-material cells
+        c.cl64.alloc(c.ncell);
+        c.cm64.alloc(c.ncell);
+        DevBuf<uint8_t> hetf(c.ncell);
+        hipLaunchKernelGGL(k_level1_split, grid1(c.ncell), dim3(256), 0, stream, c.nx, c.ny, c.nz, f.nx, f.ny, ax, ay,
+                           az, f.matL.p, f.matM.p, c.cw64.p, c.cl64.p, c.cm64.p, hetf.p);
+        CM_LAUNCH_CHECK();
+        std::vector<uint8_t> hf(c.ncell);
+        CM_HIP(hipMemcpyAsync(hf.data(), hetf.p, c.ncell, hipMemcpyDeviceToHost, stream));
+        CM_HIP(hipStreamSynchronize(stream));
+        std::vector<int> hl;
+        for (long long q = 0; q < c.ncell; ++q)
+          if (hf[q]) hl.push_back((int)q);
+        c.nhet = (int)hl.size();
+        c.het.upload(hl, stream);
+        {
+          const long long cxn = c.nx + 1, lay1 = cxn * (c.ny + 1), ncl1 = (long long)c.nx * c.ny;
+          std::vector<std::pair<long long, int>> ns;   // (node, slot)
+          ns.reserve(8 * hl.size());
+          for (int t = 0; t < (int)hl.size(); ++t) {
+            const long long q = hl[t];
+            const long long i = q % c.nx, j = (q / c.nx) % c.ny, k = q / ncl1;
+            for (int l = 0; l < 8; ++l)
+              ns.emplace_back((i + (l & 1)) + cxn * (j + ((l >> 1) & 1)) + lay1 * (k + (l >> 2)), 8 * t + l);
+          }
+          std::sort(ns.begin(), ns.end());
+          std::vector<int> hn, hp{0}, hs;
+          for (size_t q = 0; q < ns.size(); ++q) {
+            if (q == 0 || ns[q].first != ns[q - 1].first) {
+              if (q) hp.push_back((int)hs.size());
+              hn.push_back((int)ns[q].first);
+            }
+            hs.push_back(ns[q].second);
+          }
+          if (!ns.empty()) hp.push_back((int)hs.size());
+          std::vector<int> hm((size_t)c.nn, -1);
+          for (size_t q = 0; q < hn.size(); ++q) hm[hn[q]] = (int)q;
+          c.hmap.upload(hm, stream);
+          c.hptr.upload(hp, stream);
+          c.hslot.upload(hs, stream);
+        }
+        for (int a2 = 0; a2 < 3; ++a2) {
+          c.q_h[a2].assign((size_t)ncoarse[a2], 1.0 / 3.0);
+          if (sri)
+            for (int C = 0; C < ncoarse[a2]; ++C) c.q_h[a2][C] = c.ncn_h[a2][C] == 2 ? 0.25 : 0.0;
+        }
       } else {
         hipLaunchKernelGGL(k_moments_from_moments, grid1(c.ncell), dim3(256), 0, stream, c.nx, c.ny, c.nz, f.nx, f.ny,
                            f.nz, ax, ay, az, f.mom64.p, c.mom64.p);
@@ -1139,8 +1196,16 @@ struct System {
         d.cw.alloc(v.cw64.n);
         hipLaunchKernelGGL((k_cast<T, double>), grid1((long long)v.cw64.n), dim3(256), 0, stream,
                            (long long)v.cw64.n, d.cw.p, v.cw64.p);
+        d.cl1.alloc(v.ncell);
+        d.cm1.alloc(v.ncell);
+        hipLaunchKernelGGL((k_cast<T, double>), grid1(v.ncell), dim3(256), 0, stream, v.ncell, d.cl1.p, v.cl64.p);
+        hipLaunchKernelGGL((k_cast<T, double>), grid1(v.ncell), dim3(256), 0, stream, v.ncell, d.cm1.p, v.cm64.p);
         CM_LAUNCH_CHECK();
-        for (int ax = 0; ax < 3; ++ax) upload_cast(d.ph[ax], v.ph_h[ax], stream);
+        for (int ax = 0; ax < 3; ++ax) {
+          upload_cast(d.ph[ax], v.ph_h[ax], stream);
+          upload_cast(d.q1[ax], v.q_h[ax], stream);
+        }
+        d.hf.alloc(24 * (size_t)v.nhet);
       } else if (v.mom64.n) {
         d.mom.alloc(v.mom64.n);
         hipLaunchKernelGGL((k_cast<T, double>), grid1((long long)v.mom64.n), dim3(256), 0, stream,
@@ -1153,7 +1218,8 @@ struct System {
       if (!v.xval_h.empty()) upload_cast(d.xval, v.xval_h, stream);
       if (!v.r1k_h.empty()) { upload_cast(d.r1w, v.r1w_h, stream); upload_cast(d.r1k, v.r1k_h, stream); }
       if (!v.bval_h.empty()) upload_cast(d.bval, v.bval_h, stream);
-      if (l > 0 && (v.nblk || !v.r1k_h.empty())) {
+      // level 1 always has the accumulator: its heterogeneous cells add into it
+      if (l > 0 && (l == 1 || v.nblk || !v.r1k_h.empty())) {
         d.ex.alloc(3 * (size_t)v.nn);
         hipLaunchKernelGGL(k_fill<T>, grid1(3LL * v.nn), dim3(256), 0, stream, d.ex.p, 3LL * v.nn, T(0));
         CM_LAUNCH_CHECK();
@@ -1241,8 +1307,14 @@ struct System {
     a.nb_term = (int)div_up((long long)a.rw * a.nterm, 256);
     a.r1ptr = v.r1ptr.p; a.r1dof = v.r1dof.p; a.r1w = d.r1w.p; a.r1k = d.r1k.p;
     a.brow = v.brow.p; a.bcol = v.bcol.p; a.bval = d.bval.p;
-    const long long nb = a.nb_term + div_up(a.nblk, 256);
-    go(CoarseExtraL<T>{a, s, d.ex.p}, nb);   // recorded even when empty: batched systems keep one launch order
+    const int nbx = a.nb_term + div_up(a.nblk, 256);
+    // level 1: its several-material cells ride on this launch (the march gathers their forces)
+    HetArgs<T> h{};
+    if (l == 1)
+      h = HetArgs<T>{v.nhet, sri, v.het.p, v.nx, v.ny, v.nz, d.hx.p, d.hy.p, d.hz.p, d.cw.p, d.ph[0].p, d.ph[1].p,
+                     d.ph[2].p, d.hf.p};
+    // recorded even when empty: batched systems keep one launch order
+    go(CoarseExtraL<T>{a, s, d.ex.p, nbx, h}, (long long)nbx + div_up(h.n, 256));
   }
 
   // lanes per rank-one term in the sparse passes: the power of two >= the mean support (<= 64)
@@ -1306,8 +1378,12 @@ struct System {
     Level& v = lv(l);
     LevelT<T>& d = v.dat<T>();
     if (l == 1) {
-      if (sri) f(ChildMat<T, true>{d.cw.p, d.ph[0].p, d.ph[1].p, d.ph[2].p, d.ex.p});
-      else f(ChildMat<T, false>{d.cw.p, d.ph[0].p, d.ph[1].p, d.ph[2].p, d.ex.p});
+      if (sri)
+        f(SplitMat<T, true>{d.cl1.p, d.cm1.p, d.q1[0].p, d.q1[1].p, d.q1[2].p, d.ex.p, v.hmap.p, v.hptr.p, v.hslot.p,
+                            d.hf.p});
+      else
+        f(SplitMat<T, false>{d.cl1.p, d.cm1.p, d.q1[0].p, d.q1[1].p, d.q1[2].p, d.ex.p, v.hmap.p, v.hptr.p,
+                             v.hslot.p, d.hf.p});
       return;
     }
     if (l > 1) {
@@ -2632,12 +2708,12 @@ int cm_sys_level_stats(cm_sys* h, int level, int64_t* st, int cap) {
   need_level(h, level);
   cm::Level& v = h->s.lv(level);
   const System& s = h->s;
-  const int64_t vals[14] = {v.ncell, v.nirr, (int64_t)v.xrow_h.size(), (int64_t)v.xcol_h.size(),
+  const int64_t vals[15] = {v.ncell, v.nirr, (int64_t)v.xrow_h.size(), (int64_t)v.xcol_h.size(),
                             (int64_t)(v.r1ptr_h.empty() ? 0 : v.r1ptr_h.size() - 1), (int64_t)v.r1dof_h.size(),
                             v.nblk, v.blk_max, v.nrn, v.rn_max,
                             s.pick_kz<double>(v, 0), s.pick_kz<double>(v, 1), s.pick_kz<float>(v, 0),
-                            s.pick_kz<float>(v, 1)};
-  for (int q = 0; q < 14 && q < cap; ++q) st[q] = vals[q];
+                            s.pick_kz<float>(v, 1), v.nhet};
+  for (int q = 0; q < 15 && q < cap; ++q) st[q] = vals[q];
   CM_API_END
 }
 

@@ -150,13 +150,14 @@ class VoxelSystem:
 
     def level_stats(self, level):
         """Operator structure of one level: cell, irregular-cell, extra-row and rank-one counts, the
-        coarse per-node list lengths applied inside the structured march, and the z extent of a
-        march block per precision for products (kz_f64, kz_f32) and fused smoothing steps."""
-        st = np.zeros(14, np.int64)
-        check(self.lib.cm_sys_level_stats(self.h, int(level), ptr(st, ctypes.c_longlong), 14))
+        coarse per-node list lengths applied inside the structured march, the z extent of a march
+        block per precision for products (kz_f64, kz_f32) and fused smoothing steps, and (level 1)
+        the cells with several materials, applied by the heterogeneous-cell pass."""
+        st = np.zeros(15, np.int64)
+        check(self.lib.cm_sys_level_stats(self.h, int(level), ptr(st, ctypes.c_longlong), 15))
         keys = ("cells", "irregular", "extra_rows", "extra_nnz", "rank1_terms", "rank1_nnz", "block_entries",
                 "block_row_max", "rank1_node_entries", "rank1_node_max", "kz_f64", "kz_f64_smooth", "kz_f32",
-                "kz_f32_smooth")
+                "kz_f32_smooth", "het_cells")
         return dict(zip(keys, (int(v) for v in st)))
 
     def timings(self):

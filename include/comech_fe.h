@@ -108,11 +108,12 @@ int cm_sys_set_last_coarsening(cm_sys* sys, int axes);
 /* Level table: 6 int64 per level: nx, ny, nz, nodes (incl. copies), compact DOFs, extra rows. */
 int cm_sys_levels(cm_sys* sys, int* nlev, int64_t* desc, int cap);
 
-/* Operator structure of one level (diagnostics; no reference counterpart): up to 14 int64 --
+/* Operator structure of one level (diagnostics; no reference counterpart): up to 15 int64 --
  * cells, irregular cells, extra-stiffness CSR rows and entries, rank-one terms and their DOF
  * entries, coarse block-row entries and the longest block row, rank-one node-list entries and the
  * longest list of one node, then the z extent of a structured-march block for FP64 products, FP64
- * smoothing steps, FP32 products and FP32 smoothing steps. */
+ * smoothing steps, FP32 products and FP32 smoothing steps, then (level 1) the number of cells whose
+ * level-0 children carry several materials (applied by the heterogeneous-cell pass). */
 int cm_sys_level_stats(cm_sys* sys, int level, int64_t* stats, int cap);
 
 /* ---- compute (device pointers) ------------------------------------------------------------- */

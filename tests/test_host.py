@@ -121,6 +121,39 @@ def test_face_form_equals_transform_composition():
 
 
 @pytest.mark.parametrize("sri", [0, 1])
+@pytest.mark.parametrize("split", [(2, 2, 2), (2, 1, 2), (1, 2, 1)])
+def test_homogeneous_level1_cell_operator(sri, split):
+    """A level-1 cell whose level-0 children (equal halves along each split axis, one child along
+    the others) share one material: the constant-material operator with per-axis lam second
+    moments q (1/4 per split axis and 0 otherwise under SRI, 1/3 with exact integration) equals the
+    moment operator of the children's shifted moments, which is the exact Galerkin operator
+    (test_kernel_moments_are_exact_galerkin pins that one against P^T K P)."""
+    lib = _host()
+    h = np.array([1.3, 0.7, 2.1])
+    lam, mu = 2.7, 1.9
+    momL, momM = np.zeros(27), np.zeros(27)
+    V = float(np.prod(h))
+    Vc = V / np.prod(split)
+    for dz in range(split[2]):
+        for dy in range(split[1]):
+            for dx in range(split[0]):
+                d = (dx, dy, dz)
+                a = np.array([1.0 / n for n in split])
+                b = np.array([0.0 if n == 1 else (-0.5 if d[i] == 0 else 0.5) for i, n in enumerate(split)])
+                cl, cm = np.zeros(27), np.zeros(27)
+                lib.cm_host_mom_const(ctypes.c_double(lam), ctypes.c_double(Vc), sri, P(cl))
+                lib.cm_host_mom_const(ctypes.c_double(mu), ctypes.c_double(Vc), 0, P(cm))
+                lib.cm_host_mom_shift_add(P(cl), P(a), P(b), P(momL))
+                lib.cm_host_mom_shift_add(P(cm), P(a), P(b), P(momM))
+    Kmom, Kq = np.zeros((24, 24)), np.zeros((24, 24))
+    lib.cm_host_elem_mom(P(momL), P(momM), P(h), P(Kmom))
+    q = np.array([(0.25 if n == 2 else 0.0) if sri else 1.0 / 3.0 for n in split])
+    lib.cm_host_elem_q(P(h), ctypes.c_double(lam), ctypes.c_double(mu), P(q), P(Kq))
+    assert np.abs(Kq - Kmom).max() <= 1e-13 * np.abs(Kmom).max()
+    assert np.abs(Kq - Kq.T).max() <= 1e-13 * np.abs(Kq).max()
+
+
+@pytest.mark.parametrize("sri", [0, 1])
 def test_kernel_moments_are_exact_galerkin(sri):
     """Coarse cell operator from shifted child moments == P^T K P of its 2x2x2 children with random
     materials on a stretched cell (fe/gmg.py:91-97)."""

@@ -31,8 +31,6 @@ constexpr uint8_t LAB_NONE = 255;
 //   kind 0: level 0, one uint8 label per cell + (lam, mu) table
 //   kind 1: level 0, per-cell (lam, mu) values
 //   kind 2: coarse levels, 27 lam + 27 mu moments per cell
-//   kind 3: level 1, the 8 lam and 8 mu weights of its level-0 child cells (constant materials),
-//           expanded in registers into the 27 + 27 moments
 //   kind 4: level 1 split by cells: cells whose children share one material take the constant-
 //           material operator, the others go through the heterogeneous-cell pass (het_body)
 // ISO = the level-0 grid is made of identical cubes (h_x = h_y = h_z = h everywhere); the common
@@ -69,18 +67,9 @@ struct MomentMat {
 // Level-1 cells are unions of (up to) 2x2x2 level-0 cells of constant material.  Along every
 // axis a child occupies xi in its parent's coordinates as xi = a xi_c + b; the parent's moments
 // int xi^g m dV are then sums over the children of w_c prod_axes phi(g), phi = (1, b, a^2/3 + b^2)
-// (exact integration) or (1, b, b^2) (lam at the child's centre, SRI), w_c = m_c V_c.
-template <typename T, bool SRI_>
-struct ChildMat {
-  const T* __restrict__ cw;          // [16][ncell]: lam weights of children (dx + 2dy + 4dz), then mu
-  const T* __restrict__ phx;         // per coarse index: b0, q0, s0, b1, q1, s1 (q = a^2/3 + b^2, s = b^2)
-  const T* __restrict__ phy;
-  const T* __restrict__ phz;
-  T* ex;
-  static constexpr int kind = 3;
-  static constexpr bool sri = SRI_, iso = false;
-};
-
+// (exact integration) or (1, b, b^2) (lam at the child's centre, SRI), w_c = m_c V_c: the child
+// weights cw ([16][ncell]: lam weights of the children dx + 2dy + 4dz, then mu) and the per-axis
+// factor rows ph (per coarse index: b0, q0, s0, b1, q1, s1 with q = a^2/3 + b^2, s = b^2).
 // A level-1 cell whose existing level-0 children all carry one material and split every axis into
 // equal halves (or not at all) has the Galerkin operator of a constant material whose lam term is
 // integrated at the children's SRI points: elem_mid_q with the per-axis second moments q (1/4 for
@@ -365,7 +354,7 @@ struct V3Lds {
   }
 };
 
-// Extra stiffness of a coarse node (kinds 2, 3, 4): au += ex[n], ex[n] = 0; kind 4 also adds the
+// Extra stiffness of a coarse node (kinds 2, 4): au += ex[n], ex[n] = 0; kind 4 also adds the
 // forces of the node's corners of several-material cells.
 template <typename T, class M>
 __device__ __forceinline__ void add_extra(const M& mat, long long n, T au[3]) {
@@ -656,7 +645,6 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
   static_assert(TILE == 16, "one tile row per 16-lane DPP row");
   __shared__ __align__(16) T s_g[VL::size(BLOCK)];
   __shared__ T s_tL[M::kind == 0 ? 256 : 1], s_tM[M::kind == 0 ? 256 : 1];
-  __shared__ T s_ph[M::kind == 3 ? 2 * TILE * 6 : 1];
   const int tid = threadIdx.x;
   const int tx = tid % TILE, ty = tid / TILE;
   int bx, by, bz;
@@ -725,14 +713,6 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
       qyv = mat.qy[cj];
     }
   }
-  // kind 3: child factor tables of the tile's cell columns and rows (read per step from LDS)
-  if constexpr (M::kind == 3) {
-    if (tid < 2 * TILE * 6) {
-      const int ax = tid / (TILE * 6), r = tid % (TILE * 6), c = (ax ? J0 : I0) - 1 + r / 6;
-      const int nc = ax ? ny : nx;
-      s_ph[tid] = (c >= 0 && c < nc) ? (ax ? mat.phy : mat.phx)[6 * c + r % 6] : T(0);
-    }
-  }
   // staging plan (fixed for all layers): scalar e = tid + m*BLOCK of a layer is node row r,
   // entry q of the row's 3*TN contiguous scalars.  A slot outside the grid reads 0 (offset OOB);
   // a slot past the layer (e >= NST, only in the last round) writes the unused 4th value of a
@@ -773,8 +753,8 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
     for (int ab = 0; ab < 4; ++ab) VL::get(src, NL, (ty + (ab >> 1)) * TN + tx + (ab & 1), v[ab]);
     face_fwd(v, f);
   };
-  // material of cell layer kk, loaded one step ahead (kinds 0, 1, 3 and 4); kinds 0, 1 and 4 read
-  // 0 for cells outside the grid
+  // material of cell layer kk, loaded one step ahead (kinds 0, 1 and 4), 0 for cells outside the
+  // grid
   const unsigned coff = cin ? (unsigned)cxy : OOB;                   // byte offset of a label
   const unsigned coffT = cin ? (unsigned)(cxy * sizeof(T)) : OOB;     // of a T value
   const long long clb = ncl * (long long)sizeof(T);                    // one cell layer of T values
@@ -783,14 +763,8 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
   uint8_t labn[MM];
   bool labok[MM];       // block-uniform: labn belongs to a cell layer of the grid
   T lamn[MM], mun[MM];
-  T cwn[M::kind == 3 ? 16 : 1];
   auto mload = [&](int kk, int set) {
     const bool okz = kk >= 0 && kk < nz;
-    if constexpr (M::kind == 3) {
-      const long long ncell = ncl * nz;
-#pragma unroll
-      for (int q = 0; q < 16; ++q) bld(layer_rs(mat.cw, (q * ncell + kk * ncl) * (long long)sizeof(T), clb, okz), coffT, cwn[q]);
-    }
     if constexpr (M::kind == 0) {
       // a cell outside the grid or an empty layer reads 0, a real label: the step that uses labn
       // replaces it by LAB_NONE there (a select at the load would wait for it)
@@ -850,11 +824,6 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
     } else {
       act = cin && k >= 0 && k < nz;
     }
-    T cw[M::kind == 3 ? 16 : 1];
-    if constexpr (M::kind == 3) {
-#pragma unroll
-      for (int q = 0; q < 16; ++q) cw[q] = cwn[q];
-    }
     lstore(BHI, SET);         // node layer k+1, loaded MD steps ago
     if constexpr (M::kind != 2) mload(k + MM, MSET);
     gload(k + 1 + MD, SET);   // past kend: an empty descriptor (no traffic)
@@ -897,41 +866,17 @@ __device__ __forceinline__ void march_body(const int bid, GridArgs<T> g, M mat, 
         T c3[3][8];
         faces_to_cell(ulo, uhi, c3);
         T F3[3][8];
-        bool done = true;
-        if constexpr (M::kind == 2) {
-          const long long ncell = ncl * nz;
-          const long long cid = cxy + ncl * k;
-          T mL[27], mM[27];
+        // kind 2: moments of the cell (zero for a cell without stiffness)
+        const long long ncell = ncl * nz;
+        const long long cid = cxy + ncl * k;
+        T mL[27], mM[27];
 #pragma unroll
-          for (int q = 0; q < 27; ++q) {
-            mL[q] = mat.mom[q * ncell + cid];
-            mM[q] = mat.mom[(27 + q) * ncell + cid];
-          }
-          done = (mL[0] != T(0)) || (mM[0] != T(0));
-          if (done) elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
-        } else {
-          T ws = T(0);
-#pragma unroll
-          for (int q = 0; q < 16; ++q) ws += cw[q];
-          done = (ws != T(0));
-          if (done) {
-            T fxL[2][2], fxM[2][2], fyL[2][2], fyM[2][2], fzL[2][2], fzM[2][2];
-            child_factors<T, M::sri>(s_ph + 6 * tx, fxL, fxM);
-            child_factors<T, M::sri>(s_ph + TILE * 6 + 6 * ty, fyL, fyM);
-            child_factors<T, M::sri>(mat.phz + 6 * k, fzL, fzM);
-            const T gz = T(0.25) / g.hz[k];
-            {
-              T mL[27];
-              child_moments<T>(cw, fxL, fyL, fzL, mL);
-              elem_mid_mom_lam<T>(c3, F3, mL, gx, gy, gz);
-            }
-            {
-              T mM[27];
-              child_moments<T>(cw + 8, fxM, fyM, fzM, mM);
-              elem_mid_mom_mu<T>(c3, F3, mM, gx, gy, gz);
-            }
-          }
+        for (int q = 0; q < 27; ++q) {
+          mL[q] = mat.mom[q * ncell + cid];
+          mM[q] = mat.mom[(27 + q) * ncell + cid];
         }
+        const bool done = (mL[0] != T(0)) || (mM[0] != T(0));
+        if (done) elem_mid_mom<T>(c3, F3, mL, mM, gx, gy, T(0.25) / g.hz[k]);
         if (done) cell_to_faces(F3, Flo, carn);
       }
     }
@@ -1089,9 +1034,6 @@ __global__ __launch_bounds__(BLOCK) void k_block_flags(GridArgs<T> g, M mat, int
       if constexpr (M::kind == 0) any = mat.lab[cid] != LAB_NONE;
       if constexpr (M::kind == 1) any = mat.cl[cid] != T(0) || mat.cm[cid] != T(0);
       if constexpr (M::kind == 2) any = mat.mom[cid] != T(0) || mat.mom[27 * ncell + cid] != T(0);
-      if constexpr (M::kind == 3) {
-        for (int q = 0; q < 16; ++q) any |= mat.cw[q * ncell + cid] != T(0);
-      }
       if constexpr (M::kind == 4) any = mat.cl[cid] != T(0) || mat.cm[cid] != T(0);
     }
   }
@@ -1166,14 +1108,22 @@ struct CoarseExtraL {
 // blocks in Dinv are zero, so the march and the copies' step below leave d = c1 sn and
 // x += xa sn + d; the group thread completes every member's residual r (ep.y, always kept when
 // groups exist), then adds c2 Ginv r_group to d and x.
-template <typename T>
-__device__ void group_fixup(const CopyGroups<T>& ga, int g, long long nng, T* __restrict__ tc, const T* __restrict__ s,
-                            const Epi<T>& ep) {
-  const int b = ga.gptr[g], m = ga.gptr[g + 1] - b, N = 3 * m;
-  T rg[3 * GROUP_MAX];
-  for (int j = 0; j < m; ++j) {
+// MC > 0: a group of exactly MC members, every loop unrolled (the residual stays in registers and
+// the rows of Ginv load together); MC = 0: any size up to GROUP_MAX.
+template <typename T, int MC>
+__device__ __forceinline__ void group_fixup_m(const CopyGroups<T>& ga, int g, long long nng, T* __restrict__ tc,
+                                              const T* __restrict__ s, const Epi<T>& ep) {
+  const int b = ga.gptr[g], m = MC > 0 ? MC : ga.gptr[g + 1] - b, N = 3 * m;
+  // every index of rg is a compile-time constant (registers, no scratch): loops run to the bound
+  // and stop at m
+  constexpr int MB = MC > 0 ? MC : GROUP_MAX;
+  T rg[3 * MB];
+#pragma unroll
+  for (int j = 0; j < MB; ++j) {
+    if (MC == 0 && j >= m) break;
     const int slot = ga.gslot[b + j];
     const long long n = ga.tnode[slot];
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
       const T t = tc[3 * slot + q];
       tc[3 * slot + q] = T(0);
@@ -1192,17 +1142,32 @@ __device__ void group_fixup(const CopyGroups<T>& ga, int g, long long nng, T* __
     }
   }
   const T* G = ga.ginv + ga.goff[g];
-  for (int j = 0; j < m; ++j) {
+#pragma unroll
+  for (int j = 0; j < MB; ++j) {
+    if (MC == 0 && j >= m) break;
     const long long n = ga.tnode[ga.gslot[b + j]];
+#pragma unroll
     for (int q = 0; q < 3; ++q) {
       T acc = T(0);
       const T* row = G + (size_t)(3 * j + q) * N;
-      for (int k = 0; k < N; ++k) acc += row[k] * rg[k];
+#pragma unroll
+      for (int k = 0; k < 3 * MB; ++k) {
+        if (MC == 0 && k >= N) break;
+        acc += row[k] * rg[k];
+      }
       const T dz = ep.c2 * acc;
       if (ep.dout) ep.dout[3 * n + q] += dz;
       if (ep.x) ep.x[3 * n + q] += dz;
     }
   }
+}
+template <typename T>
+__device__ void group_fixup(const CopyGroups<T>& ga, int g, long long nng, T* __restrict__ tc, const T* __restrict__ s,
+                            const Epi<T>& ep) {
+  const int m = ga.gptr[g + 1] - ga.gptr[g];
+  if (m == 2) group_fixup_m<T, 2>(ga, g, nng, tc, s, ep);
+  else if (m == 3) group_fixup_m<T, 3>(ga, g, nng, tc, s, ep);
+  else group_fixup_m<T, 0>(ga, g, nng, tc, s, ep);
 }
 
 template <typename T, int EM>

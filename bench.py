@@ -544,6 +544,35 @@ def kernel_workload(args):
     print(json.dumps({"workload": "fine-level kernels, 384x256x256", **rl}), flush=True)
 
 
+def solves_workload(args):
+    """For rocprofv3 kernel traces of the solve mix: the batched ref leg alone on ``--steps`` feet of
+    the set with ``--concurrency`` systems in flight, after one warm-up foot; ``--eager`` launches
+    the V-cycles without HIP graphs (every launch visible to the trace).  Inputs are built in this
+    process (no host workers).  Prints one JSON line (not the headline metric)."""
+    import torch
+    from comech_amd import synth
+    import comech_amd.batch as CB
+    feet = timed_feet(args.steps)
+    warm = [f for f in reversed(range(N_FEET)) if f not in feet][:1]
+    descs = {f: synth.batch_foot(f) for f in feet + warm}
+    if args.eager:
+        build = CB.build_system
+
+        def build_eager(*a, **k):
+            s = build(*a, **k)
+            s.set_graphs(False)
+            return s
+        CB.build_system = build_eager
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    bs = {f: [CB.rhs(d, dev) for d in descs[f]] for f in descs}
+    log(f"inputs of {len(descs)} feet ready")
+    batch_leg(warm, descs, bs, "ref", args.tol, args.concurrency, dev, mg=TUNED)
+    infos, _, wall = batch_leg(feet, descs, bs, "ref", args.tol, args.concurrency, dev, mg=TUNED)
+    print(json.dumps({"workload": f"batched set, feet {feet}, mode ref, {args.concurrency} in flight"
+                                  + (", eager V-cycles" if args.eager else ""), **summarize(infos, wall)}), flush=True)
+
+
 def large_workload(args):
     """Large single solve (the fifth configuration): 384x256x256 voxels at 0.25 mm, 4-voxel cortical
     shells, one solve per precision form.  Prints one JSON line (not the headline metric): fine
@@ -799,10 +828,12 @@ def main():
     ap.add_argument("--concurrency", type=int, default=16, help="systems in flight on the GPU")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="batch", choices=["batch", "kernels", "large", "prep"],
+    ap.add_argument("--workload", default="batch", choices=["batch", "kernels", "solves", "large", "prep"],
                     help="batch: the headline (uneven-grid set); kernels: the fine-level kernels of the "
-                         "384x256x256 grid alone, for rocprofv3; large: the 384x256x256 solve in every mode; "
-                         "prep: engine preparation and resampling (separate JSON lines, not the headline)")
+                         "384x256x256 grid alone, for rocprofv3; solves: the batched ref leg alone, for "
+                         "rocprofv3; large: the 384x256x256 solve in every mode; prep: engine preparation "
+                         "and resampling (separate JSON lines, not the headline)")
+    ap.add_argument("--eager", action="store_true", help="solves: V-cycles launched without HIP graphs")
     ap.add_argument("--nrep", type=int, default=20, help="kernels: launches timed per kernel")
     ap.add_argument("--retune", action="store_true",
                     help="kernels: rerun the autotuner on the 384x256x256 grid and rewrite its table in "
@@ -813,7 +844,8 @@ def main():
         print("bench.py: this build measures one GPU only; every solve is independent (batched solves "
               "shard by solve across GPUs), so run it with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    {"batch": batch_workload, "kernels": kernel_workload, "large": large_workload, "prep": prep_workload}[
+    {"batch": batch_workload, "kernels": kernel_workload, "solves": solves_workload, "large": large_workload,
+     "prep": prep_workload}[
         args.workload](args)
 
 

@@ -594,7 +594,7 @@ def large_workload(args):
     f_dev = torch.from_numpy(f).to(dev)
     s = VoxelSystem(g, sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"], mu_tab=c["mu_tab"],
                     copy_orig=c["orig"][g.nnode:], irr_cells=irr_c, irr_nodes=irr_n, extra=extra, rank1=terms,
-                    device=dev, levels=args.levels)
+                    device=dev, levels=args.levels, last_axes=args.last_axes)
     table = tuning_table(s)                 # the autotuner's z extents for this grid (or tune now:
     if table is None:                        # one large solve has the GPU to itself)
         s.autotune(("f32", "f64"), nrep=5)
@@ -602,7 +602,7 @@ def large_workload(args):
         s.set_tuning(table)
     lv = s.levels()
     out = {"workload": "hindfoot raster 384x256x256 @0.25mm (4-voxel cortical shells), rtol 1e-8, FP64 true residual",
-           "ncell": g.ncell, "dofs": lv[0]["ndof"], "levels": [l["ndof"] for l in lv],
+           "ncell": g.ncell, "dofs": lv[0]["ndof"], "levels": [l["ndof"] for l in lv], "last_axes": args.last_axes,
            "t_setup_ms": round(s.timings()["total"], 1)}
     for dt, name, sz in ((torch.float32, "f32", 4), (torch.float64, "f64", 8)):
         u = torch.randn(3 * s.nn, dtype=dt, device=dev)
@@ -839,6 +839,8 @@ def main():
                     help="kernels: rerun the autotuner on the 384x256x256 grid and rewrite its table in "
                          "comech_amd/tuning.json before timing")
     ap.add_argument("--levels", type=int, default=None, help="large: force the number of multigrid levels")
+    ap.add_argument("--last-axes", type=int, default=3,
+                    help="large: axes the step to the coarsest level coarsens (x 1, y 2, z 4; 7 = fe/gmg.py's rule)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only; every solve is independent (batched solves "

@@ -235,18 +235,30 @@ def test_config5_anatomy_small_against_oracle():
         check_cn(c, ans, 1e-9, f"96x64x64 {mode}", mode=mode)
 
 
-def test_config5_full_size_properties():
-    """384x256x256 voxels at 0.25 mm (25.2 M cells, ~40 M DOFs), 4-voxel cortical shells: every
-    mode reaches an FP64 true residual <= 1e-8, and the mixed forms' von Mises fields equal the
-    plain FP64 answer's to 1e-5 (relative L2), their displacements to 1e-6."""
+@pytest.fixture(scope="module")
+def large_inputs():
     from comech_amd.engine import split_cells
-    from comech_amd.system import VoxelSystem, cell_stress_gpu
     c = synth.hindfoot_large()
     extra, terms, f = synth.solve_inputs_hindfoot(c)
     irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
+    return c, extra, terms, f, irr_c, irr_n
+
+
+@pytest.mark.parametrize("levels,last_axes", [(None, 3), (5, 7)], ids=["level-rule", "five-levels-reference"])
+def test_config5_full_size_properties(large_inputs, levels, last_axes):
+    """384x256x256 voxels at 0.25 mm (25.2 M cells, ~40 M DOFs), 4-voxel cortical shells: every
+    mode reaches an FP64 true residual <= 1e-8, and the mixed forms' von Mises fields equal the
+    plain FP64 answer's to 1e-5 (relative L2), their displacements to 1e-6.  Hierarchies: the level
+    rule (6 levels, the last step coarsening x and y) and the configuration's 5 levels with every
+    axis coarsened (fe/gmg.py:80-90), whose coarsest level (about 14k DOFs) takes the dense
+    inverse."""
+    from comech_amd.system import VoxelSystem, cell_stress_gpu
+    c, extra, terms, f, irr_c, irr_n = large_inputs
     s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
                     mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c, irr_nodes=irr_n,
-                    extra=extra, rank1=terms, device=DEV)
+                    extra=extra, rank1=terms, device=DEV, levels=levels, last_axes=last_axes)
+    if levels is not None:
+        assert len(s.levels()) == levels
     b = torch.from_numpy(f).to(DEV)
     bn = torch.linalg.vector_norm(b)
     cells = np.flatnonzero(c["active"])
@@ -255,7 +267,7 @@ def test_config5_full_size_properties():
     for mode in ("fp64", "ref", "mixed"):
         x, info = s.solve(b, tol=1e-8, maxiter=3000, mode=mode)
         true = float(torch.linalg.vector_norm(b - s.apply(x)) / bn)
-        print(f"384x256x256 {mode}: iters {info['iters']} true residual {true:.2e} "
+        print(f"384x256x256 {mode} ({len(s.levels())} levels): iters {info['iters']} true residual {true:.2e} "
               f"t_solve {info['t_solve_ms']:.0f} ms", flush=True)
         assert info["flag"] == 0 and true <= 1e-8 * (1 + 1e-9), (mode, info, true)
         out[mode] = x

@@ -1911,6 +1911,16 @@ struct System {
   int fb_iters = 0;
   double fb_drop = 10.0;
 
+  // the answer to b = 0: x = 0, converged without an iteration
+  void zero_result(double* x, long long n, cm_solve_info* info) {
+    hipLaunchKernelGGL(k_fill<double>, grid1(n), dim3(256), 0, stream, x, n, 0.0);
+    CM_LAUNCH_CHECK();
+    info->iters = 0;
+    info->flag = 0;
+    info->outer = 0;
+    info->relres = 0.0;
+  }
+
   // FP64 PCG (rb/gpusolve.py:229-254), V-cycle in FP64 (vprec 0) or FP32 (vprec 1)
   void pcg64(int vprec, const double* b, double* x, double tol, int maxiter, int check_every, cm_solve_info* info) {
     Level& v0 = lv(0);
@@ -1927,7 +1937,11 @@ struct System {
     apply<double>(0, x, r64.p, MODE_RESID, b);
     dot(n, b, b, S_BB);
     const double bn = std::sqrt(read_slot(S_BB));
-    const double thr = tol * (bn > 0 ? bn : 1.0);
+    if (bn == 0.0) {   // b = 0: x = 0 whatever x0 is (scipy's cg returns b itself)
+      zero_result(x, n, info);
+      return;
+    }
+    const double thr = tol * bn;
     precondition64(vprec, r64.p, z64.p);
     CM_HIP(hipMemcpyAsync(p64.p, z64.p, n * sizeof(double), hipMemcpyDeviceToDevice, stream));
     int cur = S_RZ0, nxt = S_RZ1;
@@ -2047,7 +2061,11 @@ struct System {
     ensure_data<double>();
     dot(n, b, b, S_BB);
     const double bn = std::sqrt(read_slot(S_BB));
-    const double thr = tol * (bn > 0 ? bn : 1.0);
+    if (bn == 0.0) {
+      zero_result(x, n, info);
+      return;
+    }
+    const double thr = tol * bn;
     int it = 0, outer = 0, flag = 1, stalls = 0;
     double rn = 0.0, last_true = 1e300;
     while (true) {

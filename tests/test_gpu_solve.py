@@ -237,6 +237,31 @@ def test_stiff_joint_convergence_is_reported_truthfully(tol, mode):
         assert true > tol and info["iters"] < 3000
 
 
+@pytest.mark.parametrize("mode", ["fp64", "ref", "mixed"])
+def test_zero_rhs_returns_zero(mode):
+    """b = 0: the answer is x = 0 without an iteration, whatever the initial guess (scipy's cg,
+    which fe/gmg.py:136 calls, returns b itself when ||b|| = 0)."""
+    from comech_amd.engine import split_cells
+    from comech_amd.system import VoxelSystem
+    c = synth.hindfoot_raster(dims=(24, 16, 16), h=3.0)
+    extra, terms, f = synth.solve_inputs_hindfoot(c)
+    irr_c, irr_n = split_cells(c["grid"], c["active"], c["cn"])
+    s = VoxelSystem(c["grid"], sri=True, active=c["active"], labels=c["labels"], lam_tab=c["lam_tab"],
+                    mu_tab=c["mu_tab"], copy_orig=c["orig"][c["grid"].nnode:], irr_cells=irr_c, irr_nodes=irr_n,
+                    extra=extra, rank1=terms, device=DEV)
+    b = torch.zeros(3 * s.nn, dtype=torch.float64, device=DEV)
+    for x0 in (None, torch.randn(3 * s.nn, dtype=torch.float64, device=DEV)):
+        x, info = s.solve(b, x0=x0, tol=1e-8, mode=mode)
+        assert info["flag"] == 0 and info["iters"] == 0 and info["relres"] == 0.0, info
+        assert int(torch.count_nonzero(x)) == 0
+    # a nonzero load afterwards still solves (the early return leaves no state behind)
+    bf = torch.from_numpy(f).to(DEV)
+    x, info = s.solve(bf, tol=1e-8, mode=mode)
+    true = float(torch.linalg.vector_norm(bf - s.apply(x)) / torch.linalg.vector_norm(bf))
+    s.close()
+    assert info["flag"] == 0 and info["iters"] > 0 and true <= 1e-8 * (1 + 1e-9)
+
+
 @pytest.mark.gpu
 def test_copy_group_blocks_same_solution_fewer_iterations():
     """Fusion (interface springs k = 1e6 between every split copy and its grid node): smoothing

@@ -2838,7 +2838,7 @@ int cm_sys_solve(cm_sys* h, int mode, const double* b, double* x, double tol, in
   CM_CHECK(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
   CM_CHECK(info != nullptr, "info is null");
   System& s = h->s;
-  if (check_every < 1) check_every = 5;
+  if (check_every < 1) check_every = 1;
   OwnStream os(s);
   hipEvent_t e0, e1;
   CM_HIP(hipEventCreate(&e0)); CM_HIP(hipEventCreate(&e1));

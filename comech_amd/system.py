@@ -194,7 +194,7 @@ class VoxelSystem:
         check(self.lib.cm_sys_vcycle(self.h, prec, ctypes.c_void_p(b.data_ptr()), ctypes.c_void_p(x.data_ptr())))
         return x
 
-    def solve(self, b, x0=None, tol=1e-8, maxiter=800, mode="fp64", check_every=5, inner_tol=1e-3):
+    def solve(self, b, x0=None, tol=1e-8, maxiter=800, mode="fp64", check_every=1, inner_tol=1e-3):
         """PCG on A x = b (FP64 device vectors, level-0 layout).  Returns (x, info)."""
         n = 3 * self.nn
         self._ck(b, torch.float64, n)

@@ -158,7 +158,7 @@ int cm_sys_smooth(cm_sys* sys, int level, int prec, const void* b, void* x, int 
  * PCG (FP32 operator, V-cycle and vectors, FP64 alpha/beta reductions, FP64 coarse solve) to
  * relative tolerance inner_tol, x += d in FP64; convergence is decided on the FP64 residual.
  * The residual is checked at iterations < check_every and at multiples of check_every
- * (rb/gpusolve.py:244).  Modes 0 and 1 confirm a converged recurrence with the FP64 true
+ * (rb/gpusolve.py:244 uses 5); check_every < 1 checks every iteration.  Modes 0 and 1 confirm a converged recurrence with the FP64 true
  * residual and, if that is still above tol (stiff systems), restart the recurrence from it
  * (residual replacement) in flexible (Polak-Ribiere) form; the reference stops on the recurrence
  * alone.  info->outer counts

@@ -409,8 +409,8 @@ def batch_workload(args):
             "what": "the reference's plain method in FP64 on the GPU (its level rule: 6,000 DOFs or 6 levels, every "
                     "axis coarsened; per-node 3x3 block Jacobi; Chebyshev degree 3), batched like the headline"}),
         "plain_fp64_sequential": summarize(pseq, w_pseq, {
-            "what": f"the same plain method one solve at a time (as the reference runs it, one process per GPU), "
-                    f"feet {sub} x 8 actions",
+            "what": f"the same plain method one solve at a time (as the reference runs it, one process per GPU; the "
+                    f"residual checked every iteration, as in every leg), feet {sub} x 8 actions",
             "ref_batched_same_feet": round(len(rsub) / w_rsub, 3),
             "ratio_ref_batched_over_plain_sequential": round((len(rsub) / w_rsub) / (len(pseq) / w_pseq), 3)}),
         "ratio_ref_over_plain": round(w_plain / w_ref, 3),
